@@ -1,0 +1,111 @@
+/*
+ * solveit_hip.h -- C ABI of the MI355X force engine (libsolveit_hip.so).
+ *
+ * Plain C: opaque handles, POD structs, raw pointers and sizes, int status
+ * codes (0 = ok, <0 = error; shp_last_error() returns the message of the last
+ * failing call on the calling thread).
+ *
+ * Two families of entry points:
+ *   - *_host: take the reference's host AoS arrays (vec4<float> position
+ *     (x,y,z,q), velocity (vx,vy,vz,species), acceleration (ax,ay,az,status),
+ *     particles.h:99-101) and return when the result is in host memory.  These
+ *     are what a C++ ForceCalculationMethod/Integrator subclass calls.
+ *   - *_dev: take device pointers to the engine's FP64 SoA state and are
+ *     stream-ordered on the handle's stream (shp_sync() waits).
+ *
+ * Status values follow structures.h:206-211: 1 inbound (target and source),
+ * 2 affects-acceleration (source only), 0 out of bounds, -1 crashed.
+ * Species codes follow structures.h:224-233 (0 monomer+, 1 dimer+, 5 BF4-, ...).
+ */
+#ifndef SOLVEIT_HIP_H
+#define SOLVEIT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHP_OK 0
+#define SHP_ERR_INVALID (-1)   /* bad argument */
+#define SHP_ERR_HIP (-2)       /* HIP runtime error */
+#define SHP_ERR_NOMEM (-3)     /* device allocation failed */
+#define SHP_ERR_UNSUPPORTED (-4)
+
+/* Arithmetic contract of a force evaluation. */
+#define SHP_PREC_REF 0   /* reference contract: float displacements, double sums */
+#define SHP_PREC_FP64 1  /* FP64 state, FP64 pair arithmetic */
+#define SHP_PREC_FP32C 2 /* FP32 pair arithmetic, compensated per-target sums */
+
+/* Symmetry (SymmetryType, structures.h:201-205). */
+#define SHP_SYM_NONE 0
+#define SHP_SYM_ROTATIONAL 1    /* nsec sectors about z (reference: 4) */
+#define SHP_SYM_TRANSLATIONAL 2 /* explicit lattice images */
+
+typedef struct shp_context* shp_handle;
+
+typedef struct shp_symmetry {
+    int type;              /* SHP_SYM_* */
+    int nsec;              /* rotational: number of sectors (>= 1) */
+    int nimg;              /* translational: number of lattice points */
+    const int* lattice;    /* host, 3*nimg ints: image = lattice * domain */
+    double domain[3];      /* translational cell size (m) */
+} shp_symmetry;
+
+/* Device-resident FP64 particle state (SoA, length n). */
+typedef struct shp_particles {
+    int n;
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* q;       /* charge in units of e */
+    const int* status;     /* ParticleStatus */
+    const int* species;    /* ParticleSpecies -> mass */
+} shp_particles;
+
+/* ---- lifetime ---- */
+int shp_version(void);
+const char* shp_last_error(void);
+int shp_create(int device, shp_handle* out);
+int shp_destroy(shp_handle h);
+int shp_sync(shp_handle h);
+/* hipStream_t of the handle (as void*); all *_dev calls are ordered on it. */
+void* shp_stream(shp_handle h);
+
+/* ---- constants of the reference (constants.h:28-47, particles.h:67-87) ---- */
+double shp_k_const(void);          /* K_CONST */
+double shp_softening(void);        /* SOFTENING (eps^2, m^2) */
+double shp_species_mass(int species);
+
+/*
+ * Direct softened Coulomb sum, reference contract, host AoS in/out.
+ * Replaces DirectForceFunction::direct / direct_rotational / direct_translational
+ * as driven by DirectForceCalculation::updateAcceleration
+ * (direct_force_method.cpp:153-190, :347-428; direct_force_method_rotational.cpp:28-94;
+ * direct_force_method_translational.cpp:28-367).
+ * Targets [i0,i1) with status 1; sources [j0,j1) with status 1 or 2.
+ * out3[3*(i-i0)+c] receives the double acceleration before the reference's
+ * rounding into float (0 for non-targets).  precision: SHP_PREC_REF or
+ * SHP_PREC_FP32C (the latter on the same float inputs).
+ */
+int shp_direct_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4,
+                    int n, int i0, int i1, int j0, int j1, int precision,
+                    const shp_symmetry* sym, double* out3);
+
+/*
+ * Direct sum on the device-resident FP64 state: every status-1 particle is a
+ * target, every status-1/2 particle a source.  ax/ay/az (device, length n)
+ * receive K*q_i/m_i * sum_j q_j (r_i - r_j)/(|r_i - r_j|^2 + eps)^1.5, zero for
+ * non-targets.  precision: SHP_PREC_FP64 or SHP_PREC_FP32C.
+ */
+int shp_direct_dev(shp_handle h, const shp_particles* p, int precision,
+                   const shp_symmetry* sym, double* ax, double* ay, double* az);
+
+/* Kernel time of the last shp_direct_dev/shp_direct_host force kernel (ms),
+ * measured with HIP events on the handle's stream. */
+double shp_last_kernel_ms(shp_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOLVEIT_HIP_H */

@@ -1,0 +1,93 @@
+// Internal definitions shared by the engine's HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/solveit_hip.h"
+
+namespace shp {
+
+// Physical constants of the reference (constants.h:28-47).  K is evaluated on
+// the host exactly as the reference does (float e*e over a long double
+// denominator) and handed to kernels as a double argument.
+double k_const_host();
+constexpr double kSoftening = 6.4e-19;  // eps^2 in m^2
+
+// Mass in amu per ParticleSpecies code (particles.h:67-87, with droplet -> 0
+// as ChargedParticles::getMass returns, particles.cpp:1081-1089).
+__host__ __device__ inline double species_mass(int s) {
+    switch (s) {
+    case 0: return 111.0;
+    case 1: return 309.0;
+    case 2: return 507.0;
+    case 4: return 198.0;
+    case 5: return 87.0;
+    case 6: return 285.0;
+    case 7: return 483.0;
+    default: return 0.0;
+    }
+}
+
+void set_error(const char* fmt, ...);
+
+// Grow-only device workspace slots owned by a context.
+enum WsSlot {
+    WS_SRC = 0,     // staged sources
+    WS_TGT,         // target index list
+    WS_KEYS,        // sort keys (in)
+    WS_KEYS2,       // sort keys (out)
+    WS_VALS2,       // sort values (out)
+    WS_PARTIAL,     // split-j partial sums
+    WS_COUNTS,      // small device counters
+    WS_CUB,         // hipcub temporary storage
+    WS_IMAGES,      // per-image transforms
+    WS_BBOX,        // bounding-box partials
+    WS_FLAGS,       // selection flags
+    WS_HOST_IN,     // device copy of host inputs
+    WS_HOST_OUT,    // device staging of host outputs
+    WS_HOST_IN2,
+    WS_HOST_IN3,
+    WS_WIDE_F64,    // float AoS widened to FP64 SoA
+    WS_WIDE_I32,
+    WS_BOX,         // bounding box (6 doubles)
+    WS_NSLOTS
+};
+
+}  // namespace shp
+
+struct shp_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    void* ws[shp::WS_NSLOTS] = {};
+    size_t ws_bytes[shp::WS_NSLOTS] = {};
+    int num_cus = 256;
+
+    void* get(int slot, size_t bytes);  // grow-only; nullptr on failure (error set)
+};
+
+#define SHP_HIP_CHECK(expr)                                                           \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            shp::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #expr,          \
+                           hipGetErrorString(e_));                                    \
+            return SHP_ERR_HIP;                                                       \
+        }                                                                             \
+    } while (0)
+
+#define SHP_REQUIRE(cond, ...)                                                        \
+    do {                                                                              \
+        if (!(cond)) {                                                                \
+            shp::set_error(__VA_ARGS__);                                              \
+            return SHP_ERR_INVALID;                                                   \
+        }                                                                             \
+    } while (0)
+
+#define SHP_WS(ptr, type, h, slot, count)                                             \
+    type* ptr = static_cast<type*>((h)->get((slot), sizeof(type) * (size_t)(count)));  \
+    if (!ptr) return SHP_ERR_NOMEM

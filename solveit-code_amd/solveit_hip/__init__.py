@@ -1,0 +1,233 @@
+"""ctypes binding of libsolveit_hip.so (the MI355X force engine's C ABI).
+
+This is the harness-side view of include/solveit_hip.h.  Importing it loads the
+in-tree library and fails loudly when it has not been built: there is no CPU
+fallback anywhere in the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsolveit_hip.so")
+
+SHP_PREC_REF = 0
+SHP_PREC_FP64 = 1
+SHP_PREC_FP32C = 2
+
+SHP_SYM_NONE = 0
+SHP_SYM_ROTATIONAL = 1
+SHP_SYM_TRANSLATIONAL = 2
+
+# ParticleStatus (structures.h:206-211) and ParticleSpecies (:224-233)
+STATUS_INBOUND = 1
+STATUS_AFFECTS = 2
+STATUS_OUT = 0
+STATUS_CRASHED = -1
+MONOMER_POS, DIMER_POS, TRIMER_POS, DROPLET, NEUTRAL, MONOMER_NEG, DIMER_NEG, TRIMER_NEG = range(8)
+
+
+class ShpError(RuntimeError):
+    pass
+
+
+class Symmetry(ctypes.Structure):
+    _fields_ = [
+        ("type", ctypes.c_int),
+        ("nsec", ctypes.c_int),
+        ("nimg", ctypes.c_int),
+        ("lattice", ctypes.POINTER(ctypes.c_int)),
+        ("domain", ctypes.c_double * 3),
+    ]
+
+
+class Particles(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("x", ctypes.c_void_p),
+        ("y", ctypes.c_void_p),
+        ("z", ctypes.c_void_p),
+        ("q", ctypes.c_void_p),
+        ("status", ctypes.c_void_p),
+        ("species", ctypes.c_void_p),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() "
+            "(hipcc --offload-arch=gfx950) first")
+    lib = ctypes.CDLL(LIB_PATH)
+    c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    fp = ctypes.POINTER(ctypes.c_float)
+    dp = ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "shp_version": (c_int, []),
+        "shp_last_error": (ctypes.c_char_p, []),
+        "shp_create": (c_int, [c_int, ctypes.POINTER(c_void_p)]),
+        "shp_destroy": (c_int, [c_void_p]),
+        "shp_sync": (c_int, [c_void_p]),
+        "shp_stream": (c_void_p, [c_void_p]),
+        "shp_k_const": (c_double, []),
+        "shp_softening": (c_double, []),
+        "shp_species_mass": (c_double, [c_int]),
+        "shp_direct_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, c_int, c_int,
+                                    c_int, ctypes.POINTER(Symmetry), dp]),
+        "shp_direct_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), c_int,
+                                   ctypes.POINTER(Symmetry), c_void_p, c_void_p, c_void_p]),
+        "shp_last_kernel_ms": (c_double, [c_void_p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib, sig
+
+
+lib, _SIGNATURES = _load()
+EXPORTED = tuple(_SIGNATURES)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise ShpError(f"{what} failed ({rc}): {lib.shp_last_error().decode()}")
+
+
+def k_const():
+    return lib.shp_k_const()
+
+
+def softening():
+    return lib.shp_softening()
+
+
+def make_symmetry(kind="none", nsec=4, lattice=None, domain=(0.0, 0.0, 0.0)):
+    """Build a Symmetry struct; keeps the lattice array alive on the struct."""
+    s = Symmetry()
+    s.type = {"none": SHP_SYM_NONE, "rotational": SHP_SYM_ROTATIONAL,
+              "translational": SHP_SYM_TRANSLATIONAL}[kind]
+    s.nsec = int(nsec)
+    s.nimg = 0
+    s.lattice = None
+    if kind == "translational":
+        lat = np.ascontiguousarray(np.asarray(lattice, dtype=np.int32).reshape(-1, 3))
+        s._lat = lat
+        s.nimg = lat.shape[0]
+        s.lattice = lat.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    for c in range(3):
+        s.domain[c] = float(domain[c])
+    return s
+
+
+class Engine:
+    """One HIP context (device, stream, workspace) of the force engine."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib.shp_create(int(device), ctypes.byref(h)), "shp_create")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            lib.shp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def stream(self):
+        return lib.shp_stream(self.h)
+
+    def sync(self):
+        _check(lib.shp_sync(self.h), "shp_sync")
+
+    def last_kernel_ms(self):
+        return lib.shp_last_kernel_ms(self.h)
+
+    def direct_host(self, pos4, vel4, acc4, i0=None, i1=None, j0=None, j1=None,
+                    precision=SHP_PREC_REF, symmetry=None):
+        """Reference-contract direct sum on host AoS float32 arrays of shape (n, 4).
+
+        Returns the (i1-i0, 3) float64 acceleration before the reference's float
+        rounding (zero rows for non-targets)."""
+        pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+        vel4 = np.ascontiguousarray(vel4, dtype=np.float32)
+        acc4 = np.ascontiguousarray(acc4, dtype=np.float32)
+        n = pos4.shape[0]
+        i0 = 0 if i0 is None else i0
+        i1 = n if i1 is None else i1
+        j0 = 0 if j0 is None else j0
+        j1 = n if j1 is None else j1
+        out = np.zeros((max(i1 - i0, 0), 3), dtype=np.float64)
+        fp = ctypes.POINTER(ctypes.c_float)
+        sym = symmetry if symmetry is not None else make_symmetry("none")
+        rc = lib.shp_direct_host(self.h, pos4.ctypes.data_as(fp), vel4.ctypes.data_as(fp),
+                                 acc4.ctypes.data_as(fp), n, i0, i1, j0, j1, precision,
+                                 ctypes.byref(sym),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        _check(rc, "shp_direct_host")
+        return out
+
+    def direct_dev(self, state, precision=SHP_PREC_FP64, symmetry=None, out=None):
+        """Direct sum on a DeviceState (torch tensors on the GPU).  Returns
+        (ax, ay, az) float64 device tensors; stream-ordered on the engine's stream."""
+        import torch
+        n = state.n
+        if out is None:
+            out = tuple(torch.empty(n, dtype=torch.float64, device=state.x.device)
+                        for _ in range(3))
+        sym = symmetry if symmetry is not None else make_symmetry("none")
+        p = state.as_struct()
+        rc = lib.shp_direct_dev(self.h, ctypes.byref(p), precision, ctypes.byref(sym),
+                                out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr())
+        _check(rc, "shp_direct_dev")
+        return out
+
+
+class DeviceState:
+    """FP64 SoA particle state resident in HBM (torch tensors as plain storage)."""
+
+    def __init__(self, x, y, z, q, status, species, vx=None, vy=None, vz=None, t_inj=None):
+        self.x, self.y, self.z, self.q = x, y, z, q
+        self.status, self.species = status, species
+        self.vx, self.vy, self.vz, self.t_inj = vx, vy, vz, t_inj
+        self.n = int(x.shape[0])
+
+    @classmethod
+    def from_numpy(cls, d, device="cuda:0"):
+        import torch
+
+        def f64(a):
+            return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+
+        def i32(a):
+            return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
+
+        kw = {}
+        for k in ("vx", "vy", "vz", "t_inj"):
+            if k in d:
+                kw[k] = f64(d[k])
+        return cls(f64(d["x"]), f64(d["y"]), f64(d["z"]), f64(d["q"]), i32(d["status"]),
+                   i32(d["species"]), **kw)
+
+    def as_struct(self):
+        p = Particles()
+        p.n = self.n
+        p.x, p.y, p.z = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr()
+        p.q = self.q.data_ptr()
+        p.status = self.status.data_ptr()
+        p.species = self.species.data_ptr()
+        return p

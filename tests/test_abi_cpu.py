@@ -1,0 +1,56 @@
+"""The C ABI library loads and exports every entry point include/*.h declares
+(CPU only: no compute call is made here)."""
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b(shp_\w+)\s*\(", txt, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert {"shp_create", "shp_destroy", "shp_direct_host", "shp_direct_dev"} <= names
+
+
+def test_library_exports_every_declared_symbol():
+    import solveit_hip
+    out = subprocess.check_output(["nm", "-D", "--defined-only", solveit_hip.LIB_PATH], text=True)
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    missing = _declared() - exported
+    assert not missing, f"declared but not exported: {sorted(missing)}"
+
+
+def test_python_binding_covers_declared_symbols():
+    import solveit_hip
+    assert _declared() <= set(solveit_hip.EXPORTED)
+
+
+def test_constants_match_oracle():
+    import oracle
+    import solveit_hip
+    assert solveit_hip.k_const() == oracle.k_const()
+    assert solveit_hip.softening() == oracle.softening()
+    for s in (0, 1, 2, 4, 5, 6, 7):
+        assert solveit_hip.lib.shp_species_mass(s) == oracle.species_mass(s)
+
+
+def test_error_path_without_device_is_reported():
+    import solveit_hip
+    # An out-of-range device must come back as an error code with a message,
+    # never a crash (here there may be no device at all).
+    with pytest.raises(solveit_hip.ShpError):
+        solveit_hip.Engine(device=4096)
+    assert solveit_hip.lib.shp_last_error()
