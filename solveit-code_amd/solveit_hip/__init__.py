@@ -59,6 +59,14 @@ def _load():
         raise ImportError(
             f"{LIB_PATH} is not built; run __graft_entry__.build() "
             "(hipcc --offload-arch=gfx950) first")
+    # torch ships its own HIP/HSA runtime (soname libamdhip64.so.7 under a
+    # different file name).  Loading torch first makes the dynamic loader bind
+    # this library's libamdhip64.so.7 / libhsa-runtime64.so.1 to those already
+    # loaded copies, so one process never initialises two HSA runtimes.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
     fp = ctypes.POINTER(ctypes.c_float)
