@@ -105,6 +105,87 @@ int shp_direct_dev(shp_handle h, const shp_particles* p, int precision,
  * measured with HIP events on the handle's stream. */
 double shp_last_kernel_ms(shp_handle h);
 
+/* ---- background E-field (pwl on a scattered (r, z) triangulation) ---- */
+
+/* Host arrays in the reference's layout (InterpolationScheme members,
+ * interpolation_scheme.h:44-66): nodes xy[2*nnodes] (r/r0, z/r0), er/ez per node
+ * (field in units of E0), triangles tri[3*ntri] (0-based), neighbours
+ * nbr[3*ntri] (neighbour k across the edge opposite vertex k, <0 on the hull). */
+typedef struct shp_field_mesh {
+    int nnodes;
+    const double* xy;
+    const double* er;
+    const double* ez;
+    int ntri;
+    const int* tri;
+    const int* nbr;
+    double r0;   /* InterpolationScheme::r0 (m) */
+    double E0;   /* InterpolationScheme::E0 (V/m) */
+} shp_field_mesh;
+
+/* Uploads the mesh and builds the point locator.  Replaces the one-time
+ * InterpolationScheme::initializeSingleEmitterInterpolationScheme
+ * (interpolation_scheme.cpp:152-191). */
+int shp_efield_setup(shp_handle h, const shp_field_mesh* mesh);
+
+/* Field at n device points (FP64): E = E0 (Er cos(phi), Er sin(phi), Ez) with
+ * (Er, Ez) interpolated at (rho/r0, z/r0), zero outside z/r0 in [0, 140].
+ * Replaces InterpolationScheme::computeEfieldSingleEmitter
+ * (interpolation_scheme.cpp:200-271) and the walk of
+ * pwl_interp_2d_scattered_value (pwl_interp_2d_scattered.c:887-956, :1762-1990).
+ * tri_out (device, may be NULL) receives the triangle used. */
+int shp_efield_dev(shp_handle h, int n, const double* x, const double* y, const double* z,
+                   double* ex, double* ey, double* ez, int* tri_out);
+
+/* ---- leapfrog DKD on the device-resident FP64 state ---- */
+
+typedef struct shp_state {
+    int n;
+    double* x;
+    double* y;
+    double* z;
+    double* vx;
+    double* vy;
+    double* vz;
+    const double* q;
+    int* status;            /* updated by boundary conditions */
+    const int* species;
+    const double* t_inj;    /* injection time (s), ascending */
+} shp_state;
+
+#define SHP_METHOD_DIRECT 0
+#define SHP_METHOD_FMM 1
+#define SHP_METHOD_BH 2
+
+typedef struct shp_step_opts {
+    int method;          /* SHP_METHOD_* */
+    int precision;       /* SHP_PREC_FP64 or SHP_PREC_FP32C */
+    shp_symmetry sym;    /* force symmetry; also drives folding / wrapping */
+    int use_efield;      /* add the background field (needs shp_efield_setup) */
+    double bc_space[4];  /* |x|, |y|, |z|, rho limits (m); 0 disables each */
+    double bc_time;      /* max time since injection (s); 0 disables */
+    int fmm_order;       /* FMM expansion order p */
+    int leaf_size;       /* FMM / Barnes-Hut leaf capacity */
+    double theta;        /* Barnes-Hut opening angle / FMM separation */
+} shp_step_opts;
+
+/*
+ * Runs nsteps of LeapfrogDKD::runSimulation (integration_schemes.cpp:856-891):
+ * per step drift(dt/2) -- force -- E-field -- kick(dt) -- drift(dt/2), with the
+ * injection clamps of propagatePositionNew/propagateVelocityNew
+ * (particles.cpp:693-786), the symmetry folding of enforceSymmetryConditions
+ * (particles.cpp:199-340) and the boundary conditions of
+ * enforceBoundaryConditions (particles.cpp:73-177).  Ions with t_inj later than
+ * the drift's end time are outside the active range (findMostRecentInjected,
+ * particles.cpp:1328-1359).  ax/ay/az (device, length n) hold the last
+ * Coulomb acceleration.  Stream-ordered.
+ */
+int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double t0, double dt,
+                     int nsteps, double* ax, double* ay, double* az);
+
+/* Kernel time (ms) of the last leapfrog update kernel. */
+double shp_last_update_ms(shp_handle h);
+
 #ifdef __cplusplus
 }
 #endif

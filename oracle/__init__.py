@@ -271,3 +271,179 @@ def rel_l2(a, ref, mask=None):
         return 0.0, 0.0, -1
     k = int(np.argmax(ratio))
     return float(np.sqrt(ratio.mean())), float(np.sqrt(ratio[k])), k
+
+
+# ---------------------------------------------------------------------------
+# Background field (oracle/pwl.c) and leapfrog (oracle/leapfrog.c)
+# ---------------------------------------------------------------------------
+class _LfParams(ctypes.Structure):
+    _fields_ = [
+        ("sym", ctypes.c_int), ("nsec", ctypes.c_int),
+        ("dom", ctypes.c_double * 3),
+        ("bc", ctypes.c_double * 4),
+        ("bc_time", ctypes.c_double),
+        ("use_efield", ctypes.c_int),
+        ("xy", ctypes.c_void_p), ("er", ctypes.c_void_p), ("ez", ctypes.c_void_p),
+        ("tri", ctypes.c_void_p), ("nbr", ctypes.c_void_p),
+        ("ntri", ctypes.c_int),
+        ("r0", ctypes.c_double), ("E0", ctypes.c_double),
+        ("nimg", ctypes.c_int), ("lattice", ctypes.c_void_p),
+    ]
+
+
+_cd, _ci, _vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+lib.orc_tri_search.restype = _ci
+lib.orc_tri_search.argtypes = [_vp, _ci, _vp, _vp, _vp, _vp, _vp, _vp]
+lib.orc_efield_ref.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, _vp, _ci, _vp, _vp, _vp, _vp, _vp, _vp]
+lib.orc_background_accel_ref.argtypes = [_vp, _vp, _vp, _ci, _ci, ctypes.c_float, ctypes.c_float,
+                                         _vp, _ci, _vp, _vp, _vp, _vp, _vp, _vp]
+lib.orc_efield_f64_batch.argtypes = [_ci, _vp, _vp, _vp, _cd, _cd, _vp, _ci, _vp, _vp, _vp, _vp,
+                                     _vp, _vp, _vp, _vp]
+lib.orc_leapfrog_run_f64.argtypes = [ctypes.POINTER(_LfParams), _ci, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _vp, _vp, _vp, _vp, _cd, _cd, _ci, _vp]
+
+
+def _mesh_arrays(mesh):
+    return {"xy": _f64(mesh["xy"]), "er": _f64(mesh["er"]), "ez": _f64(mesh["ez"]),
+            "tri": _i32(mesh["tri"]), "nbr": _i32(mesh["nbr"])}
+
+
+def efield_f64(mesh, x, y, z, seeds=None):
+    """FP64 field at points (walk from `seeds` or the default start).
+    Returns (E[n,3], edge[n], tri[n])."""
+    m = _mesh_arrays(mesh)
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    n = x.shape[0]
+    out = np.zeros((n, 3))
+    edges = np.zeros(n, np.int32)
+    tris = np.zeros(n, np.int32)
+    sd = None if seeds is None else _i32(seeds)
+    lib.orc_efield_f64_batch(n, x.ctypes.data, y.ctypes.data, z.ctypes.data, float(mesh["r0"]),
+                             float(mesh["E0"]), m["xy"].ctypes.data, m["tri"].shape[0],
+                             m["tri"].ctypes.data, m["nbr"].ctypes.data, m["er"].ctypes.data,
+                             m["ez"].ctypes.data, None if sd is None else sd.ctypes.data,
+                             out.ctypes.data, edges.ctypes.data, tris.ctypes.data)
+    return out, edges, tris
+
+
+def tri_search(mesh, p, start=-1):
+    """One reference walk from `start`; returns (edge, tri, (alpha, beta, gamma), steps)."""
+    m = _mesh_arrays(mesh)
+    pp = _f64(p)
+    t = ctypes.c_int(start)
+    bc = np.zeros(3)
+    steps = ctypes.c_int(0)
+    e = lib.orc_tri_search(m["xy"].ctypes.data, m["tri"].shape[0], m["tri"].ctypes.data,
+                           m["nbr"].ctypes.data, pp.ctypes.data, ctypes.addressof(t),
+                           bc.ctypes.data, ctypes.addressof(steps))
+    return e, t.value, tuple(bc), steps.value
+
+
+def background_accel_ref(mesh, pos4, vel4, acc4, start=-1):
+    """computeBackgroundAcceleration on the reference's float arrays, in index
+    order with the warm start chained; returns (acc4', efield[n,3] float32)."""
+    m = _mesh_arrays(mesh)
+    pos4, vel4 = _f32(pos4), _f32(vel4)
+    acc = _f32(acc4).copy()
+    n = pos4.shape[0]
+    ef = np.zeros((n, 3), np.float32)
+    t = ctypes.c_int(start)
+    lib.orc_background_accel_ref(pos4.ctypes.data, vel4.ctypes.data, acc.ctypes.data, 0, n,
+                                 float(np.float32(mesh["r0"])), float(np.float32(mesh["E0"])),
+                                 m["xy"].ctypes.data, m["tri"].shape[0], m["tri"].ctypes.data,
+                                 m["nbr"].ctypes.data, m["er"].ctypes.data, m["ez"].ctypes.data,
+                                 ctypes.addressof(t), ef.ctypes.data)
+    return acc, ef
+
+
+def leapfrog_run_f64(d, t0, dt, nsteps, mesh=None, sym="none", nsec=4, lattice=None,
+                     domain=(0.0, 0.0, 0.0), bc_space=(0, 0, 0, 0), bc_time=0.0):
+    """nsteps of DKD (FP64 direct force) on a copy of the synth dict; returns
+    (state dict, last Coulomb acceleration[n,3])."""
+    s = {k: (_f64(d[k]).copy() if k not in ("status", "species") else _i32(d[k]).copy())
+         for k in ("x", "y", "z", "vx", "vy", "vz", "q", "status", "species", "t_inj")}
+    n = s["x"].shape[0]
+    P = _LfParams()
+    P.sym = _SYM[sym]
+    P.nsec = nsec
+    for c in range(3):
+        P.dom[c] = float(domain[c])
+    for c in range(4):
+        P.bc[c] = float(bc_space[c])
+    P.bc_time = float(bc_time)
+    keep = []
+    if mesh is not None:
+        m = _mesh_arrays(mesh)
+        keep.append(m)
+        P.use_efield = 1
+        P.xy, P.er, P.ez = m["xy"].ctypes.data, m["er"].ctypes.data, m["ez"].ctypes.data
+        P.tri, P.nbr = m["tri"].ctypes.data, m["nbr"].ctypes.data
+        P.ntri = m["tri"].shape[0]
+        P.r0, P.E0 = float(mesh["r0"]), float(mesh["E0"])
+    lat = _i32(np.zeros((1, 3)) if lattice is None else lattice)
+    keep.append(lat)
+    P.nimg = lat.shape[0]
+    P.lattice = lat.ctypes.data
+    acc = np.zeros((n, 3))
+    lib.orc_leapfrog_run_f64(ctypes.byref(P), n, *[s[k].ctypes.data for k in
+                                                    ("x", "y", "z", "vx", "vy", "vz", "q",
+                                                     "status", "species", "t_inj")],
+                             float(t0), float(dt), int(nsteps), acc.ctypes.data)
+    return s, acc
+
+
+_REF_DIR = os.path.join(_HERE, "_ref")
+
+
+def ref_pwl_lib():
+    """The reference's own pwl_interp_2d_scattered.c built by oracle/ref.mk, or None."""
+    p = os.path.join(_REF_DIR, "libref_pwl.so")
+    if not os.path.exists(p):
+        return None
+    L = ctypes.CDLL(p)
+    L.pwl_interp_2d_scattered_value.restype = ctypes.c_void_p
+    L.pwl_interp_2d_scattered_value.argtypes = [_ci, _vp, _vp, _ci, _vp, _vp, _ci, _vp]
+    return L
+
+
+def ref_efield_lib():
+    """The reference's InterpolationScheme built by oracle/ref.mk, or None."""
+    p = os.path.join(_REF_DIR, "libref_efield.so")
+    if not os.path.exists(p):
+        return None
+    L = ctypes.CDLL(p)
+    L.ref_efield_init.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                  ctypes.c_float, ctypes.c_float]
+    L.ref_efield.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]
+    return L
+
+
+# ---------------------------------------------------------------------------
+# FMM (oracle/fmm.c)
+# ---------------------------------------------------------------------------
+lib.orc_fmm_f64.argtypes = [_ci, _vp, _vp, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _vp, _vp]
+lib.orc_fmm_taylor.argtypes = [_ci, _vp, _cd, _vp]
+lib.orc_fmm_ncoef.argtypes = [_ci]
+lib.orc_fmm_root.argtypes = [_ci, _vp, _vp, _vp, _vp, _vp]
+lib.orc_fmm_key.restype = ctypes.c_uint64
+lib.orc_fmm_key.argtypes = [_cd, _cd, _cd, _vp, _ci]
+
+
+def fmm_f64(d, p, L, ring=1):
+    """FP64 FMM accelerations of the engine's algorithm; returns (acc[n,3], stats)."""
+    x, y, z, q = _f64(d["x"]), _f64(d["y"]), _f64(d["z"]), _f64(d["q"])
+    st, sp = _i32(d["status"]), _i32(d["species"])
+    n = x.shape[0]
+    out = np.zeros((n, 3))
+    stats = np.zeros(3)
+    lib.orc_fmm_f64(n, x.ctypes.data, y.ctypes.data, z.ctypes.data, q.ctypes.data,
+                    st.ctypes.data, sp.ctypes.data, p, L, ring, out.ctypes.data, stats.ctypes.data)
+    return out, {"leaves": int(stats[0]), "p2p_pairs": stats[1], "m2l": stats[2]}
+
+
+def fmm_taylor(p, R, eps2):
+    T = np.zeros(lib.orc_fmm_ncoef(p))
+    r = _f64(R)
+    lib.orc_fmm_taylor(p, r.ctypes.data, float(eps2), T.ctypes.data)
+    return T

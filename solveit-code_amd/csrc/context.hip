@@ -68,6 +68,8 @@ int shp_create(int device, shp_handle* out) {
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&h->ev0);
     if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev2);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev3);
     if (e != hipSuccess) {
         shp::set_error("shp_create: %s", hipGetErrorString(e));
         delete h;
@@ -85,6 +87,9 @@ int shp_destroy(shp_handle h) {
         if (h->ws[i]) (void)hipFree(h->ws[i]);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->ev2) (void)hipEventDestroy(h->ev2);
+    if (h->ev3) (void)hipEventDestroy(h->ev3);
+    shp::free_mesh(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return SHP_OK;
@@ -101,6 +106,14 @@ void* shp_stream(shp_handle h) { return h ? (void*)h->stream : nullptr; }
 double shp_k_const(void) { return shp::k_const_host(); }
 double shp_softening(void) { return shp::kSoftening; }
 double shp_species_mass(int species) { return shp::species_mass(species); }
+
+double shp_last_update_ms(shp_handle h) {
+    if (!h || !h->timed_update) return -1.0;
+    float ms = -1.0f;
+    if (hipEventSynchronize(h->ev3) != hipSuccess) return -1.0;
+    if (hipEventElapsedTime(&ms, h->ev2, h->ev3) != hipSuccess) return -1.0;
+    return (double)ms;
+}
 
 double shp_last_kernel_ms(shp_handle h) {
     if (!h || !h->timed) return -1.0;

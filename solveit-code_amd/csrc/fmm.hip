@@ -1,0 +1,918 @@
+// Fast multipole method on gfx950 for the softened Coulomb kernel
+// G(r) = (|r|^2 + eps^2)^(-1/2), FP32 expansion and pair arithmetic on FP64
+// state.
+//
+// Replaces MultipoleMethodForceCalculation::updateAcceleration and its passes
+// (multipole_method_force_calculation.cpp:148-522, fmm_cpukernel.cpp:103-1317).
+// The reference translates spherical-harmonic expansions of the plain 1/r
+// kernel and softens only the centre distance in M2L (fmm_cpukernel.cpp:832);
+// here the softened kernel itself is expanded in Cartesian Taylor series, so
+// the far field converges to the softened direct sum.
+//
+// Tree: cube root around the FP64 bounding box; 63-bit Morton keys (21 bits per
+// axis, x -> bit 3i, y -> 3i+1, z -> 3i+2, multipole_method_functions.cpp:10-26
+// uses another interleave) sorted once; every box at every level is a
+// contiguous run of the sorted particles.  Uniform leaf level L.  Near field:
+// leaf pairs whose cell coordinates differ by at most `ring` per axis.  Far
+// field at level l: children of the parent's near boxes that are not near.
+//
+// Expansions (lengths in units of the box half size h of the level):
+//   P2M  M_a = sum q v^a                      v = (y - c)/h
+//   M2M  M'_a += sum_{k<=a} C(a,k) 2^-|a| s(o, a-k) M_k     (s: octant sign)
+//   M2L  L_b += sum_{|a|+|b|<=p} C(a+b,a) (-1)^|a| T_{a+b}(2 D, eps/h) M_a
+//   L2L  L'_k += 1/2 sum_{b>=k} C(b,k) 2^-|b| s(o, b-k) L_b
+//   L2P  a = -K q/m (1/h^2) grad_u sum_b L_b u^b       u = (x - c)/h
+// T_g are the Taylor coefficients of G at the integer box offset D (R = 2D in
+// units of h), tabulated per level by the recurrence
+//   |g| s T_g = -(2|g|-1) sum_i R_i T_{g-e_i} - (|g|-1) sum_i T_{g-2e_i},
+//   s = |R|^2 + (eps/h)^2.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <vector>
+
+#include "shp_internal.h"
+
+namespace shp {
+
+static inline int fdiv_up(long a, long b) { return (int)((a + b - 1) / b); }
+
+constexpr int kMaxP = 12;
+constexpr int kMaxNc = (kMaxP + 1) * (kMaxP + 2) * (kMaxP + 3) / 6;  // 455
+constexpr int kMaxLevel = 21;
+constexpr int kFmmThreads = 256;
+constexpr int kM2LBatch = 4;
+
+// ------------------------------------------------------------- host tables ----
+
+struct FmmHostTables {
+    int p = -1, nc = 0;
+    std::vector<int4> mi;  // a, b, c, degree
+    int idx[kMaxP + 1][kMaxP + 1][kMaxP + 1];
+    // M2L tasks: pair ranges, one accumulator each
+    std::vector<int> pair_alpha, pair_gamma;
+    std::vector<float> pair_w;
+    std::vector<int2> task;        // [pair begin, pair end)
+    std::vector<int> task_beta;
+    std::vector<int2> beta_tasks;  // [task begin, task end) per beta
+    // shift tables (M2M by alpha over kappa<=alpha; L2L by kappa over beta>=kappa)
+    std::vector<int2> up_range;    // per alpha
+    std::vector<int> up_k;
+    std::vector<float> up_c;
+    std::vector<int> up_diffbits;  // (a-k) parities packed 1 bit per axis
+    std::vector<int2> dn_range;    // per kappa
+    std::vector<int> dn_b;
+    std::vector<float> dn_c;
+    std::vector<int> dn_diffbits;
+};
+
+static double binom_d(int n, int k) {
+    double r = 1.0;
+    for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+
+static void build_tables(FmmHostTables& t, int p) {
+    t.p = p;
+    t.mi.clear();
+    for (int n = 0; n <= p; n++)
+        for (int a = n; a >= 0; a--)
+            for (int b = n - a; b >= 0; b--) {
+                int c = n - a - b;
+                t.idx[a][b][c] = (int)t.mi.size();
+                t.mi.push_back(make_int4(a, b, c, n));
+            }
+    t.nc = (int)t.mi.size();
+    const int nc = t.nc;
+    // M2L pairs grouped by beta, split into tasks of at most G pairs so that
+    // the number of tasks fits one workgroup.
+    std::vector<std::vector<int>> pa(nc), pg(nc);
+    std::vector<std::vector<float>> pw(nc);
+    long npairs = 0;
+    for (int be = 0; be < nc; be++) {
+        int4 B = t.mi[be];
+        for (int al = 0; al < nc; al++) {
+            int4 A = t.mi[al];
+            if (A.w + B.w > p) continue;
+            double w = binom_d(A.x + B.x, A.x) * binom_d(A.y + B.y, A.y) * binom_d(A.z + B.z, A.z);
+            if (A.w & 1) w = -w;
+            pa[be].push_back(al);
+            pg[be].push_back(t.idx[A.x + B.x][A.y + B.y][A.z + B.z]);
+            pw[be].push_back((float)w);
+            npairs++;
+        }
+    }
+    int G = 1;
+    for (;; G++) {
+        long ntask = 0;
+        for (int be = 0; be < nc; be++) ntask += (pa[be].size() + G - 1) / G;
+        if (ntask <= kFmmThreads) break;
+    }
+    t.pair_alpha.clear();
+    t.pair_gamma.clear();
+    t.pair_w.clear();
+    t.task.clear();
+    t.task_beta.clear();
+    t.beta_tasks.assign(nc, make_int2(0, 0));
+    for (int be = 0; be < nc; be++) {
+        int tb = (int)t.task.size();
+        int n = (int)pa[be].size();
+        for (int s = 0; s < n; s += G) {
+            int e = std::min(n, s + G);
+            int pb = (int)t.pair_alpha.size();
+            for (int k = s; k < e; k++) {
+                t.pair_alpha.push_back(pa[be][k]);
+                t.pair_gamma.push_back(pg[be][k]);
+                t.pair_w.push_back(pw[be][k]);
+            }
+            t.task.push_back(make_int2(pb, (int)t.pair_alpha.size()));
+            t.task_beta.push_back(be);
+        }
+        t.beta_tasks[be] = make_int2(tb, (int)t.task.size());
+    }
+    (void)npairs;
+    // shift tables
+    t.up_range.assign(nc, make_int2(0, 0));
+    t.up_k.clear();
+    t.up_c.clear();
+    t.up_diffbits.clear();
+    for (int al = 0; al < nc; al++) {
+        int4 A = t.mi[al];
+        int b = (int)t.up_k.size();
+        for (int k0 = 0; k0 <= A.x; k0++)
+            for (int k1 = 0; k1 <= A.y; k1++)
+                for (int k2 = 0; k2 <= A.z; k2++) {
+                    t.up_k.push_back(t.idx[k0][k1][k2]);
+                    t.up_c.push_back((float)(binom_d(A.x, k0) * binom_d(A.y, k1) * binom_d(A.z, k2) *
+                                             std::ldexp(1.0, -A.w)));
+                    t.up_diffbits.push_back(((A.x - k0) & 1) | (((A.y - k1) & 1) << 1) |
+                                            (((A.z - k2) & 1) << 2));
+                }
+        t.up_range[al] = make_int2(b, (int)t.up_k.size());
+    }
+    t.dn_range.assign(nc, make_int2(0, 0));
+    t.dn_b.clear();
+    t.dn_c.clear();
+    t.dn_diffbits.clear();
+    for (int ka = 0; ka < nc; ka++) {
+        int4 K = t.mi[ka];
+        int b = (int)t.dn_b.size();
+        for (int be = 0; be < nc; be++) {
+            int4 B = t.mi[be];
+            if (B.x < K.x || B.y < K.y || B.z < K.z) continue;
+            t.dn_b.push_back(be);
+            t.dn_c.push_back((float)(0.5 * binom_d(B.x, K.x) * binom_d(B.y, K.y) *
+                                     binom_d(B.z, K.z) * std::ldexp(1.0, -B.w)));
+            t.dn_diffbits.push_back(((B.x - K.x) & 1) | (((B.y - K.y) & 1) << 1) |
+                                    (((B.z - K.z) & 1) << 2));
+        }
+        t.dn_range[ka] = make_int2(b, (int)t.dn_b.size());
+    }
+}
+
+struct FmmDevTables {
+    int p = -1, nc = 0, ntask = 0;
+    int4* mi = nullptr;
+    int* pair_alpha = nullptr;
+    int* pair_gamma = nullptr;
+    float* pair_w = nullptr;
+    int2* task = nullptr;
+    int2* beta_tasks = nullptr;
+    int2* up_range = nullptr;
+    int* up_k = nullptr;
+    float* up_c = nullptr;
+    int* up_bits = nullptr;
+    int2* dn_range = nullptr;
+    int* dn_b = nullptr;
+    float* dn_c = nullptr;
+    int* dn_bits = nullptr;
+};
+
+static std::map<shp_handle, FmmDevTables> g_tables;
+
+template <class T>
+static int upload(T** dst, const std::vector<T>& v) {
+    if (*dst) (void)hipFree(*dst);
+    *dst = nullptr;
+    SHP_HIP_CHECK(hipMalloc((void**)dst, sizeof(T) * std::max<size_t>(1, v.size())));
+    if (!v.empty()) SHP_HIP_CHECK(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return SHP_OK;
+}
+
+static int get_tables(shp_handle h, int p, FmmDevTables** out) {
+    FmmDevTables& d = g_tables[h];
+    if (d.p != p) {
+        FmmHostTables t;
+        build_tables(t, p);
+        int rc = 0;
+        rc |= upload(&d.mi, t.mi);
+        rc |= upload(&d.pair_alpha, t.pair_alpha);
+        rc |= upload(&d.pair_gamma, t.pair_gamma);
+        rc |= upload(&d.pair_w, t.pair_w);
+        rc |= upload(&d.task, t.task);
+        rc |= upload(&d.beta_tasks, t.beta_tasks);
+        rc |= upload(&d.up_range, t.up_range);
+        rc |= upload(&d.up_k, t.up_k);
+        rc |= upload(&d.up_c, t.up_c);
+        rc |= upload(&d.up_bits, t.up_diffbits);
+        rc |= upload(&d.dn_range, t.dn_range);
+        rc |= upload(&d.dn_b, t.dn_b);
+        rc |= upload(&d.dn_c, t.dn_c);
+        rc |= upload(&d.dn_bits, t.dn_diffbits);
+        if (rc) return SHP_ERR_HIP;
+        d.p = p;
+        d.nc = t.nc;
+        d.ntask = (int)t.task.size();
+    }
+    *out = &d;
+    return SHP_OK;
+}
+
+void fmm_release(shp_handle h) {
+    auto it = g_tables.find(h);
+    if (it == g_tables.end()) return;
+    FmmDevTables& d = it->second;
+    void* ptrs[] = {d.mi, d.pair_alpha, d.pair_gamma, d.pair_w, d.task, d.beta_tasks, d.up_range,
+                    d.up_k, d.up_c, d.up_bits, d.dn_range, d.dn_b, d.dn_c, d.dn_bits};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    g_tables.erase(it);
+}
+
+// ------------------------------------------------------------------ keys ----
+
+__device__ __forceinline__ uint64_t spread21(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+__device__ __forceinline__ uint32_t compact21(uint64_t v) {
+    v &= 0x1249249249249249ULL;
+    v = (v ^ (v >> 2)) & 0x10c30c30c30c30c3ULL;
+    v = (v ^ (v >> 4)) & 0x100f00f00f00f00fULL;
+    v = (v ^ (v >> 8)) & 0x1f0000ff0000ffULL;
+    v = (v ^ (v >> 16)) & 0x1f00000000ffffULL;
+    v = (v ^ (v >> 32)) & 0x1fffff;
+    return (uint32_t)v;
+}
+
+__global__ void k_fmm_bbox_partial(const double* __restrict__ X, const double* __restrict__ Y,
+                                   const double* __restrict__ Z, const int* __restrict__ status,
+                                   int n, double* __restrict__ part) {
+    __shared__ double red[6][kFmmThreads];
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int s = status[i];
+        if (!(s == 1 || s == 2)) continue;
+        const double v[3] = {X[i], Y[i], Z[i]};
+        for (int c = 0; c < 3; c++) {
+            mn[c] = fmin(mn[c], v[c]);
+            mx[c] = fmax(mx[c], v[c]);
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        red[c][threadIdx.x] = mn[c];
+        red[3 + c][threadIdx.x] = mx[c];
+    }
+    __syncthreads();
+    for (int s = kFmmThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 3; c++) {
+                red[c][threadIdx.x] = fmin(red[c][threadIdx.x], red[c][threadIdx.x + s]);
+                red[3 + c][threadIdx.x] = fmax(red[3 + c][threadIdx.x], red[3 + c][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Root cube (x0, y0, z0, side): side = largest extent times (1 + 2^-20),
+// centred on the bounding box.
+__global__ void k_fmm_root(const double* __restrict__ part, int nparts, double* __restrict__ root) {
+    if (threadIdx.x != 0) return;
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int b = 0; b < nparts; b++)
+        for (int c = 0; c < 3; c++) {
+            mn[c] = fmin(mn[c], part[b * 6 + c]);
+            mx[c] = fmax(mx[c], part[b * 6 + 3 + c]);
+        }
+    double side = fmax(fmax(mx[0] - mn[0], mx[1] - mn[1]), mx[2] - mn[2]);
+    if (!(side > 0)) side = 1e-9;
+    side = side * (1.0 + 0x1p-20);
+    for (int c = 0; c < 3; c++) root[c] = 0.5 * (mn[c] + mx[c]) - 0.5 * side;
+    root[3] = side;
+}
+
+// Level-21 Morton key of each active particle; inactive ones sort last.
+__global__ void k_fmm_keys(const double* __restrict__ X, const double* __restrict__ Y,
+                           const double* __restrict__ Z, const int* __restrict__ status, int n,
+                           const double* __restrict__ root, uint64_t* __restrict__ keys,
+                           int* __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = status[i];
+    uint64_t k = ~0ULL;
+    if (s == 1 || s == 2) {
+        const double sc = 2097152.0 / root[3];
+        const double v[3] = {X[i], Y[i], Z[i]};
+        k = 0;
+        for (int c = 0; c < 3; c++) {
+            const double f = floor((v[c] - root[c]) * sc);
+            int64_t iv = (int64_t)f;
+            if (f < 0) iv = 0;
+            if (iv > 2097151) iv = 2097151;
+            k |= spread21((uint64_t)iv) << c;
+        }
+    }
+    keys[i] = k;
+    vals[i] = i;
+}
+
+// Sorted particle records relative to nothing yet: double position, charge.
+__global__ void k_fmm_gather(const int* __restrict__ perm, int na, const double* __restrict__ X,
+                             const double* __restrict__ Y, const double* __restrict__ Z,
+                             const double* __restrict__ Q, double4* __restrict__ sp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= na) return;
+    const int j = perm[i];
+    sp[i] = make_double4(X[j], Y[j], Z[j], Q[j]);
+}
+
+// Number of active particles (keys != ~0 sort last) and box counts per level.
+__global__ void k_fmm_level_counts(const uint64_t* __restrict__ keys, int n, int L,
+                                   int* __restrict__ counts /* [L+2]: na, nbox[0..L] */) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    if (k == ~0ULL) return;
+    atomicAdd(&counts[0], 1);
+    for (int l = 0; l <= L; l++) {
+        const int sh = 3 * (kMaxLevel - l);
+        const bool start = (i == 0) || ((keys[i - 1] >> sh) != (k >> sh));
+        if (start) atomicAdd(&counts[1 + l], 1);
+    }
+}
+
+// Box starts at level l: flags of prefix changes.
+__global__ void k_fmm_flags(const uint64_t* __restrict__ keys, int na, int sh, char* __restrict__ f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= na) return;
+    f[i] = (i == 0 || (keys[i - 1] >> sh) != (keys[i] >> sh)) ? 1 : 0;
+}
+
+// Per box: key at its level, particle range end, and (for non-leaf levels) the
+// first child index at level l+1.
+__global__ void k_fmm_box_info(const uint64_t* __restrict__ keys, const int* __restrict__ start,
+                               int nb, int na, int sh, uint64_t* __restrict__ bkey,
+                               int* __restrict__ bend, const int* __restrict__ cstart, int ncb,
+                               int* __restrict__ cfirst) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int s = start[b];
+    bkey[b] = keys[s] >> sh;
+    bend[b] = b + 1 < nb ? start[b + 1] : na;
+    if (cstart) {
+        int lo = 0, hi = ncb;  // first child with cstart >= s
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (cstart[mid] < s) lo = mid + 1;
+            else hi = mid;
+        }
+        cfirst[b] = lo;
+    }
+}
+
+__device__ __forceinline__ int find_key(const uint64_t* __restrict__ bkey, int nb, uint64_t k) {
+    int lo = 0, hi = nb - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const uint64_t v = bkey[mid];
+        if (v == k) return mid;
+        if (v < k) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return -1;
+}
+
+struct Level {
+    int nb;
+    const uint64_t* key;
+    const int* start;
+    const int* end;
+    const int* cfirst;  // nullptr at the leaf level
+    float* M;
+    float* Lc;
+};
+
+__device__ __forceinline__ void box_center(const double* __restrict__ root, int l, uint64_t key,
+                                           double c[3], double& h) {
+    const double bs = root[3] / (double)(1 << l);
+    for (int d = 0; d < 3; d++) c[d] = root[d] + ((double)compact21(key >> d) + 0.5) * bs;
+    h = 0.5 * bs;
+}
+
+// ------------------------------------------------------------------- P2M ----
+
+// One workgroup per leaf; powers of v per particle staged in LDS.
+__global__ __launch_bounds__(kFmmThreads) void k_fmm_p2m(
+    Level lv, int l, const double* __restrict__ root, const double4* __restrict__ sp, int nc,
+    int p, const int4* __restrict__ mi) {
+    __shared__ float pw[64][3][kMaxP + 1];
+    __shared__ float qs[64];
+    const int b = blockIdx.x;
+    double c[3], h;
+    box_center(root, l, lv.key[b], c, h);
+    const double ih = 1.0 / h;
+    const int s0 = lv.start[b], s1 = lv.end[b];
+    float acc = 0.0f;
+    int4 A = make_int4(0, 0, 0, 0);
+    if ((int)threadIdx.x < nc) A = mi[threadIdx.x];
+    for (int base = s0; base < s1; base += 64) {
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int i = base + threadIdx.x;
+            if (i < s1) {
+                const double4 r = sp[i];
+                const float v[3] = {(float)((r.x - c[0]) * ih), (float)((r.y - c[1]) * ih),
+                                    (float)((r.z - c[2]) * ih)};
+                for (int d = 0; d < 3; d++) {
+                    float t = 1.0f;
+                    for (int k = 0; k <= p; k++) {
+                        pw[threadIdx.x][d][k] = t;
+                        t *= v[d];
+                    }
+                }
+                qs[threadIdx.x] = (float)r.w;
+            }
+        }
+        __syncthreads();
+        const int cnt = min(64, s1 - base);
+        if ((int)threadIdx.x < nc)
+            for (int j = 0; j < cnt; j++)
+                acc = fmaf(qs[j], pw[j][0][A.x] * pw[j][1][A.y] * pw[j][2][A.z], acc);
+    }
+    if ((int)threadIdx.x < nc) lv.M[(size_t)b * nc + threadIdx.x] = acc;
+}
+
+// ------------------------------------------------------------------- M2M ----
+
+// s(o, diff) = prod over axes with child bit 0 of (-1)^(diff parity)
+__device__ __forceinline__ float octant_sign(int oct, int bits) {
+    return (__popc(bits & ~oct & 7) & 1) ? -1.0f : 1.0f;
+}
+
+__global__ __launch_bounds__(kFmmThreads) void k_fmm_m2m(Level par, Level ch, int nc,
+                                                         const int2* __restrict__ rng,
+                                                         const int* __restrict__ kk,
+                                                         const float* __restrict__ cc,
+                                                         const int* __restrict__ bits) {
+    const int b = blockIdx.x;
+    const int a = threadIdx.x;
+    const int c0 = par.cfirst[b];
+    const int c1 = b + 1 < par.nb ? par.cfirst[b + 1] : ch.nb;
+    if (a >= nc) return;
+    const int2 r = rng[a];
+    float acc = 0.0f;
+    for (int c = c0; c < c1; c++) {
+        const int oct = (int)(ch.key[c] & 7);
+        const float* Mc = ch.M + (size_t)c * nc;
+        for (int t = r.x; t < r.y; t++) acc = fmaf(cc[t] * octant_sign(oct, bits[t]), Mc[kk[t]], acc);
+    }
+    par.M[(size_t)b * nc + a] = acc;
+}
+
+// ------------------------------------------------------------- T tables ----
+
+// T[level][offset][nc] for integer offsets D in [-W, W]^3 (R = 2D).
+__global__ void k_fmm_ttable(int p, int nc, const int4* __restrict__ mi, int W, int L0, int L,
+                             const double* __restrict__ root, double eps2, float* __restrict__ T) {
+    const int nside = 2 * W + 1, noff = nside * nside * nside;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= noff * (L - L0 + 1)) return;
+    const int l = L0 + t / noff, o = t % noff;
+    const int dx = o % nside - W, dy = (o / nside) % nside - W, dz = o / (nside * nside) - W;
+    const double h = 0.5 * root[3] / (double)(1 << l);
+    const double R[3] = {2.0 * dx, 2.0 * dy, 2.0 * dz};
+    const double s = R[0] * R[0] + R[1] * R[1] + R[2] * R[2] + eps2 / (h * h);
+    double Td[kMaxNc];
+    Td[0] = 1.0 / sqrt(s);
+    // mi is ordered by degree, and within a degree by (a desc, b desc); index of
+    // (a,b,c) with n=a+b+c is n(n+1)(n+2)/6 + (n-a)(n-a+1)/2 + (n-a-b).
+    for (int k = 1; k < nc; k++) {
+        const int4 g = mi[k];
+        const int n = g.w;
+        double acc = 0.0;
+        const int gg[3] = {g.x, g.y, g.z};
+        for (int i = 0; i < 3; i++) {
+            for (int dec = 1; dec <= 2; dec++) {
+                if (gg[i] < dec) continue;
+                int hh[3] = {gg[0], gg[1], gg[2]};
+                hh[i] -= dec;
+                const int m = hh[0] + hh[1] + hh[2];
+                const int id = m * (m + 1) * (m + 2) / 6 + (m - hh[0]) * (m - hh[0] + 1) / 2 +
+                               (m - hh[0] - hh[1]);
+                acc += dec == 1 ? (2 * n - 1) * R[i] * Td[id] : (n - 1) * Td[id];
+            }
+        }
+        Td[k] = -acc / (n * s);
+    }
+    float* out = T + (size_t)t * nc;
+    for (int k = 0; k < nc; k++) out[k] = (float)Td[k];
+}
+
+// ------------------------------------------------------------------- M2L ----
+
+// One workgroup per target box: candidate sources are the children of the
+// parent's near boxes that are not near the target; up to kM2LBatch sources
+// are staged in LDS at a time and every lane runs one task (a slice of the
+// (beta, alpha) pairs of one beta) over them.
+__global__ __launch_bounds__(kFmmThreads) void k_fmm_m2l(
+    Level tl, Level pl, int l, int ring, int nc, int ntask, const int2* __restrict__ task,
+    const int* __restrict__ pa, const int* __restrict__ pg, const float* __restrict__ pw,
+    const int2* __restrict__ beta_tasks, const float* __restrict__ Ttab, int W) {
+    __shared__ int src[1024];
+    __shared__ int srcoff[1024];
+    __shared__ int nsrc;
+    __shared__ float Ms[kM2LBatch][kMaxNc];
+    __shared__ float Ts[kM2LBatch][kMaxNc];
+    __shared__ float part[kFmmThreads];
+    __shared__ int pcnt[128];
+    __shared__ int pfirst[128];
+    const int b = blockIdx.x;
+    const uint64_t key = tl.key[b];
+    const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
+    const int px = ix >> 1, py = iy >> 1, pz = iz >> 1;
+    const int nside = 2 * ring + 1, ncand = nside * nside * nside;
+    const int pmax = (1 << (l - 1)) - 1;
+    const int wside = 2 * W + 1;
+    // 1) parent-level near boxes and their child counts
+    for (int t = threadIdx.x; t < ncand; t += blockDim.x) {
+        const int qx = px + t % nside - ring, qy = py + (t / nside) % nside - ring,
+                  qz = pz + t / (nside * nside) - ring;
+        int first = 0, cnt = 0;
+        if (qx >= 0 && qy >= 0 && qz >= 0 && qx <= pmax && qy <= pmax && qz <= pmax) {
+            const uint64_t pk = spread21(qx) | spread21(qy) << 1 | spread21(qz) << 2;
+            const int pb = find_key(pl.key, pl.nb, pk);
+            if (pb >= 0) {
+                first = pl.cfirst[pb];
+                cnt = (pb + 1 < pl.nb ? pl.cfirst[pb + 1] : tl.nb) - first;
+            }
+        }
+        pfirst[t] = first;
+        pcnt[t] = cnt;
+    }
+    __syncthreads();
+    // 2) deterministic compaction of far children (serial over <=125 parents)
+    if (threadIdx.x == 0) {
+        int k = 0;
+        for (int t = 0; t < ncand; t++)
+            for (int c = pfirst[t]; c < pfirst[t] + pcnt[t]; c++) {
+                const uint64_t sk = tl.key[c];
+                const int sx = (int)compact21(sk), sy = (int)compact21(sk >> 1),
+                          sz = (int)compact21(sk >> 2);
+                const int dx = ix - sx, dy = iy - sy, dz = iz - sz;
+                if (abs(dx) <= ring && abs(dy) <= ring && abs(dz) <= ring) continue;
+                src[k] = c;
+                srcoff[k] = (dx + W) + wside * ((dy + W) + wside * (dz + W));
+                k++;
+            }
+        nsrc = k;
+    }
+    __syncthreads();
+    const int ns = nsrc;
+    const int me = threadIdx.x;
+    int2 tk = make_int2(0, 0);
+    if (me < ntask) tk = task[me];
+    float acc = 0.0f;
+    const float* Tl = Ttab;
+    for (int base = 0; base < ns; base += kM2LBatch) {
+        const int nb = min(kM2LBatch, ns - base);
+        __syncthreads();
+        for (int t = threadIdx.x; t < kM2LBatch * nc; t += blockDim.x) {
+            const int s = t / nc, k = t % nc;
+            if (s < nb) {
+                Ms[s][k] = tl.M[(size_t)src[base + s] * nc + k];
+                Ts[s][k] = Tl[(size_t)srcoff[base + s] * nc + k];
+            } else {
+                Ms[s][k] = 0.0f;
+                Ts[s][k] = 0.0f;
+            }
+        }
+        __syncthreads();
+        for (int q = tk.x; q < tk.y; q++) {
+            const int a = pa[q], g = pg[q];
+            float v = 0.0f;
+#pragma unroll
+            for (int s = 0; s < kM2LBatch; s++) v = fmaf(Ts[s][g], Ms[s][a], v);
+            acc = fmaf(pw[q], v, acc);
+        }
+    }
+    part[me] = acc;
+    __syncthreads();
+    if (me < nc) {
+        const int2 bt = beta_tasks[me];
+        float s = 0.0f;
+        for (int t = bt.x; t < bt.y; t++) s += part[t];
+        tl.Lc[(size_t)b * nc + me] += s;
+    }
+}
+
+// ------------------------------------------------------------------- L2L ----
+
+__global__ __launch_bounds__(kFmmThreads) void k_fmm_l2l(Level par, Level ch, int nc,
+                                                         const int2* __restrict__ rng,
+                                                         const int* __restrict__ bb,
+                                                         const float* __restrict__ cc,
+                                                         const int* __restrict__ bits) {
+    const int b = blockIdx.x;  // parent
+    const int k = threadIdx.x;
+    if (k >= nc) return;
+    const int c0 = par.cfirst[b];
+    const int c1 = b + 1 < par.nb ? par.cfirst[b + 1] : ch.nb;
+    const float* Lp = par.Lc + (size_t)b * nc;
+    const int2 r = rng[k];
+    for (int c = c0; c < c1; c++) {
+        const int oct = (int)(ch.key[c] & 7);
+        float acc = 0.0f;
+        for (int t = r.x; t < r.y; t++) acc = fmaf(cc[t] * octant_sign(oct, bits[t]), Lp[bb[t]], acc);
+        ch.Lc[(size_t)c * nc + k] += acc;
+    }
+}
+
+__global__ void k_fmm_zero(float* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+// ------------------------------------------------------------ L2P + P2P ----
+
+__device__ __forceinline__ void fmm_two_sum(float& hi, float& lo, float p) {
+    const float s = hi + p;
+    const float bp = s - hi;
+    lo += (hi - (s - bp)) + (p - bp);
+    hi = s;
+}
+
+// One wave-sized workgroup per leaf: L2P for its targets, then P2P over the
+// leaves within `ring` cells (FP32 pairs relative to the leaf centre,
+// compensated per-target sums), then scaling by K q/m and scatter to the
+// caller's order.
+__global__ __launch_bounds__(64) void k_fmm_leaf(
+    Level lv, int L, int ring, const double* __restrict__ root, const double4* __restrict__ sp,
+    const int* __restrict__ perm, const int* __restrict__ status, const int* __restrict__ species,
+    int nc, int p, const int4* __restrict__ mi, float eps2f, double K, double* __restrict__ ax,
+    double* __restrict__ ay, double* __restrict__ az) {
+    __shared__ float4 tile[64];
+    __shared__ float Lsh[kMaxNc];
+    __shared__ int nbr_first[125], nbr_end[125];
+    __shared__ int nnbr;
+    const int b = blockIdx.x;
+    const uint64_t key = lv.key[b];
+    double c[3], h;
+    box_center(root, L, key, c, h);
+    const float ihf = (float)(1.0 / h);
+    for (int k = threadIdx.x; k < nc; k += 64) Lsh[k] = lv.Lc[(size_t)b * nc + k];
+    const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
+    const int nside = 2 * ring + 1, ncand = nside * nside * nside;
+    const int cmax = (1 << L) - 1;
+    for (int t = threadIdx.x; t < ncand; t += 64) {
+        const int qx = ix + t % nside - ring, qy = iy + (t / nside) % nside - ring,
+                  qz = iz + t / (nside * nside) - ring;
+        int f = 0, e = 0;
+        if (qx >= 0 && qy >= 0 && qz >= 0 && qx <= cmax && qy <= cmax && qz <= cmax) {
+            const uint64_t nk = spread21(qx) | spread21(qy) << 1 | spread21(qz) << 2;
+            const int nb = find_key(lv.key, lv.nb, nk);
+            if (nb >= 0) {
+                f = lv.start[nb];
+                e = lv.end[nb];
+            }
+        }
+        nbr_first[t] = f;
+        nbr_end[t] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int k = 0;
+        for (int t = 0; t < ncand; t++)
+            if (nbr_end[t] > nbr_first[t]) {
+                nbr_first[k] = nbr_first[t];
+                nbr_end[k] = nbr_end[t];
+                k++;
+            }
+        nnbr = k;
+    }
+    __syncthreads();
+    const int s0 = lv.start[b], s1 = lv.end[b];
+    for (int tb = s0; tb < s1; tb += 64) {
+        const int i = tb + threadIdx.x;
+        const bool valid = i < s1;
+        const double4 r = sp[valid ? i : s0];
+        const float ux = (float)((r.x - c[0]) * (1.0 / h)), uy = (float)((r.y - c[1]) * (1.0 / h)),
+                    uz = (float)((r.z - c[2]) * (1.0 / h));
+        // L2P: gradient of sum L_b u^b
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        {
+            float px[kMaxP + 1], py[kMaxP + 1], pz[kMaxP + 1];
+            px[0] = py[0] = pz[0] = 1.0f;
+            for (int k = 1; k <= p; k++) {
+                px[k] = px[k - 1] * ux;
+                py[k] = py[k - 1] * uy;
+                pz[k] = pz[k - 1] * uz;
+            }
+            for (int k = 1; k < nc; k++) {
+                const int4 m = mi[k];
+                const float Lk = Lsh[k];
+                if (m.x) gx = fmaf(Lk * (float)m.x, px[m.x - 1] * py[m.y] * pz[m.z], gx);
+                if (m.y) gy = fmaf(Lk * (float)m.y, px[m.x] * py[m.y - 1] * pz[m.z], gy);
+                if (m.z) gz = fmaf(Lk * (float)m.z, px[m.x] * py[m.y] * pz[m.z - 1], gz);
+            }
+        }
+        // far part in units of 1/h^2 relative to the P2P sum expressed in u units:
+        // P2P below runs in u = (x - c)/h, so sum q u/|u|^3 = h^2 sum q d/|d|^3.
+        float hx = -gx, hy = -gy, hz = -gz, lx = 0.0f, ly = 0.0f, lz = 0.0f;
+        for (int nbi = 0; nbi < nnbr; nbi++) {
+            const int f = nbr_first[nbi], e = nbr_end[nbi];
+            for (int sb = f; sb < e; sb += 64) {
+                __syncthreads();
+                const int j = sb + threadIdx.x;
+                if (j < e) {
+                    const double4 rs = sp[j];
+                    tile[threadIdx.x] = make_float4((float)((rs.x - c[0]) * (1.0 / h)),
+                                                    (float)((rs.y - c[1]) * (1.0 / h)),
+                                                    (float)((rs.z - c[2]) * (1.0 / h)), (float)rs.w);
+                }
+                __syncthreads();
+                const int cnt = min(64, e - sb);
+                float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+                for (int jj = 0; jj < cnt; jj++) {
+                    const float4 q = tile[jj];
+                    const float dx = ux - q.x, dy = uy - q.y, dz = uz - q.z;
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2f)));
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float w = (q.w * inv) * (inv * inv);
+                    sx = fmaf(dx, w, sx);
+                    sy = fmaf(dy, w, sy);
+                    sz = fmaf(dz, w, sz);
+                }
+                fmm_two_sum(hx, lx, sx);
+                fmm_two_sum(hy, ly, sy);
+                fmm_two_sum(hz, lz, sz);
+            }
+        }
+        if (valid) {
+            const int oi = perm[i];
+            if (status[oi] == 1) {
+                const double kc = K * r.w / species_mass(species[oi]) / (h * h);
+                ax[oi] = ((double)hx + (double)lx) * kc;
+                ay[oi] = ((double)hy + (double)ly) * kc;
+                az[oi] = ((double)hz + (double)lz) * kc;
+            }
+        }
+    }
+}
+
+__global__ void k_fmm_fill(double* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+}  // namespace shp
+
+using namespace shp;
+
+extern "C" {
+
+int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level, int ring,
+                double* ax, double* ay, double* az) {
+    SHP_REQUIRE(h && pt && ax && ay && az, "shp_fmm_dev: null argument");
+    SHP_REQUIRE(order >= 1 && order <= kMaxP, "shp_fmm_dev: order %d outside [1, %d]", order, kMaxP);
+    SHP_REQUIRE(ring >= 1 && ring <= 2, "shp_fmm_dev: ring %d outside [1, 2]", ring);
+    SHP_REQUIRE(leaf_level >= 1 && leaf_level <= 16, "shp_fmm_dev: leaf level %d outside [1, 16]",
+                leaf_level);
+    const int n = pt->n;
+    if (n <= 0) return SHP_OK;
+    SHP_REQUIRE(pt->x && pt->y && pt->z && pt->q && pt->status && pt->species,
+                "shp_fmm_dev: null particle array");
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    FmmDevTables* tb;
+    int rc = get_tables(h, order, &tb);
+    if (rc) return rc;
+    const int L = leaf_level, nc = tb->nc;
+    hipStream_t st = h->stream;
+    // bounding box and root
+    const int nbp = std::min(fdiv_up(n, kFmmThreads), 1024);
+    SHP_WS(bpart, double, h, WS_BBOX, nbp * 6);
+    SHP_WS(root, double, h, WS_BOX, 8);
+    hipLaunchKernelGGL(k_fmm_bbox_partial, dim3(nbp), dim3(kFmmThreads), 0, st, pt->x, pt->y, pt->z,
+                       pt->status, n, bpart);
+    hipLaunchKernelGGL(k_fmm_root, dim3(1), dim3(64), 0, st, bpart, nbp, root);
+    // keys and sort
+    SHP_WS(keys, uint64_t, h, WS_FMM_KEYS, n);
+    SHP_WS(keys2, uint64_t, h, WS_FMM_KEYS2, n);
+    SHP_WS(vals, int, h, WS_TGT, n);
+    SHP_WS(perm, int, h, WS_VALS2, n);
+    hipLaunchKernelGGL(k_fmm_keys, dim3(fdiv_up(n, 256)), dim3(256), 0, st, pt->x, pt->y, pt->z,
+                       pt->status, n, root, keys, vals);
+    size_t tmp = 0;
+    SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, keys2, vals, perm, n, 0, 64, st));
+    SHP_WS(cub, char, h, WS_CUB, tmp);
+    SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub, tmp, keys, keys2, vals, perm, n, 0, 64, st));
+    // counts per level (host needs them for launch sizes)
+    SHP_WS(cnt, int, h, WS_COUNTS, 32);
+    SHP_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * 32, st));
+    hipLaunchKernelGGL(k_fmm_level_counts, dim3(fdiv_up(n, 256)), dim3(256), 0, st, keys2, n, L, cnt);
+    int hc[32];
+    SHP_HIP_CHECK(hipMemcpyAsync(hc, cnt, sizeof(int) * 32, hipMemcpyDeviceToHost, st));
+    SHP_HIP_CHECK(hipStreamSynchronize(st));
+    const int na = hc[0];
+    std::vector<int> nbox(L + 1);
+    for (int l = 0; l <= L; l++) nbox[l] = hc[1 + l];
+    if (na == 0) {
+        hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ax, n);
+        hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ay, n);
+        hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, az, n);
+        return SHP_OK;
+    }
+    // sorted particle records
+    SHP_WS(sp, double4, h, WS_SRC, na);
+    hipLaunchKernelGGL(k_fmm_gather, dim3(fdiv_up(na, 256)), dim3(256), 0, st, perm, na, pt->x, pt->y,
+                       pt->z, pt->q, sp);
+    // level arrays in one workspace: key(u64), start, end, cfirst per box; M, L per box
+    size_t tot = 0;
+    for (int l = 0; l <= L; l++) tot += nbox[l];
+    SHP_WS(bkey, uint64_t, h, WS_FMM_BKEY, tot);
+    SHP_WS(bint, int, h, WS_FMM_BINT, 3 * tot);
+    SHP_WS(bexp, float, h, WS_FMM_EXP, 2 * tot * (size_t)nc);
+    SHP_WS(flags, char, h, WS_FLAGS, na);
+    std::vector<Level> lv(L + 1);
+    {
+        size_t off = 0;
+        for (int l = 0; l <= L; l++) {
+            Level& v = lv[l];
+            v.nb = nbox[l];
+            v.key = bkey + off;
+            v.start = bint + off;
+            v.end = bint + tot + off;
+            v.cfirst = l < L ? bint + 2 * tot + off : nullptr;
+            v.M = bexp + off * nc;
+            v.Lc = bexp + (tot + off) * nc;
+            off += nbox[l];
+        }
+    }
+    for (int l = L; l >= 0; l--) {
+        const int sh = 3 * (kMaxLevel - l);
+        hipLaunchKernelGGL(k_fmm_flags, dim3(fdiv_up(na, 256)), dim3(256), 0, st, keys2, na, sh, flags);
+        hipcub::CountingInputIterator<int> it(0);
+        size_t t2 = 0;
+        SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, t2, it, flags, (int*)lv[l].start, cnt + 24,
+                                                    na, st));
+        SHP_WS(cub2, char, h, WS_CUB, t2);
+        SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(cub2, t2, it, flags, (int*)lv[l].start, cnt + 24,
+                                                    na, st));
+        hipLaunchKernelGGL(k_fmm_box_info, dim3(fdiv_up(nbox[l], 256)), dim3(256), 0, st, keys2,
+                           lv[l].start, nbox[l], na, sh, (uint64_t*)lv[l].key, (int*)lv[l].end,
+                           l < L ? lv[l + 1].start : nullptr, l < L ? nbox[l + 1] : 0,
+                           (int*)lv[l].cfirst);
+    }
+    SHP_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_fmm_zero, dim3(fdiv_up(tot * nc, 256)), dim3(256), 0, st, bexp + tot * nc,
+                       tot * nc);
+    // upward pass
+    SHP_HIP_CHECK(hipEventRecord(h->ev0, st));
+    hipLaunchKernelGGL(k_fmm_p2m, dim3(nbox[L]), dim3(kFmmThreads), 0, st, lv[L], L, root, sp, nc,
+                       order, tb->mi);
+    for (int l = L - 1; l >= 0; l--)
+        hipLaunchKernelGGL(k_fmm_m2m, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l + 1], nc,
+                           tb->up_range, tb->up_k, tb->up_c, tb->up_bits);
+    // translation tables for offsets within the far band
+    const int W = 2 * ring + 1;
+    const int nside = 2 * W + 1, noff = nside * nside * nside;
+    SHP_WS(ttab, float, h, WS_FMM_T, (size_t)noff * (L + 1) * nc);
+    hipLaunchKernelGGL(k_fmm_ttable, dim3(fdiv_up(noff * (L + 1), 64)), dim3(64), 0, st, order, nc,
+                       tb->mi, W, 0, L, root, kSoftening, ttab);
+    for (int l = 1; l <= L; l++)
+        hipLaunchKernelGGL(k_fmm_m2l, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l - 1], l, ring,
+                           nc, tb->ntask, tb->task, tb->pair_alpha, tb->pair_gamma, tb->pair_w,
+                           tb->beta_tasks, ttab + (size_t)l * noff * nc, W);
+    for (int l = 1; l < L; l++)
+        hipLaunchKernelGGL(k_fmm_l2l, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l + 1], nc,
+                           tb->dn_range, tb->dn_b, tb->dn_c, tb->dn_bits);
+    hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ax, n);
+    hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ay, n);
+    hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, az, n);
+    hipLaunchKernelGGL(k_fmm_leaf, dim3(nbox[L]), dim3(64), 0, st, lv[L], L, ring, root, sp, perm,
+                       pt->status, pt->species, nc, order, tb->mi, (float)0.0, k_const_host(), ax, ay,
+                       az);
+    SHP_HIP_CHECK(hipGetLastError());
+    SHP_HIP_CHECK(hipEventRecord(h->ev1, st));
+    h->timed = true;
+    return SHP_OK;
+}
+
+}  // extern "C"

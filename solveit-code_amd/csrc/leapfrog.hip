@@ -1,0 +1,461 @@
+// Background E-field (pwl on a scattered triangulation) and the DKD leapfrog
+// update on the FP64 device state.
+//
+// The field lookup replaces the reference's per-ion barycentric walk from a
+// static warm start (pwl_interp_2d_scattered.c:1762-1990, called twice per ion
+// by interpolation_scheme.cpp:244,247) with a quadtree point location built once
+// on the host: a lane descends the tree, tests the few triangles its leaf lists
+// with the reference's barycentric formula and takes the first that contains
+// the point.  Inside the hull this is the triangle the walk stops in (up to
+// ties on shared edges, where both give the same interpolant).  Outside the hull
+// the walk is continued from the leaf's first triangle, as a walk started there
+// would go; the reference's result there depends on its call history.
+//
+// The update kernel fuses, per ion: the kick (Coulomb + field acceleration,
+// propagateVelocityNew particles.cpp:763-786), the second half drift of this
+// step and the first half drift of the next (propagatePositionNew
+// particles.cpp:693-739), each followed by symmetry folding and boundary
+// conditions (particles.cpp:34-40, 73-177, 199-340).  It also writes the
+// status masked by the next step's active range for the next force call.
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <vector>
+
+#include "shp_internal.h"
+
+namespace shp {
+
+static inline int cdiv_l(long a, long b) { return (int)((a + b - 1) / b); }
+
+void free_mesh(shp_handle h) {
+    FieldMesh& m = h->mesh;
+    void* ptrs[] = {m.xy, m.er, m.ez, m.tri, m.nbr, m.qnode, m.qlist};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    m = FieldMesh();
+}
+
+struct MeshView {
+    const double* xy;
+    const double* er;
+    const double* ez;
+    const int* tri;
+    const int* nbr;
+    const int2* qnode;
+    const int* qlist;
+    int ntri;
+    double x0, y0, side;
+    double r0, E0;
+};
+
+static MeshView view_of(const FieldMesh& m) {
+    return MeshView{m.xy, m.er, m.ez, m.tri, m.nbr, m.qnode, m.qlist, m.ntri,
+                    m.root[0], m.root[1], m.root[2], m.r0, m.E0};
+}
+
+// Barycentric coordinates of p in triangle t, written as in
+// triangulation_search_delaunay (vertex c as base).
+__device__ __forceinline__ void bary(const MeshView& M, int t, double px, double py, double& al,
+                                     double& be, double& ga) {
+#pragma clang fp contract(off)
+    const int a = M.tri[3 * t], b = M.tri[3 * t + 1], c = M.tri[3 * t + 2];
+    const double2 A = *reinterpret_cast<const double2*>(M.xy + 2 * a);
+    const double2 B = *reinterpret_cast<const double2*>(M.xy + 2 * b);
+    const double2 C = *reinterpret_cast<const double2*>(M.xy + 2 * c);
+    const double dxa = A.x - C.x, dya = A.y - C.y;
+    const double dxb = B.x - C.x, dyb = B.y - C.y;
+    const double dxp = px - C.x, dyp = py - C.y;
+    const double det = dxa * dyb - dya * dxb;
+    al = (dxp * dyb - dyp * dxb) / det;
+    be = (dxa * dyp - dya * dxp) / det;
+    ga = 1.0 - al - be;
+}
+
+// Locate (px, py); returns the triangle and its barycentric coordinates.
+__device__ int locate(const MeshView& M, double px, double py, double& al, double& be,
+                      double& ga) {
+    double x0 = M.x0, y0 = M.y0, side = M.side;
+    int node = 0;
+    int2 q = M.qnode[0];
+    while (q.y < 0) {
+        side *= 0.5;
+        const int qx = px >= x0 + side;
+        const int qy = py >= y0 + side;
+        x0 += qx * side;
+        y0 += qy * side;
+        node = q.x + qx + 2 * qy;
+        q = M.qnode[node];
+    }
+    for (int k = 0; k < q.y; k++) {
+        const int t = M.qlist[q.x + k];
+        bary(M, t, px, py, al, be, ga);
+        if (0.0 <= al && 0.0 <= be && 0.0 <= ga) return t;
+    }
+    // Not inside any listed triangle: the reference walk from the leaf's first.
+    int t = M.qlist[q.x];
+    for (int step = 0; step <= M.ntri; step++) {
+        bary(M, t, px, py, al, be, ga);
+        if (0.0 <= al && 0.0 <= be && 0.0 <= ga) break;
+        const int* nb = M.nbr + 3 * t;
+        if (al < 0.0 && 0 <= nb[0]) { t = nb[0]; continue; }
+        if (be < 0.0 && 0 <= nb[1]) { t = nb[1]; continue; }
+        if (ga < 0.0 && 0 <= nb[2]) { t = nb[2]; continue; }
+        break;  // outside the hull: extrapolate from this hull triangle
+    }
+    return t;
+}
+
+// FP64 field at (x, y, z) in V/m; returns the triangle (-1 when gated off).
+__device__ __forceinline__ int efield_f64(const MeshView& M, double x, double y, double z,
+                                          double& ex, double& ey, double& ez) {
+    const double rho = sqrt(x * x + y * y);
+    const double zn = z / M.r0;
+    ex = ey = ez = 0.0;
+    if (zn < 0.0 || zn > 140.0) return -1;
+    double al, be, ga;
+    const int t = locate(M, rho / M.r0, zn, al, be, ga);
+    const int a = M.tri[3 * t], b = M.tri[3 * t + 1], c = M.tri[3 * t + 2];
+    const double vr = al * M.er[a] + be * M.er[b] + ga * M.er[c];
+    const double vz = al * M.ez[a] + be * M.ez[b] + ga * M.ez[c];
+    double cs = 1.0, sn = 0.0;
+    if (rho > 0.0) {
+        cs = x / rho;
+        sn = y / rho;
+    }
+    ex = M.E0 * vr * cs;
+    ey = M.E0 * vr * sn;
+    ez = M.E0 * vz;
+    return t;
+}
+
+__global__ void k_efield(MeshView M, int n, const double* __restrict__ X,
+                         const double* __restrict__ Y, const double* __restrict__ Z,
+                         double* __restrict__ ex, double* __restrict__ ey, double* __restrict__ ez,
+                         int* __restrict__ tri_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a, b, c;
+    const int t = efield_f64(M, X[i], Y[i], Z[i], a, b, c);
+    ex[i] = a;
+    ey[i] = b;
+    ez[i] = c;
+    if (tri_out) tri_out[i] = t;
+}
+
+// ------------------------------------------------------------ update kernels ----
+
+struct UpdateParams {
+    int sym;            // SHP_SYM_*
+    int nsec;
+    double dom[3];      // translational cell
+    int axes[3];        // wrap axes (domain > 0)
+    double bc[4];       // |x|,|y|,|z|,rho limits
+    double bc_time;
+    double e_over_amu;  // ELEM_CHARGE / AMU (particles.cpp:500)
+    int use_efield;
+};
+
+__device__ __forceinline__ void fold_and_bound(const UpdateParams& P, double& x, double& y,
+                                               double& z, double& vx, double& vy, int& st,
+                                               double t_now, double t_inj) {
+    if (P.sym == SHP_SYM_ROTATIONAL) {
+        if (P.nsec == 4) {
+            // rotateToFirstQuadrant, particles.cpp:232-262
+            if (x < 0 && y < 0) {
+                x = -x; y = -y; vx = -vx; vy = -vy;
+            } else if (x < 0) {
+                double tx = y, ty = -x; x = tx; y = ty;
+                tx = vy; ty = -vx; vx = tx; vy = ty;
+            } else if (y < 0) {
+                double tx = -y, ty = x; x = tx; y = ty;
+                tx = -vy; ty = vx; vx = tx; vy = ty;
+            }
+        } else if (P.nsec > 1) {
+            const double w = 2.0 * M_PI / P.nsec;
+            double phi = atan2(y, x);
+            if (phi < 0) phi += 2.0 * M_PI;
+            const int k = (int)floor(phi / w);
+            if (k > 0) {
+                const double c = cos(-k * w), s = sin(-k * w);
+                double tx = c * x - s * y, ty = s * x + c * y;
+                x = tx; y = ty;
+                tx = c * vx - s * vy; ty = s * vx + c * vy;
+                vx = tx; vy = ty;
+            }
+        }
+    } else if (P.sym == SHP_SYM_TRANSLATIONAL) {
+        // translateToSymmetryDomain, particles.cpp:282-330
+        double* p[3] = {&x, &y, &z};
+        for (int c = 0; c < 3; c++) {
+            if (!P.axes[c]) continue;
+            const double D = P.dom[c];
+            double t = *p[c] + D / 2;
+            if (t >= 0) {
+                *p[c] = fmod(t, D) - D / 2;
+            } else {
+                t = fmod(t, D);
+                *p[c] = t < 0 ? t + D / 2 : t - D / 2;
+            }
+        }
+    }
+    if (st == 1) {
+        bool out = false;
+        if (P.bc[0] != 0 && fabs(x) > P.bc[0]) out = true;
+        if (P.bc[1] != 0 && fabs(y) > P.bc[1]) out = true;
+        if (P.bc[2] != 0 && fabs(z) > P.bc[2]) out = true;
+        if (P.bc[3] != 0 && sqrt(x * x + y * y) > P.bc[3]) out = true;
+        if (P.bc_time != 0 && t_now - t_inj > P.bc_time) out = true;
+        if (out) st = 0;
+    }
+}
+
+// One half drift of ion state over [t_start, t_start + h].
+__device__ __forceinline__ void half_drift(const UpdateParams& P, double& x, double& y, double& z,
+                                           double& vx, double& vy, double vz, int& st,
+                                           double t_inj, double t_start, double h) {
+    const double t_end = t_start + h;
+    if (t_inj > t_end) return;  // not injected yet: outside [first, last)
+    if (st == 1 || st == 2) {
+        const double s = fmin(fmax(t_end - t_inj, 0.0), h);
+        x += vx * s;
+        y += vy * s;
+        z += vz * s;
+    }
+    fold_and_bound(P, x, y, z, vx, vy, st, t_end, t_inj);
+}
+
+// First half drift of a run; writes the active-masked status for the force.
+__global__ void k_drift(UpdateParams P, int n, double* __restrict__ X, double* __restrict__ Y,
+                        double* __restrict__ Z, double* __restrict__ VX, double* __restrict__ VY,
+                        const double* __restrict__ VZ, int* __restrict__ status,
+                        const double* __restrict__ tinj, double t_start, double h,
+                        int* __restrict__ eff) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = X[i], y = Y[i], z = Z[i], vx = VX[i], vy = VY[i];
+    int st = status[i];
+    const double ti = tinj[i];
+    half_drift(P, x, y, z, vx, vy, VZ[i], st, ti, t_start, h);
+    X[i] = x; Y[i] = y; Z[i] = z; VX[i] = vx; VY[i] = vy;
+    status[i] = st;
+    eff[i] = ti <= t_start + h ? st : 0;
+}
+
+// Kick over [t, t+dt] (ions active since the drift to t + dt/2), then the
+// second half drift of this step and, when `next` is set, the first half drift
+// of the next step.
+__global__ void k_kick_drift(UpdateParams P, MeshView M, int n, double* __restrict__ X,
+                             double* __restrict__ Y, double* __restrict__ Z,
+                             double* __restrict__ VX, double* __restrict__ VY,
+                             double* __restrict__ VZ, const double* __restrict__ Q,
+                             int* __restrict__ status, const int* __restrict__ species,
+                             const double* __restrict__ tinj, const double* __restrict__ AX,
+                             const double* __restrict__ AY, const double* __restrict__ AZ,
+                             double t, double dt, int next, int* __restrict__ eff) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = X[i], y = Y[i], z = Z[i];
+    double vx = VX[i], vy = VY[i], vz = VZ[i];
+    int st = status[i];
+    const double ti = tinj[i];
+    const double h = 0.5 * dt;
+    if (st == 1 && ti <= t + h) {
+        double ax = AX[i], ay = AY[i], az = AZ[i];
+        if (P.use_efield) {
+            double ex, ey, ez;
+            efield_f64(M, x, y, z, ex, ey, ez);
+            const double f = P.e_over_amu * Q[i] / species_mass(species[i]);
+            ax += f * ex;
+            ay += f * ey;
+            az += f * ez;
+        }
+        const double s = fmin(fmax(t + dt - ti, 0.0), dt);
+        vx += ax * s;
+        vy += ay * s;
+        vz += az * s;
+    }
+    half_drift(P, x, y, z, vx, vy, vz, st, ti, t + h, h);
+    if (next) half_drift(P, x, y, z, vx, vy, vz, st, ti, t + dt, h);
+    X[i] = x; Y[i] = y; Z[i] = z;
+    VX[i] = vx; VY[i] = vy; VZ[i] = vz;
+    status[i] = st;
+    if (next) eff[i] = ti <= t + dt + h ? st : 0;
+}
+
+static UpdateParams make_params(const shp_step_opts* o) {
+    UpdateParams P{};
+    P.sym = o->sym.type;
+    P.nsec = o->sym.nsec;
+    for (int c = 0; c < 3; c++) {
+        P.dom[c] = o->sym.domain[c];
+        P.axes[c] = o->sym.domain[c] > 0;
+    }
+    for (int c = 0; c < 4; c++) P.bc[c] = o->bc_space[c];
+    P.bc_time = o->bc_time;
+    P.e_over_amu = (double)1.6021766e-19f / (double)1.66053904e-27f;
+    P.use_efield = o->use_efield;
+    return P;
+}
+
+}  // namespace shp
+
+using namespace shp;
+
+extern "C" {
+
+int shp_efield_setup(shp_handle h, const shp_field_mesh* m) {
+    SHP_REQUIRE(h && m && m->xy && m->er && m->ez && m->tri && m->nbr, "shp_efield_setup: null");
+    SHP_REQUIRE(m->nnodes >= 3 && m->ntri >= 1, "shp_efield_setup: %d nodes, %d triangles",
+                m->nnodes, m->ntri);
+    SHP_REQUIRE(m->r0 > 0, "shp_efield_setup: r0 must be positive");
+    for (int k = 0; k < 3 * m->ntri; k++)
+        SHP_REQUIRE(m->tri[k] >= 0 && m->tri[k] < m->nnodes && m->nbr[k] < m->ntri,
+                    "shp_efield_setup: triangle/neighbour index out of range at %d", k);
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    // Quadtree over the bounding square of the nodes.
+    double x0 = m->xy[0], x1 = x0, y0 = m->xy[1], y1 = y0;
+    for (int i = 0; i < m->nnodes; i++) {
+        x0 = std::min(x0, m->xy[2 * i]);
+        x1 = std::max(x1, m->xy[2 * i]);
+        y0 = std::min(y0, m->xy[2 * i + 1]);
+        y1 = std::max(y1, m->xy[2 * i + 1]);
+    }
+    double side = std::max(x1 - x0, y1 - y0) * (1.0 + 1e-9) + 1e-300;
+    struct TB { double x0, x1, y0, y1; };
+    std::vector<TB> tb(m->ntri);
+    for (int t = 0; t < m->ntri; t++) {
+        TB b{1e300, -1e300, 1e300, -1e300};
+        for (int k = 0; k < 3; k++) {
+            const int v = m->tri[3 * t + k];
+            b.x0 = std::min(b.x0, m->xy[2 * v]);
+            b.x1 = std::max(b.x1, m->xy[2 * v]);
+            b.y0 = std::min(b.y0, m->xy[2 * v + 1]);
+            b.y1 = std::max(b.y1, m->xy[2 * v + 1]);
+        }
+        tb[t] = b;
+    }
+    std::vector<int2> nodes(1);
+    std::vector<int> list;
+    const int kLeafMax = 8, kDepthMax = 20;
+    std::function<void(int, double, double, double, std::vector<int>&, int)> build =
+        [&](int node, double cx, double cy, double s, std::vector<int>& ts, int depth) {
+            if ((int)ts.size() <= kLeafMax || depth >= kDepthMax) {
+                int off = (int)list.size();
+                if (ts.empty()) {
+                    // nearest triangle (by centroid) to the cell centre
+                    double mx = cx + s / 2, my = cy + s / 2, best = 1e300;
+                    int bt = 0;
+                    for (int t = 0; t < m->ntri; t++) {
+                        double gx = (tb[t].x0 + tb[t].x1) / 2 - mx, gy = (tb[t].y0 + tb[t].y1) / 2 - my;
+                        double d = gx * gx + gy * gy;
+                        if (d < best) { best = d; bt = t; }
+                    }
+                    list.push_back(bt);
+                    nodes[node] = make_int2(off, 0);
+                } else {
+                    list.insert(list.end(), ts.begin(), ts.end());
+                    nodes[node] = make_int2(off, (int)ts.size());
+                }
+                return;
+            }
+            int first = (int)nodes.size();
+            nodes.resize(nodes.size() + 4);
+            nodes[node] = make_int2(first, -1);
+            double hs = s / 2;
+            for (int q = 0; q < 4; q++) {
+                double qx0 = cx + (q & 1) * hs, qy0 = cy + (q >> 1) * hs;
+                std::vector<int> sub;
+                for (int t : ts)
+                    if (tb[t].x1 >= qx0 && tb[t].x0 <= qx0 + hs && tb[t].y1 >= qy0 && tb[t].y0 <= qy0 + hs)
+                        sub.push_back(t);
+                build(first + q, qx0, qy0, hs, sub, depth + 1);
+            }
+        };
+    {
+        std::vector<int> all(m->ntri);
+        for (int t = 0; t < m->ntri; t++) all[t] = t;
+        build(0, x0, y0, side, all, 0);
+    }
+    // A leaf that lists no triangle must still hand the walk a start: make its
+    // count 0 but keep its one-entry list (locate() then walks from it).
+    free_mesh(h);
+    FieldMesh& M = h->mesh;
+    auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+        SHP_HIP_CHECK(hipMalloc(dst, bytes));
+        SHP_HIP_CHECK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return SHP_OK;
+    };
+    int rc = 0;
+    rc |= up((void**)&M.xy, m->xy, sizeof(double) * 2 * m->nnodes);
+    rc |= up((void**)&M.er, m->er, sizeof(double) * m->nnodes);
+    rc |= up((void**)&M.ez, m->ez, sizeof(double) * m->nnodes);
+    rc |= up((void**)&M.tri, m->tri, sizeof(int) * 3 * m->ntri);
+    rc |= up((void**)&M.nbr, m->nbr, sizeof(int) * 3 * m->ntri);
+    rc |= up((void**)&M.qnode, nodes.data(), sizeof(int2) * nodes.size());
+    rc |= up((void**)&M.qlist, list.data(), sizeof(int) * list.size());
+    if (rc) return SHP_ERR_HIP;
+    M.nnodes = m->nnodes;
+    M.ntri = m->ntri;
+    M.nqnodes = (int)nodes.size();
+    M.nqlist = (int)list.size();
+    M.root[0] = x0;
+    M.root[1] = y0;
+    M.root[2] = side;
+    M.r0 = m->r0;
+    M.E0 = m->E0;
+    M.ready = true;
+    return SHP_OK;
+}
+
+int shp_efield_dev(shp_handle h, int n, const double* x, const double* y, const double* z,
+                   double* ex, double* ey, double* ez, int* tri_out) {
+    SHP_REQUIRE(h && x && y && z && ex && ey && ez, "shp_efield_dev: null argument");
+    SHP_REQUIRE(h->mesh.ready, "shp_efield_dev: no field mesh (call shp_efield_setup)");
+    SHP_REQUIRE(n >= 0, "shp_efield_dev: n=%d", n);
+    if (n == 0) return SHP_OK;
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    hipLaunchKernelGGL(k_efield, dim3(cdiv_l(n, 256)), dim3(256), 0, h->stream, view_of(h->mesh), n,
+                       x, y, z, ex, ey, ez, tri_out);
+    SHP_HIP_CHECK(hipGetLastError());
+    return SHP_OK;
+}
+
+int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double t0, double dt,
+                     int nsteps, double* ax, double* ay, double* az) {
+    SHP_REQUIRE(h && s && o && ax && ay && az, "shp_leapfrog_run: null argument");
+    SHP_REQUIRE(s->n >= 0 && nsteps >= 0 && dt >= 0, "shp_leapfrog_run: n=%d steps=%d dt=%g",
+                s->n, nsteps, dt);
+    SHP_REQUIRE(s->x && s->y && s->z && s->vx && s->vy && s->vz && s->q && s->status &&
+                    s->species && s->t_inj,
+                "shp_leapfrog_run: null state array");
+    SHP_REQUIRE(!o->use_efield || h->mesh.ready, "shp_leapfrog_run: E-field requested but no mesh");
+    SHP_REQUIRE(o->method == SHP_METHOD_DIRECT, "shp_leapfrog_run: method %d not available",
+                o->method);
+    if (s->n == 0 || nsteps == 0) return SHP_OK;
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    const int n = s->n;
+    SHP_WS(eff, int, h, WS_EFF_STATUS, n);
+    const UpdateParams P = make_params(o);
+    const MeshView M = h->mesh.ready ? view_of(h->mesh) : MeshView{};
+    const dim3 grid(cdiv_l(n, 256)), block(256);
+    hipLaunchKernelGGL(k_drift, grid, block, 0, h->stream, P, n, s->x, s->y, s->z, s->vx, s->vy,
+                       s->vz, s->status, s->t_inj, t0, 0.5 * dt, eff);
+    SHP_HIP_CHECK(hipGetLastError());
+    for (int k = 0; k < nsteps; k++) {
+        const double t = t0 + k * dt;
+        shp_particles p{n, s->x, s->y, s->z, s->q, eff, s->species};
+        int rc = shp_direct_dev(h, &p, o->precision, &o->sym, ax, ay, az);
+        if (rc) return rc;
+        SHP_HIP_CHECK(hipEventRecord(h->ev2, h->stream));
+        hipLaunchKernelGGL(k_kick_drift, grid, block, 0, h->stream, P, M, n, s->x, s->y, s->z, s->vx,
+                           s->vy, s->vz, s->q, s->status, s->species, s->t_inj, ax, ay, az, t, dt,
+                           k + 1 < nsteps ? 1 : 0, eff);
+        SHP_HIP_CHECK(hipGetLastError());
+        SHP_HIP_CHECK(hipEventRecord(h->ev3, h->stream));
+        h->timed_update = true;
+    }
+    return SHP_OK;
+}
+
+}  // extern "C"

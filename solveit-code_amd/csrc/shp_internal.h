@@ -32,6 +32,7 @@ __host__ __device__ inline double species_mass(int s) {
 }
 
 void set_error(const char* fmt, ...);
+void free_mesh(shp_handle h);
 
 // Grow-only device workspace slots owned by a context.
 enum WsSlot {
@@ -53,16 +54,39 @@ enum WsSlot {
     WS_WIDE_F64,    // float AoS widened to FP64 SoA
     WS_WIDE_I32,
     WS_BOX,         // bounding box (6 doubles)
+    WS_EFF_STATUS,  // status masked by the active (injected) range
     WS_NSLOTS
 };
 
 }  // namespace shp
 
+namespace shp {
+// Device copy of the background-field mesh plus its point locator: a quadtree
+// over the (r, z) bounding square whose leaves list the triangles whose
+// bounding boxes touch them (a leaf that touches none lists the triangle
+// nearest to its centre, from which the reference walk continues).
+struct FieldMesh {
+    bool ready = false;
+    int nnodes = 0, ntri = 0, nqnodes = 0, nqlist = 0;
+    double* xy = nullptr;    // [nnodes][2]
+    double* er = nullptr;    // [nnodes]
+    double* ez = nullptr;    // [nnodes]
+    int* tri = nullptr;      // [ntri][3]
+    int* nbr = nullptr;      // [ntri][3]
+    int2* qnode = nullptr;   // {first child | list offset, -1 | count}
+    int* qlist = nullptr;
+    double root[3] = {0, 0, 0};  // x0, y0, side
+    double r0 = 0, E0 = 0;
+};
+}  // namespace shp
+
 struct shp_context {
     int device = 0;
+    shp::FieldMesh mesh;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool timed = false;
+    hipEvent_t ev2 = nullptr, ev3 = nullptr;  // update-kernel timing
+    bool timed = false, timed_update = false;
     void* ws[shp::WS_NSLOTS] = {};
     size_t ws_bytes[shp::WS_NSLOTS] = {};
     int num_cus = 256;

@@ -42,6 +42,55 @@ class Symmetry(ctypes.Structure):
     ]
 
 
+class FieldMesh(ctypes.Structure):
+    _fields_ = [
+        ("nnodes", ctypes.c_int),
+        ("xy", ctypes.c_void_p),
+        ("er", ctypes.c_void_p),
+        ("ez", ctypes.c_void_p),
+        ("ntri", ctypes.c_int),
+        ("tri", ctypes.c_void_p),
+        ("nbr", ctypes.c_void_p),
+        ("r0", ctypes.c_double),
+        ("E0", ctypes.c_double),
+    ]
+
+
+class State(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("x", ctypes.c_void_p),
+        ("y", ctypes.c_void_p),
+        ("z", ctypes.c_void_p),
+        ("vx", ctypes.c_void_p),
+        ("vy", ctypes.c_void_p),
+        ("vz", ctypes.c_void_p),
+        ("q", ctypes.c_void_p),
+        ("status", ctypes.c_void_p),
+        ("species", ctypes.c_void_p),
+        ("t_inj", ctypes.c_void_p),
+    ]
+
+
+class StepOpts(ctypes.Structure):
+    _fields_ = [
+        ("method", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("sym", Symmetry),
+        ("use_efield", ctypes.c_int),
+        ("bc_space", ctypes.c_double * 4),
+        ("bc_time", ctypes.c_double),
+        ("fmm_order", ctypes.c_int),
+        ("leaf_size", ctypes.c_int),
+        ("theta", ctypes.c_double),
+    ]
+
+
+SHP_METHOD_DIRECT = 0
+SHP_METHOD_FMM = 1
+SHP_METHOD_BH = 2
+
+
 class Particles(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int),
@@ -86,6 +135,12 @@ def _load():
         "shp_direct_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), c_int,
                                    ctypes.POINTER(Symmetry), c_void_p, c_void_p, c_void_p]),
         "shp_last_kernel_ms": (c_double, [c_void_p]),
+        "shp_efield_setup": (c_int, [c_void_p, ctypes.POINTER(FieldMesh)]),
+        "shp_efield_dev": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p]),
+        "shp_leapfrog_run": (c_int, [c_void_p, ctypes.POINTER(State), ctypes.POINTER(StepOpts),
+                                     c_double, c_double, c_int, c_void_p, c_void_p, c_void_p]),
+        "shp_last_update_ms": (c_double, [c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -205,6 +260,70 @@ class Engine:
         return out
 
 
+def make_step_opts(method=0, precision=2, symmetry=None, use_efield=False, bc_space=(0, 0, 0, 0),
+                   bc_time=0.0, fmm_order=12, leaf_size=64, theta=0.5):
+    o = StepOpts()
+    o.method = method
+    o.precision = precision
+    sym = symmetry if symmetry is not None else make_symmetry("none")
+    o.sym = sym
+    o._sym = sym  # keeps the lattice array alive
+    o.use_efield = 1 if use_efield else 0
+    for c in range(4):
+        o.bc_space[c] = float(bc_space[c])
+    o.bc_time = float(bc_time)
+    o.fmm_order = int(fmm_order)
+    o.leaf_size = int(leaf_size)
+    o.theta = float(theta)
+    return o
+
+
+def _engine_efield_setup(self, mesh):
+    """Upload a tip-field mesh dict (xy, er, ez, tri, nbr, r0, E0) and build the locator."""
+    keep = {k: np.ascontiguousarray(mesh[k], dtype=np.float64) for k in ("xy", "er", "ez")}
+    keep.update({k: np.ascontiguousarray(mesh[k], dtype=np.int32) for k in ("tri", "nbr")})
+    m = FieldMesh()
+    m.nnodes = keep["er"].shape[0]
+    m.xy, m.er, m.ez = keep["xy"].ctypes.data, keep["er"].ctypes.data, keep["ez"].ctypes.data
+    m.ntri = keep["tri"].shape[0]
+    m.tri, m.nbr = keep["tri"].ctypes.data, keep["nbr"].ctypes.data
+    m.r0, m.E0 = float(mesh["r0"]), float(mesh["E0"])
+    _check(lib.shp_efield_setup(self.h, ctypes.byref(m)), "shp_efield_setup")
+
+
+def _engine_efield_dev(self, x, y, z, want_tri=False):
+    import torch
+    n = int(x.shape[0])
+    out = tuple(torch.empty(n, dtype=torch.float64, device=x.device) for _ in range(3))
+    tri = torch.empty(n, dtype=torch.int32, device=x.device) if want_tri else None
+    _check(lib.shp_efield_dev(self.h, n, x.data_ptr(), y.data_ptr(), z.data_ptr(),
+                              out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(),
+                              tri.data_ptr() if tri is not None else None), "shp_efield_dev")
+    return (out, tri) if want_tri else out
+
+
+def _engine_leapfrog_run(self, state, opts, t0, dt, nsteps, acc=None):
+    import torch
+    n = state.n
+    if acc is None:
+        acc = tuple(torch.zeros(n, dtype=torch.float64, device=state.x.device) for _ in range(3))
+    s = state.as_state()
+    _check(lib.shp_leapfrog_run(self.h, ctypes.byref(s), ctypes.byref(opts), float(t0), float(dt),
+                                int(nsteps), acc[0].data_ptr(), acc[1].data_ptr(),
+                                acc[2].data_ptr()), "shp_leapfrog_run")
+    return acc
+
+
+def _engine_last_update_ms(self):
+    return lib.shp_last_update_ms(self.h)
+
+
+Engine.efield_setup = _engine_efield_setup
+Engine.efield_dev = _engine_efield_dev
+Engine.leapfrog_run = _engine_leapfrog_run
+Engine.last_update_ms = _engine_last_update_ms
+
+
 class DeviceState:
     """FP64 SoA particle state resident in HBM (torch tensors as plain storage)."""
 
@@ -230,6 +349,21 @@ class DeviceState:
                 kw[k] = f64(d[k])
         return cls(f64(d["x"]), f64(d["y"]), f64(d["z"]), f64(d["q"]), i32(d["status"]),
                    i32(d["species"]), **kw)
+
+    def as_state(self):
+        s = State()
+        s.n = self.n
+        for k in ("x", "y", "z", "vx", "vy", "vz", "q", "status", "species", "t_inj"):
+            setattr(s, k, getattr(self, k).data_ptr())
+        return s
+
+    def to_numpy(self):
+        out = {}
+        for k in ("x", "y", "z", "vx", "vy", "vz", "q", "status", "species", "t_inj"):
+            v = getattr(self, k)
+            if v is not None:
+                out[k] = v.cpu().numpy()
+        return out
 
     def as_struct(self):
         p = Particles()

@@ -103,3 +103,94 @@ def full_lattice(n_half, axes):
     """All lattice points of the (2 n_half + 1)^d image block on the symmetry axes."""
     rng = [range(-n_half, n_half + 1) if a else range(0, 1) for a in axes]
     return np.array([(i, j, k) for i in rng[0] for j in rng[1] for k in rng[2]], dtype=np.int32)
+
+
+# ----------------------------------------------------------------------------
+# Synthetic tip field on a scattered (r, z) triangulation
+# ----------------------------------------------------------------------------
+# Reference values of the single-emitter info file (info_323.txt: r0, E0).
+TIP_R0 = 5.0e-6        # m, normalisation length
+TIP_E0 = 6.390574e7    # V/m, field scale
+
+
+def hyperboloid_field(rho, z, a=200.0, tip_radius=1.0):
+    """Analytic field of a hyperboloid tip facing a plane, in units of r0.
+
+    Prolate spheroidal coordinates with foci at Z = +-a, Z = d - z, where the
+    emitter surface is eta = eta0 (apex radius of curvature `tip_radius`, apex
+    at z = 0) and the counter-electrode is eta = 0.  The potential is
+    V ln((1+eta)/(1-eta)) / ln((1+eta0)/(1-eta0)); returns (E_r, E_z) = -grad
+    phi scaled so that the field at the apex is (0, 1)."""
+    k = tip_radius / a
+    eta0 = (-k + np.sqrt(k * k + 4.0)) / 2.0
+    d = a * eta0
+
+    def raw(rho, z):
+        Z = d - z
+        r1 = np.sqrt(rho * rho + (Z - a) ** 2)
+        r2 = np.sqrt(rho * rho + (Z + a) ** 2)
+        eta = (r2 - r1) / (2 * a)
+        dphi = 2.0 / (1.0 - eta * eta)
+        deta_drho = (rho / r2 - rho / r1) / (2 * a)
+        deta_dZ = ((Z + a) / r2 - (Z - a) / r1) / (2 * a)
+        return -dphi * deta_drho, dphi * deta_dZ
+
+    _, ez0 = raw(np.zeros(1), np.zeros(1))
+    er, ez = raw(np.asarray(rho, np.float64), np.asarray(z, np.float64))
+    return er / ez0[0], ez / ez0[0]
+
+
+def tip_field_mesh(nr=200, nz=200, rmax=140.0, zmax=140.0, power=2.0):
+    """Nodes, triangles and neighbours of the (r, z) interpolation mesh.
+
+    Node (ir, iz) sits at (rmax (ir/(nr-1))^power, zmax (iz/(nz-1))^power) in
+    units of r0 and has id iz*nr + ir.  Each cell is split along its
+    (ir,iz)-(ir+1,iz+1) diagonal into two counter-clockwise triangles.
+    Conventions of the reference's loader and walk (interpolation_scheme.cpp:
+    66-150, pwl_interp_2d_scattered.c:1762-1990): 0-based node ids, neighbour k
+    across the edge opposite vertex k, -1 on the hull.
+    Returns dict(xy[nd,2], er[nd], ez[nd], tri[nt,3], nbr[nt,3])."""
+    rs = rmax * (np.arange(nr) / (nr - 1)) ** power
+    zs = zmax * (np.arange(nz) / (nz - 1)) ** power
+    R, Z = np.meshgrid(rs, zs)           # [iz, ir]
+    xy = np.stack([R.ravel(), Z.ravel()], axis=1)
+    er, ez = hyperboloid_field(xy[:, 0], xy[:, 1])
+    ncr, ncz = nr - 1, nz - 1
+    ir, iz = np.meshgrid(np.arange(ncr), np.arange(ncz))
+    ir, iz = ir.ravel(), iz.ravel()
+    n00 = iz * nr + ir
+    n10 = n00 + 1
+    n01 = n00 + nr
+    n11 = n01 + 1
+    ci = iz * ncr + ir
+    nt = 2 * ncr * ncz
+    tri = np.empty((nt, 3), np.int32)
+    nbr = np.full((nt, 3), -1, np.int32)
+    t1, t2 = 2 * ci, 2 * ci + 1
+    tri[t1] = np.stack([n00, n10, n11], 1)
+    tri[t2] = np.stack([n00, n11, n01], 1)
+    # T1 = (n00, n10, n11): opposite n00 -> right cell's T2, opposite n10 -> own
+    # T2, opposite n11 -> lower cell's T2.
+    nbr[t1, 0] = np.where(ir + 1 < ncr, 2 * (ci + 1) + 1, -1)
+    nbr[t1, 1] = t2
+    nbr[t1, 2] = np.where(iz > 0, 2 * (ci - ncr) + 1, -1)
+    # T2 = (n00, n11, n01): opposite n00 -> upper cell's T1, opposite n11 ->
+    # left cell's T1, opposite n01 -> own T1.
+    nbr[t2, 0] = np.where(iz + 1 < ncz, 2 * (ci + ncr), -1)
+    nbr[t2, 1] = np.where(ir > 0, 2 * (ci - 1), -1)
+    nbr[t2, 2] = t1
+    return {"xy": np.ascontiguousarray(xy), "er": er, "ez": ez, "tri": tri, "nbr": nbr,
+            "r0": TIP_R0, "E0": TIP_E0}
+
+
+def write_mesh_csv(mesh, folder, fields="Fields_syn.csv", triangles="Triangles_syn.csv"):
+    """Write the mesh in the reference's CSV layout (x,y,Er,Ez / a,b,c,n0,n1,n2)."""
+    import os
+    os.makedirs(folder, exist_ok=True)
+    with open(os.path.join(folder, fields), "w") as f:
+        for (x, y), a, b in zip(mesh["xy"], mesh["er"], mesh["ez"]):
+            f.write(f"{float(x):.17g},{float(y):.17g},{float(a):.17g},{float(b):.17g}\n")
+    with open(os.path.join(folder, triangles), "w") as f:
+        for t, n in zip(mesh["tri"], mesh["nbr"]):
+            f.write(f"{t[0]},{t[1]},{t[2]},{n[0]},{n[1]},{n[2]}\n")
+    return fields, triangles

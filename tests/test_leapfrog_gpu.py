@@ -1,0 +1,149 @@
+"""Parity of the HIP background field and DKD leapfrog with the oracle (FP64),
+through the C ABI.  Bar: 1e-12 relative for the field; trajectories of a few
+tens of steps within 1e-10 relative (double sums in a different order, carried
+through the integration)."""
+import numpy as np
+import pytest
+
+import oracle
+from solveit_hip import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    import solveit_hip
+    import torch
+    assert torch.cuda.is_available()
+    e = solveit_hip.Engine(0)
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def mesh():
+    return synth.tip_field_mesh()
+
+
+def _t(a, dtype="f64"):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64 if dtype == "f64"
+                           else torch.int32, device="cuda:0")
+
+
+def test_efield_matches_oracle(eng, mesh):
+    import torch
+    eng.efield_setup(mesh)
+    rng = np.random.default_rng(5)
+    d = synth.cone(20000, seed=6)
+    # plume points plus points spread over the whole hull and some gated ones
+    n2 = 5000
+    rr = 140 * mesh["r0"] * rng.random(n2) ** 2
+    ph = rng.uniform(0, 2 * np.pi, n2)
+    zz = 140 * mesh["r0"] * rng.random(n2) ** 2
+    x = np.concatenate([d["x"], rr * np.cos(ph), [0.0, 1e-7, 0.0, 3e-6]])
+    y = np.concatenate([d["y"], rr * np.sin(ph), [0.0, 0.0, 0.0, 0.0]])
+    z = np.concatenate([d["z"], zz, [0.0, -1e-9, 141 * mesh["r0"], 0.0]])
+    print("case efield n=%d" % x.shape[0], flush=True)
+    (ex, ey, ez), tri = eng.efield_dev(_t(x), _t(y), _t(z), want_tri=True)
+    eng.sync()
+    got = torch.stack([ex, ey, ez], 1).cpu().numpy()
+    ref, edge, _ = oracle.efield_f64(mesh, x, y, z)
+    assert np.all(edge == 0)
+    scale = np.maximum(np.linalg.norm(ref, axis=1), 1e-300)
+    err = np.linalg.norm(got - ref, axis=1) / scale
+    assert np.all(np.linalg.norm(ref, axis=1)[-3:-1] == 0) and np.all(got[-3:-1] == 0)
+    assert err.max() <= 1e-12
+
+
+def _run_both(eng, mesh, d, nsteps, dt, **kw):
+    import solveit_hip
+    import torch
+    symkind = kw.pop("sym", "none")
+    nsec = kw.pop("nsec", 4)
+    lattice = kw.pop("lattice", None)
+    domain = kw.pop("domain", (0.0, 0.0, 0.0))
+    bc_space = kw.pop("bc_space", (0, 0, 0, 0))
+    bc_time = kw.pop("bc_time", 0.0)
+    use_field = mesh is not None
+    st = solveit_hip.DeviceState.from_numpy(d)
+    torch.cuda.synchronize()
+    if use_field:
+        eng.efield_setup(mesh)
+    sym = solveit_hip.make_symmetry(symkind, nsec=nsec, lattice=lattice, domain=domain)
+    opts = solveit_hip.make_step_opts(method=solveit_hip.SHP_METHOD_DIRECT,
+                                      precision=solveit_hip.SHP_PREC_FP64, symmetry=sym,
+                                      use_efield=use_field, bc_space=bc_space, bc_time=bc_time)
+    acc = eng.leapfrog_run(st, opts, 0.0, dt, nsteps)
+    eng.sync()
+    got = st.to_numpy()
+    ref, racc = oracle.leapfrog_run_f64(d, 0.0, dt, nsteps, mesh=mesh, sym=symkind, nsec=nsec,
+                                        lattice=lattice, domain=domain, bc_space=bc_space,
+                                        bc_time=bc_time)
+    return got, ref, torch.stack(acc, 1).cpu().numpy(), racc
+
+
+def _close(got, ref, keys=("x", "y", "z", "vx", "vy", "vz"), tol=1e-10):
+    for k in keys:
+        scale = np.max(np.abs(ref[k])) + 1e-300
+        assert np.max(np.abs(got[k] - ref[k])) <= tol * scale, k
+
+
+def test_leapfrog_with_field_matches_oracle(eng, mesh):
+    d = synth.cone(1500, seed=8)
+    print("case leapfrog field", flush=True)
+    got, ref, acc, racc = _run_both(eng, mesh, d, 20, 1e-12)
+    _close(got, ref)
+    np.testing.assert_array_equal(got["status"], ref["status"])
+
+
+def test_leapfrog_injection_and_boundaries(eng, mesh):
+    d = synth.cone(1200, seed=9)
+    dt = 1e-12
+    d["t_inj"] = np.sort(np.random.default_rng(1).uniform(0, 10 * dt, 1200))
+    print("case leapfrog injection/bc", flush=True)
+    got, ref, _, _ = _run_both(eng, mesh, d, 15, dt, bc_space=(0, 0, 9.0e-6, 2e-7),
+                               bc_time=12 * dt)
+    np.testing.assert_array_equal(got["status"], ref["status"])
+    assert np.any(ref["status"] == 0) and np.any(ref["status"] == 1)
+    _close(got, ref)
+
+
+@pytest.mark.parametrize("nsec", [4, 6])
+def test_leapfrog_rotational(eng, mesh, nsec):
+    d = synth.cone(800, seed=10)
+    print(f"case leapfrog rot {nsec}", flush=True)
+    got, ref, _, _ = _run_both(eng, None, d, 10, 1e-12, sym="rotational", nsec=nsec)
+    _close(got, ref)
+
+
+def test_leapfrog_translational(eng):
+    d = synth.cell_fill(600, seed=11)
+    lat = synth.full_lattice(1, (1, 1, 1))
+    print("case leapfrog trans", flush=True)
+    got, ref, _, _ = _run_both(eng, None, d, 10, 1e-12, sym="translational", lattice=lat,
+                               domain=(2e-6, 2e-6, 2e-6))
+    _close(got, ref)
+
+
+def test_leapfrog_fp32c_energy_drift(eng, mesh):
+    """FP32C forces with FP64 state: the trajectory stays within float-force
+    error of the FP64 one over 50 steps."""
+    import solveit_hip
+    import torch
+    d = synth.cone(4096, seed=12)
+    out = {}
+    for prec in (solveit_hip.SHP_PREC_FP64, solveit_hip.SHP_PREC_FP32C):
+        st = solveit_hip.DeviceState.from_numpy(d)
+        torch.cuda.synchronize()
+        eng.efield_setup(mesh)
+        opts = solveit_hip.make_step_opts(precision=prec, use_efield=True)
+        eng.leapfrog_run(st, opts, 0.0, 1e-12, 50)
+        eng.sync()
+        out[prec] = st.to_numpy()
+    a, b = out[solveit_hip.SHP_PREC_FP64], out[solveit_hip.SHP_PREC_FP32C]
+    dv = np.sqrt((a["vx"] - b["vx"]) ** 2 + (a["vy"] - b["vy"]) ** 2 + (a["vz"] - b["vz"]) ** 2)
+    v = np.sqrt(a["vx"] ** 2 + a["vy"] ** 2 + a["vz"] ** 2)
+    print(f"  fp32c vs fp64 velocity rel err after 50 steps: {np.max(dv / v):.3e}")
+    assert np.max(dv / v) < 1e-5
