@@ -1,11 +1,17 @@
 #!/usr/bin/env python3
 """Throughput of the force engine's hot path on one MI355X.
 
-One step = one force evaluation of the whole synthetic cone plume
-(N = 1,048,576 ions, 5 % -1e) with the engine's FP32-compensated path, inputs
-resident in HBM.  The FP64 GPU direct path is timed once on the same input as
-the accuracy reference (rel L2 on a seeded 8192-target subsample) and as the
-speed-up denominator.  Prints one JSON line.
+One step = one DKD leapfrog step of the whole synthetic cone plume
+(N = 1,048,576 ions, 10 um long, 20 deg, lognormal radius, 5 % -1e) with FP64
+state: the FMM force (FP32 expansions and pairs, compensated sums, order p=10,
+near field two leaf cells wide), the pwl background field and the fused
+kick/drift update.  Inputs are resident in HBM when the timed region starts.
+
+Also measured, on the same state: the force error of the FMM against the GPU
+FP64 direct sum on a seeded 8192-ion subsample (relative L2 as
+integration_schemes.cpp:411-455 defines it), one FP64 direct leapfrog step
+(the speed-up denominator), per-pass FMM kernel times, and a CPU baseline (the
+oracle's FP64 direct sum on a bounded target sample).  Prints one JSON line.
 """
 import argparse
 import json
@@ -19,13 +25,12 @@ for _p in (ROOT, os.path.join(ROOT, "solveit-code_amd")):
         sys.path.insert(0, _p)
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak (dense)
-FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_PAIR = 20
 
 
-def cpu_baseline(d, n_targets, seconds_hint=""):
+def cpu_baseline(d, n_targets):
     """The oracle's FP64 direct sum (single thread) on a bounded target sample of
-    the same workload; scaled to particle-steps/s of a full N x N step."""
+    the same workload, scaled to particle-steps/s of a full N x N direct step."""
     import oracle
     n = d["x"].shape[0]
     t0 = time.perf_counter()
@@ -33,71 +38,102 @@ def cpu_baseline(d, n_targets, seconds_hint=""):
     dt = time.perf_counter() - t0
     pairs_per_s = n_targets * n / dt
     return {"value": pairs_per_s / n, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n_targets} targets x {n} sources of the same plume, oracle FP64 direct "
-                      f"(1 thread, {dt:.1f} s, {pairs_per_s:.3e} pairs/s)"}
+            "sample": f"oracle FP64 direct sum, {n_targets} targets x {n} sources of the same "
+                      f"plume ({dt:.1f} s, {pairs_per_s:.3e} pairs/s), scaled to one N x N step"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--order", type=int, default=10)
+    ap.add_argument("--leaf-size", type=int, default=20)
+    ap.add_argument("--dt", type=float, default=1e-12)
     ap.add_argument("--cpu-targets", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
+              "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
     import torch
 
+    import oracle
     import solveit_hip
     from solveit_hip import synth
 
     n = args.n
     d = synth.cone(n)
+    mesh = synth.tip_field_mesh()
+    eng = solveit_hip.Engine(0)
+    eng.efield_setup(mesh)
     st = solveit_hip.DeviceState.from_numpy(d)
     torch.cuda.synchronize()
-    eng = solveit_hip.Engine(0)
-    out = tuple(torch.empty(n, dtype=torch.float64, device="cuda:0") for _ in range(3))
-
-    for _ in range(args.warmup):
-        eng.direct_dev(st, precision=solveit_hip.SHP_PREC_FP32C, out=out)
+    acc = tuple(torch.zeros(n, dtype=torch.float64, device="cuda:0") for _ in range(3))
+    opts = solveit_hip.make_step_opts(method=solveit_hip.SHP_METHOD_FMM, use_efield=True,
+                                      fmm_order=args.order, leaf_size=args.leaf_size, fmm_ring=2)
+    t = 0.0
+    eng.leapfrog_run(st, opts, t, args.dt, args.warmup, acc=acc)
+    t += args.warmup * args.dt
     eng.sync()
     torch.cuda.synchronize()
-    kernel_ms = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.direct_dev(st, precision=solveit_hip.SHP_PREC_FP32C, out=out)
-        kernel_ms.append(eng.last_kernel_ms())
+    eng.leapfrog_run(st, opts, t, args.dt, args.steps, acc=acc)
     eng.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    t += args.steps * args.dt
     ms_per_step = wall * 1e3 / args.steps
-    a32 = torch.stack(out, 1).cpu().numpy()
-
-    # FP64 direct on the GPU: accuracy reference and speed-up denominator.
-    out64 = tuple(torch.empty(n, dtype=torch.float64, device="cuda:0") for _ in range(3))
-    eng.sync()
-    t1 = time.perf_counter()
-    eng.direct_dev(st, precision=solveit_hip.SHP_PREC_FP64, out=out64)
-    eng.sync()
-    fp64_ms = (time.perf_counter() - t1) * 1e3
-    a64 = torch.stack(out64, 1).cpu().numpy()
-    rng = np.random.default_rng(8192)
-    sub = rng.choice(n, size=min(8192, n), replace=False)
-    import oracle
-    l2, l2max, _ = oracle.rel_l2(a32[sub], a64[sub])
-
-    nsrc = int(np.sum((d["status"] == 1) | (d["status"] == 2)))
-    ntgt = int(np.sum(d["status"] == 1))
-    pairs = float(ntgt) * float(nsrc)
-    kms = float(np.mean(kernel_ms))
-    achieved = FLOP_PER_PAIR * pairs / (kms * 1e-3) / 1e12
     value = n * args.steps / wall
+
+    # Per-pass kernel times over the same number of single steps.
+    passes = []
+    upd_ms = []
+    for _ in range(args.steps):
+        eng.leapfrog_run(st, opts, t, args.dt, 1, acc=acc)
+        t += args.dt
+        passes.append(eng.fmm_stats())
+        upd_ms.append(eng.last_update_ms())
+    eng.sync()
+    mean = {k: float(np.mean([p[k] for p in passes])) for k in passes[0]}
+
+    # Accuracy of the FMM force against the GPU FP64 direct sum on this state.
+    a_f = torch.stack(eng.fmm_dev(st, order=args.order, leaf_size=args.leaf_size, ring=2), 1)
+    eng.sync()
+    a_fmm = a_f.cpu().numpy()
+    fp64_step_ms = None
+    l2 = l2max = None
+    if not args.no_fp64:
+        a_d = torch.stack(eng.direct_dev(st, precision=solveit_hip.SHP_PREC_FP64), 1)
+        eng.sync()
+        a_dir = a_d.cpu().numpy()
+        sub = np.random.default_rng(8192).choice(n, size=min(8192, n), replace=False)
+        l2, l2max, _ = oracle.rel_l2(a_fmm[sub], a_dir[sub])
+        opts64 = solveit_hip.make_step_opts(method=solveit_hip.SHP_METHOD_DIRECT,
+                                            precision=solveit_hip.SHP_PREC_FP64, use_efield=True)
+        eng.sync()
+        t1 = time.perf_counter()
+        eng.leapfrog_run(st, opts64, t, args.dt, 1, acc=acc)
+        eng.sync()
+        fp64_step_ms = (time.perf_counter() - t1) * 1e3
+
+    m2l_flop = 2.0 * mean["m2l_madds"] * mean["m2l_pairs"]
+    p2p_flop = FLOP_PER_PAIR * mean["p2p_pairs"]
+    m2l_tf = m2l_flop / (mean["m2l_ms"] * 1e-3) / 1e12
+    leaf_tf = p2p_flop / (mean["leaf_ms"] * 1e-3) / 1e12
+    if mean["m2l_ms"] >= mean["leaf_ms"]:
+        roof = {"kernel": "k_fmm_m2l", "achieved": m2l_tf, "kernel_ms": mean["m2l_ms"],
+                "flop_per_launch": m2l_flop}
+    else:
+        roof = {"kernel": "k_fmm_leaf", "achieved": leaf_tf, "kernel_ms": mean["leaf_ms"],
+                "flop_per_launch": p2p_flop}
+    roof.update({"bound": "fp32-vector", "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": roof["achieved"] / FP32_PEAK_TFLOPS, "traffic": None})
 
     line = {
         "metric": "particle-steps/s at N=1M, FP32-comp FMM vs FP64 direct at rel L2 err<=1e-5",
@@ -113,27 +149,27 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": f"cone plume N={n} (10 um, 20 deg, lognormal r, 5% -1e), "
-                        "direct P2P FP32 pair arithmetic with compensated per-target sums, "
-                        "one force evaluation per step",
+            "workload": f"cone plume N={n} (10 um, 20 deg, lognormal r sigma 0.5 x 100 nm, "
+                        "5% -1e), DKD leapfrog step with FMM force (FP32 expansions and pairs, "
+                        "compensated sums) + pwl tip field, FP64 state",
             "n": n,
-            "method": "direct-fp32c",
-            "rel_l2_vs_fp64_gpu": l2,
-            "max_rel_vs_fp64_gpu": l2max,
-            "fp64_direct_ms": fp64_ms,
-            "speedup_vs_fp64_direct": fp64_ms / ms_per_step,
+            "method": "fmm",
+            "order": args.order,
+            "leaf_level": int(mean["leaf_level"]),
+            "near_ring": 2,
+            "rel_l2_vs_fp64_direct": l2,
+            "max_rel_vs_fp64_direct": l2max,
+            "fp64_direct_step_ms": fp64_step_ms,
+            "speedup_vs_fp64_direct": (fp64_step_ms / ms_per_step) if fp64_step_ms else None,
+            "pass_ms": {k: mean[k] for k in ("p2m_m2m_ms", "ttable_ms", "m2l_ms", "l2l_ms",
+                                             "leaf_ms")},
+            "update_kernel_ms": float(np.mean(upd_ms)),
+            "m2l_pairs": mean["m2l_pairs"],
+            "p2p_pairs": mean["p2p_pairs"],
+            "m2l_tflops": m2l_tf,
+            "p2p_tflops": leaf_tf,
         },
-        "roofline": {
-            "bound": "fp32-vector",
-            "kernel": "k_direct_f32c",
-            "achieved": achieved,
-            "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel_ms": kms,
-            "flop_per_launch": FLOP_PER_PAIR * pairs,
-        },
+        "roofline": roof,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(d, args.cpu_targets)

@@ -102,8 +102,31 @@ int shp_direct_dev(shp_handle h, const shp_particles* p, int precision,
                    const shp_symmetry* sym, double* ax, double* ay, double* az);
 
 /* Kernel time of the last shp_direct_dev/shp_direct_host force kernel (ms),
- * measured with HIP events on the handle's stream. */
+ * measured with HIP events on the handle's stream (for shp_fmm_dev: the span
+ * from P2M to the last leaf kernel). */
 double shp_last_kernel_ms(shp_handle h);
+
+/*
+ * Fast multipole evaluation of the same softened sum on the device-resident
+ * FP64 state (FP32 expansions and pairs, compensated per-target sums).
+ * Replaces MultipoleMethodForceCalculation::updateAcceleration and the passes
+ * it drives (multipole_method_force_calculation.cpp:148-522: P2P, P2M, M2M,
+ * M2L, L2L, L2P of fmm_cpukernel.cpp:103-1317).  order: expansion order p
+ * (1..12); leaf_level: uniform leaf level of the octree (0 = choose the
+ * smallest level whose mean leaf occupancy is <= leaf_size); ring: near-field
+ * width in leaf cells (1 or 2; the far field starts beyond it).
+ */
+int shp_fmm_dev(shp_handle h, const shp_particles* p, int order, int leaf_level, int leaf_size,
+                int ring, double* ax, double* ay, double* az);
+
+/* Leaf level used by the last shp_fmm_dev call on this handle. */
+int shp_fmm_last_leaf_level(shp_handle h);
+
+/* Measurements of the last shp_fmm_dev call (waits for it): out[0..4] = ms of
+ * P2M+M2M, translation tables, M2L, L2L, L2P+P2P; out[5] = M2L box pairs;
+ * out[6] = P2P particle pairs; out[7] = leaf level; out[8] = multiply-adds per
+ * M2L box pair. */
+int shp_fmm_stats(shp_handle h, double* out);
 
 /* ---- background E-field (pwl on a scattered (r, z) triangulation) ---- */
 
@@ -166,7 +189,9 @@ typedef struct shp_step_opts {
     double bc_time;      /* max time since injection (s); 0 disables */
     int fmm_order;       /* FMM expansion order p */
     int leaf_size;       /* FMM / Barnes-Hut leaf capacity */
-    double theta;        /* Barnes-Hut opening angle / FMM separation */
+    double theta;        /* Barnes-Hut opening angle */
+    int fmm_ring;        /* FMM near-field width in leaf cells (1 or 2) */
+    int leaf_level;      /* FMM uniform leaf level (0 = from leaf_size) */
 } shp_step_opts;
 
 /*

@@ -70,6 +70,8 @@ int shp_create(int device, shp_handle* out) {
     if (e == hipSuccess) e = hipEventCreate(&h->ev1);
     if (e == hipSuccess) e = hipEventCreate(&h->ev2);
     if (e == hipSuccess) e = hipEventCreate(&h->ev3);
+    for (int k = 0; k < 8 && e == hipSuccess; k++) e = hipEventCreate(&h->fev[k]);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->fmm_stats, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) {
         shp::set_error("shp_create: %s", hipGetErrorString(e));
         delete h;
@@ -89,7 +91,11 @@ int shp_destroy(shp_handle h) {
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->ev2) (void)hipEventDestroy(h->ev2);
     if (h->ev3) (void)hipEventDestroy(h->ev3);
+    for (int k = 0; k < 8; k++)
+        if (h->fev[k]) (void)hipEventDestroy(h->fev[k]);
+    if (h->fmm_stats) (void)hipFree(h->fmm_stats);
     shp::free_mesh(h);
+    shp::fmm_release(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return SHP_OK;

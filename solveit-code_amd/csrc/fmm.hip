@@ -108,7 +108,7 @@ static void build_tables(FmmHostTables& t, int p) {
     for (;; G++) {
         long ntask = 0;
         for (int be = 0; be < nc; be++) ntask += (pa[be].size() + G - 1) / G;
-        if (ntask <= kFmmThreads) break;
+        if (ntask <= 2 * kFmmThreads) break;
     }
     t.pair_alpha.clear();
     t.pair_gamma.clear();
@@ -174,6 +174,7 @@ static void build_tables(FmmHostTables& t, int p) {
 
 struct FmmDevTables {
     int p = -1, nc = 0, ntask = 0;
+    long npairs = 0;  // (alpha, beta) pairs of one M2L
     int4* mi = nullptr;
     int* pair_alpha = nullptr;
     int* pair_gamma = nullptr;
@@ -225,12 +226,17 @@ static int get_tables(shp_handle h, int p, FmmDevTables** out) {
         d.p = p;
         d.nc = t.nc;
         d.ntask = (int)t.task.size();
+        d.npairs = (long)t.pair_alpha.size();
     }
     *out = &d;
     return SHP_OK;
 }
 
+static std::map<shp_handle, int> g_last_level;
+static int& last_leaf_level(shp_handle h) { return g_last_level[h]; }
+
 void fmm_release(shp_handle h) {
+    g_last_level.erase(h);
     auto it = g_tables.find(h);
     if (it == g_tables.end()) return;
     FmmDevTables& d = it->second;
@@ -344,19 +350,40 @@ __global__ void k_fmm_gather(const int* __restrict__ perm, int na, const double*
     sp[i] = make_double4(X[j], Y[j], Z[j], Q[j]);
 }
 
-// Number of active particles (keys != ~0 sort last) and box counts per level.
-__global__ void k_fmm_level_counts(const uint64_t* __restrict__ keys, int n, int L,
-                                   int* __restrict__ counts /* [L+2]: na, nbox[0..L] */) {
+// Number of active particles (keys != ~0 sort last) and, per level, the number
+// of particles that open a new box there: bin[lmin] counts particles whose key
+// first differs from the previous one at level lmin (so they open boxes at all
+// levels >= lmin).  counts = {na, bin[0..21]}.
+__global__ __launch_bounds__(256) void k_fmm_level_hist(const uint64_t* __restrict__ keys, int n,
+                                                        int* __restrict__ counts) {
+    __shared__ int bins[kMaxLevel + 1];
+    if (threadIdx.x <= kMaxLevel) bins[threadIdx.x] = 0;
+    __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t k = keys[i];
-    if (k == ~0ULL) return;
-    atomicAdd(&counts[0], 1);
-    for (int l = 0; l <= L; l++) {
-        const int sh = 3 * (kMaxLevel - l);
-        const bool start = (i == 0) || ((keys[i - 1] >> sh) != (k >> sh));
-        if (start) atomicAdd(&counts[1 + l], 1);
+    if (i < n) {
+        const uint64_t k = keys[i];
+        if (k != ~0ULL) {
+            if (i + 1 == n || keys[i + 1] == ~0ULL) counts[0] = i + 1;
+            int lmin = 0;
+            if (i > 0) {
+                const uint64_t x = k ^ keys[i - 1];
+                lmin = x == 0 ? kMaxLevel + 1 : kMaxLevel - (63 - __clzll((long long)x)) / 3;
+            }
+            if (lmin <= kMaxLevel) atomicAdd(&bins[lmin], 1);
+        }
     }
+    __syncthreads();
+    if (threadIdx.x <= kMaxLevel && bins[threadIdx.x]) atomicAdd(&counts[1 + threadIdx.x], bins[threadIdx.x]);
+}
+
+__global__ void k_fmm_u64fill(unsigned long long* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ULL;
+}
+
+__global__ void k_fmm_ifill(int* __restrict__ p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // Box starts at level l: flags of prefix changes.
@@ -430,9 +457,10 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_p2m(
     box_center(root, l, lv.key[b], c, h);
     const double ih = 1.0 / h;
     const int s0 = lv.start[b], s1 = lv.end[b];
-    float acc = 0.0f;
-    int4 A = make_int4(0, 0, 0, 0);
-    if ((int)threadIdx.x < nc) A = mi[threadIdx.x];
+    float acc0 = 0.0f, acc1 = 0.0f;
+    const int a0 = threadIdx.x, a1 = threadIdx.x + kFmmThreads;
+    const int4 A0 = a0 < nc ? mi[a0] : make_int4(0, 0, 0, 0);
+    const int4 A1 = a1 < nc ? mi[a1] : make_int4(0, 0, 0, 0);
     for (int base = s0; base < s1; base += 64) {
         __syncthreads();
         if (threadIdx.x < 64) {
@@ -453,11 +481,15 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_p2m(
         }
         __syncthreads();
         const int cnt = min(64, s1 - base);
-        if ((int)threadIdx.x < nc)
+        if (a0 < nc)
             for (int j = 0; j < cnt; j++)
-                acc = fmaf(qs[j], pw[j][0][A.x] * pw[j][1][A.y] * pw[j][2][A.z], acc);
+                acc0 = fmaf(qs[j], pw[j][0][A0.x] * pw[j][1][A0.y] * pw[j][2][A0.z], acc0);
+        if (a1 < nc)
+            for (int j = 0; j < cnt; j++)
+                acc1 = fmaf(qs[j], pw[j][0][A1.x] * pw[j][1][A1.y] * pw[j][2][A1.z], acc1);
     }
-    if ((int)threadIdx.x < nc) lv.M[(size_t)b * nc + threadIdx.x] = acc;
+    if (a0 < nc) lv.M[(size_t)b * nc + a0] = acc0;
+    if (a1 < nc) lv.M[(size_t)b * nc + a1] = acc1;
 }
 
 // ------------------------------------------------------------------- M2M ----
@@ -473,18 +505,18 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_m2m(Level par, Level ch, in
                                                          const float* __restrict__ cc,
                                                          const int* __restrict__ bits) {
     const int b = blockIdx.x;
-    const int a = threadIdx.x;
     const int c0 = par.cfirst[b];
     const int c1 = b + 1 < par.nb ? par.cfirst[b + 1] : ch.nb;
-    if (a >= nc) return;
-    const int2 r = rng[a];
-    float acc = 0.0f;
-    for (int c = c0; c < c1; c++) {
-        const int oct = (int)(ch.key[c] & 7);
-        const float* Mc = ch.M + (size_t)c * nc;
-        for (int t = r.x; t < r.y; t++) acc = fmaf(cc[t] * octant_sign(oct, bits[t]), Mc[kk[t]], acc);
+    for (int a = threadIdx.x; a < nc; a += blockDim.x) {
+        const int2 r = rng[a];
+        float acc = 0.0f;
+        for (int c = c0; c < c1; c++) {
+            const int oct = (int)(ch.key[c] & 7);
+            const float* Mc = ch.M + (size_t)c * nc;
+            for (int t = r.x; t < r.y; t++) acc = fmaf(cc[t] * octant_sign(oct, bits[t]), Mc[kk[t]], acc);
+        }
+        par.M[(size_t)b * nc + a] = acc;
     }
-    par.M[(size_t)b * nc + a] = acc;
 }
 
 // ------------------------------------------------------------- T tables ----
@@ -529,19 +561,22 @@ __global__ void k_fmm_ttable(int p, int nc, const int4* __restrict__ mi, int W, 
 // ------------------------------------------------------------------- M2L ----
 
 // One workgroup per target box: candidate sources are the children of the
-// parent's near boxes that are not near the target; up to kM2LBatch sources
-// are staged in LDS at a time and every lane runs one task (a slice of the
-// (beta, alpha) pairs of one beta) over them.
+// parent's near boxes that are not near the target, listed in parent order by
+// wave 0 with ballot compaction (deterministic).  Up to kM2LBatch sources are
+// staged in LDS at a time; each lane runs up to two tasks (slices of the
+// (beta, alpha) pairs of one beta) over them, and the task partials of each
+// beta are summed in task order.
 __global__ __launch_bounds__(kFmmThreads) void k_fmm_m2l(
     Level tl, Level pl, int l, int ring, int nc, int ntask, const int2* __restrict__ task,
     const int* __restrict__ pa, const int* __restrict__ pg, const float* __restrict__ pw,
-    const int2* __restrict__ beta_tasks, const float* __restrict__ Ttab, int W) {
+    const int2* __restrict__ beta_tasks, const float* __restrict__ Ttab, int W,
+    unsigned long long* __restrict__ stats) {
     __shared__ int src[1024];
     __shared__ int srcoff[1024];
     __shared__ int nsrc;
     __shared__ float Ms[kM2LBatch][kMaxNc];
     __shared__ float Ts[kM2LBatch][kMaxNc];
-    __shared__ float part[kFmmThreads];
+    __shared__ float part[2 * kFmmThreads];
     __shared__ int pcnt[128];
     __shared__ int pfirst[128];
     const int b = blockIdx.x;
@@ -551,7 +586,7 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_m2l(
     const int nside = 2 * ring + 1, ncand = nside * nside * nside;
     const int pmax = (1 << (l - 1)) - 1;
     const int wside = 2 * W + 1;
-    // 1) parent-level near boxes and their child counts
+    // 1) parent-level near boxes and their child ranges
     for (int t = threadIdx.x; t < ncand; t += blockDim.x) {
         const int qx = px + t % nside - ring, qy = py + (t / nside) % nside - ring,
                   qz = pz + t / (nside * nside) - ring;
@@ -568,29 +603,41 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_m2l(
         pcnt[t] = cnt;
     }
     __syncthreads();
-    // 2) deterministic compaction of far children (serial over <=125 parents)
-    if (threadIdx.x == 0) {
+    // 2) far children, 8 parents x 8 child slots per wave-64 step
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
         int k = 0;
-        for (int t = 0; t < ncand; t++)
-            for (int c = pfirst[t]; c < pfirst[t] + pcnt[t]; c++) {
+        for (int t0 = 0; t0 < ncand; t0 += 8) {
+            const int t = t0 + (lane >> 3), cslot = lane & 7;
+            bool keep = false;
+            int c = 0, off = 0;
+            if (t < ncand && cslot < pcnt[t]) {
+                c = pfirst[t] + cslot;
                 const uint64_t sk = tl.key[c];
-                const int sx = (int)compact21(sk), sy = (int)compact21(sk >> 1),
-                          sz = (int)compact21(sk >> 2);
-                const int dx = ix - sx, dy = iy - sy, dz = iz - sz;
-                if (abs(dx) <= ring && abs(dy) <= ring && abs(dz) <= ring) continue;
-                src[k] = c;
-                srcoff[k] = (dx + W) + wside * ((dy + W) + wside * (dz + W));
-                k++;
+                const int dx = ix - (int)compact21(sk), dy = iy - (int)compact21(sk >> 1),
+                          dz = iz - (int)compact21(sk >> 2);
+                keep = !(abs(dx) <= ring && abs(dy) <= ring && abs(dz) <= ring);
+                off = (dx + W) + wside * ((dy + W) + wside * (dz + W));
             }
-        nsrc = k;
+            const unsigned long long m = __ballot(keep);
+            if (keep) {
+                const int pos = k + __popcll(m & ((1ULL << lane) - 1ULL));
+                src[pos] = c;
+                srcoff[pos] = off;
+            }
+            k += __popcll(m);
+        }
+        if (lane == 0) {
+            nsrc = k;
+            atomicAdd(stats, (unsigned long long)k);
+        }
     }
     __syncthreads();
     const int ns = nsrc;
     const int me = threadIdx.x;
-    int2 tk = make_int2(0, 0);
-    if (me < ntask) tk = task[me];
-    float acc = 0.0f;
-    const float* Tl = Ttab;
+    const int2 tk0 = me < ntask ? task[me] : make_int2(0, 0);
+    const int2 tk1 = me + kFmmThreads < ntask ? task[me + kFmmThreads] : make_int2(0, 0);
+    float acc0 = 0.0f, acc1 = 0.0f;
     for (int base = 0; base < ns; base += kM2LBatch) {
         const int nb = min(kM2LBatch, ns - base);
         __syncthreads();
@@ -598,28 +645,36 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_m2l(
             const int s = t / nc, k = t % nc;
             if (s < nb) {
                 Ms[s][k] = tl.M[(size_t)src[base + s] * nc + k];
-                Ts[s][k] = Tl[(size_t)srcoff[base + s] * nc + k];
+                Ts[s][k] = Ttab[(size_t)srcoff[base + s] * nc + k];
             } else {
                 Ms[s][k] = 0.0f;
                 Ts[s][k] = 0.0f;
             }
         }
         __syncthreads();
-        for (int q = tk.x; q < tk.y; q++) {
+        for (int q = tk0.x; q < tk0.y; q++) {
             const int a = pa[q], g = pg[q];
             float v = 0.0f;
 #pragma unroll
             for (int s = 0; s < kM2LBatch; s++) v = fmaf(Ts[s][g], Ms[s][a], v);
-            acc = fmaf(pw[q], v, acc);
+            acc0 = fmaf(pw[q], v, acc0);
+        }
+        for (int q = tk1.x; q < tk1.y; q++) {
+            const int a = pa[q], g = pg[q];
+            float v = 0.0f;
+#pragma unroll
+            for (int s = 0; s < kM2LBatch; s++) v = fmaf(Ts[s][g], Ms[s][a], v);
+            acc1 = fmaf(pw[q], v, acc1);
         }
     }
-    part[me] = acc;
+    part[me] = acc0;
+    part[me + kFmmThreads] = acc1;
     __syncthreads();
-    if (me < nc) {
-        const int2 bt = beta_tasks[me];
+    for (int be = threadIdx.x; be < nc; be += blockDim.x) {
+        const int2 bt = beta_tasks[be];
         float s = 0.0f;
         for (int t = bt.x; t < bt.y; t++) s += part[t];
-        tl.Lc[(size_t)b * nc + me] += s;
+        tl.Lc[(size_t)b * nc + be] += s;
     }
 }
 
@@ -631,17 +686,17 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_l2l(Level par, Level ch, in
                                                          const float* __restrict__ cc,
                                                          const int* __restrict__ bits) {
     const int b = blockIdx.x;  // parent
-    const int k = threadIdx.x;
-    if (k >= nc) return;
     const int c0 = par.cfirst[b];
     const int c1 = b + 1 < par.nb ? par.cfirst[b + 1] : ch.nb;
     const float* Lp = par.Lc + (size_t)b * nc;
-    const int2 r = rng[k];
-    for (int c = c0; c < c1; c++) {
-        const int oct = (int)(ch.key[c] & 7);
-        float acc = 0.0f;
-        for (int t = r.x; t < r.y; t++) acc = fmaf(cc[t] * octant_sign(oct, bits[t]), Lp[bb[t]], acc);
-        ch.Lc[(size_t)c * nc + k] += acc;
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        const int2 r = rng[k];
+        for (int c = c0; c < c1; c++) {
+            const int oct = (int)(ch.key[c] & 7);
+            float acc = 0.0f;
+            for (int t = r.x; t < r.y; t++) acc = fmaf(cc[t] * octant_sign(oct, bits[t]), Lp[bb[t]], acc);
+            ch.Lc[(size_t)c * nc + k] += acc;
+        }
     }
 }
 
@@ -666,17 +721,19 @@ __device__ __forceinline__ void fmm_two_sum(float& hi, float& lo, float p) {
 __global__ __launch_bounds__(64) void k_fmm_leaf(
     Level lv, int L, int ring, const double* __restrict__ root, const double4* __restrict__ sp,
     const int* __restrict__ perm, const int* __restrict__ status, const int* __restrict__ species,
-    int nc, int p, const int4* __restrict__ mi, float eps2f, double K, double* __restrict__ ax,
+    int nc, int p, const int4* __restrict__ mi, double K, unsigned long long* __restrict__ stats,
+    double* __restrict__ ax,
     double* __restrict__ ay, double* __restrict__ az) {
     __shared__ float4 tile[64];
     __shared__ float Lsh[kMaxNc];
+    __shared__ float pwr[64][3][kMaxP + 1];
     __shared__ int nbr_first[125], nbr_end[125];
     __shared__ int nnbr;
     const int b = blockIdx.x;
     const uint64_t key = lv.key[b];
     double c[3], h;
     box_center(root, L, key, c, h);
-    const float ihf = (float)(1.0 / h);
+    const float eps2f = (float)(kSoftening / (h * h));
     for (int k = threadIdx.x; k < nc; k += 64) Lsh[k] = lv.Lc[(size_t)b * nc + k];
     const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
     const int nside = 2 * ring + 1, ncand = nside * nside * nside;
@@ -697,18 +754,21 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
         nbr_end[t] = e;
     }
     __syncthreads();
+    const int s0 = lv.start[b], s1 = lv.end[b];
     if (threadIdx.x == 0) {
         int k = 0;
+        unsigned long long nsrc = 0;
         for (int t = 0; t < ncand; t++)
             if (nbr_end[t] > nbr_first[t]) {
                 nbr_first[k] = nbr_first[t];
                 nbr_end[k] = nbr_end[t];
+                nsrc += (unsigned long long)(nbr_end[t] - nbr_first[t]);
                 k++;
             }
         nnbr = k;
+        atomicAdd(stats + 1, nsrc * (unsigned long long)(s1 - s0));
     }
     __syncthreads();
-    const int s0 = lv.start[b], s1 = lv.end[b];
     for (int tb = s0; tb < s1; tb += 64) {
         const int i = tb + threadIdx.x;
         const bool valid = i < s1;
@@ -718,19 +778,19 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
         // L2P: gradient of sum L_b u^b
         float gx = 0.0f, gy = 0.0f, gz = 0.0f;
         {
-            float px[kMaxP + 1], py[kMaxP + 1], pz[kMaxP + 1];
-            px[0] = py[0] = pz[0] = 1.0f;
+            float (*pw)[kMaxP + 1] = pwr[threadIdx.x];
+            pw[0][0] = pw[1][0] = pw[2][0] = 1.0f;
             for (int k = 1; k <= p; k++) {
-                px[k] = px[k - 1] * ux;
-                py[k] = py[k - 1] * uy;
-                pz[k] = pz[k - 1] * uz;
+                pw[0][k] = pw[0][k - 1] * ux;
+                pw[1][k] = pw[1][k - 1] * uy;
+                pw[2][k] = pw[2][k - 1] * uz;
             }
             for (int k = 1; k < nc; k++) {
                 const int4 m = mi[k];
                 const float Lk = Lsh[k];
-                if (m.x) gx = fmaf(Lk * (float)m.x, px[m.x - 1] * py[m.y] * pz[m.z], gx);
-                if (m.y) gy = fmaf(Lk * (float)m.y, px[m.x] * py[m.y - 1] * pz[m.z], gy);
-                if (m.z) gz = fmaf(Lk * (float)m.z, px[m.x] * py[m.y] * pz[m.z - 1], gz);
+                if (m.x) gx = fmaf(Lk * (float)m.x, pw[0][m.x - 1] * pw[1][m.y] * pw[2][m.z], gx);
+                if (m.y) gy = fmaf(Lk * (float)m.y, pw[0][m.x] * pw[1][m.y - 1] * pw[2][m.z], gy);
+                if (m.z) gz = fmaf(Lk * (float)m.z, pw[0][m.x] * pw[1][m.y] * pw[2][m.z - 1], gz);
             }
         }
         // far part in units of 1/h^2 relative to the P2P sum expressed in u units:
@@ -788,13 +848,15 @@ using namespace shp;
 
 extern "C" {
 
-int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level, int ring,
-                double* ax, double* ay, double* az) {
+int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level, int leaf_size,
+                int ring, double* ax, double* ay, double* az) {
     SHP_REQUIRE(h && pt && ax && ay && az, "shp_fmm_dev: null argument");
     SHP_REQUIRE(order >= 1 && order <= kMaxP, "shp_fmm_dev: order %d outside [1, %d]", order, kMaxP);
     SHP_REQUIRE(ring >= 1 && ring <= 2, "shp_fmm_dev: ring %d outside [1, 2]", ring);
-    SHP_REQUIRE(leaf_level >= 1 && leaf_level <= 16, "shp_fmm_dev: leaf level %d outside [1, 16]",
+    SHP_REQUIRE(leaf_level >= 0 && leaf_level <= 16, "shp_fmm_dev: leaf level %d outside [0, 16]",
                 leaf_level);
+    SHP_REQUIRE(leaf_level > 0 || leaf_size >= 1, "shp_fmm_dev: leaf_size %d must be >= 1",
+                leaf_size);
     const int n = pt->n;
     if (n <= 0) return SHP_OK;
     SHP_REQUIRE(pt->x && pt->y && pt->z && pt->q && pt->status && pt->species,
@@ -803,7 +865,8 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     FmmDevTables* tb;
     int rc = get_tables(h, order, &tb);
     if (rc) return rc;
-    const int L = leaf_level, nc = tb->nc;
+    int L = leaf_level;
+    const int nc = tb->nc;
     hipStream_t st = h->stream;
     // bounding box and root
     const int nbp = std::min(fdiv_up(n, kFmmThreads), 1024);
@@ -823,16 +886,27 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, keys2, vals, perm, n, 0, 64, st));
     SHP_WS(cub, char, h, WS_CUB, tmp);
     SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub, tmp, keys, keys2, vals, perm, n, 0, 64, st));
-    // counts per level (host needs them for launch sizes)
+    // box counts per level (the host needs them for launch sizes)
     SHP_WS(cnt, int, h, WS_COUNTS, 32);
-    SHP_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * 32, st));
-    hipLaunchKernelGGL(k_fmm_level_counts, dim3(fdiv_up(n, 256)), dim3(256), 0, st, keys2, n, L, cnt);
+    hipLaunchKernelGGL(k_fmm_ifill, dim3(1), dim3(32), 0, st, cnt, 32, 0);
+    hipLaunchKernelGGL(k_fmm_level_hist, dim3(fdiv_up(n, 256)), dim3(256), 0, st, keys2, n, cnt);
     int hc[32];
     SHP_HIP_CHECK(hipMemcpyAsync(hc, cnt, sizeof(int) * 32, hipMemcpyDeviceToHost, st));
     SHP_HIP_CHECK(hipStreamSynchronize(st));
     const int na = hc[0];
+    std::vector<int> nbox_all(kMaxLevel + 1);
+    for (int l = 0, acc = 0; l <= kMaxLevel; l++) {
+        acc += hc[1 + l];
+        nbox_all[l] = acc;
+    }
+    if (L <= 0) {
+        // smallest level whose mean leaf occupancy is at most leaf_size
+        L = 1;
+        while (L < 16 && (double)na / std::max(1, nbox_all[L]) > (double)leaf_size) L++;
+    }
+    last_leaf_level(h) = L;
     std::vector<int> nbox(L + 1);
-    for (int l = 0; l <= L; l++) nbox[l] = hc[1 + l];
+    for (int l = 0; l <= L; l++) nbox[l] = nbox_all[l];
     if (na == 0) {
         hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ax, n);
         hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ay, n);
@@ -883,35 +957,64 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     SHP_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_fmm_zero, dim3(fdiv_up(tot * nc, 256)), dim3(256), 0, st, bexp + tot * nc,
                        tot * nc);
+    hipLaunchKernelGGL(k_fmm_u64fill, dim3(1), dim3(64), 0, st, h->fmm_stats, 8);
+    hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ax, n);
+    hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ay, n);
+    hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, az, n);
     // upward pass
     SHP_HIP_CHECK(hipEventRecord(h->ev0, st));
+    SHP_HIP_CHECK(hipEventRecord(h->fev[0], st));
     hipLaunchKernelGGL(k_fmm_p2m, dim3(nbox[L]), dim3(kFmmThreads), 0, st, lv[L], L, root, sp, nc,
                        order, tb->mi);
     for (int l = L - 1; l >= 0; l--)
         hipLaunchKernelGGL(k_fmm_m2m, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l + 1], nc,
                            tb->up_range, tb->up_k, tb->up_c, tb->up_bits);
+    SHP_HIP_CHECK(hipEventRecord(h->fev[1], st));
     // translation tables for offsets within the far band
     const int W = 2 * ring + 1;
     const int nside = 2 * W + 1, noff = nside * nside * nside;
     SHP_WS(ttab, float, h, WS_FMM_T, (size_t)noff * (L + 1) * nc);
     hipLaunchKernelGGL(k_fmm_ttable, dim3(fdiv_up(noff * (L + 1), 64)), dim3(64), 0, st, order, nc,
                        tb->mi, W, 0, L, root, kSoftening, ttab);
+    SHP_HIP_CHECK(hipEventRecord(h->fev[2], st));
     for (int l = 1; l <= L; l++)
         hipLaunchKernelGGL(k_fmm_m2l, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l - 1], l, ring,
                            nc, tb->ntask, tb->task, tb->pair_alpha, tb->pair_gamma, tb->pair_w,
-                           tb->beta_tasks, ttab + (size_t)l * noff * nc, W);
+                           tb->beta_tasks, ttab + (size_t)l * noff * nc, W, h->fmm_stats);
+    SHP_HIP_CHECK(hipEventRecord(h->fev[3], st));
     for (int l = 1; l < L; l++)
         hipLaunchKernelGGL(k_fmm_l2l, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l + 1], nc,
                            tb->dn_range, tb->dn_b, tb->dn_c, tb->dn_bits);
-    hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ax, n);
-    hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ay, n);
-    hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, az, n);
+    SHP_HIP_CHECK(hipEventRecord(h->fev[4], st));
     hipLaunchKernelGGL(k_fmm_leaf, dim3(nbox[L]), dim3(64), 0, st, lv[L], L, ring, root, sp, perm,
-                       pt->status, pt->species, nc, order, tb->mi, (float)0.0, k_const_host(), ax, ay,
-                       az);
+                       pt->status, pt->species, nc, order, tb->mi, k_const_host(), h->fmm_stats,
+                       ax, ay, az);
     SHP_HIP_CHECK(hipGetLastError());
+    SHP_HIP_CHECK(hipEventRecord(h->fev[5], st));
     SHP_HIP_CHECK(hipEventRecord(h->ev1, st));
     h->timed = true;
+    h->fmm_timed = true;
+    h->fmm_npairs = (double)tb->npairs;
+    return SHP_OK;
+}
+
+int shp_fmm_last_leaf_level(shp_handle h) { return h ? last_leaf_level(h) : -1; }
+
+int shp_fmm_stats(shp_handle h, double* out) {
+    SHP_REQUIRE(h && out, "shp_fmm_stats: null argument");
+    SHP_REQUIRE(h->fmm_timed, "shp_fmm_stats: no FMM evaluation on this handle yet");
+    SHP_HIP_CHECK(hipEventSynchronize(h->fev[5]));
+    for (int k = 0; k < 5; k++) {
+        float ms = 0.0f;
+        SHP_HIP_CHECK(hipEventElapsedTime(&ms, h->fev[k], h->fev[k + 1]));
+        out[k] = ms;
+    }
+    unsigned long long c[2];
+    SHP_HIP_CHECK(hipMemcpy(c, h->fmm_stats, sizeof(c), hipMemcpyDeviceToHost));
+    out[5] = (double)c[0];
+    out[6] = (double)c[1];
+    out[7] = last_leaf_level(h);
+    out[8] = h->fmm_npairs;
     return SHP_OK;
 }
 

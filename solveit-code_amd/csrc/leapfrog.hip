@@ -430,8 +430,10 @@ int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double 
                     s->species && s->t_inj,
                 "shp_leapfrog_run: null state array");
     SHP_REQUIRE(!o->use_efield || h->mesh.ready, "shp_leapfrog_run: E-field requested but no mesh");
-    SHP_REQUIRE(o->method == SHP_METHOD_DIRECT, "shp_leapfrog_run: method %d not available",
-                o->method);
+    SHP_REQUIRE(o->method == SHP_METHOD_DIRECT || o->method == SHP_METHOD_FMM,
+                "shp_leapfrog_run: method %d not available", o->method);
+    SHP_REQUIRE(o->method != SHP_METHOD_FMM || o->sym.type == SHP_SYM_NONE,
+                "shp_leapfrog_run: the FMM path has no symmetry images");
     if (s->n == 0 || nsteps == 0) return SHP_OK;
     SHP_HIP_CHECK(hipSetDevice(h->device));
     const int n = s->n;
@@ -445,7 +447,10 @@ int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double 
     for (int k = 0; k < nsteps; k++) {
         const double t = t0 + k * dt;
         shp_particles p{n, s->x, s->y, s->z, s->q, eff, s->species};
-        int rc = shp_direct_dev(h, &p, o->precision, &o->sym, ax, ay, az);
+        int rc = o->method == SHP_METHOD_FMM
+                     ? shp_fmm_dev(h, &p, o->fmm_order, o->leaf_level, o->leaf_size,
+                                   o->fmm_ring > 0 ? o->fmm_ring : 2, ax, ay, az)
+                     : shp_direct_dev(h, &p, o->precision, &o->sym, ax, ay, az);
         if (rc) return rc;
         SHP_HIP_CHECK(hipEventRecord(h->ev2, h->stream));
         hipLaunchKernelGGL(k_kick_drift, grid, block, 0, h->stream, P, M, n, s->x, s->y, s->z, s->vx,

@@ -33,6 +33,7 @@ __host__ __device__ inline double species_mass(int s) {
 
 void set_error(const char* fmt, ...);
 void free_mesh(shp_handle h);
+void fmm_release(shp_handle h);
 
 // Grow-only device workspace slots owned by a context.
 enum WsSlot {
@@ -55,6 +56,12 @@ enum WsSlot {
     WS_WIDE_I32,
     WS_BOX,         // bounding box (6 doubles)
     WS_EFF_STATUS,  // status masked by the active (injected) range
+    WS_FMM_KEYS,
+    WS_FMM_KEYS2,
+    WS_FMM_BKEY,    // box keys of all levels
+    WS_FMM_BINT,    // box start/end/first-child of all levels
+    WS_FMM_EXP,     // multipole and local coefficients
+    WS_FMM_T,       // translation tables
     WS_NSLOTS
 };
 
@@ -86,6 +93,10 @@ struct shp_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev2 = nullptr, ev3 = nullptr;  // update-kernel timing
+    hipEvent_t fev[8] = {};                   // FMM pass timing
+    unsigned long long* fmm_stats = nullptr;  // device counters of the last FMM call
+    bool fmm_timed = false;
+    double fmm_npairs = 0;
     bool timed = false, timed_update = false;
     void* ws[shp::WS_NSLOTS] = {};
     size_t ws_bytes[shp::WS_NSLOTS] = {};

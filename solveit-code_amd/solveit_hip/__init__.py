@@ -83,6 +83,8 @@ class StepOpts(ctypes.Structure):
         ("fmm_order", ctypes.c_int),
         ("leaf_size", ctypes.c_int),
         ("theta", ctypes.c_double),
+        ("fmm_ring", ctypes.c_int),
+        ("leaf_level", ctypes.c_int),
     ]
 
 
@@ -141,6 +143,10 @@ def _load():
         "shp_leapfrog_run": (c_int, [c_void_p, ctypes.POINTER(State), ctypes.POINTER(StepOpts),
                                      c_double, c_double, c_int, c_void_p, c_void_p, c_void_p]),
         "shp_last_update_ms": (c_double, [c_void_p]),
+        "shp_fmm_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), c_int, c_int, c_int, c_int,
+                                c_void_p, c_void_p, c_void_p]),
+        "shp_fmm_last_leaf_level": (c_int, [c_void_p]),
+        "shp_fmm_stats": (c_int, [c_void_p, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -261,7 +267,7 @@ class Engine:
 
 
 def make_step_opts(method=0, precision=2, symmetry=None, use_efield=False, bc_space=(0, 0, 0, 0),
-                   bc_time=0.0, fmm_order=12, leaf_size=64, theta=0.5):
+                   bc_time=0.0, fmm_order=10, leaf_size=20, theta=0.5, fmm_ring=2, leaf_level=0):
     o = StepOpts()
     o.method = method
     o.precision = precision
@@ -275,6 +281,8 @@ def make_step_opts(method=0, precision=2, symmetry=None, use_efield=False, bc_sp
     o.fmm_order = int(fmm_order)
     o.leaf_size = int(leaf_size)
     o.theta = float(theta)
+    o.fmm_ring = int(fmm_ring)
+    o.leaf_level = int(leaf_level)
     return o
 
 
@@ -312,6 +320,36 @@ def _engine_leapfrog_run(self, state, opts, t0, dt, nsteps, acc=None):
                                 int(nsteps), acc[0].data_ptr(), acc[1].data_ptr(),
                                 acc[2].data_ptr()), "shp_leapfrog_run")
     return acc
+
+
+def _engine_fmm_dev(self, state, order=8, leaf_level=0, leaf_size=32, ring=2, out=None):
+    """FMM accelerations on a DeviceState; returns (ax, ay, az) float64 device tensors."""
+    import torch
+    n = state.n
+    if out is None:
+        out = tuple(torch.empty(n, dtype=torch.float64, device=state.x.device) for _ in range(3))
+    p = state.as_struct()
+    _check(lib.shp_fmm_dev(self.h, ctypes.byref(p), int(order), int(leaf_level), int(leaf_size),
+                           int(ring), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr()),
+           "shp_fmm_dev")
+    return out
+
+
+def _engine_fmm_leaf_level(self):
+    return lib.shp_fmm_last_leaf_level(self.h)
+
+
+def _engine_fmm_stats(self):
+    out = (ctypes.c_double * 9)()
+    _check(lib.shp_fmm_stats(self.h, out), "shp_fmm_stats")
+    keys = ("p2m_m2m_ms", "ttable_ms", "m2l_ms", "l2l_ms", "leaf_ms", "m2l_pairs",
+            "p2p_pairs", "leaf_level", "m2l_madds")
+    return dict(zip(keys, list(out)))
+
+
+Engine.fmm_dev = _engine_fmm_dev
+Engine.fmm_leaf_level = _engine_fmm_leaf_level
+Engine.fmm_stats = _engine_fmm_stats
 
 
 def _engine_last_update_ms(self):
