@@ -447,3 +447,16 @@ def fmm_taylor(p, R, eps2):
     r = _f64(R)
     lib.orc_fmm_taylor(p, r.ctypes.data, float(eps2), T.ctypes.data)
     return T
+
+
+lib.orc_fmm_counts.argtypes = [_ci, _vp, _vp, _vp, _vp, _ci, _ci, _vp]
+
+
+def fmm_counts(d, L, ring=2):
+    """(M2L box pairs, P2P particle pairs, leaves) of the FMM tree definition."""
+    x, y, z = _f64(d["x"]), _f64(d["y"]), _f64(d["z"])
+    st = _i32(d["status"])
+    out = np.zeros(3)
+    lib.orc_fmm_counts(x.shape[0], x.ctypes.data, y.ctypes.data, z.ctypes.data, st.ctypes.data,
+                       int(L), int(ring), out.ctypes.data)
+    return {"m2l": out[0], "p2p_pairs": out[1], "leaves": int(out[2])}

@@ -454,3 +454,83 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
     free(ki);
     free(B.mi);
 }
+
+/* Interaction counts of the FMM tree definition above, without expansions:
+ * out[0] = M2L box pairs over all levels, out[1] = P2P particle pairs
+ * (self pairs included), out[2] = leaf boxes. */
+static int cmp_u64(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : (x > y);
+}
+static int find_u64(const uint64_t* k, int nb, uint64_t key) {
+    int lo = 0, hi = nb - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        if (k[mid] == key) return mid;
+        if (k[mid] < key) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+void orc_fmm_counts(int n, const double* x, const double* y, const double* z, const int* status,
+                    int L, int ring, double* out) {
+    double root[4];
+    orc_fmm_root(n, x, y, z, status, root);
+    uint64_t* k = malloc(sizeof(uint64_t) * (n > 0 ? n : 1));
+    int na = 0;
+    for (int i = 0; i < n; i++)
+        if (status[i] == 1 || status[i] == 2) k[na++] = orc_fmm_key(x[i], y[i], z[i], root, L);
+    qsort(k, na, sizeof(uint64_t), cmp_u64);
+    uint64_t** bk = calloc(L + 1, sizeof(uint64_t*));
+    int** bc = calloc(L + 1, sizeof(int*));
+    int* nb = calloc(L + 1, sizeof(int));
+    for (int l = L; l >= 0; l--) {
+        int sh = 3 * (L - l);
+        bk[l] = malloc(sizeof(uint64_t) * (na + 1));
+        bc[l] = malloc(sizeof(int) * (na + 1));
+        int m = 0;
+        for (int i = 0; i < na; i++) {
+            uint64_t v = k[i] >> sh;
+            if (m == 0 || bk[l][m - 1] != v) { bk[l][m] = v; bc[l][m] = 0; m++; }
+            bc[l][m - 1]++;
+        }
+        nb[l] = m;
+    }
+    double m2l = 0, p2p = 0;
+    for (int l = 1; l <= L; l++) {
+        int64_t pmax = ((int64_t)1 << (l - 1)) - 1;
+        for (int b = 0; b < nb[l]; b++) {
+            int64_t ix = compact21(bk[l][b]), iy = compact21(bk[l][b] >> 1), iz = compact21(bk[l][b] >> 2);
+            for (int dz = -ring; dz <= ring; dz++)
+                for (int dy = -ring; dy <= ring; dy++)
+                    for (int dx = -ring; dx <= ring; dx++) {
+                        int64_t qx = (ix >> 1) + dx, qy = (iy >> 1) + dy, qz = (iz >> 1) + dz;
+                        if (qx < 0 || qy < 0 || qz < 0 || qx > pmax || qy > pmax || qz > pmax) continue;
+                        if (find_u64(bk[l - 1], nb[l - 1], spread21(qx) | spread21(qy) << 1 | spread21(qz) << 2) < 0)
+                            continue;
+                        for (int o = 0; o < 8; o++) {
+                            int64_t cx = 2 * qx + (o & 1), cy = 2 * qy + ((o >> 1) & 1), cz = 2 * qz + ((o >> 2) & 1);
+                            if (llabs(cx - ix) <= ring && llabs(cy - iy) <= ring && llabs(cz - iz) <= ring) continue;
+                            if (find_u64(bk[l], nb[l], spread21(cx) | spread21(cy) << 1 | spread21(cz) << 2) >= 0)
+                                m2l += 1;
+                        }
+                    }
+        }
+    }
+    int64_t cmax = ((int64_t)1 << L) - 1;
+    for (int b = 0; b < nb[L]; b++) {
+        int64_t ix = compact21(bk[L][b]), iy = compact21(bk[L][b] >> 1), iz = compact21(bk[L][b] >> 2);
+        double s = 0;
+        for (int dz = -ring; dz <= ring; dz++)
+            for (int dy = -ring; dy <= ring; dy++)
+                for (int dx = -ring; dx <= ring; dx++) {
+                    int64_t qx = ix + dx, qy = iy + dy, qz = iz + dz;
+                    if (qx < 0 || qy < 0 || qz < 0 || qx > cmax || qy > cmax || qz > cmax) continue;
+                    int j = find_u64(bk[L], nb[L], spread21(qx) | spread21(qy) << 1 | spread21(qz) << 2);
+                    if (j >= 0) s += bc[L][j];
+                }
+        p2p += s * bc[L][b];
+    }
+    out[0] = m2l; out[1] = p2p; out[2] = nb[L];
+    for (int l = 0; l <= L; l++) { free(bk[l]); free(bc[l]); }
+    free(bk); free(bc); free(nb); free(k);
+}

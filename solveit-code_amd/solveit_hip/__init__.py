@@ -172,6 +172,14 @@ def softening():
     return lib.shp_softening()
 
 
+def _torch_ready(device):
+    """Device calls run on the engine's own stream: wait for torch's current
+    stream first so its writes to the inputs and outputs have landed.  Every
+    wrapper below synchronizes the engine's stream before returning tensors."""
+    import torch
+    torch.cuda.current_stream(device).synchronize()
+
+
 def make_symmetry(kind="none", nsec=4, lattice=None, domain=(0.0, 0.0, 0.0)):
     """Build a Symmetry struct; keeps the lattice array alive on the struct."""
     s = Symmetry()
@@ -260,9 +268,11 @@ class Engine:
                         for _ in range(3))
         sym = symmetry if symmetry is not None else make_symmetry("none")
         p = state.as_struct()
+        _torch_ready(state.x.device)
         rc = lib.shp_direct_dev(self.h, ctypes.byref(p), precision, ctypes.byref(sym),
                                 out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr())
         _check(rc, "shp_direct_dev")
+        self.sync()
         return out
 
 
@@ -304,9 +314,11 @@ def _engine_efield_dev(self, x, y, z, want_tri=False):
     n = int(x.shape[0])
     out = tuple(torch.empty(n, dtype=torch.float64, device=x.device) for _ in range(3))
     tri = torch.empty(n, dtype=torch.int32, device=x.device) if want_tri else None
+    _torch_ready(x.device)
     _check(lib.shp_efield_dev(self.h, n, x.data_ptr(), y.data_ptr(), z.data_ptr(),
                               out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(),
                               tri.data_ptr() if tri is not None else None), "shp_efield_dev")
+    self.sync()
     return (out, tri) if want_tri else out
 
 
@@ -316,9 +328,11 @@ def _engine_leapfrog_run(self, state, opts, t0, dt, nsteps, acc=None):
     if acc is None:
         acc = tuple(torch.zeros(n, dtype=torch.float64, device=state.x.device) for _ in range(3))
     s = state.as_state()
+    _torch_ready(state.x.device)
     _check(lib.shp_leapfrog_run(self.h, ctypes.byref(s), ctypes.byref(opts), float(t0), float(dt),
                                 int(nsteps), acc[0].data_ptr(), acc[1].data_ptr(),
                                 acc[2].data_ptr()), "shp_leapfrog_run")
+    self.sync()
     return acc
 
 
@@ -329,9 +343,11 @@ def _engine_fmm_dev(self, state, order=8, leaf_level=0, leaf_size=32, ring=2, ou
     if out is None:
         out = tuple(torch.empty(n, dtype=torch.float64, device=state.x.device) for _ in range(3))
     p = state.as_struct()
+    _torch_ready(state.x.device)
     _check(lib.shp_fmm_dev(self.h, ctypes.byref(p), int(order), int(leaf_level), int(leaf_size),
                            int(ring), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr()),
            "shp_fmm_dev")
+    self.sync()
     return out
 
 

@@ -71,7 +71,8 @@ def test_ref_ranges_and_status(eng):
         got = eng.direct_host(pos4, vel4, acc4, i0, i1, j0, j1)
         ref = oracle.direct_ref(pos4, vel4, acc4, i0, i1, j0, j1)
         assert got.shape == ref.shape
-        assert _rowerr(got, ref).max() <= TOL_FP64
+        if got.size:
+            assert _rowerr(got, ref).max() <= TOL_FP64
 
 
 @pytest.mark.parametrize("nsec", [4, 6])
