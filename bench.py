@@ -122,7 +122,7 @@ def main():
         eng.sync()
         fp64_step_ms = (time.perf_counter() - t1) * 1e3
 
-    m2l_flop = 2.0 * mean["m2l_madds"] * mean["m2l_pairs"]
+    m2l_flop = 2.0 * mean["m2l_madds"]
     p2p_flop = FLOP_PER_PAIR * mean["p2p_pairs"]
     m2l_tf = m2l_flop / (mean["m2l_ms"] * 1e-3) / 1e12
     leaf_tf = p2p_flop / (mean["leaf_ms"] * 1e-3) / 1e12

@@ -124,8 +124,8 @@ int shp_fmm_last_leaf_level(shp_handle h);
 
 /* Measurements of the last shp_fmm_dev call (waits for it): out[0..4] = ms of
  * P2M+M2M, translation tables, M2L, L2L, L2P+P2P; out[5] = M2L box pairs;
- * out[6] = P2P particle pairs; out[7] = leaf level; out[8] = multiply-adds per
- * M2L box pair. */
+ * out[6] = P2P particle pairs; out[7] = leaf level; out[8] = total M2L
+ * multiply-adds (expansion terms actually evaluated, summed over box pairs). */
 int shp_fmm_stats(shp_handle h, double* out);
 
 /* ---- background E-field (pwl on a scattered (r, z) triangulation) ---- */

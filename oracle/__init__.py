@@ -460,3 +460,16 @@ def fmm_counts(d, L, ring=2):
     lib.orc_fmm_counts(x.shape[0], x.ctypes.data, y.ctypes.data, z.ctypes.data, st.ctypes.data,
                        int(L), int(ring), out.ctypes.data)
     return {"m2l": out[0], "p2p_pairs": out[1], "leaves": int(out[2])}
+
+
+lib.orc_fmm_set_shell_orders.argtypes = [_vp, _ci]
+
+
+def fmm_set_shell_orders(orders=None):
+    """M2L expansion order per Chebyshev shell of the box offset (index = shell);
+    None restores the single order p."""
+    if orders is None:
+        lib.orc_fmm_set_shell_orders(None, 0)
+        return
+    a = _i32(orders)
+    lib.orc_fmm_set_shell_orders(a.ctypes.data, a.shape[0])

@@ -190,6 +190,12 @@ static void box_center(const Tree* t, int l, uint64_t key, double c[3], double* 
     *h = 0.5 * bs;
 }
 
+/* Optional M2L order per Chebyshev shell of the box offset (0 = use p). */
+static int g_pshell[8];
+void orc_fmm_set_shell_orders(const int* ps, int n) {
+    for (int k = 0; k < 8; k++) g_pshell[k] = (ps && k < n) ? ps[k] : 0;
+}
+
 typedef struct { uint64_t k; int i; } KI;
 static int cmp_ki(const void* a, const void* b) {
     const KI* x = a;
@@ -348,14 +354,20 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
                             double R[3] = {(ca[0] - cs[0]) / h, (ca[1] - cs[1]) / h, (ca[2] - cs[2]) / h};
                             orc_fmm_taylor(p, R, eps2 / (h * h), T);
                             const double* Ms = t.M[l] + (size_t)ch * nc;
+                            int64_t shell = llabs(sx - ix[0]);
+                            if (llabs(sy - ix[1]) > shell) shell = llabs(sy - ix[1]);
+                            if (llabs(sz - ix[2]) > shell) shell = llabs(sz - ix[2]);
+                            const int pp = (shell < 8 && g_pshell[shell] > 0 && g_pshell[shell] < p)
+                                               ? g_pshell[shell] : p;
                             for (int bb = 0; bb < nc; bb++) {
                                 int* Bm = B.mi[bb];
                                 int nb_ = Bm[0] + Bm[1] + Bm[2];
+                                if (nb_ > pp) break;
                                 double acc = 0.0;
                                 for (int a = 0; a < nc; a++) {
                                     int* A = B.mi[a];
                                     int na_ = A[0] + A[1] + A[2];
-                                    if (na_ + nb_ > p) break;
+                                    if (na_ + nb_ > pp) break;
                                     double w = binom(A[0] + Bm[0], A[0]) * binom(A[1] + Bm[1], A[1]) *
                                                binom(A[2] + Bm[2], A[2]);
                                     if (na_ & 1) w = -w;

@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <utility>
 #include <vector>
 
 #include "shp_internal.h"
@@ -189,6 +190,10 @@ struct FmmDevTables {
     int* dn_b = nullptr;
     float* dn_c = nullptr;
     int* dn_bits = nullptr;
+    float* mscale = nullptr;
+    float* dscale = nullptr;
+    float* lscale = nullptr;
+    float* zeros = nullptr;
 };
 
 static std::map<shp_handle, FmmDevTables> g_tables;
@@ -222,6 +227,21 @@ static int get_tables(shp_handle h, int p, FmmDevTables** out) {
         rc |= upload(&d.dn_b, t.dn_b);
         rc |= upload(&d.dn_c, t.dn_c);
         rc |= upload(&d.dn_bits, t.dn_diffbits);
+        // scalings of the unrolled M2L: Mhat_a = (-1)^|a| M_a / a!,
+        // D_g = g! T_g, L_b += Lhat_b / b!
+        std::vector<float> ms(t.nc), ds(t.nc), ls(t.nc), zero(t.nc, 0.0f);
+        auto fact = [](int k) { double f = 1; for (int i = 2; i <= k; i++) f *= i; return f; };
+        for (int k = 0; k < t.nc; k++) {
+            const int4 m = t.mi[k];
+            const double f = fact(m.x) * fact(m.y) * fact(m.z);
+            ms[k] = (float)((m.w & 1 ? -1.0 : 1.0) / f);
+            ds[k] = (float)f;
+            ls[k] = (float)(1.0 / f);
+        }
+        rc |= upload(&d.mscale, ms);
+        rc |= upload(&d.dscale, ds);
+        rc |= upload(&d.lscale, ls);
+        rc |= upload(&d.zeros, zero);
         if (rc) return SHP_ERR_HIP;
         d.p = p;
         d.nc = t.nc;
@@ -241,7 +261,8 @@ void fmm_release(shp_handle h) {
     if (it == g_tables.end()) return;
     FmmDevTables& d = it->second;
     void* ptrs[] = {d.mi, d.pair_alpha, d.pair_gamma, d.pair_w, d.task, d.beta_tasks, d.up_range,
-                    d.up_k, d.up_c, d.up_bits, d.dn_range, d.dn_b, d.dn_c, d.dn_bits};
+                    d.up_k, d.up_c, d.up_bits, d.dn_range, d.dn_b, d.dn_c, d.dn_bits,
+                    d.mscale, d.dscale, d.lscale, d.zeros};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     g_tables.erase(it);
@@ -299,20 +320,30 @@ __global__ void k_fmm_bbox_partial(const double* __restrict__ X, const double* _
 }
 
 // Root cube (x0, y0, z0, side): side = largest extent times (1 + 2^-20),
-// centred on the bounding box.
-__global__ void k_fmm_root(const double* __restrict__ part, int nparts, double* __restrict__ root) {
-    if (threadIdx.x != 0) return;
+// centred on the bounding box; root[4..9] = bounding box min, max.  One wave.
+__global__ __launch_bounds__(64) void k_fmm_root(const double* __restrict__ part, int nparts,
+                                                 double* __restrict__ root) {
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-    for (int b = 0; b < nparts; b++)
+    for (int b = threadIdx.x; b < nparts; b += 64)
         for (int c = 0; c < 3; c++) {
             mn[c] = fmin(mn[c], part[b * 6 + c]);
             mx[c] = fmax(mx[c], part[b * 6 + 3 + c]);
         }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int c = 0; c < 3; c++) {
+            mn[c] = fmin(mn[c], __shfl_xor(mn[c], off));
+            mx[c] = fmax(mx[c], __shfl_xor(mx[c], off));
+        }
+    if (threadIdx.x != 0) return;
     double side = fmax(fmax(mx[0] - mn[0], mx[1] - mn[1]), mx[2] - mn[2]);
     if (!(side > 0)) side = 1e-9;
     side = side * (1.0 + 0x1p-20);
     for (int c = 0; c < 3; c++) root[c] = 0.5 * (mn[c] + mx[c]) - 0.5 * side;
     root[3] = side;
+    for (int c = 0; c < 3; c++) {
+        root[4 + c] = mn[c];
+        root[7 + c] = mx[c];
+    }
 }
 
 // Level-21 Morton key of each active particle; inactive ones sort last.
@@ -678,6 +709,218 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_m2l(
     }
 }
 
+// ------------------------------------------------- M2L, unrolled per lane ----
+//
+// With Mhat_a = (-1)^|a| M_a / a!, D_g = g! T_g and Lhat_b = b! L_b the M2L
+// contraction is Lhat_b = sum_{|a|+|b|<=p} D_{a+b} Mhat_a: no weights, and
+// the (a, b, a+b) index triples are compile-time constants.  A lane owns one
+// target box and walks the far-band offsets in the same order as every other
+// lane of its wave, so D_{a+b}(offset) is a uniform (scalar) operand and the
+// multiply-adds unroll into straight-line code; only Mhat of the source box is
+// a per-lane (vector) load.  The outputs are split into passes by the degree of
+// b so that the accumulators fit in registers.  The expansion order is chosen
+// per Chebyshev shell of the offset (3, 4, 5 box widths).
+
+constexpr int cidx(int a, int b, int c) {
+    return (a + b + c) * (a + b + c + 1) * (a + b + c + 2) / 6 + (b + c) * (b + c + 1) / 2 + c;
+}
+constexpr int dstart(int n) { return n * (n + 1) * (n + 2) / 6; }
+
+struct PairIdx {
+    short ia, ib, ig;
+};
+
+template <int N>
+struct PairArr {
+    PairIdx v[N > 0 ? N : 1];
+};
+
+constexpr int count_pairs(int P, int BLO, int BHI) {
+    int k = 0;
+    for (int na = 0; na <= P; na++)
+        for (int nb = BLO; nb <= BHI; nb++)
+            if (na + nb <= P) k += (na + 1) * (na + 2) / 2 * ((nb + 1) * (nb + 2) / 2);
+    return k;
+}
+
+// alpha-major list so that each Mhat_a is loaded once and then dead
+template <int P, int BLO, int BHI>
+constexpr PairArr<count_pairs(P, BLO, BHI)> make_pairs() {
+    PairArr<count_pairs(P, BLO, BHI)> out{};
+    int k = 0;
+    for (int na = 0; na <= P; na++)
+        for (int a1 = na; a1 >= 0; a1--)
+            for (int a2 = na - a1; a2 >= 0; a2--) {
+                const int a3 = na - a1 - a2;
+                for (int nb = BLO; nb <= BHI; nb++) {
+                    if (na + nb > P) continue;
+                    for (int b1 = nb; b1 >= 0; b1--)
+                        for (int b2 = nb - b1; b2 >= 0; b2--) {
+                            const int b3 = nb - b1 - b2;
+                            out.v[k].ia = (short)cidx(a1, a2, a3);
+                            out.v[k].ib = (short)(cidx(b1, b2, b3) - dstart(BLO));
+                            out.v[k].ig = (short)cidx(a1 + b1, a2 + b2, a3 + b3);
+                            k++;
+                        }
+                }
+            }
+    return out;
+}
+
+template <int P, int BLO, int BHI>
+struct PairTable {
+    static constexpr int n = count_pairs(P, BLO, BHI);
+    static constexpr PairArr<n> t = make_pairs<P, BLO, BHI>();
+};
+
+constexpr int kTermChunk = 1024;  // terms per fold expression (compiler nesting limit)
+
+template <int P, int BLO, int BHI, int OFF, size_t... I>
+__device__ __forceinline__ void m2l_terms(float* acc, const float* __restrict__ Ms,
+                                          const float* __restrict__ D, std::index_sequence<I...>) {
+    using T = PairTable<P, BLO, BHI>;
+    ((acc[T::t.v[OFF + I].ib] =
+          fmaf(D[T::t.v[OFF + I].ig], Ms[T::t.v[OFF + I].ia], acc[T::t.v[OFF + I].ib])),
+     ...);
+}
+
+template <int P, int BLO, int BHI, size_t... C>
+__device__ __forceinline__ void m2l_chunks(float* acc, const float* __restrict__ Ms,
+                                           const float* __restrict__ D, std::index_sequence<C...>) {
+    constexpr int n = PairTable<P, BLO, BHI>::n;
+    (m2l_terms<P, BLO, BHI, (int)C * kTermChunk>(
+         acc, Ms, D,
+         std::make_index_sequence<((int)C + 1) * kTermChunk <= n ? kTermChunk
+                                                                 : n - (int)C * kTermChunk>{}),
+     ...);
+}
+
+template <int P, int BLO, int BHI>
+__device__ __forceinline__ void m2l_apply(float* acc, const float* __restrict__ Ms,
+                                          const float* __restrict__ D) {
+    if constexpr (P >= BLO) {
+        constexpr int hi = BHI < P ? BHI : P;
+        constexpr int n = PairTable<P, BLO, hi>::n;
+        m2l_chunks<P, BLO, hi>(acc, Ms, D,
+                               std::make_index_sequence<(n + kTermChunk - 1) / kTermChunk>{});
+    }
+}
+
+struct M2LLevel {
+    int l, nb, ring;
+    const uint64_t* key;
+    const float* Mhat;   // [box][nc]
+    float* Lc;           // [box][nc]
+    const float* D;      // [offset][nc], offsets (dx+5) + 11 (dy+5) + 121 (dz+5)
+    const int* grid;     // dense box index grid over the occupied cells
+    int lo[3], dim[3];
+};
+
+template <int P3, int P4, int P5, int BLO, int BHI>
+__global__ __launch_bounds__(64) void k_fmm_m2l_unrolled(M2LLevel lv, int nc,
+                                                         const float* __restrict__ lscale,
+                                                         const float* __restrict__ zeros,
+                                                         unsigned long long* __restrict__ stats) {
+    constexpr int NB = dstart(BHI + 1) - dstart(BLO);
+    float acc[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) acc[k] = 0.0f;
+    const int b = blockIdx.x * 64 + threadIdx.x;
+    const bool valid = b < lv.nb;
+    const uint64_t key = lv.key[valid ? b : 0];
+    const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
+    const int cmax = (1 << lv.l) - 1;
+    int used[3] = {0, 0, 0};
+    for (int shell = 3; shell <= 5; shell++) {
+        for (int o = 0; o < 1331; o++) {
+            const int dx = o % 11 - 5, dy = (o / 11) % 11 - 5, dz = o / 121 - 5;
+            const int sh = max(abs(dx), max(abs(dy), abs(dz)));
+            if (sh != shell) continue;  // uniform
+            const int sx = ix - dx, sy = iy - dy, sz = iz - dz;
+            bool ok = valid && sx >= 0 && sy >= 0 && sz >= 0 && sx <= cmax && sy <= cmax && sz <= cmax &&
+                      abs((sx >> 1) - (ix >> 1)) <= lv.ring && abs((sy >> 1) - (iy >> 1)) <= lv.ring &&
+                      abs((sz >> 1) - (iz >> 1)) <= lv.ring;
+            int s = -1;
+            if (ok) {
+                const int gx = sx - lv.lo[0], gy = sy - lv.lo[1], gz = sz - lv.lo[2];
+                if (gx >= 0 && gy >= 0 && gz >= 0 && gx < lv.dim[0] && gy < lv.dim[1] && gz < lv.dim[2])
+                    s = lv.grid[gx + lv.dim[0] * (gy + lv.dim[1] * gz)];
+            }
+            used[shell - 3] += s >= 0;
+            const float* Ms = s >= 0 ? lv.Mhat + (size_t)s * nc : zeros;
+            const float* D = lv.D + (size_t)o * nc;
+            if (shell == 3) m2l_apply<P3, BLO, BHI>(acc, Ms, D);
+            else if (shell == 4) m2l_apply<P4, BLO, BHI>(acc, Ms, D);
+            else m2l_apply<P5, BLO, BHI>(acc, Ms, D);
+        }
+    }
+    if (valid) {
+        float* Lb = lv.Lc + (size_t)b * nc + dstart(BLO);
+        const float* sc = lscale + dstart(BLO);
+#pragma unroll
+        for (int k = 0; k < NB; k++) Lb[k] += acc[k] * sc[k];
+    }
+    if (BLO == 0 && stats) {
+        // one count per wave and shell: sum of the lanes' source counts
+        for (int k = 0; k < 3; k++) {
+            int u = used[k];
+            for (int off = 32; off > 0; off >>= 1) u += __shfl_xor(u, off);
+            if (threadIdx.x == 0) {
+                atomicAdd(stats, (unsigned long long)u);
+                atomicAdd(stats + 2 + k, (unsigned long long)u);
+            }
+        }
+    }
+}
+
+// Orders per Chebyshev shell (3, 4, 5) of the unrolled M2L: p, p-2, p-3 for
+// the compiled orders with a two-cell near field; 0 when not available.
+static void m2l_orders(int p, int ring, int& p3, int& p4, int& p5) {
+    p3 = p4 = p5 = 0;
+    if (ring != 2) return;
+    if (p == 4 || p == 8 || p == 10) {
+        p3 = p;
+        p4 = p - 2;
+        p5 = p - 3;
+    }
+}
+
+template <int P3, int P4, int P5>
+static void launch_m2l_passes(const M2LLevel& m, int nc, const float* lscale, const float* zeros,
+                              unsigned long long* stats, hipStream_t st) {
+    const dim3 g(fdiv_up(m.nb, 64)), b(64);
+    hipLaunchKernelGGL((k_fmm_m2l_unrolled<P3, P4, P5, 0, 4>), g, b, 0, st, m, nc, lscale, zeros, stats);
+    if constexpr (P3 >= 5)
+        hipLaunchKernelGGL((k_fmm_m2l_unrolled<P3, P4, P5, 5, 6>), g, b, 0, st, m, nc, lscale, zeros, stats);
+    if constexpr (P3 >= 7)
+        hipLaunchKernelGGL((k_fmm_m2l_unrolled<P3, P4, P5, 7, 8>), g, b, 0, st, m, nc, lscale, zeros, stats);
+    if constexpr (P3 >= 9)
+        hipLaunchKernelGGL((k_fmm_m2l_unrolled<P3, P4, P5, 9, 10>), g, b, 0, st, m, nc, lscale, zeros, stats);
+}
+
+static void launch_m2l_unrolled(const M2LLevel& m, int nc, int p3, const FmmDevTables* tb,
+                                unsigned long long* stats, hipStream_t st) {
+    if (p3 == 10) launch_m2l_passes<10, 8, 7>(m, nc, tb->lscale, tb->zeros, stats, st);
+    else if (p3 == 8) launch_m2l_passes<8, 6, 5>(m, nc, tb->lscale, tb->zeros, stats, st);
+    else if (p3 == 4) launch_m2l_passes<4, 2, 1>(m, nc, tb->lscale, tb->zeros, stats, st);
+}
+
+__global__ void k_fmm_scale(float* __restrict__ v, const float* __restrict__ sc, int nc, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] *= sc[i % nc];
+}
+
+__global__ void k_fmm_grid_scatter(const uint64_t* __restrict__ key, int nb, int lo0, int lo1, int lo2,
+                                   int d0, int d1, int d2, int* __restrict__ grid) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const uint64_t k = key[b];
+    const int gx = (int)compact21(k) - lo0, gy = (int)compact21(k >> 1) - lo1,
+              gz = (int)compact21(k >> 2) - lo2;
+    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < d0 && gy < d1 && gz < d2)
+        grid[gx + d0 * (gy + d1 * gz)] = b;
+}
+
 // ------------------------------------------------------------------- L2L ----
 
 __global__ __launch_bounds__(kFmmThreads) void k_fmm_l2l(Level par, Level ch, int nc,
@@ -871,7 +1114,7 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     // bounding box and root
     const int nbp = std::min(fdiv_up(n, kFmmThreads), 1024);
     SHP_WS(bpart, double, h, WS_BBOX, nbp * 6);
-    SHP_WS(root, double, h, WS_BOX, 8);
+    SHP_WS(root, double, h, WS_BOX, 16);
     hipLaunchKernelGGL(k_fmm_bbox_partial, dim3(nbp), dim3(kFmmThreads), 0, st, pt->x, pt->y, pt->z,
                        pt->status, n, bpart);
     hipLaunchKernelGGL(k_fmm_root, dim3(1), dim3(64), 0, st, bpart, nbp, root);
@@ -891,7 +1134,9 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     hipLaunchKernelGGL(k_fmm_ifill, dim3(1), dim3(32), 0, st, cnt, 32, 0);
     hipLaunchKernelGGL(k_fmm_level_hist, dim3(fdiv_up(n, 256)), dim3(256), 0, st, keys2, n, cnt);
     int hc[32];
+    double hroot[10];
     SHP_HIP_CHECK(hipMemcpyAsync(hc, cnt, sizeof(int) * 32, hipMemcpyDeviceToHost, st));
+    SHP_HIP_CHECK(hipMemcpyAsync(hroot, root, sizeof(double) * 10, hipMemcpyDeviceToHost, st));
     SHP_HIP_CHECK(hipStreamSynchronize(st));
     const int na = hc[0];
     std::vector<int> nbox_all(kMaxLevel + 1);
@@ -976,11 +1221,69 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     SHP_WS(ttab, float, h, WS_FMM_T, (size_t)noff * (L + 1) * nc);
     hipLaunchKernelGGL(k_fmm_ttable, dim3(fdiv_up(noff * (L + 1), 64)), dim3(64), 0, st, order, nc,
                        tb->mi, W, 0, L, root, kSoftening, ttab);
+    // Unrolled M2L (ring 2, orders with compiled pair lists) needs a dense box
+    // grid per level over the occupied cells; very large grids use the
+    // table-driven kernel instead.
+    int p3 = 0, p4 = 0, p5 = 0;
+    m2l_orders(order, ring, p3, p4, p5);
+    std::vector<int> glo(3 * (L + 1)), gdim(3 * (L + 1));
+    size_t gtot = 0;
+    for (int l = 0; l <= L; l++) {
+        const double sc = std::ldexp(1.0, l) / hroot[3];
+        size_t cells = 1;
+        for (int c = 0; c < 3; c++) {
+            const int top = (1 << l) - 1;
+            int lo = (int)std::floor((hroot[4 + c] - hroot[c]) * sc) - 1;
+            int hi = (int)std::floor((hroot[7 + c] - hroot[c]) * sc) + 1;
+            lo = std::max(0, std::min(lo, top));
+            hi = std::max(lo, std::min(hi, top));
+            glo[3 * l + c] = lo;
+            gdim[3 * l + c] = hi - lo + 1;
+            cells *= (size_t)(hi - lo + 1);
+        }
+        gtot += cells;
+    }
+    const bool unrolled = p3 > 0 && gtot <= ((size_t)1 << 27);
     SHP_HIP_CHECK(hipEventRecord(h->fev[2], st));
-    for (int l = 1; l <= L; l++)
-        hipLaunchKernelGGL(k_fmm_m2l, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l - 1], l, ring,
-                           nc, tb->ntask, tb->task, tb->pair_alpha, tb->pair_gamma, tb->pair_w,
-                           tb->beta_tasks, ttab + (size_t)l * noff * nc, W, h->fmm_stats);
+    if (unrolled) {
+        SHP_WS(grid, int, h, WS_FMM_GRID, gtot);
+        hipLaunchKernelGGL(k_fmm_ifill, dim3(fdiv_up((long)gtot, 256)), dim3(256), 0, st, grid, (int)gtot, -1);
+        hipLaunchKernelGGL(k_fmm_scale, dim3(fdiv_up((long)tot * nc, 256)), dim3(256), 0, st, bexp, tb->mscale,
+                           nc, tot * (size_t)nc);
+        hipLaunchKernelGGL(k_fmm_scale, dim3(fdiv_up((long)noff * (L + 1) * nc, 256)), dim3(256), 0, st, ttab,
+                           tb->dscale, nc, (size_t)noff * (L + 1) * nc);
+        size_t goff = 0;
+        for (int l = 0; l <= L; l++) {
+            const size_t cells = (size_t)gdim[3 * l] * gdim[3 * l + 1] * gdim[3 * l + 2];
+            int* gl = grid + goff;
+            hipLaunchKernelGGL(k_fmm_grid_scatter, dim3(fdiv_up(nbox[l], 256)), dim3(256), 0, st, lv[l].key,
+                               nbox[l], glo[3 * l], glo[3 * l + 1], glo[3 * l + 2], gdim[3 * l],
+                               gdim[3 * l + 1], gdim[3 * l + 2], gl);
+            if (l >= 1) {
+                M2LLevel m{};
+                m.l = l;
+                m.nb = nbox[l];
+                m.ring = ring;
+                m.key = lv[l].key;
+                m.Mhat = lv[l].M;
+                m.Lc = lv[l].Lc;
+                m.D = ttab + (size_t)l * noff * nc;
+                m.grid = gl;
+                for (int c = 0; c < 3; c++) {
+                    m.lo[c] = glo[3 * l + c];
+                    m.dim[c] = gdim[3 * l + c];
+                }
+                launch_m2l_unrolled(m, nc, p3, tb, h->fmm_stats, st);
+            }
+            goff += cells;
+        }
+    } else {
+        for (int l = 1; l <= L; l++)
+            hipLaunchKernelGGL(k_fmm_m2l, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l - 1], l,
+                               ring, nc, tb->ntask, tb->task, tb->pair_alpha, tb->pair_gamma, tb->pair_w,
+                               tb->beta_tasks, ttab + (size_t)l * noff * nc, W, h->fmm_stats);
+    }
+    SHP_HIP_CHECK(hipGetLastError());
     SHP_HIP_CHECK(hipEventRecord(h->fev[3], st));
     for (int l = 1; l < L; l++)
         hipLaunchKernelGGL(k_fmm_l2l, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l + 1], nc,
@@ -994,7 +1297,12 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     SHP_HIP_CHECK(hipEventRecord(h->ev1, st));
     h->timed = true;
     h->fmm_timed = true;
-    h->fmm_npairs = (double)tb->npairs;
+    // multiply-adds per M2L box pair for each shell (or the single order)
+    auto madds_of = [](int p) { return p <= 0 ? 0.0 : binom_d(p + 6, 6); };
+    h->fmm_unrolled = unrolled;
+    h->fmm_shell_madds[0] = unrolled ? madds_of(p3) : (double)tb->npairs;
+    h->fmm_shell_madds[1] = unrolled ? madds_of(p4) : (double)tb->npairs;
+    h->fmm_shell_madds[2] = unrolled ? madds_of(p5) : (double)tb->npairs;
     return SHP_OK;
 }
 
@@ -1009,12 +1317,16 @@ int shp_fmm_stats(shp_handle h, double* out) {
         SHP_HIP_CHECK(hipEventElapsedTime(&ms, h->fev[k], h->fev[k + 1]));
         out[k] = ms;
     }
-    unsigned long long c[2];
+    unsigned long long c[5];
     SHP_HIP_CHECK(hipMemcpy(c, h->fmm_stats, sizeof(c), hipMemcpyDeviceToHost));
     out[5] = (double)c[0];
     out[6] = (double)c[1];
     out[7] = last_leaf_level(h);
-    out[8] = h->fmm_npairs;
+    if (h->fmm_unrolled)
+        out[8] = c[2] * h->fmm_shell_madds[0] + c[3] * h->fmm_shell_madds[1] +
+                 c[4] * h->fmm_shell_madds[2];
+    else
+        out[8] = c[0] * h->fmm_shell_madds[0];
     return SHP_OK;
 }
 

@@ -62,6 +62,7 @@ enum WsSlot {
     WS_FMM_BINT,    // box start/end/first-child of all levels
     WS_FMM_EXP,     // multipole and local coefficients
     WS_FMM_T,       // translation tables
+    WS_FMM_GRID,    // dense box-index grids of all levels
     WS_NSLOTS
 };
 
@@ -96,7 +97,8 @@ struct shp_context {
     hipEvent_t fev[8] = {};                   // FMM pass timing
     unsigned long long* fmm_stats = nullptr;  // device counters of the last FMM call
     bool fmm_timed = false;
-    double fmm_npairs = 0;
+    bool fmm_unrolled = false;
+    double fmm_shell_madds[3] = {0, 0, 0};  // multiply-adds per M2L pair, shells 3/4/5
     bool timed = false, timed_update = false;
     void* ws[shp::WS_NSLOTS] = {};
     size_t ws_bytes[shp::WS_NSLOTS] = {};

@@ -65,3 +65,25 @@ def test_fmm_converges_to_direct():
         errs.append(oracle.rel_l2(a, ref)[0])
     assert errs[0] > errs[1] > errs[2]
     assert errs[2] < 2e-5
+
+
+def test_shell_orders():
+    """Per-shell truncation: orders equal to p change nothing; the engine's
+    reduced orders (p, p-2, p-3 on shells 3/4/5) stay within 1e-5 of the direct
+    sum at p=10 and are less accurate than the uniform order."""
+    d = synth.cone(8192, seed=5)
+    ref = oracle.direct_f64(d)
+    base, _ = oracle.fmm_f64(d, 10, 5, ring=2)
+    try:
+        oracle.fmm_set_shell_orders([10] * 8)
+        same, _ = oracle.fmm_f64(d, 10, 5, ring=2)
+        np.testing.assert_array_equal(same, base)
+        oracle.fmm_set_shell_orders([0, 0, 0, 10, 8, 7])
+        red, _ = oracle.fmm_f64(d, 10, 5, ring=2)
+        oracle.fmm_set_shell_orders([0, 0, 0, 10, 2, 2])
+        crude, _ = oracle.fmm_f64(d, 10, 5, ring=2)
+    finally:
+        oracle.fmm_set_shell_orders(None)
+    e_base, e_red, e_crude = (oracle.rel_l2(a, ref)[0] for a in (base, red, crude))
+    assert e_base <= e_red < 1e-5
+    assert e_crude > 3 * e_red
