@@ -194,6 +194,11 @@ struct FmmDevTables {
     float* dscale = nullptr;
     float* lscale = nullptr;
     float* zeros = nullptr;
+    // far-offset list of the tiled M2L (ring 2) and its chunk bounds
+    short* olist = nullptr;
+    int olist_n = 0, olist_s4 = 0, olist_s5 = 0;
+    int* cbound = nullptr;
+    int cbound_key = -1;  // (p3, chunks) the bounds were made for
 };
 
 static std::map<shp_handle, FmmDevTables> g_tables;
@@ -262,7 +267,7 @@ void fmm_release(shp_handle h) {
     FmmDevTables& d = it->second;
     void* ptrs[] = {d.mi, d.pair_alpha, d.pair_gamma, d.pair_w, d.task, d.beta_tasks, d.up_range,
                     d.up_k, d.up_c, d.up_bits, d.dn_range, d.dn_b, d.dn_c, d.dn_bits,
-                    d.mscale, d.dscale, d.lscale, d.zeros};
+                    d.mscale, d.dscale, d.lscale, d.zeros, d.olist, d.cbound};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     g_tables.erase(it);
@@ -807,7 +812,8 @@ __device__ __forceinline__ void m2l_apply(float* acc, const float* __restrict__ 
 }
 
 struct M2LLevel {
-    int l, nb, ring;
+    int l, nb;
+    long gbase;          // index of the level's first box in the all-level box numbering
     const uint64_t* key;
     const float* Mhat;   // [box][nc]
     float* Lc;           // [box][nc]
@@ -816,56 +822,101 @@ struct M2LLevel {
     int lo[3], dim[3];
 };
 
+// All M2L levels of one evaluation.  Work unit = (tile of 64 Morton-consecutive
+// boxes of one level, chunk of the far-offset list); the offset list is sorted
+// by Chebyshev shell (3, 4, 5) and cut into chunks of equal cost per pass.
+constexpr int kM2LMaxLev = 20;
+struct M2LPlan {
+    int nlev, ntile, nchunk, ring, nc;
+    int s4, s5;                       // first offset-list entries of shells 4 and 5
+    int tile0[kM2LMaxLev + 1];        // first tile of each level
+    M2LLevel lv[kM2LMaxLev];
+};
+
+// The four output-degree ranges computed by separate kernels (registers).
+constexpr int kM2LPasses = 4;
+constexpr int kPassLo[kM2LPasses] = {0, 5, 7, 9};
+constexpr int kPassHi[kM2LPasses] = {4, 6, 8, 10};
+
+constexpr int pass_pairs(int P, int lo, int hi) { return P >= lo ? count_pairs(P, lo, hi < P ? hi : P) : 0; }
+
 template <int P3, int P4, int P5, int BLO, int BHI>
-__global__ __launch_bounds__(64) void k_fmm_m2l_unrolled(M2LLevel lv, int nc,
-                                                         const float* __restrict__ lscale,
-                                                         const float* __restrict__ zeros,
-                                                         unsigned long long* __restrict__ stats) {
+__global__ __launch_bounds__(64) void k_fmm_m2l_tiled(const M2LPlan pl, const short* __restrict__ olist,
+                                                      const int* __restrict__ cbound,
+                                                      const float* __restrict__ lscale,
+                                                      const float* __restrict__ zeros,
+                                                      float* __restrict__ part, size_t part_stride,
+                                                      unsigned long long* __restrict__ stats) {
     constexpr int NB = dstart(BHI + 1) - dstart(BLO);
+    constexpr bool w3 = pass_pairs(P3, BLO, BHI) > 0, w4 = pass_pairs(P4, BLO, BHI) > 0,
+                   w5 = pass_pairs(P5, BLO, BHI) > 0;
+    // Blocks are dealt round-robin to the 8 XCDs: renumber so that each XCD
+    // runs a contiguous range of tiles (neighbouring tiles share source boxes
+    // in that XCD's L2).
+    const int nblk = pl.ntile * pl.nchunk;
+    const int per = (nblk + 7) >> 3;
+    const int w = blockIdx.x;
+    const int n = (w & 7) * per + (w >> 3);
+    if (n >= nblk) return;
+    const int tile = n % pl.ntile, chunk = n / pl.ntile;
+    int li = 0;
+    while (li + 1 < pl.nlev && pl.tile0[li + 1] <= tile) li++;
+    const M2LLevel& lv = pl.lv[li];
+    const int nc = pl.nc;
     float acc[NB];
 #pragma unroll
     for (int k = 0; k < NB; k++) acc[k] = 0.0f;
-    const int b = blockIdx.x * 64 + threadIdx.x;
+    const int b = (tile - pl.tile0[li]) * 64 + threadIdx.x;
     const bool valid = b < lv.nb;
     const uint64_t key = lv.key[valid ? b : 0];
     const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
     const int cmax = (1 << lv.l) - 1;
     int used[3] = {0, 0, 0};
-    for (int shell = 3; shell <= 5; shell++) {
-        for (int o = 0; o < 1331; o++) {
-            const int dx = o % 11 - 5, dy = (o / 11) % 11 - 5, dz = o / 121 - 5;
-            const int sh = max(abs(dx), max(abs(dy), abs(dz)));
-            if (sh != shell) continue;  // uniform
-            const int sx = ix - dx, sy = iy - dy, sz = iz - dz;
-            bool ok = valid && sx >= 0 && sy >= 0 && sz >= 0 && sx <= cmax && sy <= cmax && sz <= cmax &&
-                      abs((sx >> 1) - (ix >> 1)) <= lv.ring && abs((sy >> 1) - (iy >> 1)) <= lv.ring &&
-                      abs((sz >> 1) - (iz >> 1)) <= lv.ring;
-            int s = -1;
-            if (ok) {
-                const int gx = sx - lv.lo[0], gy = sy - lv.lo[1], gz = sz - lv.lo[2];
-                if (gx >= 0 && gy >= 0 && gz >= 0 && gx < lv.dim[0] && gy < lv.dim[1] && gz < lv.dim[2])
-                    s = lv.grid[gx + lv.dim[0] * (gy + lv.dim[1] * gz)];
-            }
-            used[shell - 3] += s >= 0;
-            const float* Ms = s >= 0 ? lv.Mhat + (size_t)s * nc : zeros;
-            const float* D = lv.D + (size_t)o * nc;
-            if (shell == 3) m2l_apply<P3, BLO, BHI>(acc, Ms, D);
-            else if (shell == 4) m2l_apply<P4, BLO, BHI>(acc, Ms, D);
-            else m2l_apply<P5, BLO, BHI>(acc, Ms, D);
+    const int i1 = cbound[chunk + 1];
+    for (int i = cbound[chunk]; i < i1; i++) {
+        const int shell = i < pl.s4 ? 3 : (i < pl.s5 ? 4 : 5);  // uniform
+        if ((shell == 3 && !w3) || (shell == 4 && !w4) || (shell == 5 && !w5)) continue;
+        const int o = olist[i];
+        const int dx = o % 11 - 5, dy = (o / 11) % 11 - 5, dz = o / 121 - 5;
+        const int sx = ix - dx, sy = iy - dy, sz = iz - dz;
+        const bool ok = valid && sx >= 0 && sy >= 0 && sz >= 0 && sx <= cmax && sy <= cmax &&
+                        sz <= cmax && abs((sx >> 1) - (ix >> 1)) <= pl.ring &&
+                        abs((sy >> 1) - (iy >> 1)) <= pl.ring && abs((sz >> 1) - (iz >> 1)) <= pl.ring;
+        int s = -1;
+        if (ok) {
+            const int gx = sx - lv.lo[0], gy = sy - lv.lo[1], gz = sz - lv.lo[2];
+            if (gx >= 0 && gy >= 0 && gz >= 0 && gx < lv.dim[0] && gy < lv.dim[1] && gz < lv.dim[2])
+                s = lv.grid[gx + lv.dim[0] * (gy + lv.dim[1] * gz)];
+        }
+        used[shell - 3] += s >= 0;
+        const float* Ms = s >= 0 ? lv.Mhat + (size_t)s * nc : zeros;
+        const float* D = lv.D + (size_t)o * nc;
+        if (shell == 3) {
+            if constexpr (w3) m2l_apply<P3, BLO, BHI>(acc, Ms, D);
+        } else if (shell == 4) {
+            if constexpr (w4) m2l_apply<P4, BLO, BHI>(acc, Ms, D);
+        } else {
+            if constexpr (w5) m2l_apply<P5, BLO, BHI>(acc, Ms, D);
         }
     }
     if (valid) {
-        float* Lb = lv.Lc + (size_t)b * nc + dstart(BLO);
-        const float* sc = lscale + dstart(BLO);
+        if (pl.nchunk == 1) {
+            float* Lb = lv.Lc + (size_t)b * nc + dstart(BLO);
+            const float* sc = lscale + dstart(BLO);
 #pragma unroll
-        for (int k = 0; k < NB; k++) Lb[k] += acc[k] * sc[k];
+            for (int k = 0; k < NB; k++) Lb[k] += acc[k] * sc[k];
+        } else {
+            float* Pb = part + chunk * part_stride + (size_t)(lv.gbase + b) * nc + dstart(BLO);
+#pragma unroll
+            for (int k = 0; k < NB; k++) Pb[k] = acc[k];
+        }
     }
     if (BLO == 0 && stats) {
         // one count per wave and shell: sum of the lanes' source counts
         for (int k = 0; k < 3; k++) {
             int u = used[k];
             for (int off = 32; off > 0; off >>= 1) u += __shfl_xor(u, off);
-            if (threadIdx.x == 0) {
+            if (threadIdx.x == 0 && u) {
                 atomicAdd(stats, (unsigned long long)u);
                 atomicAdd(stats + 2 + k, (unsigned long long)u);
             }
@@ -873,8 +924,58 @@ __global__ __launch_bounds__(64) void k_fmm_m2l_unrolled(M2LLevel lv, int nc,
     }
 }
 
-// Orders per Chebyshev shell (3, 4, 5) of the unrolled M2L: p, p-2, p-3 for
-// the compiled orders with a two-cell near field; 0 when not available.
+// L[box][k] += lscale[k] * sum over chunks of the partial sums (fixed order).
+__global__ void k_fmm_m2l_reduce(const float* __restrict__ part, size_t stride, int nchunk, int nc,
+                                 const float* __restrict__ lscale, float* __restrict__ Lc, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int c = 0; c < nchunk; c++) s += part[c * stride + i];
+    Lc[i] += s * lscale[i % nc];
+}
+
+// Offsets of shells 3..5 (ring 2) sorted by shell; chunk bounds per pass such
+// that each chunk carries about the same number of multiply-adds.
+static void m2l_offsets(std::vector<short>& ol, int& s4, int& s5) {
+    ol.clear();
+    for (int shell = 3; shell <= 5; shell++) {
+        if (shell == 4) s4 = (int)ol.size();
+        if (shell == 5) s5 = (int)ol.size();
+        for (int o = 0; o < 1331; o++) {
+            const int dx = o % 11 - 5, dy = (o / 11) % 11 - 5, dz = o / 121 - 5;
+            if (std::max(std::abs(dx), std::max(std::abs(dy), std::abs(dz))) == shell) ol.push_back((short)o);
+        }
+    }
+}
+
+static std::vector<int> m2l_chunk_bounds(int p3, int p4, int p5, int S, int s4, int s5, int nol) {
+    std::vector<int> cb;
+    for (int ps = 0; ps < kM2LPasses; ps++) {
+        const int c3 = pass_pairs(p3, kPassLo[ps], kPassHi[ps]);
+        const int c4 = pass_pairs(p4, kPassLo[ps], kPassHi[ps]);
+        const int c5 = pass_pairs(p5, kPassLo[ps], kPassHi[ps]);
+        auto cost = [&](int i) {
+            const int c = i < s4 ? c3 : (i < s5 ? c4 : c5);
+            return c ? (double)c + 64.0 : 0.0;  // + box lookup and loads
+        };
+        double tot = 0;
+        for (int i = 0; i < nol; i++) tot += cost(i);
+        cb.push_back(0);
+        double run = 0;
+        int c = 1;
+        for (int i = 0; i < nol && c < S; i++) {
+            run += cost(i);
+            while (c < S && run >= tot * c / S) {
+                cb.push_back(i + 1);
+                c++;
+            }
+        }
+        while ((int)cb.size() < (ps + 1) * (S + 1) - 1) cb.push_back(nol);
+        cb.push_back(nol);
+    }
+    return cb;
+}
+
 static void m2l_orders(int p, int ring, int& p3, int& p4, int& p5) {
     p3 = p4 = p5 = 0;
     if (ring != 2) return;
@@ -886,23 +987,51 @@ static void m2l_orders(int p, int ring, int& p3, int& p4, int& p5) {
 }
 
 template <int P3, int P4, int P5>
-static void launch_m2l_passes(const M2LLevel& m, int nc, const float* lscale, const float* zeros,
+static void launch_m2l_passes(const M2LPlan& pl, const short* olist, const int* cbound,
+                              const float* lscale, const float* zeros, float* part, size_t stride,
                               unsigned long long* stats, hipStream_t st) {
-    const dim3 g(fdiv_up(m.nb, 64)), b(64);
-    hipLaunchKernelGGL((k_fmm_m2l_unrolled<P3, P4, P5, 0, 4>), g, b, 0, st, m, nc, lscale, zeros, stats);
+    const int nblk = pl.ntile * pl.nchunk;
+    const dim3 g(8 * ((nblk + 7) / 8)), b(64);
+    const int S1 = pl.nchunk + 1;
+    hipLaunchKernelGGL((k_fmm_m2l_tiled<P3, P4, P5, 0, 4>), g, b, 0, st, pl, olist, cbound, lscale,
+                       zeros, part, stride, stats);
     if constexpr (P3 >= 5)
-        hipLaunchKernelGGL((k_fmm_m2l_unrolled<P3, P4, P5, 5, 6>), g, b, 0, st, m, nc, lscale, zeros, stats);
+        hipLaunchKernelGGL((k_fmm_m2l_tiled<P3, P4, P5, 5, 6>), g, b, 0, st, pl, olist, cbound + S1,
+                           lscale, zeros, part, stride, stats);
     if constexpr (P3 >= 7)
-        hipLaunchKernelGGL((k_fmm_m2l_unrolled<P3, P4, P5, 7, 8>), g, b, 0, st, m, nc, lscale, zeros, stats);
+        hipLaunchKernelGGL((k_fmm_m2l_tiled<P3, P4, P5, 7, 8>), g, b, 0, st, pl, olist, cbound + 2 * S1,
+                           lscale, zeros, part, stride, stats);
     if constexpr (P3 >= 9)
-        hipLaunchKernelGGL((k_fmm_m2l_unrolled<P3, P4, P5, 9, 10>), g, b, 0, st, m, nc, lscale, zeros, stats);
+        hipLaunchKernelGGL((k_fmm_m2l_tiled<P3, P4, P5, 9, 10>), g, b, 0, st, pl, olist, cbound + 3 * S1,
+                           lscale, zeros, part, stride, stats);
 }
 
-static void launch_m2l_unrolled(const M2LLevel& m, int nc, int p3, const FmmDevTables* tb,
-                                unsigned long long* stats, hipStream_t st) {
-    if (p3 == 10) launch_m2l_passes<10, 8, 7>(m, nc, tb->lscale, tb->zeros, stats, st);
-    else if (p3 == 8) launch_m2l_passes<8, 6, 5>(m, nc, tb->lscale, tb->zeros, stats, st);
-    else if (p3 == 4) launch_m2l_passes<4, 2, 1>(m, nc, tb->lscale, tb->zeros, stats, st);
+static void launch_m2l_tiled(const M2LPlan& pl, int p3, const FmmDevTables* tb, const int* cbound,
+                             float* part, size_t stride, unsigned long long* stats, hipStream_t st) {
+    if (p3 == 10) launch_m2l_passes<10, 8, 7>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st);
+    else if (p3 == 8) launch_m2l_passes<8, 6, 5>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st);
+    else if (p3 == 4) launch_m2l_passes<4, 2, 1>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st);
+}
+
+// Offset list and chunk bounds of the tiled M2L, made once per (order, chunks).
+static int m2l_bounds(FmmDevTables* tb, int p3, int p4, int p5, int S, const int** out) {
+    if (!tb->olist) {
+        std::vector<short> ol;
+        int s4 = 0, s5 = 0;
+        m2l_offsets(ol, s4, s5);
+        if (upload(&tb->olist, ol) != SHP_OK) return SHP_ERR_HIP;
+        tb->olist_n = (int)ol.size();
+        tb->olist_s4 = s4;
+        tb->olist_s5 = s5;
+    }
+    const int key = p3 * 64 + S;
+    if (tb->cbound_key != key) {
+        const std::vector<int> cb = m2l_chunk_bounds(p3, p4, p5, S, tb->olist_s4, tb->olist_s5, tb->olist_n);
+        if (upload(&tb->cbound, cb) != SHP_OK) return SHP_ERR_HIP;
+        tb->cbound_key = key;
+    }
+    *out = tb->cbound;
+    return SHP_OK;
 }
 
 __global__ void k_fmm_scale(float* __restrict__ v, const float* __restrict__ sc, int nc, size_t n) {
@@ -1253,29 +1382,61 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
         hipLaunchKernelGGL(k_fmm_scale, dim3(fdiv_up((long)noff * (L + 1) * nc, 256)), dim3(256), 0, st, ttab,
                            tb->dscale, nc, (size_t)noff * (L + 1) * nc);
         size_t goff = 0;
+        std::vector<int*> gl(L + 1);
         for (int l = 0; l <= L; l++) {
-            const size_t cells = (size_t)gdim[3 * l] * gdim[3 * l + 1] * gdim[3 * l + 2];
-            int* gl = grid + goff;
+            gl[l] = grid + goff;
             hipLaunchKernelGGL(k_fmm_grid_scatter, dim3(fdiv_up(nbox[l], 256)), dim3(256), 0, st, lv[l].key,
                                nbox[l], glo[3 * l], glo[3 * l + 1], glo[3 * l + 2], gdim[3 * l],
-                               gdim[3 * l + 1], gdim[3 * l + 2], gl);
-            if (l >= 1) {
-                M2LLevel m{};
+                               gdim[3 * l + 1], gdim[3 * l + 2], gl[l]);
+            goff += (size_t)gdim[3 * l] * gdim[3 * l + 1] * gdim[3 * l + 2];
+        }
+        if (L >= 2) {
+            M2LPlan pl{};
+            pl.ring = ring;
+            pl.nc = nc;
+            size_t gbase = 0;
+            for (int l = 0; l < 2; l++) gbase += nbox[l];
+            const size_t first = gbase;
+            for (int l = 2; l <= L; l++) {
+                M2LLevel& m = pl.lv[pl.nlev];
                 m.l = l;
                 m.nb = nbox[l];
-                m.ring = ring;
+                m.gbase = (long)(gbase - first);
                 m.key = lv[l].key;
                 m.Mhat = lv[l].M;
                 m.Lc = lv[l].Lc;
                 m.D = ttab + (size_t)l * noff * nc;
-                m.grid = gl;
+                m.grid = gl[l];
                 for (int c = 0; c < 3; c++) {
                     m.lo[c] = glo[3 * l + c];
                     m.dim[c] = gdim[3 * l + c];
                 }
-                launch_m2l_unrolled(m, nc, p3, tb, h->fmm_stats, st);
+                pl.tile0[pl.nlev] = pl.ntile;
+                pl.ntile += fdiv_up(nbox[l], 64);
+                pl.nlev++;
+                gbase += nbox[l];
             }
-            goff += cells;
+            pl.tile0[pl.nlev] = pl.ntile;
+            // Enough work units to give every SIMD several waves; the partial
+            // sums of the offset chunks take S x (boxes of levels >= 2) x nc floats.
+            const size_t nrow = gbase - first;
+            int S = std::max(1, std::min(32, fdiv_up(8 * 4 * h->num_cus, std::max(1, pl.ntile))));
+            while (S > 1 && (size_t)S * nrow * nc * sizeof(float) > ((size_t)4 << 30)) S--;
+            pl.nchunk = S;
+            const int* cb = nullptr;
+            SHP_REQUIRE(m2l_bounds(tb, p3, p4, p5, S, &cb) == SHP_OK, "shp_fmm_dev: M2L chunk table");
+            pl.s4 = tb->olist_s4;  // set by m2l_bounds on first use
+            pl.s5 = tb->olist_s5;
+            float* part = nullptr;
+            const size_t stride = nrow * nc;
+            if (S > 1) {
+                SHP_WS(pw, float, h, WS_FMM_PART, (size_t)S * stride);
+                part = pw;
+            }
+            launch_m2l_tiled(pl, p3, tb, cb, part, stride, h->fmm_stats, st);
+            if (S > 1)
+                hipLaunchKernelGGL(k_fmm_m2l_reduce, dim3(fdiv_up((long)stride, 256)), dim3(256), 0, st,
+                                   part, stride, S, nc, tb->lscale, lv[2].Lc, stride);
         }
     } else {
         for (int l = 1; l <= L; l++)

@@ -63,6 +63,7 @@ enum WsSlot {
     WS_FMM_EXP,     // multipole and local coefficients
     WS_FMM_T,       // translation tables
     WS_FMM_GRID,    // dense box-index grids of all levels
+    WS_FMM_PART,    // M2L partial sums of the offset chunks
     WS_NSLOTS
 };
 
