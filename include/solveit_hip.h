@@ -122,11 +122,21 @@ int shp_fmm_dev(shp_handle h, const shp_particles* p, int order, int leaf_level,
 /* Leaf level used by the last shp_fmm_dev call on this handle. */
 int shp_fmm_last_leaf_level(shp_handle h);
 
-/* Measurements of the last shp_fmm_dev call (waits for it): out[0..4] = ms of
- * P2M+M2M, translation tables, M2L, L2L, L2P+P2P; out[5] = M2L box pairs;
- * out[6] = P2P particle pairs; out[7] = leaf level; out[8] = total M2L
- * multiply-adds (expansion terms actually evaluated, summed over box pairs). */
+/* Measurements of the last shp_fmm_dev call (waits for it), 11 values:
+ * out[0..4] = ms of P2M+M2M, translation tables, M2L, L2L, L2P+P2P;
+ * out[5] = M2L box pairs; out[6] = P2P particle pairs; out[7] = leaf level;
+ * out[8] = M2L multiply-adds (expansion terms of the box pairs that exist);
+ * out[9] = ms of the first M2L kernel (output degrees 0..4, the longest);
+ * out[10] = its multiply-adds.  No counterpart in the reference (its
+ * updateAcceleration has no pass timers). */
 int shp_fmm_stats(shp_handle h, double* out);
+
+/* Sums of the shp_fmm_stats values over every shp_fmm_dev call (including the
+ * calls inside shp_leapfrog_run) since the last reset: out[0..10] as above,
+ * summed; out[11] = number of calls.  out may be NULL; reset != 0 clears the
+ * sums after reading.  Waits for the last call.  Measurement only: no
+ * counterpart in the reference. */
+int shp_fmm_timing(shp_handle h, double* out, int reset);
 
 /* ---- background E-field (pwl on a scattered (r, z) triangulation) ---- */
 

@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <utility>
 #include <vector>
@@ -45,6 +46,14 @@ constexpr int kMaxNc = (kMaxP + 1) * (kMaxP + 2) * (kMaxP + 3) / 6;  // 455
 constexpr int kMaxLevel = 21;
 constexpr int kFmmThreads = 256;
 constexpr int kM2LBatch = 4;
+constexpr int kShiftLanes = 64;  // one wave per parent box in M2M / L2L
+
+// Index of the coefficient (a, b, c) in the degree-major order (within a
+// degree: a descending, then b descending) and first index of degree n.
+constexpr int cidx(int a, int b, int c) {
+    return (a + b + c) * (a + b + c + 1) * (a + b + c + 2) / 6 + (b + c) * (b + c + 1) / 2 + c;
+}
+constexpr int dstart(int n) { return n * (n + 1) * (n + 2) / 6; }
 
 // ------------------------------------------------------------- host tables ----
 
@@ -58,15 +67,14 @@ struct FmmHostTables {
     std::vector<int2> task;        // [pair begin, pair end)
     std::vector<int> task_beta;
     std::vector<int2> beta_tasks;  // [task begin, task end) per beta
-    // shift tables (M2M by alpha over kappa<=alpha; L2L by kappa over beta>=kappa)
-    std::vector<int2> up_range;    // per alpha
-    std::vector<int> up_k;
-    std::vector<float> up_c;
-    std::vector<int> up_diffbits;  // (a-k) parities packed 1 bit per axis
-    std::vector<int2> dn_range;    // per kappa
-    std::vector<int> dn_b;
-    std::vector<float> dn_c;
-    std::vector<int> dn_diffbits;
+    // M2M / L2L term lists of the 64 lanes of a shift workgroup, stored
+    // lane-interleaved (entry t of lane l at t * 64 + l) and padded with zero
+    // terms to the longest lane: {src | dst << 12 | last << 24, coefficient}.
+    // A lane accumulates coef * S[src] and stores to dst when `last` is set.
+    // M2M: src = class * nc + k in the octant transform, dst = output a;
+    // L2L: src = parent coefficient b, dst = class * nc + k.
+    std::vector<int2> m2m_term, l2l_term;
+    int m2m_nt = 0, l2l_nt = 0;
 };
 
 static double binom_d(int n, int k) {
@@ -134,42 +142,65 @@ static void build_tables(FmmHostTables& t, int p) {
         t.beta_tasks[be] = make_int2(tb, (int)t.task.size());
     }
     (void)npairs;
-    // shift tables
-    t.up_range.assign(nc, make_int2(0, 0));
-    t.up_k.clear();
-    t.up_c.clear();
-    t.up_diffbits.clear();
-    for (int al = 0; al < nc; al++) {
-        int4 A = t.mi[al];
-        int b = (int)t.up_k.size();
-        for (int k0 = 0; k0 <= A.x; k0++)
-            for (int k1 = 0; k1 <= A.y; k1++)
-                for (int k2 = 0; k2 <= A.z; k2++) {
-                    t.up_k.push_back(t.idx[k0][k1][k2]);
-                    t.up_c.push_back((float)(binom_d(A.x, k0) * binom_d(A.y, k1) * binom_d(A.z, k2) *
-                                             std::ldexp(1.0, -A.w)));
-                    t.up_diffbits.push_back(((A.x - k0) & 1) | (((A.y - k1) & 1) << 1) |
-                                            (((A.z - k2) & 1) << 2));
-                }
-        t.up_range[al] = make_int2(b, (int)t.up_k.size());
-    }
-    t.dn_range.assign(nc, make_int2(0, 0));
-    t.dn_b.clear();
-    t.dn_c.clear();
-    t.dn_diffbits.clear();
-    for (int ka = 0; ka < nc; ka++) {
-        int4 K = t.mi[ka];
-        int b = (int)t.dn_b.size();
-        for (int be = 0; be < nc; be++) {
-            int4 B = t.mi[be];
-            if (B.x < K.x || B.y < K.y || B.z < K.z) continue;
-            t.dn_b.push_back(be);
-            t.dn_c.push_back((float)(0.5 * binom_d(B.x, K.x) * binom_d(B.y, K.y) *
-                                     binom_d(B.z, K.z) * std::ldexp(1.0, -B.w)));
-            t.dn_diffbits.push_back(((B.x - K.x) & 1) | (((B.y - K.y) & 1) << 1) |
-                                    (((B.z - K.z) & 1) << 2));
+    // Shift term lists: outputs (M2M: a; L2L: (k, class of b)) are dealt
+    // longest first onto the least loaded lane; each output is a run of terms
+    // ending with `last`.
+    struct Term { int src, dst; float c; };
+    auto deal = [](std::vector<std::vector<Term>>& outs, std::vector<int2>& table, int& nt) {
+        std::vector<int> order(outs.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return outs[x].size() > outs[y].size(); });
+        std::vector<std::vector<int2>> lane(kShiftLanes);
+        std::vector<size_t> load(kShiftLanes, 0);
+        for (int i : order) {
+            if (outs[i].empty()) continue;
+            const int ln = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            for (size_t j = 0; j < outs[i].size(); j++) {
+                const Term& e = outs[i][j];
+                const int last = j + 1 == outs[i].size() ? 1 : 0;
+                int bits;
+                std::memcpy(&bits, &e.c, sizeof(bits));
+                lane[ln].push_back(make_int2(e.src | (e.dst << 12) | (last << 24), bits));
+            }
+            load[ln] += outs[i].size();
         }
-        t.dn_range[ka] = make_int2(b, (int)t.dn_b.size());
+        nt = (int)*std::max_element(load.begin(), load.end());
+        table.assign((size_t)nt * kShiftLanes, make_int2(0, 0));
+        for (int ln = 0; ln < kShiftLanes; ln++)
+            for (size_t j = 0; j < lane[ln].size(); j++) table[j * kShiftLanes + ln] = lane[ln][j];
+    };
+    {
+        // M2M: M'_a = 2^-|a| sum_{k<=a} C(a,k) W_k[class(a-k)]
+        std::vector<std::vector<Term>> outs(nc);
+        for (int a = 0; a < nc; a++) {
+            const int4 A = t.mi[a];
+            for (int kx = 0; kx <= A.x; kx++)
+                for (int ky = 0; ky <= A.y; ky++)
+                    for (int kz = 0; kz <= A.z; kz++) {
+                        const int cls = ((A.x - kx) & 1) | (((A.y - ky) & 1) << 1) | (((A.z - kz) & 1) << 2);
+                        const double c = binom_d(A.x, kx) * binom_d(A.y, ky) * binom_d(A.z, kz) *
+                                         std::ldexp(1.0, -A.w);
+                        outs[a].push_back({cls * nc + t.idx[kx][ky][kz], a, (float)c});
+                    }
+        }
+        deal(outs, t.m2m_term, t.m2m_nt);
+    }
+    {
+        // L2L: Q_r[k] = 1/2 sum_{b>=k, class(b)=r} C(b,k) 2^-|b| L_b
+        std::vector<std::vector<Term>> outs(8 * (size_t)nc);
+        for (int k = 0; k < nc; k++) {
+            const int4 K = t.mi[k];
+            for (int b = 0; b < nc; b++) {
+                const int4 B = t.mi[b];
+                if (B.x < K.x || B.y < K.y || B.z < K.z) continue;
+                const int cls = (B.x & 1) | ((B.y & 1) << 1) | ((B.z & 1) << 2);
+                const double c = 0.5 * binom_d(B.x, K.x) * binom_d(B.y, K.y) * binom_d(B.z, K.z) *
+                                 std::ldexp(1.0, -B.w);
+                outs[(size_t)cls * nc + k].push_back({b, cls * nc + k, (float)c});
+            }
+        }
+        deal(outs, t.l2l_term, t.l2l_nt);
     }
 }
 
@@ -182,14 +213,9 @@ struct FmmDevTables {
     float* pair_w = nullptr;
     int2* task = nullptr;
     int2* beta_tasks = nullptr;
-    int2* up_range = nullptr;
-    int* up_k = nullptr;
-    float* up_c = nullptr;
-    int* up_bits = nullptr;
-    int2* dn_range = nullptr;
-    int* dn_b = nullptr;
-    float* dn_c = nullptr;
-    int* dn_bits = nullptr;
+    int2* m2m_term = nullptr;
+    int2* l2l_term = nullptr;
+    int m2m_nt = 0, l2l_nt = 0;
     float* mscale = nullptr;
     float* dscale = nullptr;
     float* lscale = nullptr;
@@ -224,14 +250,10 @@ static int get_tables(shp_handle h, int p, FmmDevTables** out) {
         rc |= upload(&d.pair_w, t.pair_w);
         rc |= upload(&d.task, t.task);
         rc |= upload(&d.beta_tasks, t.beta_tasks);
-        rc |= upload(&d.up_range, t.up_range);
-        rc |= upload(&d.up_k, t.up_k);
-        rc |= upload(&d.up_c, t.up_c);
-        rc |= upload(&d.up_bits, t.up_diffbits);
-        rc |= upload(&d.dn_range, t.dn_range);
-        rc |= upload(&d.dn_b, t.dn_b);
-        rc |= upload(&d.dn_c, t.dn_c);
-        rc |= upload(&d.dn_bits, t.dn_diffbits);
+        rc |= upload(&d.m2m_term, t.m2m_term);
+        rc |= upload(&d.l2l_term, t.l2l_term);
+        d.m2m_nt = t.m2m_nt;
+        d.l2l_nt = t.l2l_nt;
         // scalings of the unrolled M2L: Mhat_a = (-1)^|a| M_a / a!,
         // D_g = g! T_g, L_b += Lhat_b / b!
         std::vector<float> ms(t.nc), ds(t.nc), ls(t.nc), zero(t.nc, 0.0f);
@@ -265,8 +287,8 @@ void fmm_release(shp_handle h) {
     auto it = g_tables.find(h);
     if (it == g_tables.end()) return;
     FmmDevTables& d = it->second;
-    void* ptrs[] = {d.mi, d.pair_alpha, d.pair_gamma, d.pair_w, d.task, d.beta_tasks, d.up_range,
-                    d.up_k, d.up_c, d.up_bits, d.dn_range, d.dn_b, d.dn_c, d.dn_bits,
+    void* ptrs[] = {d.mi, d.pair_alpha, d.pair_gamma, d.pair_w, d.task, d.beta_tasks, d.m2m_term,
+                    d.l2l_term,
                     d.mscale, d.dscale, d.lscale, d.zeros, d.olist, d.cbound};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -530,28 +552,55 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_p2m(
 
 // ------------------------------------------------------------------- M2M ----
 
-// s(o, diff) = prod over axes with child bit 0 of (-1)^(diff parity)
-__device__ __forceinline__ float octant_sign(int oct, int bits) {
-    return (__popc(bits & ~oct & 7) & 1) ? -1.0f : 1.0f;
-}
-
-__global__ __launch_bounds__(kFmmThreads) void k_fmm_m2m(Level par, Level ch, int nc,
-                                                         const int2* __restrict__ rng,
-                                                         const int* __restrict__ kk,
-                                                         const float* __restrict__ cc,
-                                                         const int* __restrict__ bits) {
-    const int b = blockIdx.x;
+// The octant sign s(o, g) = prod over axes with child bit 0 of (-1)^g_axis
+// depends on g only through its parity class, so the children of a parent are
+// first combined by a Walsh-Hadamard transform over the octant index:
+//   W_k[r] = sum_children s(o, r) M_k(child)        (r = parity class)
+//   M'_a   = 2^-|a| sum_{k<=a} C(a,k) W_k[class(a-k)]
+// which makes the shift cost one triangular sum per parent instead of one per
+// child.  One wave per parent; the triangular sums run from flat per-lane term
+// lists (uniform trip count, no divergent nested loops).
+__global__ __launch_bounds__(kShiftLanes) void k_fmm_m2m(Level par, Level ch, int nc,
+                                                         const int2* __restrict__ term, int nt) {
+    extern __shared__ float sm[];
+    float* W = sm;  // [8][nc]: child moments by octant, then by parity class
+    const int b = blockIdx.x, t = threadIdx.x;
     const int c0 = par.cfirst[b];
     const int c1 = b + 1 < par.nb ? par.cfirst[b + 1] : ch.nb;
-    for (int a = threadIdx.x; a < nc; a += blockDim.x) {
-        const int2 r = rng[a];
-        float acc = 0.0f;
-        for (int c = c0; c < c1; c++) {
-            const int oct = (int)(ch.key[c] & 7);
-            const float* Mc = ch.M + (size_t)c * nc;
-            for (int t = r.x; t < r.y; t++) acc = fmaf(cc[t] * octant_sign(oct, bits[t]), Mc[kk[t]], acc);
+    for (int i = t; i < 8 * nc; i += kShiftLanes) W[i] = 0.0f;
+    __syncthreads();
+    for (int c = c0; c < c1; c++) {
+        const int o = (int)(ch.key[c] & 7);
+        const float* Mc = ch.M + (size_t)c * nc;
+        for (int k = t; k < nc; k += kShiftLanes) W[o * nc + k] = Mc[k];
+    }
+    __syncthreads();
+    for (int k = t; k < nc; k += kShiftLanes) {
+        float u[8];
+#pragma unroll
+        for (int o = 0; o < 8; o++) u[o] = W[o * nc + k];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int o = 0; o < 8; o++)
+                if (!(o & (1 << d))) {
+                    const float x = u[o], y = u[o | (1 << d)];
+                    u[o] = x + y;            // class bit 0: sum over the axis
+                    u[o | (1 << d)] = y - x;  // class bit 1: upper minus lower child
+                }
+#pragma unroll
+        for (int o = 0; o < 8; o++) W[o * nc + k] = u[o];
+    }
+    __syncthreads();
+    float* Mp = par.M + (size_t)b * nc;
+    float acc = 0.0f;
+    for (int j = 0; j < nt; j++) {
+        const int2 e = term[j * kShiftLanes + t];
+        acc = fmaf(__int_as_float(e.y), W[e.x & 0xfff], acc);
+        if (e.x >> 24) {
+            Mp[(e.x >> 12) & 0xfff] = acc;
+            acc = 0.0f;
         }
-        par.M[(size_t)b * nc + a] = acc;
     }
 }
 
@@ -726,11 +775,6 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_m2l(
 // b so that the accumulators fit in registers.  The expansion order is chosen
 // per Chebyshev shell of the offset (3, 4, 5 box widths).
 
-constexpr int cidx(int a, int b, int c) {
-    return (a + b + c) * (a + b + c + 1) * (a + b + c + 2) / 6 + (b + c) * (b + c + 1) / 2 + c;
-}
-constexpr int dstart(int n) { return n * (n + 1) * (n + 2) / 6; }
-
 struct PairIdx {
     short ia, ib, ig;
 };
@@ -822,16 +866,60 @@ struct M2LLevel {
     int lo[3], dim[3];
 };
 
-// All M2L levels of one evaluation.  Work unit = (tile of 64 Morton-consecutive
-// boxes of one level, chunk of the far-offset list); the offset list is sorted
-// by Chebyshev shell (3, 4, 5) and cut into chunks of equal cost per pass.
+// All M2L levels of one evaluation.  The boxes of levels >= 2 are grouped by
+// (level, octant in the parent) with Morton order kept inside a group; a tile
+// is up to 64 boxes of one group, so all lanes of a wave share the child
+// octant and with it the list of valid far offsets (875 for a near field of
+// two cells: the offsets whose source parent is within two cells of the
+// target parent).  Work unit = (tile, chunk of that offset list); the lists
+// are sorted by Chebyshev shell (3, 4, 5) and cut into chunks of equal cost
+// per pass.
 constexpr int kM2LMaxLev = 20;
 struct M2LPlan {
-    int nlev, ntile, nchunk, ring, nc;
-    int s4, s5;                       // first offset-list entries of shells 4 and 5
-    int tile0[kM2LMaxLev + 1];        // first tile of each level
-    M2LLevel lv[kM2LMaxLev];
+    int nlev, ntile, nchunk, nc;
+    int nol, s4, s5;                  // offsets per octant list; first entries of shells 4 and 5
+    const int4* tiles;                // {first sorted box, count, level index, octant}
+    const int* sorted;                // box index within its level, grouped by (level, octant)
+    M2LLevel lv[kM2LMaxLev];          // level index li = level - 2
 };
+
+// Grouping of the boxes of levels >= 2 by (level, octant): sort keys.
+struct LevelOffs {
+    int nl;
+    int off[kM2LMaxLev + 1];  // level index li covers [off[li], off[li+1])
+};
+
+__global__ void k_fmm_oct_keys(const uint64_t* __restrict__ bkey, LevelOffs lo,
+                               unsigned short* __restrict__ okey, int* __restrict__ oval) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= lo.off[lo.nl]) return;
+    int li = 0;
+    while (li + 1 < lo.nl && lo.off[li + 1] <= i) li++;
+    okey[i] = (unsigned short)(li * 8 + (int)(bkey[i] & 7));
+    oval[i] = i - lo.off[li];
+}
+
+__global__ void k_fmm_seg_bounds(const unsigned short* __restrict__ sk, int n, int* __restrict__ start,
+                                 int* __restrict__ end) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = sk[i];
+    if (i == 0 || sk[i - 1] != k) start[k] = i;
+    if (i == n - 1 || sk[i + 1] != k) end[k] = i + 1;
+}
+
+// Tiles of at most 64 boxes per (level, octant) group; entries up to `ub` that
+// are not needed get count 0 (their workgroups return at once).
+__global__ void k_fmm_tile_table(const int* __restrict__ start, const int* __restrict__ end, int nseg,
+                                 int ub, int4* __restrict__ tiles) {
+    if (threadIdx.x != 0) return;
+    int t = 0;
+    for (int sg = 0; sg < nseg; sg++) {
+        const int cnt = end[sg] - start[sg];
+        for (int k = 0; k < cnt && t < ub; k += 64) tiles[t++] = make_int4(start[sg] + k, min(64, cnt - k), sg >> 3, sg & 7);
+    }
+    for (; t < ub; t++) tiles[t] = make_int4(0, 0, 0, 0);
+}
 
 // The four output-degree ranges computed by separate kernels (registers).
 constexpr int kM2LPasses = 4;
@@ -859,16 +947,17 @@ __global__ __launch_bounds__(64) void k_fmm_m2l_tiled(const M2LPlan pl, const sh
     const int n = (w & 7) * per + (w >> 3);
     if (n >= nblk) return;
     const int tile = n % pl.ntile, chunk = n / pl.ntile;
-    int li = 0;
-    while (li + 1 < pl.nlev && pl.tile0[li + 1] <= tile) li++;
-    const M2LLevel& lv = pl.lv[li];
+    const int4 tl = pl.tiles[tile];
+    if (tl.y == 0) return;
+    const M2LLevel& lv = pl.lv[tl.z];
+    const short* ol = olist + tl.w * pl.nol;
     const int nc = pl.nc;
     float acc[NB];
 #pragma unroll
     for (int k = 0; k < NB; k++) acc[k] = 0.0f;
-    const int b = (tile - pl.tile0[li]) * 64 + threadIdx.x;
-    const bool valid = b < lv.nb;
-    const uint64_t key = lv.key[valid ? b : 0];
+    const bool valid = (int)threadIdx.x < tl.y;
+    const int b = valid ? pl.sorted[tl.x + threadIdx.x] : 0;
+    const uint64_t key = lv.key[b];
     const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
     const int cmax = (1 << lv.l) - 1;
     int used[3] = {0, 0, 0};
@@ -876,12 +965,12 @@ __global__ __launch_bounds__(64) void k_fmm_m2l_tiled(const M2LPlan pl, const sh
     for (int i = cbound[chunk]; i < i1; i++) {
         const int shell = i < pl.s4 ? 3 : (i < pl.s5 ? 4 : 5);  // uniform
         if ((shell == 3 && !w3) || (shell == 4 && !w4) || (shell == 5 && !w5)) continue;
-        const int o = olist[i];
+        const int o = ol[i];
         const int dx = o % 11 - 5, dy = (o / 11) % 11 - 5, dz = o / 121 - 5;
         const int sx = ix - dx, sy = iy - dy, sz = iz - dz;
-        const bool ok = valid && sx >= 0 && sy >= 0 && sz >= 0 && sx <= cmax && sy <= cmax &&
-                        sz <= cmax && abs((sx >> 1) - (ix >> 1)) <= pl.ring &&
-                        abs((sy >> 1) - (iy >> 1)) <= pl.ring && abs((sz >> 1) - (iz >> 1)) <= pl.ring;
+        // the octant's list holds only offsets whose source parent is near the
+        // target parent; what remains is the edge of the level
+        const bool ok = valid && sx >= 0 && sy >= 0 && sz >= 0 && sx <= cmax && sy <= cmax && sz <= cmax;
         int s = -1;
         if (ok) {
             const int gx = sx - lv.lo[0], gy = sy - lv.lo[1], gz = sz - lv.lo[2];
@@ -936,15 +1025,29 @@ __global__ void k_fmm_m2l_reduce(const float* __restrict__ part, size_t stride, 
 
 // Offsets of shells 3..5 (ring 2) sorted by shell; chunk bounds per pass such
 // that each chunk carries about the same number of multiply-adds.
-static void m2l_offsets(std::vector<short>& ol, int& s4, int& s5) {
+static void m2l_offsets(std::vector<short>& ol, int& nol, int& s4, int& s5) {
+    // per child octant c: offsets o (source = target - o) of shells 3..5 whose
+    // source parent, floor((c - o) / 2) per axis, is within two cells
     ol.clear();
-    for (int shell = 3; shell <= 5; shell++) {
-        if (shell == 4) s4 = (int)ol.size();
-        if (shell == 5) s5 = (int)ol.size();
-        for (int o = 0; o < 1331; o++) {
-            const int dx = o % 11 - 5, dy = (o / 11) % 11 - 5, dz = o / 121 - 5;
-            if (std::max(std::abs(dx), std::max(std::abs(dy), std::abs(dz))) == shell) ol.push_back((short)o);
+    for (int c = 0; c < 8; c++) {
+        const int cc[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
+        const size_t base = ol.size();
+        for (int shell = 3; shell <= 5; shell++) {
+            if (shell == 4) s4 = (int)(ol.size() - base);
+            if (shell == 5) s5 = (int)(ol.size() - base);
+            for (int o = 0; o < 1331; o++) {
+                const int d[3] = {o % 11 - 5, (o / 11) % 11 - 5, o / 121 - 5};
+                if (std::max(std::abs(d[0]), std::max(std::abs(d[1]), std::abs(d[2]))) != shell) continue;
+                bool ok = true;
+                for (int a = 0; a < 3; a++) {
+                    const int q = cc[a] - d[a];
+                    const int pq = q >= 0 ? q / 2 : -((-q + 1) / 2);  // floor(q / 2)
+                    ok = ok && pq >= -2 && pq <= 2;
+                }
+                if (ok) ol.push_back((short)o);
+            }
         }
+        nol = (int)(ol.size() - base);  // the same for every octant (875)
     }
 }
 
@@ -989,12 +1092,14 @@ static void m2l_orders(int p, int ring, int& p3, int& p4, int& p5) {
 template <int P3, int P4, int P5>
 static void launch_m2l_passes(const M2LPlan& pl, const short* olist, const int* cbound,
                               const float* lscale, const float* zeros, float* part, size_t stride,
-                              unsigned long long* stats, hipStream_t st) {
+                              unsigned long long* stats, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     const int nblk = pl.ntile * pl.nchunk;
     const dim3 g(8 * ((nblk + 7) / 8)), b(64);
     const int S1 = pl.nchunk + 1;
+    (void)hipEventRecord(e0, st);
     hipLaunchKernelGGL((k_fmm_m2l_tiled<P3, P4, P5, 0, 4>), g, b, 0, st, pl, olist, cbound, lscale,
                        zeros, part, stride, stats);
+    (void)hipEventRecord(e1, st);
     if constexpr (P3 >= 5)
         hipLaunchKernelGGL((k_fmm_m2l_tiled<P3, P4, P5, 5, 6>), g, b, 0, st, pl, olist, cbound + S1,
                            lscale, zeros, part, stride, stats);
@@ -1007,20 +1112,24 @@ static void launch_m2l_passes(const M2LPlan& pl, const short* olist, const int* 
 }
 
 static void launch_m2l_tiled(const M2LPlan& pl, int p3, const FmmDevTables* tb, const int* cbound,
-                             float* part, size_t stride, unsigned long long* stats, hipStream_t st) {
-    if (p3 == 10) launch_m2l_passes<10, 8, 7>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st);
-    else if (p3 == 8) launch_m2l_passes<8, 6, 5>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st);
-    else if (p3 == 4) launch_m2l_passes<4, 2, 1>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st);
+                             float* part, size_t stride, unsigned long long* stats, hipStream_t st,
+                             hipEvent_t e0, hipEvent_t e1) {
+    if (p3 == 10)
+        launch_m2l_passes<10, 8, 7>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st, e0, e1);
+    else if (p3 == 8)
+        launch_m2l_passes<8, 6, 5>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st, e0, e1);
+    else if (p3 == 4)
+        launch_m2l_passes<4, 2, 1>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st, e0, e1);
 }
 
 // Offset list and chunk bounds of the tiled M2L, made once per (order, chunks).
 static int m2l_bounds(FmmDevTables* tb, int p3, int p4, int p5, int S, const int** out) {
     if (!tb->olist) {
         std::vector<short> ol;
-        int s4 = 0, s5 = 0;
-        m2l_offsets(ol, s4, s5);
+        int nol = 0, s4 = 0, s5 = 0;
+        m2l_offsets(ol, nol, s4, s5);
         if (upload(&tb->olist, ol) != SHP_OK) return SHP_ERR_HIP;
-        tb->olist_n = (int)ol.size();
+        tb->olist_n = nol;
         tb->olist_s4 = s4;
         tb->olist_s5 = s5;
     }
@@ -1052,23 +1161,59 @@ __global__ void k_fmm_grid_scatter(const uint64_t* __restrict__ key, int nb, int
 
 // ------------------------------------------------------------------- L2L ----
 
-__global__ __launch_bounds__(kFmmThreads) void k_fmm_l2l(Level par, Level ch, int nc,
-                                                         const int2* __restrict__ rng,
-                                                         const int* __restrict__ bb,
-                                                         const float* __restrict__ cc,
-                                                         const int* __restrict__ bits) {
-    const int b = blockIdx.x;  // parent
+// L2L with the same octant factorisation: split the parent's sum by the parity
+// class of b,
+//   Q_r[k] = 1/2 sum_{b>=k, class(b)=r} C(b,k) 2^-|b| L_b,
+// then each child takes L'_k += s(o, k) sum_r s(o, r) Q_r[k] (an inverse
+// transform over r per k).  One wave per parent; flat per-lane term lists.
+__global__ __launch_bounds__(kShiftLanes) void k_fmm_l2l(Level par, Level ch, int nc,
+                                                         const int4* __restrict__ mi,
+                                                         const int2* __restrict__ term, int nt) {
+    extern __shared__ float sm[];
+    float* Lp = sm;      // [nc]
+    float* Q = sm + nc;  // [8][nc]: by parity class, then by child octant
+    const int b = blockIdx.x, t = threadIdx.x;
     const int c0 = par.cfirst[b];
     const int c1 = b + 1 < par.nb ? par.cfirst[b + 1] : ch.nb;
-    const float* Lp = par.Lc + (size_t)b * nc;
-    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
-        const int2 r = rng[k];
-        for (int c = c0; c < c1; c++) {
-            const int oct = (int)(ch.key[c] & 7);
-            float acc = 0.0f;
-            for (int t = r.x; t < r.y; t++) acc = fmaf(cc[t] * octant_sign(oct, bits[t]), Lp[bb[t]], acc);
-            ch.Lc[(size_t)c * nc + k] += acc;
+    const float* Lsrc = par.Lc + (size_t)b * nc;
+    for (int k = t; k < nc; k += kShiftLanes) Lp[k] = Lsrc[k];
+    for (int i = t; i < 8 * nc; i += kShiftLanes) Q[i] = 0.0f;
+    __syncthreads();
+    float acc = 0.0f;
+    for (int j = 0; j < nt; j++) {
+        const int2 e = term[j * kShiftLanes + t];
+        acc = fmaf(__int_as_float(e.y), Lp[e.x & 0xfff], acc);
+        if (e.x >> 24) {
+            Q[(e.x >> 12) & 0xfff] = acc;
+            acc = 0.0f;
         }
+    }
+    __syncthreads();
+    for (int k = t; k < nc; k += kShiftLanes) {
+        float u[8];
+#pragma unroll
+        for (int o = 0; o < 8; o++) u[o] = Q[o * nc + k];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int o = 0; o < 8; o++)
+                if (!(o & (1 << d))) {
+                    const float x = u[o], y = u[o | (1 << d)];
+                    u[o] = x - y;             // child bit 0: s = (-1)^class bit
+                    u[o | (1 << d)] = x + y;  // child bit 1: s = +1
+                }
+        const int4 K = mi[k];
+#pragma unroll
+        for (int o = 0; o < 8; o++) {
+            const int par_k = ((o & 1) ? 0 : K.x) + ((o & 2) ? 0 : K.y) + ((o & 4) ? 0 : K.z);
+            Q[o * nc + k] = (par_k & 1) ? -u[o] : u[o];
+        }
+    }
+    __syncthreads();
+    for (int c = c0; c < c1; c++) {
+        const int o = (int)(ch.key[c] & 7);
+        float* Lc = ch.Lc + (size_t)c * nc;
+        for (int k = t; k < nc; k += kShiftLanes) Lc[k] += Q[o * nc + k];
     }
 }
 
@@ -1086,103 +1231,147 @@ __device__ __forceinline__ void fmm_two_sum(float& hi, float& lo, float p) {
     hi = s;
 }
 
-// One wave-sized workgroup per leaf: L2P for its targets, then P2P over the
-// leaves within `ring` cells (FP32 pairs relative to the leaf centre,
-// compensated per-target sums), then scaling by K q/m and scatter to the
-// caller's order.
+// Dense cell -> box index grid of one level (nullptr: search the sorted keys).
+struct GridView {
+    const int* g;
+    int lo[3], dim[3];
+};
+
+__device__ __forceinline__ int find_cell(const Level& lv, const GridView& gv, int qx, int qy, int qz) {
+    if (gv.g) {
+        const int gx = qx - gv.lo[0], gy = qy - gv.lo[1], gz = qz - gv.lo[2];
+        if (gx < 0 || gy < 0 || gz < 0 || gx >= gv.dim[0] || gy >= gv.dim[1] || gz >= gv.dim[2]) return -1;
+        return gv.g[gx + gv.dim[0] * (gy + gv.dim[1] * gz)];
+    }
+    return find_key(lv.key, lv.nb, spread21(qx) | spread21(qy) << 1 | spread21(qz) << 2);
+}
+
+// One wave per leaf.  The near-field sources (particles of the leaves within
+// `ring` cells) are listed once as runs of the sorted particles; the leaf's
+// targets are then processed in blocks of T = min(n, 64) lanes, with
+// G = 64 / T lane groups that split the sources and the L2P terms, so a leaf
+// of ~20 targets keeps ~60 of the 64 lanes busy.  Pairs are FP32 relative to
+// the leaf centre (units of h) with compensated per-target sums; the group
+// partials are added in FP64, scaled by K q/m and scattered to the caller's
+// order.
 __global__ __launch_bounds__(64) void k_fmm_leaf(
-    Level lv, int L, int ring, const double* __restrict__ root, const double4* __restrict__ sp,
-    const int* __restrict__ perm, const int* __restrict__ status, const int* __restrict__ species,
-    int nc, int p, const int4* __restrict__ mi, double K, unsigned long long* __restrict__ stats,
-    double* __restrict__ ax,
-    double* __restrict__ ay, double* __restrict__ az) {
+    Level lv, int L, int ring, GridView gv, const double* __restrict__ root,
+    const double4* __restrict__ sp, const int* __restrict__ perm, const int* __restrict__ status,
+    const int* __restrict__ species, int nc, int p, double K, unsigned long long* __restrict__ stats,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
     __shared__ float4 tile[64];
     __shared__ float Lsh[kMaxNc];
-    __shared__ float pwr[64][3][kMaxP + 1];
-    __shared__ int nbr_first[125], nbr_end[125];
-    __shared__ int nnbr;
-    const int b = blockIdx.x;
+    __shared__ int rf[128], rl[128], rpre[129];
+    __shared__ double red[3][64];
+    const int b = blockIdx.x, lane = threadIdx.x;
     const uint64_t key = lv.key[b];
     double c[3], h;
     box_center(root, L, key, c, h);
+    const double ih = 1.0 / h;
     const float eps2f = (float)(kSoftening / (h * h));
-    for (int k = threadIdx.x; k < nc; k += 64) Lsh[k] = lv.Lc[(size_t)b * nc + k];
+    for (int k = lane; k < nc; k += 64) Lsh[k] = lv.Lc[(size_t)b * nc + k];
     const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
     const int nside = 2 * ring + 1, ncand = nside * nside * nside;
     const int cmax = (1 << L) - 1;
-    for (int t = threadIdx.x; t < ncand; t += 64) {
-        const int qx = ix + t % nside - ring, qy = iy + (t / nside) % nside - ring,
-                  qz = iz + t / (nside * nside) - ring;
+    // near-field runs, compacted in candidate order
+    int nrun = 0;
+    for (int t0 = 0; t0 < ncand; t0 += 64) {
+        const int t = t0 + lane;
         int f = 0, e = 0;
-        if (qx >= 0 && qy >= 0 && qz >= 0 && qx <= cmax && qy <= cmax && qz <= cmax) {
-            const uint64_t nk = spread21(qx) | spread21(qy) << 1 | spread21(qz) << 2;
-            const int nb = find_key(lv.key, lv.nb, nk);
-            if (nb >= 0) {
-                f = lv.start[nb];
-                e = lv.end[nb];
+        if (t < ncand) {
+            const int qx = ix + t % nside - ring, qy = iy + (t / nside) % nside - ring,
+                      qz = iz + t / (nside * nside) - ring;
+            if (qx >= 0 && qy >= 0 && qz >= 0 && qx <= cmax && qy <= cmax && qz <= cmax) {
+                const int nb = find_cell(lv, gv, qx, qy, qz);
+                if (nb >= 0) {
+                    f = lv.start[nb];
+                    e = lv.end[nb];
+                }
             }
         }
-        nbr_first[t] = f;
-        nbr_end[t] = e;
+        const bool has = e > f;
+        const unsigned long long m = __ballot(has);
+        if (has) {
+            const int pos = nrun + __popcll(m & ((1ull << lane) - 1ull));
+            rf[pos] = f;
+            rl[pos] = e - f;
+        }
+        nrun += __popcll(m);
     }
     __syncthreads();
+    // prefix sums of the run lengths (wave scan)
+    int carry = 0;
+    for (int r0 = 0; r0 < nrun; r0 += 64) {
+        int v = r0 + lane < nrun ? rl[r0 + lane] : 0;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(v, d);
+            if (lane >= d) v += y;
+        }
+        if (r0 + lane < nrun) rpre[r0 + lane + 1] = carry + v;
+        carry += __shfl(v, 63);
+    }
+    if (lane == 0) rpre[0] = 0;
+    __syncthreads();
+    const int nsrc = carry;
     const int s0 = lv.start[b], s1 = lv.end[b];
-    if (threadIdx.x == 0) {
-        int k = 0;
-        unsigned long long nsrc = 0;
-        for (int t = 0; t < ncand; t++)
-            if (nbr_end[t] > nbr_first[t]) {
-                nbr_first[k] = nbr_first[t];
-                nbr_end[k] = nbr_end[t];
-                nsrc += (unsigned long long)(nbr_end[t] - nbr_first[t]);
-                k++;
-            }
-        nnbr = k;
-        atomicAdd(stats + 1, nsrc * (unsigned long long)(s1 - s0));
-    }
-    __syncthreads();
-    for (int tb = s0; tb < s1; tb += 64) {
-        const int i = tb + threadIdx.x;
-        const bool valid = i < s1;
+    const int nt = s1 - s0;
+    if (lane == 0) atomicAdd(stats + 1, (unsigned long long)nsrc * (unsigned long long)nt);
+    const int T = nt < 64 ? nt : 64, G = 64 / T;
+    const int g = lane / T, tl = lane - g * T;
+    const bool active = g < G;
+    for (int tb = s0; tb < s1; tb += T) {
+        const int i = tb + tl;
+        const bool valid = active && i < s1;
         const double4 r = sp[valid ? i : s0];
-        const float ux = (float)((r.x - c[0]) * (1.0 / h)), uy = (float)((r.y - c[1]) * (1.0 / h)),
-                    uz = (float)((r.z - c[2]) * (1.0 / h));
-        // L2P: gradient of sum L_b u^b
+        const float ux = (float)((r.x - c[0]) * ih), uy = (float)((r.y - c[1]) * ih),
+                    uz = (float)((r.z - c[2]) * ih);
+        // L2P share of this group: gradient of sum L_b u^b over b_x = g mod G
         float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-        {
-            float (*pw)[kMaxP + 1] = pwr[threadIdx.x];
-            pw[0][0] = pw[1][0] = pw[2][0] = 1.0f;
-            for (int k = 1; k <= p; k++) {
-                pw[0][k] = pw[0][k - 1] * ux;
-                pw[1][k] = pw[1][k - 1] * uy;
-                pw[2][k] = pw[2][k - 1] * uz;
-            }
-            for (int k = 1; k < nc; k++) {
-                const int4 m = mi[k];
-                const float Lk = Lsh[k];
-                if (m.x) gx = fmaf(Lk * (float)m.x, pw[0][m.x - 1] * pw[1][m.y] * pw[2][m.z], gx);
-                if (m.y) gy = fmaf(Lk * (float)m.y, pw[0][m.x] * pw[1][m.y - 1] * pw[2][m.z], gy);
-                if (m.z) gz = fmaf(Lk * (float)m.z, pw[0][m.x] * pw[1][m.y] * pw[2][m.z - 1], gz);
+        if (active) {
+            for (int a = g; a <= p; a += G) {
+                float xa = 1.0f, xam1 = 0.0f;
+                for (int k = 0; k < a; k++) {
+                    xam1 = xa;
+                    xa *= ux;
+                }
+                float yb = 1.0f, ybm1 = 0.0f;
+                for (int bb = 0; a + bb <= p; bb++) {
+                    float zc = 1.0f, zcm1 = 0.0f;
+                    for (int cc = 0; a + bb + cc <= p; cc++) {
+                        const float Lk = Lsh[cidx(a, bb, cc)];
+                        gx = fmaf(Lk * (float)a, xam1 * yb * zc, gx);
+                        gy = fmaf(Lk * (float)bb, xa * ybm1 * zc, gy);
+                        gz = fmaf(Lk * (float)cc, xa * yb * zcm1, gz);
+                        zcm1 = zc;
+                        zc *= uz;
+                    }
+                    ybm1 = yb;
+                    yb *= uy;
+                }
             }
         }
         // far part in units of 1/h^2 relative to the P2P sum expressed in u units:
         // P2P below runs in u = (x - c)/h, so sum q u/|u|^3 = h^2 sum q d/|d|^3.
         float hx = -gx, hy = -gy, hz = -gz, lx = 0.0f, ly = 0.0f, lz = 0.0f;
-        for (int nbi = 0; nbi < nnbr; nbi++) {
-            const int f = nbr_first[nbi], e = nbr_end[nbi];
-            for (int sb = f; sb < e; sb += 64) {
-                __syncthreads();
-                const int j = sb + threadIdx.x;
-                if (j < e) {
-                    const double4 rs = sp[j];
-                    tile[threadIdx.x] = make_float4((float)((rs.x - c[0]) * (1.0 / h)),
-                                                    (float)((rs.y - c[1]) * (1.0 / h)),
-                                                    (float)((rs.z - c[2]) * (1.0 / h)), (float)rs.w);
+        for (int jb = 0; jb < nsrc; jb += 64) {
+            __syncthreads();
+            const int j = jb + lane;
+            if (j < nsrc) {
+                int lo = 0, hi = nrun - 1;  // run holding source j: last r with rpre[r] <= j
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (rpre[mid] <= j) lo = mid;
+                    else hi = mid - 1;
                 }
-                __syncthreads();
-                const int cnt = min(64, e - sb);
-                float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-                for (int jj = 0; jj < cnt; jj++) {
+                const double4 rs = sp[rf[lo] + (j - rpre[lo])];
+                tile[lane] = make_float4((float)((rs.x - c[0]) * ih), (float)((rs.y - c[1]) * ih),
+                                         (float)((rs.z - c[2]) * ih), (float)rs.w);
+            }
+            __syncthreads();
+            const int cnt = min(64, nsrc - jb);
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            if (active) {
+                for (int jj = g; jj < cnt; jj += G) {
                     const float4 q = tile[jj];
                     const float dx = ux - q.x, dy = uy - q.y, dz = uz - q.z;
                     const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2f)));
@@ -1192,18 +1381,28 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
                     sy = fmaf(dy, w, sy);
                     sz = fmaf(dz, w, sz);
                 }
-                fmm_two_sum(hx, lx, sx);
-                fmm_two_sum(hy, ly, sy);
-                fmm_two_sum(hz, lz, sz);
             }
+            fmm_two_sum(hx, lx, sx);
+            fmm_two_sum(hy, ly, sy);
+            fmm_two_sum(hz, lz, sz);
         }
-        if (valid) {
+        red[0][lane] = (double)hx + (double)lx;
+        red[1][lane] = (double)hy + (double)ly;
+        red[2][lane] = (double)hz + (double)lz;
+        __syncthreads();
+        if (g == 0 && valid) {
+            double sxd = 0.0, syd = 0.0, szd = 0.0;
+            for (int gg = 0; gg < G; gg++) {
+                sxd += red[0][gg * T + tl];
+                syd += red[1][gg * T + tl];
+                szd += red[2][gg * T + tl];
+            }
             const int oi = perm[i];
             if (status[oi] == 1) {
                 const double kc = K * r.w / species_mass(species[oi]) / (h * h);
-                ax[oi] = ((double)hx + (double)lx) * kc;
-                ay[oi] = ((double)hy + (double)ly) * kc;
-                az[oi] = ((double)hz + (double)lz) * kc;
+                ax[oi] = sxd * kc;
+                ay[oi] = syd * kc;
+                az[oi] = szd * kc;
             }
         }
     }
@@ -1212,6 +1411,44 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
 __global__ void k_fmm_fill(double* __restrict__ p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
+}
+
+// Folds the measurements of the last FMM call into the handle: its events
+// and counters are complete once the stream has passed them (the next call's
+// host readback, or an explicit wait here).
+static int fmm_fold(shp_handle h) {
+    if (!h->fmm_pending) return SHP_OK;
+    SHP_HIP_CHECK(hipEventSynchronize(h->fev[5]));
+    double v[kFmmStats] = {};
+    for (int k = 0; k < 5; k++) {
+        float ms = 0.0f;
+        SHP_HIP_CHECK(hipEventElapsedTime(&ms, h->fev[k], h->fev[k + 1]));
+        v[k] = ms;
+    }
+    unsigned long long c[5];
+    SHP_HIP_CHECK(hipMemcpy(c, h->fmm_stats, sizeof(c), hipMemcpyDeviceToHost));
+    v[5] = (double)c[0];
+    v[6] = (double)c[1];
+    v[7] = h->fmm_leaf_level;
+    if (h->fmm_unrolled) {
+        for (int k = 0; k < 3; k++) {
+            v[8] += (double)c[2 + k] * h->fmm_shell_madds[k];
+            v[10] += (double)c[2 + k] * h->fmm_shell_madds0[k];
+        }
+    } else {
+        v[8] = v[10] = (double)c[0] * h->fmm_shell_madds[0];
+    }
+    float ms0 = 0.0f;
+    SHP_HIP_CHECK(hipEventElapsedTime(&ms0, h->fev[6], h->fev[7]));
+    v[9] = ms0;
+    for (int k = 0; k < kFmmStats; k++) {
+        h->fmm_last[k] = v[k];
+        h->fmm_acc[k] += v[k];
+    }
+    h->fmm_acc_n++;
+    h->fmm_pending = false;
+    h->fmm_have_last = true;
+    return SHP_OK;
 }
 
 }  // namespace shp
@@ -1267,6 +1504,12 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     SHP_HIP_CHECK(hipMemcpyAsync(hc, cnt, sizeof(int) * 32, hipMemcpyDeviceToHost, st));
     SHP_HIP_CHECK(hipMemcpyAsync(hroot, root, sizeof(double) * 10, hipMemcpyDeviceToHost, st));
     SHP_HIP_CHECK(hipStreamSynchronize(st));
+    {
+        // the previous call's events and counters are complete: fold them
+        // before this call resets the counters
+        const int frc = fmm_fold(h);
+        if (frc != SHP_OK) return frc;
+    }
     const int na = hc[0];
     std::vector<int> nbox_all(kMaxLevel + 1);
     for (int l = 0, acc = 0; l <= kMaxLevel; l++) {
@@ -1341,8 +1584,8 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     hipLaunchKernelGGL(k_fmm_p2m, dim3(nbox[L]), dim3(kFmmThreads), 0, st, lv[L], L, root, sp, nc,
                        order, tb->mi);
     for (int l = L - 1; l >= 0; l--)
-        hipLaunchKernelGGL(k_fmm_m2m, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l + 1], nc,
-                           tb->up_range, tb->up_k, tb->up_c, tb->up_bits);
+        hipLaunchKernelGGL(k_fmm_m2m, dim3(nbox[l]), dim3(kShiftLanes), 8 * nc * sizeof(float), st,
+                           lv[l], lv[l + 1], nc, tb->m2m_term, tb->m2m_nt);
     SHP_HIP_CHECK(hipEventRecord(h->fev[1], st));
     // translation tables for offsets within the far band
     const int W = 2 * ring + 1;
@@ -1373,6 +1616,7 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
         gtot += cells;
     }
     const bool unrolled = p3 > 0 && gtot <= ((size_t)1 << 27);
+    GridView leafgrid{};  // leaf-level cell grid for the near-field lookups (built with the M2L grids)
     SHP_HIP_CHECK(hipEventRecord(h->fev[2], st));
     if (unrolled) {
         SHP_WS(grid, int, h, WS_FMM_GRID, gtot);
@@ -1390,13 +1634,19 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
                                gdim[3 * l + 1], gdim[3 * l + 2], gl[l]);
             goff += (size_t)gdim[3 * l] * gdim[3 * l + 1] * gdim[3 * l + 2];
         }
+        leafgrid.g = gl[L];
+        for (int c = 0; c < 3; c++) {
+            leafgrid.lo[c] = glo[3 * L + c];
+            leafgrid.dim[c] = gdim[3 * L + c];
+        }
         if (L >= 2) {
             M2LPlan pl{};
-            pl.ring = ring;
             pl.nc = nc;
             size_t gbase = 0;
             for (int l = 0; l < 2; l++) gbase += nbox[l];
             const size_t first = gbase;
+            LevelOffs lofs{};
+            int ub = 0;  // tiles of the (level, octant) groups, at most
             for (int l = 2; l <= L; l++) {
                 M2LLevel& m = pl.lv[pl.nlev];
                 m.l = l;
@@ -1411,12 +1661,35 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
                     m.lo[c] = glo[3 * l + c];
                     m.dim[c] = gdim[3 * l + c];
                 }
-                pl.tile0[pl.nlev] = pl.ntile;
-                pl.ntile += fdiv_up(nbox[l], 64);
+                lofs.off[pl.nlev] = (int)(gbase - first);
+                ub += fdiv_up(nbox[l], 64) + 8;
                 pl.nlev++;
                 gbase += nbox[l];
             }
-            pl.tile0[pl.nlev] = pl.ntile;
+            lofs.nl = pl.nlev;
+            lofs.off[pl.nlev] = (int)(gbase - first);
+            // group the boxes by (level, octant), Morton order within a group
+            const int nb2 = (int)(gbase - first);
+            const int nseg = 8 * pl.nlev;
+            SHP_WS(okey, unsigned short, h, WS_FMM_OKEY, 2 * (size_t)nb2);
+            SHP_WS(oval, int, h, WS_FMM_OVAL, 2 * (size_t)nb2);
+            SHP_WS(segb, int, h, WS_FMM_SEG, 2 * (size_t)nseg);
+            SHP_WS(tiles, int4, h, WS_FMM_TILES, (size_t)ub);
+            hipLaunchKernelGGL(k_fmm_oct_keys, dim3(fdiv_up(nb2, 256)), dim3(256), 0, st, bkey + first, lofs,
+                               okey, oval);
+            size_t tsz = 0;
+            SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tsz, okey, okey + nb2, oval, oval + nb2,
+                                                             nb2, 0, 8, st));
+            SHP_WS(cubt, char, h, WS_CUB, tsz);
+            SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cubt, tsz, okey, okey + nb2, oval, oval + nb2,
+                                                             nb2, 0, 8, st));
+            hipLaunchKernelGGL(k_fmm_ifill, dim3(fdiv_up(2 * nseg, 256)), dim3(256), 0, st, segb, 2 * nseg, 0);
+            hipLaunchKernelGGL(k_fmm_seg_bounds, dim3(fdiv_up(nb2, 256)), dim3(256), 0, st, okey + nb2, nb2,
+                               segb, segb + nseg);
+            hipLaunchKernelGGL(k_fmm_tile_table, dim3(1), dim3(64), 0, st, segb, segb + nseg, nseg, ub, tiles);
+            pl.tiles = tiles;
+            pl.sorted = oval + nb2;
+            pl.ntile = ub;
             // Enough work units to give every SIMD several waves; the partial
             // sums of the offset chunks take S x (boxes of levels >= 2) x nc floats.
             const size_t nrow = gbase - first;
@@ -1425,7 +1698,8 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
             pl.nchunk = S;
             const int* cb = nullptr;
             SHP_REQUIRE(m2l_bounds(tb, p3, p4, p5, S, &cb) == SHP_OK, "shp_fmm_dev: M2L chunk table");
-            pl.s4 = tb->olist_s4;  // set by m2l_bounds on first use
+            pl.nol = tb->olist_n;  // set by m2l_bounds on first use
+            pl.s4 = tb->olist_s4;
             pl.s5 = tb->olist_s5;
             float* part = nullptr;
             const size_t stride = nrow * nc;
@@ -1433,37 +1707,49 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
                 SHP_WS(pw, float, h, WS_FMM_PART, (size_t)S * stride);
                 part = pw;
             }
-            launch_m2l_tiled(pl, p3, tb, cb, part, stride, h->fmm_stats, st);
+            launch_m2l_tiled(pl, p3, tb, cb, part, stride, h->fmm_stats, st, h->fev[6], h->fev[7]);
             if (S > 1)
                 hipLaunchKernelGGL(k_fmm_m2l_reduce, dim3(fdiv_up((long)stride, 256)), dim3(256), 0, st,
                                    part, stride, S, nc, tb->lscale, lv[2].Lc, stride);
         }
+        if (L < 2) {
+            SHP_HIP_CHECK(hipEventRecord(h->fev[6], st));
+            SHP_HIP_CHECK(hipEventRecord(h->fev[7], st));
+        }
     } else {
+        // table-driven M2L; timed as the first pass
+        SHP_HIP_CHECK(hipEventRecord(h->fev[6], st));
         for (int l = 1; l <= L; l++)
             hipLaunchKernelGGL(k_fmm_m2l, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l - 1], l,
                                ring, nc, tb->ntask, tb->task, tb->pair_alpha, tb->pair_gamma, tb->pair_w,
                                tb->beta_tasks, ttab + (size_t)l * noff * nc, W, h->fmm_stats);
+        SHP_HIP_CHECK(hipEventRecord(h->fev[7], st));
     }
     SHP_HIP_CHECK(hipGetLastError());
     SHP_HIP_CHECK(hipEventRecord(h->fev[3], st));
     for (int l = 1; l < L; l++)
-        hipLaunchKernelGGL(k_fmm_l2l, dim3(nbox[l]), dim3(kFmmThreads), 0, st, lv[l], lv[l + 1], nc,
-                           tb->dn_range, tb->dn_b, tb->dn_c, tb->dn_bits);
+        hipLaunchKernelGGL(k_fmm_l2l, dim3(nbox[l]), dim3(kShiftLanes), 9 * nc * sizeof(float), st,
+                           lv[l], lv[l + 1], nc, tb->mi, tb->l2l_term, tb->l2l_nt);
     SHP_HIP_CHECK(hipEventRecord(h->fev[4], st));
-    hipLaunchKernelGGL(k_fmm_leaf, dim3(nbox[L]), dim3(64), 0, st, lv[L], L, ring, root, sp, perm,
-                       pt->status, pt->species, nc, order, tb->mi, k_const_host(), h->fmm_stats,
-                       ax, ay, az);
+    hipLaunchKernelGGL(k_fmm_leaf, dim3(nbox[L]), dim3(64), 0, st, lv[L], L, ring, leafgrid, root, sp,
+                       perm, pt->status, pt->species, nc, order, k_const_host(), h->fmm_stats, ax, ay,
+                       az);
     SHP_HIP_CHECK(hipGetLastError());
     SHP_HIP_CHECK(hipEventRecord(h->fev[5], st));
     SHP_HIP_CHECK(hipEventRecord(h->ev1, st));
     h->timed = true;
-    h->fmm_timed = true;
-    // multiply-adds per M2L box pair for each shell (or the single order)
+    h->fmm_pending = true;
+    // multiply-adds per M2L box pair for each shell (or the single order), all
+    // passes and the first pass (output degrees 0..4)
     auto madds_of = [](int p) { return p <= 0 ? 0.0 : binom_d(p + 6, 6); };
+    auto madds0_of = [](int p) { return p <= 0 ? 0.0 : (double)pass_pairs(p, kPassLo[0], kPassHi[0]); };
     h->fmm_unrolled = unrolled;
-    h->fmm_shell_madds[0] = unrolled ? madds_of(p3) : (double)tb->npairs;
-    h->fmm_shell_madds[1] = unrolled ? madds_of(p4) : (double)tb->npairs;
-    h->fmm_shell_madds[2] = unrolled ? madds_of(p5) : (double)tb->npairs;
+    h->fmm_leaf_level = L;
+    const int ps[3] = {p3, p4, p5};
+    for (int k = 0; k < 3; k++) {
+        h->fmm_shell_madds[k] = unrolled ? madds_of(ps[k]) : (double)tb->npairs;
+        h->fmm_shell_madds0[k] = unrolled ? madds0_of(ps[k]) : (double)tb->npairs;
+    }
     return SHP_OK;
 }
 
@@ -1471,23 +1757,25 @@ int shp_fmm_last_leaf_level(shp_handle h) { return h ? last_leaf_level(h) : -1; 
 
 int shp_fmm_stats(shp_handle h, double* out) {
     SHP_REQUIRE(h && out, "shp_fmm_stats: null argument");
-    SHP_REQUIRE(h->fmm_timed, "shp_fmm_stats: no FMM evaluation on this handle yet");
-    SHP_HIP_CHECK(hipEventSynchronize(h->fev[5]));
-    for (int k = 0; k < 5; k++) {
-        float ms = 0.0f;
-        SHP_HIP_CHECK(hipEventElapsedTime(&ms, h->fev[k], h->fev[k + 1]));
-        out[k] = ms;
+    SHP_REQUIRE(h->fmm_have_last || h->fmm_pending, "shp_fmm_stats: no FMM evaluation on this handle yet");
+    const int rc = fmm_fold(h);
+    if (rc != SHP_OK) return rc;
+    for (int k = 0; k < kFmmStats; k++) out[k] = h->fmm_last[k];
+    return SHP_OK;
+}
+
+int shp_fmm_timing(shp_handle h, double* out, int reset) {
+    SHP_REQUIRE(h, "shp_fmm_timing: null handle");
+    const int rc = fmm_fold(h);
+    if (rc != SHP_OK) return rc;
+    if (out) {
+        for (int k = 0; k < kFmmStats; k++) out[k] = h->fmm_acc[k];
+        out[kFmmStats] = (double)h->fmm_acc_n;
     }
-    unsigned long long c[5];
-    SHP_HIP_CHECK(hipMemcpy(c, h->fmm_stats, sizeof(c), hipMemcpyDeviceToHost));
-    out[5] = (double)c[0];
-    out[6] = (double)c[1];
-    out[7] = last_leaf_level(h);
-    if (h->fmm_unrolled)
-        out[8] = c[2] * h->fmm_shell_madds[0] + c[3] * h->fmm_shell_madds[1] +
-                 c[4] * h->fmm_shell_madds[2];
-    else
-        out[8] = c[0] * h->fmm_shell_madds[0];
+    if (reset) {
+        for (int k = 0; k < kFmmStats; k++) h->fmm_acc[k] = 0.0;
+        h->fmm_acc_n = 0;
+    }
     return SHP_OK;
 }
 

@@ -31,6 +31,9 @@ __host__ __device__ inline double species_mass(int s) {
     }
 }
 
+// Values reported per FMM call (shp_fmm_stats / shp_fmm_timing).
+constexpr int kFmmStats = 11;
+
 void set_error(const char* fmt, ...);
 void free_mesh(shp_handle h);
 void fmm_release(shp_handle h);
@@ -64,6 +67,10 @@ enum WsSlot {
     WS_FMM_T,       // translation tables
     WS_FMM_GRID,    // dense box-index grids of all levels
     WS_FMM_PART,    // M2L partial sums of the offset chunks
+    WS_FMM_OKEY,    // (level, octant) group keys of the M2L boxes (in, out)
+    WS_FMM_OVAL,    // box indices grouped by (level, octant) (in, out)
+    WS_FMM_SEG,     // group start/end
+    WS_FMM_TILES,   // M2L tiles
     WS_NSLOTS
 };
 
@@ -97,9 +104,15 @@ struct shp_context {
     hipEvent_t ev2 = nullptr, ev3 = nullptr;  // update-kernel timing
     hipEvent_t fev[8] = {};                   // FMM pass timing
     unsigned long long* fmm_stats = nullptr;  // device counters of the last FMM call
-    bool fmm_timed = false;
-    bool fmm_unrolled = false;
-    double fmm_shell_madds[3] = {0, 0, 0};  // multiply-adds per M2L pair, shells 3/4/5
+    // FMM measurements: the last call's events and counters are folded into
+    // fmm_last / fmm_acc once the stream has passed them (see shp_fmm_stats).
+    bool fmm_pending = false, fmm_have_last = false, fmm_unrolled = false;
+    int fmm_leaf_level = 0;
+    double fmm_shell_madds[3] = {0, 0, 0};   // multiply-adds per M2L pair, shells 3/4/5
+    double fmm_shell_madds0[3] = {0, 0, 0};  // the same, first M2L pass only
+    double fmm_last[shp::kFmmStats] = {};
+    double fmm_acc[shp::kFmmStats] = {};
+    long fmm_acc_n = 0;
     bool timed = false, timed_update = false;
     void* ws[shp::WS_NSLOTS] = {};
     size_t ws_bytes[shp::WS_NSLOTS] = {};

@@ -147,6 +147,7 @@ def _load():
                                 c_void_p, c_void_p, c_void_p]),
         "shp_fmm_last_leaf_level": (c_int, [c_void_p]),
         "shp_fmm_stats": (c_int, [c_void_p, dp]),
+        "shp_fmm_timing": (c_int, [c_void_p, dp, c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -355,17 +356,30 @@ def _engine_fmm_leaf_level(self):
     return lib.shp_fmm_last_leaf_level(self.h)
 
 
+FMM_STAT_KEYS = ("p2m_m2m_ms", "ttable_ms", "m2l_ms", "l2l_ms", "leaf_ms", "m2l_pairs",
+                 "p2p_pairs", "leaf_level", "m2l_madds", "m2l_pass0_ms", "m2l_pass0_madds")
+
+
 def _engine_fmm_stats(self):
-    out = (ctypes.c_double * 9)()
+    """Measurements of the last FMM evaluation (dict of FMM_STAT_KEYS)."""
+    out = (ctypes.c_double * len(FMM_STAT_KEYS))()
     _check(lib.shp_fmm_stats(self.h, out), "shp_fmm_stats")
-    keys = ("p2m_m2m_ms", "ttable_ms", "m2l_ms", "l2l_ms", "leaf_ms", "m2l_pairs",
-            "p2p_pairs", "leaf_level", "m2l_madds")
-    return dict(zip(keys, list(out)))
+    return dict(zip(FMM_STAT_KEYS, list(out)))
+
+
+def _engine_fmm_timing(self, reset=False):
+    """Sums of the per-call FMM measurements since the last reset, plus "calls"."""
+    out = (ctypes.c_double * (len(FMM_STAT_KEYS) + 1))()
+    _check(lib.shp_fmm_timing(self.h, out, 1 if reset else 0), "shp_fmm_timing")
+    d = dict(zip(FMM_STAT_KEYS, list(out)))
+    d["calls"] = int(out[len(FMM_STAT_KEYS)])
+    return d
 
 
 Engine.fmm_dev = _engine_fmm_dev
 Engine.fmm_leaf_level = _engine_fmm_leaf_level
 Engine.fmm_stats = _engine_fmm_stats
+Engine.fmm_timing = _engine_fmm_timing
 
 
 def _engine_last_update_ms(self):
