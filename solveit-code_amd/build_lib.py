@@ -23,6 +23,7 @@ FLAGS = [
     "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
+    "-fconstexpr-steps=200000000",  # compile-time M2L term tables
 ]
 
 

@@ -792,27 +792,50 @@ constexpr int count_pairs(int P, int BLO, int BHI) {
     return k;
 }
 
-// alpha-major list so that each Mhat_a is loaded once and then dead
+// Multi-index of each coefficient index (degree-major order), up to degree 12.
+struct Tri {
+    short x, y, z;
+};
+struct TriArr {
+    Tri v[kMaxNc];
+};
+constexpr TriArr make_multi() {
+    TriArr t{};
+    int k = 0;
+    for (int n = 0; n <= kMaxP; n++)
+        for (int a = n; a >= 0; a--)
+            for (int b = n - a; b >= 0; b--) t.v[k++] = {(short)a, (short)b, (short)(n - a - b)};
+    return t;
+}
+constexpr TriArr kMulti = make_multi();
+
+// Term order: blocks of kAlphaBlock source coefficients (held in registers),
+// inside a block by the index g = a + b of the translation coefficient D_g,
+// so that each D_g (a wave-uniform scalar) is live only across its own terms.
+constexpr int kAlphaBlock = 32;
 template <int P, int BLO, int BHI>
 constexpr PairArr<count_pairs(P, BLO, BHI)> make_pairs() {
     PairArr<count_pairs(P, BLO, BHI)> out{};
     int k = 0;
-    for (int na = 0; na <= P; na++)
-        for (int a1 = na; a1 >= 0; a1--)
-            for (int a2 = na - a1; a2 >= 0; a2--) {
-                const int a3 = na - a1 - a2;
-                for (int nb = BLO; nb <= BHI; nb++) {
-                    if (na + nb > P) continue;
-                    for (int b1 = nb; b1 >= 0; b1--)
-                        for (int b2 = nb - b1; b2 >= 0; b2--) {
-                            const int b3 = nb - b1 - b2;
-                            out.v[k].ia = (short)cidx(a1, a2, a3);
-                            out.v[k].ib = (short)(cidx(b1, b2, b3) - dstart(BLO));
-                            out.v[k].ig = (short)cidx(a1 + b1, a2 + b2, a3 + b3);
-                            k++;
-                        }
-                }
+    const int nca = P >= BLO ? dstart(P - BLO + 1) : 0;  // sources with |a| <= P - BLO
+    const int ncg = dstart(P + 1);
+    for (int a0 = 0; a0 < nca; a0 += kAlphaBlock) {
+        const int a1 = a0 + kAlphaBlock < nca ? a0 + kAlphaBlock : nca;
+        for (int ig = 0; ig < ncg; ig++) {
+            const Tri g = kMulti.v[ig];
+            for (int ia = a0; ia < a1; ia++) {
+                const Tri a = kMulti.v[ia];
+                const int bx = g.x - a.x, by = g.y - a.y, bz = g.z - a.z;
+                if (bx < 0 || by < 0 || bz < 0) continue;
+                const int nb = bx + by + bz;
+                if (nb < BLO || nb > BHI) continue;
+                out.v[k].ia = (short)ia;
+                out.v[k].ib = (short)(cidx(bx, by, bz) - dstart(BLO));
+                out.v[k].ig = (short)ig;
+                k++;
             }
+        }
+    }
     return out;
 }
 
