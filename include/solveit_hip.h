@@ -138,6 +138,27 @@ int shp_fmm_stats(shp_handle h, double* out);
  * counterpart in the reference. */
 int shp_fmm_timing(shp_handle h, double* out, int reset);
 
+/*
+ * Barnes-Hut evaluation, reference (FP32) arithmetic.  Replaces
+ * BarnesHutForceCalculation::updateAcceleration (barnes_hut_force_calculation.cpp:574-715:
+ * generateTree :258-331, computeCellProperties :337-390, computeForceFromTree
+ * :417-450, barnesHutKernelCalculation :539-569) and its rotational
+ * (barnes_hut_force_calculation_rotational.cpp:28-135, four quadrants: sym->nsec
+ * must be 4) and translational (barnes_hut_force_calculation_translational.cpp:190-280,
+ * lattice images in the given order) variants.  One ion per leaf (cells of
+ * depth 21 hold buckets), monopole at the centre of charge, accepted iff
+ * theta (d - delta) > side.  The root is one cube around the sources'
+ * bounding box (largest extent x (1 + 2^-20)), where the reference lays a grid
+ * of root cells.  Sources: status 1 or 2; targets: status 1.
+ */
+int shp_bh_dev(shp_handle h, const shp_particles* p, float theta, const shp_symmetry* sym,
+               double* ax, double* ay, double* az);
+/* The same on the reference's host AoS arrays for ions [i0, i1) (sources and
+ * targets); out3[3*(i-i0)+c] receives the float acceleration the reference
+ * accumulates, as double (0 for non-targets). */
+int shp_bh_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4, int n,
+                int i0, int i1, float theta, const shp_symmetry* sym, double* out3);
+
 /* ---- background E-field (pwl on a scattered (r, z) triangulation) ---- */
 
 /* Host arrays in the reference's layout (InterpolationScheme members,

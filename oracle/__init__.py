@@ -473,3 +473,37 @@ def fmm_set_shell_orders(orders=None):
         return
     a = _i32(orders)
     lib.orc_fmm_set_shell_orders(a.ctypes.data, a.shape[0])
+
+
+lib.orc_bh_ref.restype = _ci
+lib.orc_bh_ref.argtypes = [_vp, _vp, _vp, _ci, _ci, _ci, ctypes.c_float, _ci, _ci, _vp, _vp, _vp]
+lib.orc_bh_root.argtypes = [_vp, _vp, _ci, _ci, _vp]
+
+
+def bh_ref(pos4, vel4, acc4, i0=0, i1=None, theta=0.5, sym="none", lattice=None,
+           domain=(0.0, 0.0, 0.0)):
+    """Barnes-Hut restatement (reference float arithmetic, single root cube) on
+    AoS float32 arrays (n, 4).  Returns (out (i1-i0, 3) float64, number of
+    cells including empty ones)."""
+    pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+    vel4 = np.ascontiguousarray(vel4, dtype=np.float32)
+    acc4 = np.ascontiguousarray(acc4, dtype=np.float32)
+    n = pos4.shape[0]
+    i1 = n if i1 is None else i1
+    lat = _i32(np.zeros((1, 3)) if lattice is None else lattice)
+    dom = _f64(domain)
+    out = np.zeros((max(i1 - i0, 0), 3))
+    ncell = lib.orc_bh_ref(pos4.ctypes.data, vel4.ctypes.data, acc4.ctypes.data, n, i0, i1,
+                           float(theta), _SYM[sym], lat.shape[0], lat.ctypes.data, dom.ctypes.data,
+                           out.ctypes.data)
+    return out, ncell
+
+
+def bh_root(pos4, acc4, i0=0, i1=None):
+    """Float root cube (x0, y0, z0, side) of the Barnes-Hut tree."""
+    pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+    acc4 = np.ascontiguousarray(acc4, dtype=np.float32)
+    i1 = pos4.shape[0] if i1 is None else i1
+    r = np.zeros(4, dtype=np.float32)
+    lib.orc_bh_root(pos4.ctypes.data, acc4.ctypes.data, i0, i1, r.ctypes.data)
+    return r

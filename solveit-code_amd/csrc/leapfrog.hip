@@ -430,7 +430,8 @@ int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double 
                     s->species && s->t_inj,
                 "shp_leapfrog_run: null state array");
     SHP_REQUIRE(!o->use_efield || h->mesh.ready, "shp_leapfrog_run: E-field requested but no mesh");
-    SHP_REQUIRE(o->method == SHP_METHOD_DIRECT || o->method == SHP_METHOD_FMM,
+    SHP_REQUIRE(o->method == SHP_METHOD_DIRECT || o->method == SHP_METHOD_FMM ||
+                    o->method == SHP_METHOD_BH,
                 "shp_leapfrog_run: method %d not available", o->method);
     SHP_REQUIRE(o->method != SHP_METHOD_FMM || o->sym.type == SHP_SYM_NONE,
                 "shp_leapfrog_run: the FMM path has no symmetry images");
@@ -450,6 +451,8 @@ int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double 
         int rc = o->method == SHP_METHOD_FMM
                      ? shp_fmm_dev(h, &p, o->fmm_order, o->leaf_level, o->leaf_size,
                                    o->fmm_ring > 0 ? o->fmm_ring : 2, ax, ay, az)
+                 : o->method == SHP_METHOD_BH
+                     ? shp_bh_dev(h, &p, (float)o->theta, &o->sym, ax, ay, az)
                      : shp_direct_dev(h, &p, o->precision, &o->sym, ax, ay, az);
         if (rc) return rc;
         SHP_HIP_CHECK(hipEventRecord(h->ev2, h->stream));

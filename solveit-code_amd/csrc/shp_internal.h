@@ -71,6 +71,10 @@ enum WsSlot {
     WS_FMM_OVAL,    // box indices grouped by (level, octant) (in, out)
     WS_FMM_SEG,     // group start/end
     WS_FMM_TILES,   // M2L tiles
+    WS_BH_SRC,      // Barnes-Hut source list
+    WS_BH_SP,       // Barnes-Hut ions in key order
+    WS_BH_SI,       // their original indices
+    WS_BH_NODES,    // Barnes-Hut tree nodes
     WS_NSLOTS
 };
 

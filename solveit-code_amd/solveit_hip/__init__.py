@@ -148,6 +148,10 @@ def _load():
         "shp_fmm_last_leaf_level": (c_int, [c_void_p]),
         "shp_fmm_stats": (c_int, [c_void_p, dp]),
         "shp_fmm_timing": (c_int, [c_void_p, dp, c_int]),
+        "shp_bh_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), ctypes.c_float,
+                               ctypes.POINTER(Symmetry), c_void_p, c_void_p, c_void_p]),
+        "shp_bh_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, ctypes.c_float,
+                                ctypes.POINTER(Symmetry), dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -273,6 +277,42 @@ class Engine:
         rc = lib.shp_direct_dev(self.h, ctypes.byref(p), precision, ctypes.byref(sym),
                                 out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr())
         _check(rc, "shp_direct_dev")
+        self.sync()
+        return out
+
+    def bh_host(self, pos4, vel4, acc4, i0=None, i1=None, theta=0.5, symmetry=None):
+        """Barnes-Hut (reference FP32 arithmetic) on host AoS float32 arrays of
+        shape (n, 4); ions [i0, i1) are sources and targets.  Returns the
+        (i1-i0, 3) float64 copy of the float acceleration (zero rows for
+        non-targets)."""
+        pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+        vel4 = np.ascontiguousarray(vel4, dtype=np.float32)
+        acc4 = np.ascontiguousarray(acc4, dtype=np.float32)
+        n = pos4.shape[0]
+        i0 = 0 if i0 is None else i0
+        i1 = n if i1 is None else i1
+        out = np.zeros((max(i1 - i0, 0), 3), dtype=np.float64)
+        fp = ctypes.POINTER(ctypes.c_float)
+        sym = symmetry if symmetry is not None else make_symmetry("none")
+        rc = lib.shp_bh_host(self.h, pos4.ctypes.data_as(fp), vel4.ctypes.data_as(fp),
+                             acc4.ctypes.data_as(fp), n, i0, i1, float(theta), ctypes.byref(sym),
+                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        _check(rc, "shp_bh_host")
+        return out
+
+    def bh_dev(self, state, theta=0.5, symmetry=None, out=None):
+        """Barnes-Hut on a DeviceState; returns (ax, ay, az) float64 device tensors."""
+        import torch
+        n = state.n
+        if out is None:
+            out = tuple(torch.empty(n, dtype=torch.float64, device=state.x.device)
+                        for _ in range(3))
+        sym = symmetry if symmetry is not None else make_symmetry("none")
+        p = state.as_struct()
+        _torch_ready(state.x.device)
+        rc = lib.shp_bh_dev(self.h, ctypes.byref(p), float(theta), ctypes.byref(sym),
+                            out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr())
+        _check(rc, "shp_bh_dev")
         self.sync()
         return out
 
