@@ -11,7 +11,9 @@ Also measured, on the same state: the force error of the FMM against the GPU
 FP64 direct sum on a seeded 8192-ion subsample (relative L2 as
 integration_schemes.cpp:411-455 defines it), one FP64 direct leapfrog step
 (the speed-up denominator), per-pass FMM kernel times, and a CPU baseline (the
-oracle's FP64 direct sum on a bounded target sample).  Prints one JSON line.
+oracle's FP64 direct sum on a bounded target sample), and the relative drift of
+the total energy (kinetic + pair potential + field work) over the timed steps.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -82,6 +84,11 @@ def main():
     t += args.warmup * args.dt
     eng.sync()
     torch.cuda.synchronize()
+    # Total energy at the start of the timed steps (FP64 pair sum for PE).
+    x0, y0, z0 = st.x.clone(), st.y.clone(), st.z.clone()
+    ke0, pe0 = eng.energy(st)
+    eng.sync()
+    torch.cuda.synchronize()
     eng.fmm_timing(reset=True)
     t0 = time.perf_counter()
     eng.leapfrog_run(st, opts, t, args.dt, args.steps, acc=acc)
@@ -95,6 +102,14 @@ def main():
     tm = eng.fmm_timing(reset=True)
     calls = max(1, tm["calls"])
     mean = {k: tm[k] / calls for k in solveit_hip.FMM_STAT_KEYS}
+
+    # Energy drift over the timed steps: KE + PE + work of the background
+    # field along the reference's z-then-radial path from the start positions.
+    ke1, pe1 = eng.energy(st)
+    fe = eng.field_energy(x0, y0, z0, st.x, st.y, st.z, st.q, nsteps=4 * args.steps)
+    dfe = float(fe[(st.status == 1) | (st.status == 2)].sum().item())
+    e_start = ke0 + pe0
+    drift = (ke1 + pe1 + dfe - e_start) / abs(e_start)
 
     # Accuracy of the FMM force against the GPU FP64 direct sum on this state.
     a_f = torch.stack(eng.fmm_dev(st, order=args.order, leaf_size=args.leaf_size, ring=2), 1)
@@ -157,6 +172,8 @@ def main():
             "pass_ms": {k: mean[k] for k in ("p2m_m2m_ms", "ttable_ms", "m2l_ms", "l2l_ms",
                                              "leaf_ms", "m2l_pass0_ms")},
             "fmm_calls_timed": tm["calls"],
+            "energy_drift_rel": drift,
+            "energy_start_J": {"kinetic": ke0, "potential": pe0},
             "m2l_pairs": mean["m2l_pairs"],
             "p2p_pairs": mean["p2p_pairs"],
             "m2l_tflops": m2l_tf,

@@ -159,6 +159,28 @@ int shp_bh_dev(shp_handle h, const shp_particles* p, float theta, const shp_symm
 int shp_bh_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4, int n,
                 int i0, int i1, float theta, const shp_symmetry* sym, double* out3);
 
+/* ---- energies (FP64) ---- */
+
+/* Kinetic and potential energy (J) of the ions with status 1 or 2 (the caller
+ * marks the injected set): out[0] = sum 0.5 amu m |v|^2; out[1] = 1/2 sum over
+ * ordered pairs i != j of e^2/(4 pi eps0) q_i q_j / sqrt(r^2 + eps^2), an FP64
+ * direct pair sum.  Replaces ParticleSystem::computeChargedKineticEnergy and
+ * computeChargedElectricPotentialEnergy (particles.cpp:535-578) as summed and
+ * halved by Integrator::updateEnergyIntervalData (integration_schemes.cpp:799-835).
+ * Waits for the result (out is host memory). */
+int shp_energy_dev(shp_handle h, const shp_particles* p, const double* vx, const double* vy,
+                   const double* vz, double* out);
+
+/* Field energy: fe[i] += -e q_i (sum_k E_z dz/ns + sum_k (E_x dx/ns + E_y dy/ns))
+ * along the path (x0, y0, z0) -> (x0, y0, z) -> (x, y, z) with left Riemann sums
+ * of ns = nsteps_arr[i] (or nsteps when nsteps_arr is NULL) points per leg, E
+ * from the mesh of shp_efield_setup.  Replaces
+ * ParticleSystem::updateChargedFieldEnergy (particles.cpp:598-638).  Device
+ * arrays; stream-ordered. */
+int shp_field_energy_dev(shp_handle h, int n, const double* x0, const double* y0,
+                         const double* z0, const double* x, const double* y, const double* z,
+                         const double* q, const int* nsteps_arr, int nsteps, double* fe);
+
 /* ---- background E-field (pwl on a scattered (r, z) triangulation) ---- */
 
 /* Host arrays in the reference's layout (InterpolationScheme members,

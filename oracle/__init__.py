@@ -507,3 +507,42 @@ def bh_root(pos4, acc4, i0=0, i1=None):
     r = np.zeros(4, dtype=np.float32)
     lib.orc_bh_root(pos4.ctypes.data, acc4.ctypes.data, i0, i1, r.ctypes.data)
     return r
+
+
+lib.orc_energy_f64.argtypes = [_ci] + [_vp] * 10
+lib.orc_field_energy_f64.argtypes = [_cd, _cd, _vp, _ci, _vp, _vp, _vp, _vp, _ci] + [_vp] * 8 + [_ci, _vp]
+
+
+def energy_f64(d):
+    """(kinetic, potential) energy in J of the ions with status 1 or 2 (FP64,
+    long double sums)."""
+    x, y, z, q = _f64(d["x"]), _f64(d["y"]), _f64(d["z"]), _f64(d["q"])
+    vx, vy, vz = _f64(d["vx"]), _f64(d["vy"]), _f64(d["vz"])
+    st, sp = _i32(d["status"]), _i32(d["species"])
+    out = np.zeros(2)
+    lib.orc_energy_f64(x.shape[0], x.ctypes.data, y.ctypes.data, z.ctypes.data, q.ctypes.data,
+                       vx.ctypes.data, vy.ctypes.data, vz.ctypes.data, st.ctypes.data,
+                       sp.ctypes.data, out.ctypes.data)
+    return out[0], out[1]
+
+
+def field_energy_f64(mesh, p0, p1, q, nsteps=2, fe=None):
+    """Field energy increments along the reference's z-then-radial path from
+    p0 = (x0, y0, z0) to p1 = (x, y, z)."""
+    m = _mesh_arrays(mesh)
+    x0, y0, z0 = (_f64(a) for a in p0)
+    x, y, z = (_f64(a) for a in p1)
+    q = _f64(q)
+    n = x.shape[0]
+    fe = np.zeros(n) if fe is None else _f64(fe).copy()
+    ns_arr = None
+    if not np.isscalar(nsteps):
+        ns_arr = _i32(nsteps)
+    lib.orc_field_energy_f64(float(mesh["r0"]), float(mesh["E0"]), m["xy"].ctypes.data,
+                             m["tri"].shape[0], m["tri"].ctypes.data, m["nbr"].ctypes.data,
+                             m["er"].ctypes.data, m["ez"].ctypes.data, n, x0.ctypes.data,
+                             y0.ctypes.data, z0.ctypes.data, x.ctypes.data, y.ctypes.data,
+                             z.ctypes.data, q.ctypes.data,
+                             None if ns_arr is None else ns_arr.ctypes.data,
+                             0 if ns_arr is not None else int(nsteps), fe.ctypes.data)
+    return fe

@@ -143,6 +143,95 @@ __global__ void k_efield(MeshView M, int n, const double* __restrict__ X,
     if (tri_out) tri_out[i] = t;
 }
 
+// ------------------------------------------------------------------ energies ----
+// FP64 restatement of the reference's energy bookkeeping (particles.cpp:535-638,
+// integration_schemes.cpp:799-835); the particle constants are the float ones
+// of constants.h:28-47 widened to double.
+
+constexpr double kElemChargeD = (double)1.6021766e-19f;
+constexpr double kAmuD = (double)1.66053904e-27f;
+constexpr int kEnergyThreads = 256;
+
+__device__ __forceinline__ bool present(int s) { return s == 1 || s == 2; }
+
+// Per-block partial sums of 0.5 amu m v^2.
+__global__ __launch_bounds__(kEnergyThreads) void k_energy_ke(
+    int n, const double* __restrict__ vx, const double* __restrict__ vy,
+    const double* __restrict__ vz, const int* __restrict__ st, const int* __restrict__ sp,
+    double* __restrict__ part) {
+    __shared__ double red[kEnergyThreads];
+    double a = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (present(st[i]))
+            a += 0.5 * kAmuD * species_mass(sp[i]) * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = kEnergyThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Per-block partial sums of q_i sum_{j != i} q_j / sqrt(r^2 + eps^2) over
+// LDS-staged source tiles (FP64).
+__global__ __launch_bounds__(kEnergyThreads) void k_energy_pe(
+    int n, const double* __restrict__ X, const double* __restrict__ Y,
+    const double* __restrict__ Z, const double* __restrict__ Q, const int* __restrict__ st,
+    double eps2, double* __restrict__ part) {
+    __shared__ double4 tile[kEnergyThreads];
+    __shared__ double red[kEnergyThreads];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool tgt = i < n && present(st[i]);
+    const double xi = tgt ? X[i] : 0.0, yi = tgt ? Y[i] : 0.0, zi = tgt ? Z[i] : 0.0;
+    double a = 0.0;
+    for (int j0 = 0; j0 < n; j0 += kEnergyThreads) {
+        __syncthreads();
+        const int j = j0 + threadIdx.x;
+        tile[threadIdx.x] = j < n && present(st[j]) ? make_double4(X[j], Y[j], Z[j], Q[j])
+                                                   : make_double4(0.0, 0.0, 0.0, 0.0);
+        __syncthreads();
+        const int cnt = min(kEnergyThreads, n - j0);
+        for (int k = 0; k < cnt; k++) {
+            const double4 p = tile[k];
+            const double dx = xi - p.x, dy = yi - p.y, dz = zi - p.z;
+            const double w = p.w / sqrt(dx * dx + dy * dy + dz * dz + eps2);
+            a += j0 + k == i ? 0.0 : w;
+        }
+    }
+    red[threadIdx.x] = tgt ? Q[i] * a : 0.0;
+    __syncthreads();
+    for (int s = kEnergyThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Field energy along the reference's z-then-radial path (left Riemann sums).
+__global__ void k_field_energy(MeshView M, int n, const double* __restrict__ X0,
+                               const double* __restrict__ Y0, const double* __restrict__ Z0,
+                               const double* __restrict__ X, const double* __restrict__ Y,
+                               const double* __restrict__ Z, const double* __restrict__ Q,
+                               const int* __restrict__ ns_arr, int nsteps, double* __restrict__ fe) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ns = ns_arr ? ns_arr[i] : nsteps;
+    const double x0 = X0[i], y0 = Y0[i], z0 = Z0[i];
+    const double dx = X[i] - x0, dy = Y[i] - y0, dz = Z[i] - z0;
+    double fz = 0.0, fr = 0.0, ex, ey, ez;
+    for (int k = 0; k < ns; k++) {
+        efield_f64(M, x0, y0, z0 + k * dz / ns, ex, ey, ez);
+        fz += ez * dz / ns;
+    }
+    const double zt = z0 + dz;
+    for (int k = 0; k < ns; k++) {
+        efield_f64(M, x0 + k * dx / ns, y0 + k * dy / ns, zt, ex, ey, ez);
+        fr += ex * dx / ns + ey * dy / ns;
+    }
+    fe[i] += (fz + fr) * (-kElemChargeD * Q[i]);
+}
+
 // ------------------------------------------------------------ update kernels ----
 
 struct UpdateParams {
@@ -463,6 +552,52 @@ int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double 
         SHP_HIP_CHECK(hipEventRecord(h->ev3, h->stream));
         h->timed_update = true;
     }
+    return SHP_OK;
+}
+
+int shp_energy_dev(shp_handle h, const shp_particles* p, const double* vx, const double* vy,
+                   const double* vz, double* out) {
+    SHP_REQUIRE(h && p && out, "shp_energy_dev: null argument");
+    SHP_REQUIRE(p->n >= 0, "shp_energy_dev: n=%d", p->n);
+    out[0] = out[1] = 0.0;
+    const int n = p->n;
+    if (n == 0) return SHP_OK;
+    SHP_REQUIRE(p->x && p->y && p->z && p->q && p->status && p->species && vx && vy && vz,
+                "shp_energy_dev: null array");
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    const int nb_ke = std::min(cdiv_l(n, kEnergyThreads), 1024);
+    const int nb_pe = cdiv_l(n, kEnergyThreads);
+    SHP_WS(part, double, h, WS_PARTIAL, (size_t)nb_ke + nb_pe);
+    hipLaunchKernelGGL(k_energy_ke, dim3(nb_ke), dim3(kEnergyThreads), 0, h->stream, n, vx, vy, vz,
+                       p->status, p->species, part);
+    hipLaunchKernelGGL(k_energy_pe, dim3(nb_pe), dim3(kEnergyThreads), 0, h->stream, n, p->x, p->y,
+                       p->z, p->q, p->status, kSoftening, part + nb_ke);
+    SHP_HIP_CHECK(hipGetLastError());
+    std::vector<double> hp((size_t)nb_ke + nb_pe);
+    SHP_HIP_CHECK(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost,
+                                 h->stream));
+    SHP_HIP_CHECK(hipStreamSynchronize(h->stream));
+    long double ke = 0.0L, pe = 0.0L;
+    for (int b = 0; b < nb_ke; b++) ke += hp[b];
+    for (int b = 0; b < nb_pe; b++) pe += hp[nb_ke + b];
+    out[0] = (double)ke;
+    out[1] = (double)(0.5L * pe * (long double)(k_const_host() * kAmuD));
+    return SHP_OK;
+}
+
+int shp_field_energy_dev(shp_handle h, int n, const double* x0, const double* y0, const double* z0,
+                         const double* x, const double* y, const double* z, const double* q,
+                         const int* nsteps_arr, int nsteps, double* fe) {
+    SHP_REQUIRE(h && fe, "shp_field_energy_dev: null argument");
+    SHP_REQUIRE(n >= 0, "shp_field_energy_dev: n=%d", n);
+    SHP_REQUIRE(h->mesh.ready, "shp_field_energy_dev: no field mesh (shp_efield_setup)");
+    SHP_REQUIRE(nsteps_arr || nsteps >= 1, "shp_field_energy_dev: nsteps=%d", nsteps);
+    if (n == 0) return SHP_OK;
+    SHP_REQUIRE(x0 && y0 && z0 && x && y && z && q, "shp_field_energy_dev: null array");
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    hipLaunchKernelGGL(k_field_energy, dim3(cdiv_l(n, 256)), dim3(256), 0, h->stream,
+                       view_of(h->mesh), n, x0, y0, z0, x, y, z, q, nsteps_arr, nsteps, fe);
+    SHP_HIP_CHECK(hipGetLastError());
     return SHP_OK;
 }
 

@@ -148,6 +148,10 @@ def _load():
         "shp_fmm_last_leaf_level": (c_int, [c_void_p]),
         "shp_fmm_stats": (c_int, [c_void_p, dp]),
         "shp_fmm_timing": (c_int, [c_void_p, dp, c_int]),
+        "shp_energy_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), c_void_p, c_void_p,
+                                   c_void_p, dp]),
+        "shp_field_energy_dev": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
         "shp_bh_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), ctypes.c_float,
                                ctypes.POINTER(Symmetry), c_void_p, c_void_p, c_void_p]),
         "shp_bh_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, ctypes.c_float,
@@ -315,6 +319,32 @@ class Engine:
         _check(rc, "shp_bh_dev")
         self.sync()
         return out
+
+    def energy(self, state):
+        """(kinetic, potential) energy in J of a DeviceState's ions with status 1 or 2."""
+        out = (ctypes.c_double * 2)()
+        p = state.as_struct()
+        _torch_ready(state.x.device)
+        rc = lib.shp_energy_dev(self.h, ctypes.byref(p), state.vx.data_ptr(),
+                                state.vy.data_ptr(), state.vz.data_ptr(), out)
+        _check(rc, "shp_energy_dev")
+        return out[0], out[1]
+
+    def field_energy(self, x0, y0, z0, x, y, z, q, nsteps=2, fe=None):
+        """Adds the field energy along the reference's z-then-radial path to `fe`
+        (float64 device tensors; nsteps an int or an int32 tensor); returns fe."""
+        import torch
+        if fe is None:
+            fe = torch.zeros_like(x)
+        ns_arr = nsteps.data_ptr() if hasattr(nsteps, "data_ptr") else None
+        ns = 0 if ns_arr is not None else int(nsteps)
+        _torch_ready(x.device)
+        rc = lib.shp_field_energy_dev(self.h, x.shape[0], x0.data_ptr(), y0.data_ptr(),
+                                      z0.data_ptr(), x.data_ptr(), y.data_ptr(), z.data_ptr(),
+                                      q.data_ptr(), ns_arr, ns, fe.data_ptr())
+        _check(rc, "shp_field_energy_dev")
+        self.sync()
+        return fe
 
 
 def make_step_opts(method=0, precision=2, symmetry=None, use_efield=False, bc_space=(0, 0, 0, 0),
