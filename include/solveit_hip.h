@@ -264,6 +264,36 @@ int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double 
 /* Kernel time (ms) of the last leapfrog update kernel. */
 double shp_last_update_ms(shp_handle h);
 
+/* ---- derivatives (FP64; no counterpart in the reference) ---- */
+
+/* Force JVP: tangent of the FP64 direct acceleration (shp_direct_dev,
+ * SHP_PREC_FP64) for position tangents (dx, dy, dz) and charge tangents dq
+ * (any may be NULL = 0):
+ *   da_i = k_i sum_j q_j H(r_ij)(dx_i - dx_j) + k_i sum_j dq_j G(r_ij) + K dq_i/m_i sum_j q_j G(r_ij),
+ * G(r) = r s^-3/2, H(r) = I s^-3/2 - 3 r r^T s^-5/2, s = |r|^2 + eps^2: an
+ * N-body sum with a rank-2 kernel.  Zero for non-targets. */
+int shp_force_jvp_dev(shp_handle h, const shp_particles* p, const double* dx, const double* dy,
+                      const double* dz, const double* dq, double* oax, double* oay, double* oaz);
+
+/* Force VJP: for an acceleration cotangent (bax, bay, baz) writes (or, with
+ * accumulate != 0, adds) the cotangents of positions (xb, yb, zb) and, when qb
+ * is not NULL, of charges. */
+int shp_force_vjp_dev(shp_handle h, const shp_particles* p, const double* bax, const double* bay,
+                      const double* baz, double* xb, double* yb, double* zb, double* qb,
+                      int accumulate);
+
+/* VJP of one DKD leapfrog step (shp_leapfrog_run with the FP64 direct force,
+ * all ions injected, no background field or boundary conditions; status 1
+ * ions drift and kick, status 2 drift, others are frozen) taken from the
+ * state s at the start of the step: for cotangents of the end state
+ * (xb1 .. vzb1) writes the cotangents of the start state (xb0 .. vzb0) and
+ * adds the charge cotangent into qb.  Checkpointed reverse sweeps call it once
+ * per step from the last step back. */
+int shp_leapfrog_step_vjp_dev(shp_handle h, const shp_state* s, double dt, const double* xb1,
+                              const double* yb1, const double* zb1, const double* vxb1,
+                              const double* vyb1, const double* vzb1, double* xb0, double* yb0,
+                              double* zb0, double* vxb0, double* vyb0, double* vzb0, double* qb);
+
 #ifdef __cplusplus
 }
 #endif

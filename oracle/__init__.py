@@ -546,3 +546,72 @@ def field_energy_f64(mesh, p0, p1, q, nsteps=2, fe=None):
                              None if ns_arr is None else ns_arr.ctypes.data,
                              0 if ns_arr is not None else int(nsteps), fe.ctypes.data)
     return fe
+
+
+lib.orc_force_jvp_f64.argtypes = [_ci] + [_vp] * 11
+lib.orc_force_vjp_f64.argtypes = [_ci] + [_vp] * 9
+
+
+def force_jvp_f64(d, dx, dy, dz, dq):
+    """Tangent (n, 3) of the FP64 direct acceleration for position/charge tangents."""
+    x, y, z, q = _f64(d["x"]), _f64(d["y"]), _f64(d["z"]), _f64(d["q"])
+    st, sp = _i32(d["status"]), _i32(d["species"])
+    dx, dy, dz, dq = _f64(dx), _f64(dy), _f64(dz), _f64(dq)
+    out = np.zeros((x.shape[0], 3))
+    lib.orc_force_jvp_f64(x.shape[0], x.ctypes.data, y.ctypes.data, z.ctypes.data, q.ctypes.data,
+                          st.ctypes.data, sp.ctypes.data, dx.ctypes.data, dy.ctypes.data,
+                          dz.ctypes.data, dq.ctypes.data, out.ctypes.data)
+    return out
+
+
+def force_vjp_f64(d, b3):
+    """Cotangents ((n, 3) positions, (n,) charges) of the FP64 direct
+    acceleration for an acceleration cotangent b3 (n, 3)."""
+    x, y, z, q = _f64(d["x"]), _f64(d["y"]), _f64(d["z"]), _f64(d["q"])
+    st, sp = _i32(d["status"]), _i32(d["species"])
+    b3 = _f64(b3)
+    xb = np.zeros((x.shape[0], 3))
+    qb = np.zeros(x.shape[0])
+    lib.orc_force_vjp_f64(x.shape[0], x.ctypes.data, y.ctypes.data, z.ctypes.data, q.ctypes.data,
+                          st.ctypes.data, sp.ctypes.data, b3.ctypes.data, xb.ctypes.data,
+                          qb.ctypes.data)
+    return xb, qb
+
+
+def rollout_grad_f64(d, dt, nsteps, grad_final):
+    """Gradient of an objective of the final state after nsteps DKD steps
+    (FP64 direct force, all ions injected, no field or boundaries) with respect
+    to the initial positions, velocities and charges, by the discrete adjoint.
+    grad_final(x, v) -> (xbar (n, 3), vbar (n, 3)) at the final state.
+    Returns (xbar0, vbar0, qbar, final state dict)."""
+    cur = {k: np.array(v, copy=True) for k, v in d.items()}
+    moving = (cur["status"] == 1) | (cur["status"] == 2)
+    kick = cur["status"] == 1
+    states = []
+    for _ in range(nsteps):
+        states.append({k: np.array(v, copy=True) for k, v in cur.items()})
+        h = {k: np.array(v, copy=True) for k, v in cur.items()}
+        for c, vc in (("x", "vx"), ("y", "vy"), ("z", "vz")):
+            h[c] = np.where(moving, cur[c] + 0.5 * dt * cur[vc], cur[c])
+        a = direct_f64(h)
+        for ax, (c, vc) in enumerate((("x", "vx"), ("y", "vy"), ("z", "vz"))):
+            cur[vc] = np.where(kick, cur[vc] + dt * a[:, ax], cur[vc])
+            cur[c] = np.where(moving, h[c] + 0.5 * dt * cur[vc], h[c])
+    x1 = np.stack([cur["x"], cur["y"], cur["z"]], 1)
+    v1 = np.stack([cur["vx"], cur["vy"], cur["vz"]], 1)
+    xb, vb = grad_final(x1, v1)
+    xb, vb = np.array(xb, float), np.array(vb, float)
+    qb = np.zeros(x1.shape[0])
+    mv = moving[:, None]
+    for s in reversed(states):
+        h = {k: np.array(v, copy=True) for k, v in s.items()}
+        for c, vc in (("x", "vx"), ("y", "vy"), ("z", "vz")):
+            h[c] = np.where(moving, s[c] + 0.5 * dt * s[vc], s[c])
+        w = np.where(mv, vb + 0.5 * dt * xb, vb)
+        b = np.where(kick[:, None], dt * w, 0.0)
+        jx, jq = force_vjp_f64(h, b)
+        qb += jq
+        xh = xb + jx
+        xb = xh
+        vb = np.where(mv, w + 0.5 * dt * xh, w)
+    return xb, vb, qb, cur

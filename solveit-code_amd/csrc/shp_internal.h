@@ -75,6 +75,8 @@ enum WsSlot {
     WS_BH_SP,       // Barnes-Hut ions in key order
     WS_BH_SI,       // their original indices
     WS_BH_NODES,    // Barnes-Hut tree nodes
+    WS_ADJ_U,       // force VJP: scaled cotangents
+    WS_ADJ_TMP,     // leapfrog VJP: half-step positions and cotangents
     WS_NSLOTS
 };
 

@@ -152,6 +152,11 @@ def _load():
                                    c_void_p, dp]),
         "shp_field_energy_dev": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+        "shp_force_jvp_dev": (c_int, [c_void_p, ctypes.POINTER(Particles)] + [c_void_p] * 7),
+        "shp_force_vjp_dev": (c_int, [c_void_p, ctypes.POINTER(Particles)] + [c_void_p] * 7
+                              + [c_int]),
+        "shp_leapfrog_step_vjp_dev": (c_int, [c_void_p, ctypes.POINTER(State), c_double]
+                                      + [c_void_p] * 13),
         "shp_bh_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), ctypes.c_float,
                                ctypes.POINTER(Symmetry), c_void_p, c_void_p, c_void_p]),
         "shp_bh_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, ctypes.c_float,
@@ -345,6 +350,50 @@ class Engine:
         _check(rc, "shp_field_energy_dev")
         self.sync()
         return fe
+
+    def force_jvp(self, state, dx=None, dy=None, dz=None, dq=None):
+        """Tangent (ax, ay, az) of the FP64 direct acceleration (device tensors)."""
+        import torch
+        n = state.n
+        out = tuple(torch.empty(n, dtype=torch.float64, device=state.x.device) for _ in range(3))
+        p = state.as_struct()
+        ptr = lambda t: None if t is None else t.data_ptr()
+        _torch_ready(state.x.device)
+        rc = lib.shp_force_jvp_dev(self.h, ctypes.byref(p), ptr(dx), ptr(dy), ptr(dz), ptr(dq),
+                                   out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr())
+        _check(rc, "shp_force_jvp_dev")
+        self.sync()
+        return out
+
+    def force_vjp(self, state, bx, by, bz):
+        """Cotangents ((xb, yb, zb), qb) for an acceleration cotangent."""
+        import torch
+        n = state.n
+        out = tuple(torch.empty(n, dtype=torch.float64, device=state.x.device) for _ in range(4))
+        p = state.as_struct()
+        _torch_ready(state.x.device)
+        rc = lib.shp_force_vjp_dev(self.h, ctypes.byref(p), bx.data_ptr(), by.data_ptr(),
+                                   bz.data_ptr(), out[0].data_ptr(), out[1].data_ptr(),
+                                   out[2].data_ptr(), out[3].data_ptr(), 0)
+        _check(rc, "shp_force_vjp_dev")
+        self.sync()
+        return out[:3], out[3]
+
+    def leapfrog_step_vjp(self, state, dt, xb1, vb1, qb):
+        """VJP of one DKD step from `state` (start of the step): xb1, vb1 are
+        3-tuples of device tensors; returns (xb0, vb0) and adds into qb."""
+        import torch
+        n = state.n
+        out = tuple(torch.empty(n, dtype=torch.float64, device=state.x.device) for _ in range(6))
+        s = state.as_state()
+        _torch_ready(state.x.device)
+        rc = lib.shp_leapfrog_step_vjp_dev(self.h, ctypes.byref(s), float(dt),
+                                           *(t.data_ptr() for t in xb1),
+                                           *(t.data_ptr() for t in vb1),
+                                           *(t.data_ptr() for t in out), qb.data_ptr())
+        _check(rc, "shp_leapfrog_step_vjp_dev")
+        self.sync()
+        return out[:3], out[3:]
 
 
 def make_step_opts(method=0, precision=2, symmetry=None, use_efield=False, bc_space=(0, 0, 0, 0),
