@@ -294,6 +294,37 @@ int shp_leapfrog_step_vjp_dev(shp_handle h, const shp_state* s, double dt, const
                               const double* vyb1, const double* vzb1, double* xb0, double* yb0,
                               double* zb0, double* vxb0, double* vyb0, double* vzb0, double* qb);
 
+/* ---- batched plumes (no counterpart in the reference, which runs one plume per process) ---- */
+
+/* A batch is the concatenated device SoA state of ninst independent plumes:
+ * instance k holds ions [offsets[k], offsets[k+1]) and its ions interact only
+ * with each other.  The plan holds the work list of a persistent direct-sum
+ * launch (tiles of targets, largest instances first, pulled from an atomic
+ * counter) for precision SHP_PREC_FP32C or SHP_PREC_FP64. */
+typedef struct shp_batch_plan_s* shp_batch_plan;
+int shp_batch_create(shp_handle h, int ninst, const int* offsets, int precision,
+                     shp_batch_plan* out);
+int shp_batch_destroy(shp_batch_plan b);
+/* Times the candidate tile shapes (workgroup size x targets per lane) on the
+ * given state and keeps the fastest; ms_out (may be NULL) receives the
+ * average time of each candidate, chosen its index. */
+int shp_batch_tune(shp_batch_plan b, const shp_particles* all, double* ms_out, int* chosen);
+int shp_batch_set_config(shp_batch_plan b, int config);
+/* Accelerations of every instance (one persistent launch); stream-ordered. */
+int shp_batch_forces(shp_batch_plan b, const shp_particles* all, double* ax, double* ay,
+                     double* az);
+/* nsteps DKD steps of every instance (shp_leapfrog_run semantics, no
+ * symmetry); with use_graph the step is captured once in a HIP graph (time
+ * kept in device memory) and replayed.  Stream-ordered. */
+int shp_batch_leapfrog(shp_batch_plan b, shp_state* all, const shp_step_opts* o, double t0,
+                       double dt, int nsteps, int use_graph, double* ax, double* ay, double* az);
+/* VJP of one DKD step of every instance (FP64 force VJP, semantics of
+ * shp_leapfrog_step_vjp_dev) from the batch state s at the start of the step.
+ * Cotangents are component-major device arrays of 3n doubles (x components,
+ * then y, then z); qb (n) accumulates the charge cotangent. */
+int shp_batch_step_vjp(shp_batch_plan b, const shp_state* s, double dt, const double* xb1,
+                       const double* vb1, double* xb0, double* vb0, double* qb);
+
 #ifdef __cplusplus
 }
 #endif

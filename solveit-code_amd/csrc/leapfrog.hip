@@ -341,9 +341,11 @@ __global__ void k_kick_drift(UpdateParams P, MeshView M, int n, double* __restri
                              int* __restrict__ status, const int* __restrict__ species,
                              const double* __restrict__ tinj, const double* __restrict__ AX,
                              const double* __restrict__ AY, const double* __restrict__ AZ,
-                             double t, double dt, int next, int* __restrict__ eff) {
+                             double t, double dt, int next, int* __restrict__ eff,
+                             const double* __restrict__ tdev) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (tdev) t = *tdev;  // time held in device memory (graph replays)
     double x = X[i], y = Y[i], z = Z[i];
     double vx = VX[i], vy = VY[i], vz = VZ[i];
     int st = status[i];
@@ -385,6 +387,40 @@ static UpdateParams make_params(const shp_step_opts* o) {
     P.e_over_amu = (double)1.6021766e-19f / (double)1.66053904e-27f;
     P.use_efield = o->use_efield;
     return P;
+}
+
+}  // namespace shp
+
+namespace shp {
+
+__global__ void k_time_advance(double* __restrict__ t, double dt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *t += dt;
+}
+
+// First half drift of a run; writes the effective (injected) status.
+int lf_begin(shp_handle h, const shp_state* s, const shp_step_opts* o, double t0, double dt, int* eff) {
+    const UpdateParams P = make_params(o);
+    const int n = s->n;
+    hipLaunchKernelGGL(k_drift, dim3(cdiv_l(n, 256)), dim3(256), 0, h->stream, P, n, s->x, s->y, s->z,
+                       s->vx, s->vy, s->vz, s->status, s->t_inj, t0, 0.5 * dt, eff);
+    SHP_HIP_CHECK(hipGetLastError());
+    return SHP_OK;
+}
+
+// Kick, second half drift and (next != 0) the next first half drift; the
+// step's start time is *tdev when tdev is given (then advanced by dt).
+int lf_kick_drift(shp_handle h, const shp_state* s, const shp_step_opts* o, double t, const double* tdev,
+                  double dt, int next, const double* ax, const double* ay, const double* az, int* eff) {
+    SHP_REQUIRE(!o->use_efield || h->mesh.ready, "leapfrog: E-field requested but no mesh");
+    const UpdateParams P = make_params(o);
+    const MeshView M = h->mesh.ready ? view_of(h->mesh) : MeshView{};
+    const int n = s->n;
+    hipLaunchKernelGGL(k_kick_drift, dim3(cdiv_l(n, 256)), dim3(256), 0, h->stream, P, M, n, s->x, s->y,
+                       s->z, s->vx, s->vy, s->vz, s->q, s->status, s->species, s->t_inj, ax, ay, az, t,
+                       dt, next, eff, tdev);
+    if (tdev) hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(64), 0, h->stream, (double*)tdev, dt);
+    SHP_HIP_CHECK(hipGetLastError());
+    return SHP_OK;
 }
 
 }  // namespace shp
@@ -547,7 +583,7 @@ int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double 
         SHP_HIP_CHECK(hipEventRecord(h->ev2, h->stream));
         hipLaunchKernelGGL(k_kick_drift, grid, block, 0, h->stream, P, M, n, s->x, s->y, s->z, s->vx,
                            s->vy, s->vz, s->q, s->status, s->species, s->t_inj, ax, ay, az, t, dt,
-                           k + 1 < nsteps ? 1 : 0, eff);
+                           k + 1 < nsteps ? 1 : 0, eff, nullptr);
         SHP_HIP_CHECK(hipGetLastError());
         SHP_HIP_CHECK(hipEventRecord(h->ev3, h->stream));
         h->timed_update = true;

@@ -35,6 +35,11 @@ __host__ __device__ inline double species_mass(int s) {
 constexpr int kFmmStats = 11;
 
 void set_error(const char* fmt, ...);
+
+// Leapfrog building blocks shared with the batched driver (leapfrog.hip).
+int lf_begin(shp_handle h, const shp_state* s, const shp_step_opts* o, double t0, double dt, int* eff);
+int lf_kick_drift(shp_handle h, const shp_state* s, const shp_step_opts* o, double t, const double* tdev,
+                  double dt, int next, const double* ax, const double* ay, const double* az, int* eff);
 void free_mesh(shp_handle h);
 void fmm_release(shp_handle h);
 
@@ -77,6 +82,7 @@ enum WsSlot {
     WS_BH_NODES,    // Barnes-Hut tree nodes
     WS_ADJ_U,       // force VJP: scaled cotangents
     WS_ADJ_TMP,     // leapfrog VJP: half-step positions and cotangents
+    WS_BATCH_TUNE,  // autotuner scratch outputs
     WS_NSLOTS
 };
 

@@ -157,6 +157,19 @@ def _load():
                               + [c_int]),
         "shp_leapfrog_step_vjp_dev": (c_int, [c_void_p, ctypes.POINTER(State), c_double]
                                       + [c_void_p] * 13),
+        "shp_batch_create": (c_int, [c_void_p, c_int, ctypes.POINTER(c_int), c_int,
+                                     ctypes.POINTER(c_void_p)]),
+        "shp_batch_destroy": (c_int, [c_void_p]),
+        "shp_batch_tune": (c_int, [c_void_p, ctypes.POINTER(Particles), dp,
+                                   ctypes.POINTER(c_int)]),
+        "shp_batch_set_config": (c_int, [c_void_p, c_int]),
+        "shp_batch_forces": (c_int, [c_void_p, ctypes.POINTER(Particles), c_void_p, c_void_p,
+                                     c_void_p]),
+        "shp_batch_leapfrog": (c_int, [c_void_p, ctypes.POINTER(State), ctypes.POINTER(StepOpts),
+                                       c_double, c_double, c_int, c_int, c_void_p, c_void_p,
+                                       c_void_p]),
+        "shp_batch_step_vjp": (c_int, [c_void_p, ctypes.POINTER(State), c_double, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_void_p]),
         "shp_bh_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), ctypes.c_float,
                                ctypes.POINTER(Symmetry), c_void_p, c_void_p, c_void_p]),
         "shp_bh_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, ctypes.c_float,
@@ -394,6 +407,90 @@ class Engine:
         _check(rc, "shp_leapfrog_step_vjp_dev")
         self.sync()
         return out[:3], out[3:]
+
+    def batch(self, offsets, precision=SHP_PREC_FP32C):
+        """Plan for a batch of independent plumes held in one concatenated
+        DeviceState; instance k owns ions [offsets[k], offsets[k+1])."""
+        return BatchPlan(self, offsets, precision)
+
+
+class BatchPlan:
+    """Persistent-launch direct sum over many plume instances (C ABI shp_batch_*)."""
+
+    N_CONFIGS = 4
+
+    def __init__(self, eng, offsets, precision):
+        self.eng = eng
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        self.ninst = off.shape[0] - 1
+        self.n = int(off[-1])
+        h = ctypes.c_void_p()
+        _check(lib.shp_batch_create(eng.h, self.ninst, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                    precision, ctypes.byref(h)), "shp_batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.shp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tune(self, state):
+        """Times the tile-shape candidates; returns (ms per candidate, chosen index)."""
+        ms = (ctypes.c_double * self.N_CONFIGS)()
+        ch = ctypes.c_int()
+        p = state.as_struct()
+        _torch_ready(state.x.device)
+        _check(lib.shp_batch_tune(self.h, ctypes.byref(p), ms, ctypes.byref(ch)), "shp_batch_tune")
+        return list(ms), ch.value
+
+    def set_config(self, k):
+        _check(lib.shp_batch_set_config(self.h, int(k)), "shp_batch_set_config")
+
+    def forces(self, state, out=None):
+        import torch
+        if out is None:
+            out = tuple(torch.empty(self.n, dtype=torch.float64, device=state.x.device)
+                        for _ in range(3))
+        p = state.as_struct()
+        _torch_ready(state.x.device)
+        _check(lib.shp_batch_forces(self.h, ctypes.byref(p), out[0].data_ptr(), out[1].data_ptr(),
+                                    out[2].data_ptr()), "shp_batch_forces")
+        self.eng.sync()
+        return out
+
+    def step_vjp(self, state, dt, xb1, vb1, qb):
+        """VJP of one DKD step of every instance; xb1, vb1 are (3, n) float64
+        device tensors (component-major); returns (xb0, vb0) and adds into qb."""
+        import torch
+        xb0 = torch.empty_like(xb1)
+        vb0 = torch.empty_like(vb1)
+        s = state.as_state()
+        _torch_ready(state.x.device)
+        _check(lib.shp_batch_step_vjp(self.h, ctypes.byref(s), float(dt), xb1.data_ptr(),
+                                      vb1.data_ptr(), xb0.data_ptr(), vb0.data_ptr(), qb.data_ptr()),
+               "shp_batch_step_vjp")
+        self.eng.sync()
+        return xb0, vb0
+
+    def leapfrog(self, state, opts, t0, dt, nsteps, use_graph=True, acc=None):
+        import torch
+        if acc is None:
+            acc = tuple(torch.zeros(self.n, dtype=torch.float64, device=state.x.device)
+                        for _ in range(3))
+        s = state.as_state()
+        _torch_ready(state.x.device)
+        _check(lib.shp_batch_leapfrog(self.h, ctypes.byref(s), ctypes.byref(opts), float(t0),
+                                      float(dt), int(nsteps), 1 if use_graph else 0,
+                                      acc[0].data_ptr(), acc[1].data_ptr(), acc[2].data_ptr()),
+               "shp_batch_leapfrog")
+        self.eng.sync()
+        return acc
 
 
 def make_step_opts(method=0, precision=2, symmetry=None, use_efield=False, bc_space=(0, 0, 0, 0),
