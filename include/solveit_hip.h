@@ -119,6 +119,18 @@ double shp_last_kernel_ms(shp_handle h);
 int shp_fmm_dev(shp_handle h, const shp_particles* p, int order, int leaf_level, int leaf_size,
                 int ring, double* ax, double* ay, double* az);
 
+/* FMM with symmetry images, the FMM counterpart of shp_direct_dev's
+ * rotational and translational sums: replaces the rotational and
+ * translational multipole methods (multipole_method_force_calculation_rotational.cpp:9-333,
+ * multipole_method_force_calculation_translational.cpp:9-520).  The images
+ * (rotations by 2 pi k / nsec about z, k = 1..nsec-1, or the lattice points
+ * other than (0,0,0), which must be present) enter the tree as source-only
+ * ions (status 2), so every image of every ion is a source and the ions
+ * themselves are the targets (self excluded, images of self included, as in
+ * the reference's image sums).  sym NULL or SHP_SYM_NONE = shp_fmm_dev. */
+int shp_fmm_sym_dev(shp_handle h, const shp_particles* p, const shp_symmetry* sym, int order,
+                    int leaf_level, int leaf_size, int ring, double* ax, double* ay, double* az);
+
 /* Leaf level used by the last shp_fmm_dev call on this handle. */
 int shp_fmm_last_leaf_level(shp_handle h);
 

@@ -1436,6 +1436,36 @@ __global__ void k_fmm_fill(double* __restrict__ p, int n) {
     if (i < n) p[i] = 0.0;
 }
 
+// Symmetry images as source-only ions: block 0 holds the ions themselves
+// (their own status), block k >= 1 the k-th non-identity image (status 2 for
+// ions that are sources, 0 otherwise).  img[k] = (c, s, 0, 0) for a rotation
+// about z or (ox, oy, oz, 1) for a translation; img[0] is the identity.
+__global__ void k_fmm_images(int n, int nblk, const double4* __restrict__ img,
+                             const double* __restrict__ X, const double* __restrict__ Y,
+                             const double* __restrict__ Z, const double* __restrict__ Q,
+                             const int* __restrict__ st, const int* __restrict__ sp,
+                             double* __restrict__ X2, double* __restrict__ Y2, double* __restrict__ Z2,
+                             double* __restrict__ Q2, int* __restrict__ st2, int* __restrict__ sp2) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * nblk) return;
+    const int k = (int)(t / n), i = (int)(t % n);
+    const double4 m = img[k];
+    const double x = X[i], y = Y[i], z = Z[i];
+    if (m.w != 0.0) {
+        X2[t] = x + m.x;
+        Y2[t] = y + m.y;
+        Z2[t] = z + m.z;
+    } else {
+        X2[t] = m.x * x - m.y * y;
+        Y2[t] = m.y * x + m.x * y;
+        Z2[t] = z;
+    }
+    Q2[t] = Q[i];
+    sp2[t] = sp[i];
+    const int s = st[i];
+    st2[t] = k == 0 ? s : ((s == 1 || s == 2) ? 2 : 0);
+}
+
 // Folds the measurements of the last FMM call into the handle: its events
 // and counters are complete once the stream has passed them (the next call's
 // host readback, or an explicit wait here).
@@ -1799,6 +1829,72 @@ int shp_fmm_timing(shp_handle h, double* out, int reset) {
         for (int k = 0; k < kFmmStats; k++) h->fmm_acc[k] = 0.0;
         h->fmm_acc_n = 0;
     }
+    return SHP_OK;
+}
+
+int shp_fmm_sym_dev(shp_handle h, const shp_particles* pt, const shp_symmetry* sym, int order,
+                    int leaf_level, int leaf_size, int ring, double* ax, double* ay, double* az) {
+    SHP_REQUIRE(h && pt && ax && ay && az, "shp_fmm_sym_dev: null argument");
+    const int symt = sym ? sym->type : SHP_SYM_NONE;
+    if (symt == SHP_SYM_NONE) return shp_fmm_dev(h, pt, order, leaf_level, leaf_size, ring, ax, ay, az);
+    const int n = pt->n;
+    if (n <= 0) return SHP_OK;
+    SHP_REQUIRE(pt->x && pt->y && pt->z && pt->q && pt->status && pt->species,
+                "shp_fmm_sym_dev: null particle array");
+    std::vector<double4> im;
+    im.push_back(make_double4(1.0, 0.0, 0.0, 0.0));
+    if (symt == SHP_SYM_ROTATIONAL) {
+        SHP_REQUIRE(sym->nsec >= 1 && sym->nsec <= 64, "shp_fmm_sym_dev: nsec=%d", sym->nsec);
+        for (int k = 1; k < sym->nsec; k++) {
+            double c, sn;
+            if ((4 * k) % sym->nsec == 0) {  // exact quadrant rotations
+                const int qd = (4 * k / sym->nsec) & 3;
+                const int cs[4] = {1, 0, -1, 0};
+                c = cs[qd];
+                sn = cs[(qd + 3) & 3];
+            } else {
+                const double ang = 2.0 * 3.14159265358979323846 * k / sym->nsec;
+                c = cos(ang);
+                sn = sin(ang);
+            }
+            im.push_back(make_double4(c, sn, 0.0, 0.0));
+        }
+    } else {
+        SHP_REQUIRE(symt == SHP_SYM_TRANSLATIONAL && sym->nimg >= 1 && sym->lattice,
+                    "shp_fmm_sym_dev: translational symmetry needs a lattice");
+        bool centre = false;
+        for (int k = 0; k < sym->nimg; k++) {
+            const int* l = sym->lattice + 3 * k;
+            if (l[0] == 0 && l[1] == 0 && l[2] == 0) {
+                centre = true;
+                continue;
+            }
+            im.push_back(make_double4(l[0] * sym->domain[0], l[1] * sym->domain[1],
+                                      l[2] * sym->domain[2], 1.0));
+        }
+        SHP_REQUIRE(centre, "shp_fmm_sym_dev: the lattice must contain the central cell (0,0,0)");
+    }
+    const int nblk = (int)im.size();
+    SHP_REQUIRE((size_t)n * nblk < ((size_t)1 << 31), "shp_fmm_sym_dev: %d ions x %d images too many", n, nblk);
+    const int ntot = n * nblk;
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    SHP_WS(dimg, double4, h, WS_FMM_SYM_IMG, nblk);
+    SHP_WS(dd, double, h, WS_FMM_SYM_D, 7 * (size_t)ntot);
+    SHP_WS(di, int, h, WS_FMM_SYM_I, 2 * (size_t)ntot);
+    SHP_HIP_CHECK(hipMemcpyAsync(dimg, im.data(), sizeof(double4) * nblk, hipMemcpyHostToDevice, h->stream));
+    double *X2 = dd, *Y2 = dd + ntot, *Z2 = dd + 2 * (size_t)ntot, *Q2 = dd + 3 * (size_t)ntot;
+    double *A2 = dd + 4 * (size_t)ntot;
+    int *st2 = di, *sp2 = di + ntot;
+    hipLaunchKernelGGL(k_fmm_images, dim3(fdiv_up(ntot, 256)), dim3(256), 0, h->stream, n, nblk, dimg, pt->x,
+                       pt->y, pt->z, pt->q, pt->status, pt->species, X2, Y2, Z2, Q2, st2, sp2);
+    SHP_HIP_CHECK(hipGetLastError());
+    shp_particles all{ntot, X2, Y2, Z2, Q2, st2, sp2};
+    int rc = shp_fmm_dev(h, &all, order, leaf_level, leaf_size, ring, A2, A2 + ntot, A2 + 2 * (size_t)ntot);
+    if (rc) return rc;
+    SHP_HIP_CHECK(hipMemcpyAsync(ax, A2, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(ay, A2 + ntot, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(az, A2 + 2 * (size_t)ntot, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                 h->stream));
     return SHP_OK;
 }
 

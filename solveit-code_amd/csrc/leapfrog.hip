@@ -558,8 +558,6 @@ int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double 
     SHP_REQUIRE(o->method == SHP_METHOD_DIRECT || o->method == SHP_METHOD_FMM ||
                     o->method == SHP_METHOD_BH,
                 "shp_leapfrog_run: method %d not available", o->method);
-    SHP_REQUIRE(o->method != SHP_METHOD_FMM || o->sym.type == SHP_SYM_NONE,
-                "shp_leapfrog_run: the FMM path has no symmetry images");
     if (s->n == 0 || nsteps == 0) return SHP_OK;
     SHP_HIP_CHECK(hipSetDevice(h->device));
     const int n = s->n;
@@ -574,8 +572,8 @@ int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double 
         const double t = t0 + k * dt;
         shp_particles p{n, s->x, s->y, s->z, s->q, eff, s->species};
         int rc = o->method == SHP_METHOD_FMM
-                     ? shp_fmm_dev(h, &p, o->fmm_order, o->leaf_level, o->leaf_size,
-                                   o->fmm_ring > 0 ? o->fmm_ring : 2, ax, ay, az)
+                     ? shp_fmm_sym_dev(h, &p, &o->sym, o->fmm_order, o->leaf_level, o->leaf_size,
+                                       o->fmm_ring > 0 ? o->fmm_ring : 2, ax, ay, az)
                  : o->method == SHP_METHOD_BH
                      ? shp_bh_dev(h, &p, (float)o->theta, &o->sym, ax, ay, az)
                      : shp_direct_dev(h, &p, o->precision, &o->sym, ax, ay, az);

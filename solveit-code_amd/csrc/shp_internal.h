@@ -83,6 +83,9 @@ enum WsSlot {
     WS_ADJ_U,       // force VJP: scaled cotangents
     WS_ADJ_TMP,     // leapfrog VJP: half-step positions and cotangents
     WS_BATCH_TUNE,  // autotuner scratch outputs
+    WS_FMM_SYM_IMG, // FMM symmetry image transforms
+    WS_FMM_SYM_D,   // FMM ions with their images: positions, charges, accelerations
+    WS_FMM_SYM_I,   // their status and species
     WS_NSLOTS
 };
 

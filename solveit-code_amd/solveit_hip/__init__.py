@@ -146,6 +146,8 @@ def _load():
         "shp_fmm_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), c_int, c_int, c_int, c_int,
                                 c_void_p, c_void_p, c_void_p]),
         "shp_fmm_last_leaf_level": (c_int, [c_void_p]),
+        "shp_fmm_sym_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(Symmetry),
+                                    c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
         "shp_fmm_stats": (c_int, [c_void_p, dp]),
         "shp_fmm_timing": (c_int, [c_void_p, dp, c_int]),
         "shp_energy_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), c_void_p, c_void_p,
@@ -553,17 +555,24 @@ def _engine_leapfrog_run(self, state, opts, t0, dt, nsteps, acc=None):
     return acc
 
 
-def _engine_fmm_dev(self, state, order=8, leaf_level=0, leaf_size=32, ring=2, out=None):
-    """FMM accelerations on a DeviceState; returns (ax, ay, az) float64 device tensors."""
+def _engine_fmm_dev(self, state, order=8, leaf_level=0, leaf_size=32, ring=2, out=None,
+                    symmetry=None):
+    """FMM accelerations on a DeviceState (with symmetry images when given);
+    returns (ax, ay, az) float64 device tensors."""
     import torch
     n = state.n
     if out is None:
         out = tuple(torch.empty(n, dtype=torch.float64, device=state.x.device) for _ in range(3))
     p = state.as_struct()
     _torch_ready(state.x.device)
-    _check(lib.shp_fmm_dev(self.h, ctypes.byref(p), int(order), int(leaf_level), int(leaf_size),
-                           int(ring), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr()),
-           "shp_fmm_dev")
+    if symmetry is None:
+        _check(lib.shp_fmm_dev(self.h, ctypes.byref(p), int(order), int(leaf_level),
+                               int(leaf_size), int(ring), out[0].data_ptr(), out[1].data_ptr(),
+                               out[2].data_ptr()), "shp_fmm_dev")
+    else:
+        _check(lib.shp_fmm_sym_dev(self.h, ctypes.byref(p), ctypes.byref(symmetry), int(order),
+                                   int(leaf_level), int(leaf_size), int(ring), out[0].data_ptr(),
+                                   out[1].data_ptr(), out[2].data_ptr()), "shp_fmm_sym_dev")
     self.sync()
     return out
 
