@@ -112,9 +112,14 @@ double shp_last_kernel_ms(shp_handle h);
  * Replaces MultipoleMethodForceCalculation::updateAcceleration and the passes
  * it drives (multipole_method_force_calculation.cpp:148-522: P2P, P2M, M2M,
  * M2L, L2L, L2P of fmm_cpukernel.cpp:103-1317).  order: expansion order p
- * (1..12); leaf_level: uniform leaf level of the octree (0 = choose the
- * smallest level whose mean leaf occupancy is <= leaf_size); ring: near-field
- * width in leaf cells (1 or 2; the far field starts beyond it).
+ * (1..12); leaf_level: uniform leaf level of the octree (0 = choose it: let
+ * Lr be the smallest level whose mean leaf occupancy is <= leaf_size; for
+ * orders 4, 8 and 10 with ring 2 the levels max(2, Lr-2) .. Lr+1 are
+ * compared by an estimate of the pass times from exact counts of near-field
+ * particle pairs and far-band box pairs (a function of the ion positions
+ * only), otherwise Lr is used); ring: near-field width in leaf cells (1 or 2;
+ * the far field starts beyond it).  The reference chooses its depth from N
+ * alone (setOptimumLevel, multipole_method_force_calculation.cpp:835-881).
  */
 int shp_fmm_dev(shp_handle h, const shp_particles* p, int order, int leaf_level, int leaf_size,
                 int ring, double* ax, double* ay, double* az);

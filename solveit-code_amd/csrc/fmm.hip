@@ -952,7 +952,7 @@ constexpr int kPassHi[kM2LPasses] = {4, 6, 8, 10};
 constexpr int pass_pairs(int P, int lo, int hi) { return P >= lo ? count_pairs(P, lo, hi < P ? hi : P) : 0; }
 
 template <int P3, int P4, int P5, int BLO, int BHI>
-__global__ __launch_bounds__(64) void k_fmm_m2l_tiled(const M2LPlan pl, const short* __restrict__ olist,
+__global__ __launch_bounds__(64, BLO == 0 ? 4 : (BLO == 7 ? 3 : 1)) void k_fmm_m2l_tiled(const M2LPlan pl, const short* __restrict__ olist,
                                                       const int* __restrict__ cbound,
                                                       const float* __restrict__ lscale,
                                                       const float* __restrict__ zeros,
@@ -969,7 +969,7 @@ __global__ __launch_bounds__(64) void k_fmm_m2l_tiled(const M2LPlan pl, const sh
     const int w = blockIdx.x;
     const int n = (w & 7) * per + (w >> 3);
     if (n >= nblk) return;
-    const int tile = n % pl.ntile, chunk = n / pl.ntile;
+    const int tile = n / pl.nchunk, chunk = n % pl.nchunk;
     const int4 tl = pl.tiles[tile];
     if (tl.y == 0) return;
     const M2LLevel& lv = pl.lv[tl.z];
@@ -1101,6 +1101,20 @@ static std::vector<int> m2l_chunk_bounds(int p3, int p4, int p5, int S, int s4, 
     }
     return cb;
 }
+
+// Estimated ms of the level-dependent passes of a leaf level on MI355X,
+// fitted to measured pass times of the unrolled M2L (p = 10, ring 2; 16 plume
+// states and levels, within 8%) and of the leaf kernel: per box of a level, a
+// far-band offset costs 0.022 ns when its source cell is empty and 0.27 ns
+// more when it holds a box (the pass is bound by the source expansion loads,
+// so the cost scales with the coefficient count), and the upward and downward
+// passes 7 ns; a near-field particle pair costs 0.45 ps and a leaf box 11 ns
+// of leaf-kernel set-up (near-field run list, L2P; within 13%).
+static double fmm_m2l_cost_ms(int nc, int nbox, double far_pairs) {
+    const double s = nc / 286.0;
+    return s * (2.16e-8 * 875.0 * nbox + 2.7e-7 * far_pairs) + 7.0e-6 * nbox;
+}
+static double fmm_leaf_cost_ms(int nbox, double p2p_pairs) { return 4.5e-10 * p2p_pairs + 1.12e-5 * nbox; }
 
 static void m2l_orders(int p, int ring, int& p3, int& p4, int& p5) {
     p3 = p4 = p5 = 0;
@@ -1269,24 +1283,88 @@ __device__ __forceinline__ int find_cell(const Level& lv, const GridView& gv, in
     return find_key(lv.key, lv.nb, spread21(qx) | spread21(qy) << 1 | spread21(qz) << 2);
 }
 
-// One wave per leaf.  The near-field sources (particles of the leaves within
-// `ring` cells) are listed once as runs of the sorted particles; the leaf's
-// targets are then processed in blocks of T = min(n, 64) lanes, with
-// G = 64 / T lane groups that split the sources and the L2P terms, so a leaf
-// of ~20 targets keeps ~60 of the 64 lanes busy.  Pairs are FP32 relative to
+// Work of a uniform leaf level l, counted over every stride-th box b of the
+// level: out[0] += n_b x (particles in the near cells of b) = P2P pairs;
+// out[1] += far-band boxes of b = occupied children of the parent's near
+// boxes minus the occupied near boxes of b = M2L box pairs.
+__global__ __launch_bounds__(256) void k_fmm_level_cost(Level lv, GridView g, Level par, GridView gp,
+                                                        int ring, int stride,
+                                                        unsigned long long* __restrict__ out) {
+    const long b = ((long)blockIdx.x * blockDim.x + threadIdx.x) * stride;
+    unsigned long long pp = 0, fb = 0;
+    if (b < lv.nb) {
+        const uint64_t key = lv.key[b];
+        const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
+        long nsum = 0;
+        int occ = 0;
+        for (int dz = -ring; dz <= ring; dz++)
+            for (int dy = -ring; dy <= ring; dy++)
+                for (int dx = -ring; dx <= ring; dx++) {
+                    const int c = find_cell(lv, g, ix + dx, iy + dy, iz + dz);
+                    if (c >= 0) {
+                        nsum += lv.end[c] - lv.start[c];
+                        occ++;
+                    }
+                }
+        long nch = 0;
+        for (int dz = -ring; dz <= ring; dz++)
+            for (int dy = -ring; dy <= ring; dy++)
+                for (int dx = -ring; dx <= ring; dx++) {
+                    const int c = find_cell(par, gp, (ix >> 1) + dx, (iy >> 1) + dy, (iz >> 1) + dz);
+                    if (c >= 0) nch += (c + 1 < par.nb ? par.cfirst[c + 1] : lv.nb) - par.cfirst[c];
+                }
+        pp = (unsigned long long)(lv.end[b] - lv.start[b]) * (unsigned long long)nsum;
+        fb = (unsigned long long)(nch - occ);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        pp += __shfl_xor(pp, off);
+        fb += __shfl_xor(fb, off);
+    }
+    if ((threadIdx.x & 63) == 0 && (pp || fb)) {
+        atomicAdd(out, pp);
+        atomicAdd(out + 1, fb);
+    }
+}
+
+// Leaf work units: a leaf's targets in runs of kLeafUnit, one wave each, so
+// that the dense leaves of a plume core spread over many waves.
+constexpr int kLeafUnit = 64;
+
+__global__ void k_fmm_unit_count(const int* __restrict__ start, const int* __restrict__ end, int nb,
+                                 int* __restrict__ cnt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) cnt[b] = max(1, (end[b] - start[b] + kLeafUnit - 1) / kLeafUnit);
+}
+
+__global__ void k_fmm_unit_fill(const int* __restrict__ cnt, const int* __restrict__ offs, int nb,
+                                int2* __restrict__ units, int* __restrict__ nunits) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    for (int k = 0; k < cnt[b]; k++) units[offs[b] + k] = make_int2(b, k);
+    if (b == nb - 1) *nunits = offs[b] + cnt[b];
+}
+
+// One wave per leaf work unit.  The near-field sources (particles of the
+// leaves within `ring` cells) are listed as runs of the sorted particles; the
+// unit's targets are then processed in blocks of T lanes, with G = 64 / T lane
+// groups that split the sources and the L2P terms, so a leaf of ~20 targets
+// keeps ~60 of the 64 lanes busy.  Pairs are FP32 relative to
 // the leaf centre (units of h) with compensated per-target sums; the group
 // partials are added in FP64, scaled by K q/m and scattered to the caller's
 // order.
 __global__ __launch_bounds__(64) void k_fmm_leaf(
     Level lv, int L, int ring, GridView gv, const double* __restrict__ root,
     const double4* __restrict__ sp, const int* __restrict__ perm, const int* __restrict__ status,
-    const int* __restrict__ species, int nc, int p, double K, unsigned long long* __restrict__ stats,
-    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
+    const int* __restrict__ species, int nc, int p, double K, const int2* __restrict__ units,
+    const int* __restrict__ nunits, unsigned long long* __restrict__ stats, double* __restrict__ ax,
+    double* __restrict__ ay, double* __restrict__ az) {
     __shared__ float4 tile[64];
     __shared__ float Lsh[kMaxNc];
     __shared__ int rf[128], rl[128], rpre[129];
     __shared__ double red[3][64];
-    const int b = blockIdx.x, lane = threadIdx.x;
+    if ((int)blockIdx.x >= *nunits) return;
+    const int2 unit = units[blockIdx.x];
+    const int b = unit.x, lane = threadIdx.x;
     const uint64_t key = lv.key[b];
     double c[3], h;
     box_center(root, L, key, c, h);
@@ -1336,15 +1414,21 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
     if (lane == 0) rpre[0] = 0;
     __syncthreads();
     const int nsrc = carry;
-    const int s0 = lv.start[b], s1 = lv.end[b];
+    const int s0 = lv.start[b] + kLeafUnit * unit.y, s1 = min(s0 + kLeafUnit, lv.end[b]);
     const int nt = s1 - s0;
     if (lane == 0) atomicAdd(stats + 1, (unsigned long long)nsrc * (unsigned long long)nt);
-    const int T = nt < 64 ? nt : 64, G = 64 / T;
-    const int g = lane / T, tl = lane - g * T;
-    const bool active = g < G;
-    for (int tb = s0; tb < s1; tb += T) {
+    for (int tb = s0; tb < s1;) {
+        // T targets x G source groups per block: 64 targets while more than 53
+        // remain, then 32 while more than 32 remain (two source groups), then
+        // all that remain (64 / T groups), so every block keeps most lanes busy
+        const int rem = s1 - tb;
+        const int T = rem >= 54 ? 64 : (rem > 32 ? 32 : rem), G = 64 / T;
+        const int g = lane / T, tl = lane - g * T;
+        const bool active = g < G;
         const int i = tb + tl;
-        const bool valid = active && i < s1;
+        const int tend = min(tb + T, s1);
+        tb += T;
+        const bool valid = active && i < tend;
         const double4 r = sp[valid ? i : s0];
         const float ux = (float)((r.x - c[0]) * ih), uy = (float)((r.y - c[1]) * ih),
                     uz = (float)((r.z - c[2]) * ih);
@@ -1569,47 +1653,79 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
         acc += hc[1 + l];
         nbox_all[l] = acc;
     }
-    if (L <= 0) {
-        // smallest level whose mean leaf occupancy is at most leaf_size
-        L = 1;
-        while (L < 16 && (double)na / std::max(1, nbox_all[L]) > (double)leaf_size) L++;
-    }
-    last_leaf_level(h) = L;
-    std::vector<int> nbox(L + 1);
-    for (int l = 0; l <= L; l++) nbox[l] = nbox_all[l];
+    // smallest level whose mean leaf occupancy is at most leaf_size
+    int Lrule = 1;
+    while (Lrule < 16 && (double)na / std::max(1, nbox_all[Lrule]) > (double)leaf_size) Lrule++;
     if (na == 0) {
+        last_leaf_level(h) = L > 0 ? L : Lrule;
         hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ax, n);
         hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ay, n);
         hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, az, n);
         return SHP_OK;
     }
+    int p3 = 0, p4 = 0, p5 = 0;
+    m2l_orders(order, ring, p3, p4, p5);
+    // Dense cell -> box grids per level over the occupied cells (unrolled M2L,
+    // near-field lookups, leaf-level choice); very large grids fall back to the
+    // table-driven M2L and key searches.
+    std::vector<int> glo, gdim;
+    auto grid_cells = [&](int Lmax) {
+        glo.assign(3 * (Lmax + 1), 0);
+        gdim.assign(3 * (Lmax + 1), 1);
+        size_t cells_tot = 0;
+        for (int l = 0; l <= Lmax; l++) {
+            const double sc = std::ldexp(1.0, l) / hroot[3];
+            size_t cells = 1;
+            for (int c = 0; c < 3; c++) {
+                const int top = (1 << l) - 1;
+                int lo = (int)std::floor((hroot[4 + c] - hroot[c]) * sc) - 1;
+                int hi = (int)std::floor((hroot[7 + c] - hroot[c]) * sc) + 1;
+                lo = std::max(0, std::min(lo, top));
+                hi = std::max(lo, std::min(hi, top));
+                glo[3 * l + c] = lo;
+                gdim[3 * l + c] = hi - lo + 1;
+                cells *= (size_t)(hi - lo + 1);
+            }
+            cells_tot += cells;
+        }
+        return cells_tot;
+    };
+    constexpr size_t kGridCells = (size_t)1 << 27;
+    // With leaf_level 0 and the unrolled M2L, the levels are built one deeper
+    // than the occupancy rule and the leaf level is chosen by estimated cost.
+    int Lb = L > 0 ? L : Lrule;
+    const bool choose = L <= 0 && p3 > 0 && Lrule >= 2 && Lrule < 16 && grid_cells(Lrule + 1) <= kGridCells;
+    if (choose) Lb = Lrule + 1;
+    const size_t gtot = grid_cells(Lb);
+    const bool unrolled = p3 > 0 && gtot <= kGridCells;
+    std::vector<int> nbox(Lb + 1);
+    for (int l = 0; l <= Lb; l++) nbox[l] = nbox_all[l];
     // sorted particle records
     SHP_WS(sp, double4, h, WS_SRC, na);
     hipLaunchKernelGGL(k_fmm_gather, dim3(fdiv_up(na, 256)), dim3(256), 0, st, perm, na, pt->x, pt->y,
                        pt->z, pt->q, sp);
-    // level arrays in one workspace: key(u64), start, end, cfirst per box; M, L per box
-    size_t tot = 0;
-    for (int l = 0; l <= L; l++) tot += nbox[l];
-    SHP_WS(bkey, uint64_t, h, WS_FMM_BKEY, tot);
-    SHP_WS(bint, int, h, WS_FMM_BINT, 3 * tot);
-    SHP_WS(bexp, float, h, WS_FMM_EXP, 2 * tot * (size_t)nc);
+    // level arrays in one workspace: key (u64), start, end, cfirst per box
+    size_t totb = 0;
+    for (int l = 0; l <= Lb; l++) totb += nbox[l];
+    SHP_WS(bkey, uint64_t, h, WS_FMM_BKEY, totb);
+    SHP_WS(bint, int, h, WS_FMM_BINT, 3 * totb);
     SHP_WS(flags, char, h, WS_FLAGS, na);
-    std::vector<Level> lv(L + 1);
+    std::vector<Level> lv(Lb + 1);
     {
         size_t off = 0;
-        for (int l = 0; l <= L; l++) {
+        for (int l = 0; l <= Lb; l++) {
             Level& v = lv[l];
             v.nb = nbox[l];
             v.key = bkey + off;
             v.start = bint + off;
-            v.end = bint + tot + off;
-            v.cfirst = l < L ? bint + 2 * tot + off : nullptr;
-            v.M = bexp + off * nc;
-            v.Lc = bexp + (tot + off) * nc;
+            v.end = bint + totb + off;
+            v.cfirst = l < Lb ? bint + 2 * totb + off : nullptr;
+            v.M = nullptr;
+            v.Lc = nullptr;
             off += nbox[l];
         }
     }
-    for (int l = L; l >= 0; l--) {
+    for (int l = Lb; l >= 0; l--) {
         const int sh = 3 * (kMaxLevel - l);
         hipLaunchKernelGGL(k_fmm_flags, dim3(fdiv_up(na, 256)), dim3(256), 0, st, keys2, na, sh, flags);
         hipcub::CountingInputIterator<int> it(0);
@@ -1621,8 +1737,79 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
                                                     na, st));
         hipLaunchKernelGGL(k_fmm_box_info, dim3(fdiv_up(nbox[l], 256)), dim3(256), 0, st, keys2,
                            lv[l].start, nbox[l], na, sh, (uint64_t*)lv[l].key, (int*)lv[l].end,
-                           l < L ? lv[l + 1].start : nullptr, l < L ? nbox[l + 1] : 0,
+                           l < Lb ? lv[l + 1].start : nullptr, l < Lb ? nbox[l + 1] : 0,
                            (int*)lv[l].cfirst);
+    }
+    SHP_HIP_CHECK(hipGetLastError());
+    std::vector<int*> gl(Lb + 1, nullptr);
+    auto gview = [&](int l) {
+        GridView g{};
+        if (unrolled) {
+            g.g = gl[l];
+            for (int c = 0; c < 3; c++) {
+                g.lo[c] = glo[3 * l + c];
+                g.dim[c] = gdim[3 * l + c];
+            }
+        }
+        return g;
+    };
+    if (unrolled) {
+        SHP_WS(grid, int, h, WS_FMM_GRID, gtot);
+        hipLaunchKernelGGL(k_fmm_ifill, dim3(fdiv_up((long)gtot, 256)), dim3(256), 0, st, grid, (int)gtot, -1);
+        size_t goff = 0;
+        for (int l = 0; l <= Lb; l++) {
+            gl[l] = grid + goff;
+            hipLaunchKernelGGL(k_fmm_grid_scatter, dim3(fdiv_up(nbox[l], 256)), dim3(256), 0, st, lv[l].key,
+                               nbox[l], glo[3 * l], glo[3 * l + 1], glo[3 * l + 2], gdim[3 * l],
+                               gdim[3 * l + 1], gdim[3 * l + 2], gl[l]);
+            goff += (size_t)gdim[3 * l] * gdim[3 * l + 1] * gdim[3 * l + 2];
+        }
+    }
+    if (choose) {
+        // P2P pairs and M2L far-band box pairs of the candidate leaf levels
+        // Lrule - 2 .. Lrule + 1, counted on up to 65536 boxes per level.
+        const int L0 = std::max(2, Lrule - 2);
+        SHP_WS(lcost, unsigned long long, h, WS_FMM_LCOST, 2 * (Lb + 1));
+        hipLaunchKernelGGL(k_fmm_u64fill, dim3(1), dim3(64), 0, st, lcost, 2 * (Lb + 1));
+        std::vector<int> stride(Lb + 1, 1);
+        for (int l = L0; l <= Lb; l++) {
+            stride[l] = std::max(1, fdiv_up(nbox[l], 1 << 16));
+            const int ns = fdiv_up(nbox[l], stride[l]);
+            hipLaunchKernelGGL(k_fmm_level_cost, dim3(fdiv_up(ns, 256)), dim3(256), 0, st, lv[l], gview(l),
+                               lv[l - 1], gview(l - 1), ring, stride[l], lcost + 2 * l);
+        }
+        unsigned long long hcost[2 * (kMaxLevel + 1)];
+        SHP_HIP_CHECK(hipMemcpyAsync(hcost, lcost, sizeof(unsigned long long) * 2 * (Lb + 1),
+                                     hipMemcpyDeviceToHost, st));
+        SHP_HIP_CHECK(hipStreamSynchronize(st));
+        double best = 0.0, m2l = 0.0;
+        for (int l = L0; l <= Lb; l++) {
+            const double sc = (double)nbox[l] / fdiv_up(nbox[l], stride[l]);
+            m2l += fmm_m2l_cost_ms(nc, nbox[l], sc * (double)hcost[2 * l + 1]);
+            const double c = m2l + fmm_leaf_cost_ms(nbox[l], sc * (double)hcost[2 * l]);
+            if (l == L0 || c < best) {
+                best = c;
+                L = l;
+            }
+        }
+    } else if (L <= 0) {
+        L = Lrule;
+    }
+    last_leaf_level(h) = L;
+    nbox.resize(L + 1);
+    lv.resize(L + 1);
+    lv[L].cfirst = nullptr;  // leaf level
+    // expansions M, L per box of levels 0..L
+    size_t tot = 0;
+    for (int l = 0; l <= L; l++) tot += nbox[l];
+    SHP_WS(bexp, float, h, WS_FMM_EXP, 2 * tot * (size_t)nc);
+    {
+        size_t off = 0;
+        for (int l = 0; l <= L; l++) {
+            lv[l].M = bexp + off * nc;
+            lv[l].Lc = bexp + (tot + off) * nc;
+            off += nbox[l];
+        }
     }
     SHP_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_fmm_zero, dim3(fdiv_up(tot * nc, 256)), dim3(256), 0, st, bexp + tot * nc,
@@ -1649,49 +1836,13 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     // Unrolled M2L (ring 2, orders with compiled pair lists) needs a dense box
     // grid per level over the occupied cells; very large grids use the
     // table-driven kernel instead.
-    int p3 = 0, p4 = 0, p5 = 0;
-    m2l_orders(order, ring, p3, p4, p5);
-    std::vector<int> glo(3 * (L + 1)), gdim(3 * (L + 1));
-    size_t gtot = 0;
-    for (int l = 0; l <= L; l++) {
-        const double sc = std::ldexp(1.0, l) / hroot[3];
-        size_t cells = 1;
-        for (int c = 0; c < 3; c++) {
-            const int top = (1 << l) - 1;
-            int lo = (int)std::floor((hroot[4 + c] - hroot[c]) * sc) - 1;
-            int hi = (int)std::floor((hroot[7 + c] - hroot[c]) * sc) + 1;
-            lo = std::max(0, std::min(lo, top));
-            hi = std::max(lo, std::min(hi, top));
-            glo[3 * l + c] = lo;
-            gdim[3 * l + c] = hi - lo + 1;
-            cells *= (size_t)(hi - lo + 1);
-        }
-        gtot += cells;
-    }
-    const bool unrolled = p3 > 0 && gtot <= ((size_t)1 << 27);
-    GridView leafgrid{};  // leaf-level cell grid for the near-field lookups (built with the M2L grids)
+    const GridView leafgrid = gview(L);  // leaf-level cell grid for the near-field lookups
     SHP_HIP_CHECK(hipEventRecord(h->fev[2], st));
     if (unrolled) {
-        SHP_WS(grid, int, h, WS_FMM_GRID, gtot);
-        hipLaunchKernelGGL(k_fmm_ifill, dim3(fdiv_up((long)gtot, 256)), dim3(256), 0, st, grid, (int)gtot, -1);
         hipLaunchKernelGGL(k_fmm_scale, dim3(fdiv_up((long)tot * nc, 256)), dim3(256), 0, st, bexp, tb->mscale,
                            nc, tot * (size_t)nc);
         hipLaunchKernelGGL(k_fmm_scale, dim3(fdiv_up((long)noff * (L + 1) * nc, 256)), dim3(256), 0, st, ttab,
                            tb->dscale, nc, (size_t)noff * (L + 1) * nc);
-        size_t goff = 0;
-        std::vector<int*> gl(L + 1);
-        for (int l = 0; l <= L; l++) {
-            gl[l] = grid + goff;
-            hipLaunchKernelGGL(k_fmm_grid_scatter, dim3(fdiv_up(nbox[l], 256)), dim3(256), 0, st, lv[l].key,
-                               nbox[l], glo[3 * l], glo[3 * l + 1], glo[3 * l + 2], gdim[3 * l],
-                               gdim[3 * l + 1], gdim[3 * l + 2], gl[l]);
-            goff += (size_t)gdim[3 * l] * gdim[3 * l + 1] * gdim[3 * l + 2];
-        }
-        leafgrid.g = gl[L];
-        for (int c = 0; c < 3; c++) {
-            leafgrid.lo[c] = glo[3 * L + c];
-            leafgrid.dim[c] = gdim[3 * L + c];
-        }
         if (L >= 2) {
             M2LPlan pl{};
             pl.nc = nc;
@@ -1784,9 +1935,26 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
         hipLaunchKernelGGL(k_fmm_l2l, dim3(nbox[l]), dim3(kShiftLanes), 9 * nc * sizeof(float), st,
                            lv[l], lv[l + 1], nc, tb->mi, tb->l2l_term, tb->l2l_nt);
     SHP_HIP_CHECK(hipEventRecord(h->fev[4], st));
-    hipLaunchKernelGGL(k_fmm_leaf, dim3(nbox[L]), dim3(64), 0, st, lv[L], L, ring, leafgrid, root, sp,
-                       perm, pt->status, pt->species, nc, order, k_const_host(), h->fmm_stats, ax, ay,
-                       az);
+    {
+        // leaf work units (at most nbox + na / kLeafUnit of them)
+        const int nbl = nbox[L];
+        const int umax = nbl + na / kLeafUnit + 1;
+        SHP_WS(ucnt, int, h, WS_FMM_UCNT, 2 * (size_t)nbl + 1);
+        SHP_WS(units, int2, h, WS_FMM_UNITS, (size_t)umax);
+        int* uoffs = ucnt + nbl;
+        int* nunits = ucnt + 2 * nbl;
+        hipLaunchKernelGGL(k_fmm_unit_count, dim3(fdiv_up(nbl, 256)), dim3(256), 0, st, lv[L].start, lv[L].end,
+                           nbl, ucnt);
+        size_t tsz = 0;
+        SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tsz, ucnt, uoffs, nbl, st));
+        SHP_WS(cubu, char, h, WS_CUB, tsz);
+        SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cubu, tsz, ucnt, uoffs, nbl, st));
+        hipLaunchKernelGGL(k_fmm_unit_fill, dim3(fdiv_up(nbl, 256)), dim3(256), 0, st, ucnt, uoffs, nbl, units,
+                           nunits);
+        hipLaunchKernelGGL(k_fmm_leaf, dim3(umax), dim3(64), 0, st, lv[L], L, ring, leafgrid, root, sp, perm,
+                           pt->status, pt->species, nc, order, k_const_host(), units, nunits, h->fmm_stats,
+                           ax, ay, az);
+    }
     SHP_HIP_CHECK(hipGetLastError());
     SHP_HIP_CHECK(hipEventRecord(h->fev[5], st));
     SHP_HIP_CHECK(hipEventRecord(h->ev1, st));

@@ -86,6 +86,9 @@ enum WsSlot {
     WS_FMM_SYM_IMG, // FMM symmetry image transforms
     WS_FMM_SYM_D,   // FMM ions with their images: positions, charges, accelerations
     WS_FMM_SYM_I,   // their status and species
+    WS_FMM_LCOST,   // FMM leaf-level work estimates
+    WS_FMM_UCNT,    // FMM leaf work-unit counts and offsets
+    WS_FMM_UNITS,   // FMM leaf work units (leaf, target run)
     WS_NSLOTS
 };
 
