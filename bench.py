@@ -30,18 +30,58 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak (dense)
 FLOP_PER_PAIR = 20
 
 
-def cpu_baseline(d, n_targets):
-    """The oracle's FP64 direct sum (single thread) on a bounded target sample of
-    the same workload, scaled to particle-steps/s of a full N x N direct step."""
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE of rocprofv3
+    --pmc runs of this bench, scaled by the counters' reading for a device copy
+    of known size), from profiles/traffic.json; None when not profiled."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return None if t is None else t["fetch_bytes"] + t["write_bytes"]
+
+
+def copy_calibration(nbytes=256 << 20):
+    """A plain device copy of nbytes (torch copy kernel), run before the
+    workload so that --pmc runs of this bench can scale FETCH_SIZE and
+    WRITE_SIZE by what they read for a known number of bytes."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def cpu_baseline(d, n_targets, threads=None):
+    """The oracle's FP64 direct sum (the reference's direct method restated,
+    one target per loop iteration) on the host cores: n_targets targets of the
+    same plume on one thread, then threads x n_targets targets split over
+    `threads` threads (ctypes releases the GIL), each scaled to
+    particle-steps/s of a full N x N direct step."""
+    from concurrent.futures import ThreadPoolExecutor
     import oracle
     n = d["x"].shape[0]
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
     oracle.direct_f64(d, i0=0, i1=n_targets)
-    dt = time.perf_counter() - t0
-    pairs_per_s = n_targets * n / dt
-    return {"value": pairs_per_s / n, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP64 direct sum, {n_targets} targets x {n} sources of the same "
-                      f"plume ({dt:.1f} s, {pairs_per_s:.3e} pairs/s), scaled to one N x N step"}
+    dt1 = time.perf_counter() - t0
+    rate1 = n_targets * n / dt1
+    nt = min(n, threads * n_targets)
+    bounds = [(k * nt // threads, (k + 1) * nt // threads) for k in range(threads)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda b: oracle.direct_f64(d, i0=b[0], i1=b[1]), bounds))
+    dtm = time.perf_counter() - t0
+    ratem = nt * n / dtm
+    return {"value": ratem / n, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "value_1core": rate1 / n,
+            "sample": f"oracle FP64 direct sum over {n} sources of the same plume: {n_targets} "
+                      f"targets on 1 thread ({dt1:.1f} s, {rate1:.3e} pairs/s) and {nt} targets "
+                      f"on {threads} threads ({dtm:.1f} s, {ratem:.3e} pairs/s), scaled to one "
+                      f"N x N step"}
 
 
 def main():
@@ -56,6 +96,8 @@ def main():
     ap.add_argument("--cpu-targets", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 256 MiB device copy first (reference kernel for --pmc byte counts)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -65,10 +107,11 @@ def main():
     import numpy as np
     import torch
 
-    import oracle
     import solveit_hip
     from solveit_hip import synth
 
+    if args.copy_calibration:
+        copy_calibration()
     n = args.n
     d = synth.cone(n)
     mesh = synth.tip_field_mesh()
@@ -114,15 +157,22 @@ def main():
     # Accuracy of the FMM force against the GPU FP64 direct sum on this state.
     a_f = torch.stack(eng.fmm_dev(st, order=args.order, leaf_size=args.leaf_size, ring=2), 1)
     eng.sync()
-    a_fmm = a_f.cpu().numpy()
     fp64_step_ms = None
-    l2 = l2max = None
+    l2 = l2max = l2_all = None
     if not args.no_fp64:
         a_d = torch.stack(eng.direct_dev(st, precision=solveit_hip.SHP_PREC_FP64), 1)
         eng.sync()
-        a_dir = a_d.cpu().numpy()
-        sub = np.random.default_rng(8192).choice(n, size=min(8192, n), replace=False)
-        l2, l2max, _ = oracle.rel_l2(a_fmm[sub], a_dir[sub])
+        # Integrator::compareAcceleration's metric (GPU, FP64) on a seeded
+        # 8192-ion target subsample and on every ion
+        sub = torch.as_tensor(np.sort(np.random.default_rng(8192).choice(n, size=min(8192, n),
+                                                                         replace=False)),
+                              device="cuda:0")
+        m_sub = eng.compare_accel(st.status[sub].contiguous(),
+                                  tuple(a_f[sub, k].contiguous() for k in range(3)),
+                                  tuple(a_d[sub, k].contiguous() for k in range(3)))
+        m_all = eng.compare_accel(st.status, tuple(a_f[:, k].contiguous() for k in range(3)),
+                                  tuple(a_d[:, k].contiguous() for k in range(3)))
+        l2, l2max, l2_all = m_sub["l2"], m_sub["max"], m_all["l2"]
         opts64 = solveit_hip.make_step_opts(method=solveit_hip.SHP_METHOD_DIRECT,
                                             precision=solveit_hip.SHP_PREC_FP64, use_efield=True)
         eng.sync()
@@ -133,14 +183,18 @@ def main():
 
     m2l_tf = 2.0 * mean["m2l_madds"] / (mean["m2l_ms"] * 1e-3) / 1e12
     leaf_tf = FLOP_PER_PAIR * mean["p2p_pairs"] / (mean["leaf_ms"] * 1e-3) / 1e12
-    # Dominant kernel: the first M2L pass (output degrees 0..4).  Algorithmic
-    # flops = 2 x the multiply-adds of the box pairs that exist.
-    pass0_flop = 2.0 * mean["m2l_pass0_madds"]
-    roof = {"kernel": "k_fmm_m2l_tiled<10,8,7,0,4>",
-            "achieved": pass0_flop / (mean["m2l_pass0_ms"] * 1e-3) / 1e12,
-            "kernel_ms": mean["m2l_pass0_ms"], "flop_per_launch": pass0_flop,
-            "bound": "fp32-vector", "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "traffic": None}
+    # Dominant kernel of the timed steps: the leaf kernel (P2P + L2P;
+    # algorithmic flops = 20 per near-field pair) or the first M2L pass (output
+    # degrees 0..4; 2 flops per multiply-add of the box pairs that exist),
+    # whichever has the longer mean launch time.
+    if mean["leaf_ms"] >= mean["m2l_pass0_ms"]:
+        kname, kms, kflop = "k_fmm_leaf", mean["leaf_ms"], FLOP_PER_PAIR * mean["p2p_pairs"]
+    else:
+        kname, kms, kflop = ("k_fmm_m2l_tiled<10,8,7,0,4>", mean["m2l_pass0_ms"],
+                             2.0 * mean["m2l_pass0_madds"])
+    roof = {"kernel": kname, "achieved": kflop / (kms * 1e-3) / 1e12, "kernel_ms": kms,
+            "flop_per_launch": kflop, "bound": "fp32-vector", "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "traffic": profiled_traffic(kname)}
     roof["frac"] = roof["achieved"] / FP32_PEAK_TFLOPS
 
     line = {
@@ -163,10 +217,12 @@ def main():
             "n": n,
             "method": "fmm",
             "order": args.order,
-            "leaf_level": int(mean["leaf_level"]),
+            "leaf_level": int(round(mean["leaf_level"])),
+            "leaf_level_mean": mean["leaf_level"],
             "near_ring": 2,
             "rel_l2_vs_fp64_direct": l2,
             "max_rel_vs_fp64_direct": l2max,
+            "rel_l2_all_ions_vs_fp64_direct": l2_all,
             "fp64_direct_step_ms": fp64_step_ms,
             "speedup_vs_fp64_direct": (fp64_step_ms / ms_per_step) if fp64_step_ms else None,
             "pass_ms": {k: mean[k] for k in ("p2m_m2m_ms", "ttable_ms", "m2l_ms", "l2l_ms",

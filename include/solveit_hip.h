@@ -155,6 +155,15 @@ int shp_fmm_stats(shp_handle h, double* out);
  * counterpart in the reference. */
 int shp_fmm_timing(shp_handle h, double* out, int reset);
 
+/* The leaf-level choice of the last shp_fmm_dev call with leaf_level 0:
+ * one row per candidate level, (level, near-field particle pairs, far-band
+ * box pairs of that level, estimated ms of the level-dependent passes), in
+ * increasing level order; at most cap rows are written to out (4 doubles
+ * each).  Returns the number of candidate rows (0 when the level was given or
+ * the rule was used), or a negative error code.  No counterpart in the
+ * reference. */
+int shp_fmm_level_estimates(shp_handle h, double* out, int cap);
+
 /*
  * Barnes-Hut evaluation, reference (FP32) arithmetic.  Replaces
  * BarnesHutForceCalculation::updateAcceleration (barnes_hut_force_calculation.cpp:574-715:
@@ -197,6 +206,20 @@ int shp_energy_dev(shp_handle h, const shp_particles* p, const double* vx, const
 int shp_field_energy_dev(shp_handle h, int n, const double* x0, const double* y0,
                          const double* z0, const double* x, const double* y, const double* z,
                          const double* q, const int* nsteps_arr, int nsteps, double* fe);
+
+/* ---- accuracy metric ---- */
+
+/* Accuracy of an acceleration field against a reference one, as
+ * Integrator::compareAcceleration computes it (integration_schemes.cpp:374-470),
+ * in FP64: over the ions with status 1, rel_i = |a_i - r_i|^2 / |r_i|^2 (0 when
+ * both are 0); out[0] = number of such ions, out[1] = sqrt(mean rel) (NaN
+ * when there are none, as in the reference), out[2] = sqrt(max rel), out[3] =
+ * the first index of the maximum (0 when every rel is 0), i.e. the values of
+ * ForceCalculationMethod::writeL2NormInfo.  Device arrays of length n; a
+ * deterministic two-level reduction; waits for the result (out is host memory). */
+int shp_compare_accel(shp_handle h, int n, const int* status, const double* ax, const double* ay,
+                      const double* az, const double* rx, const double* ry, const double* rz,
+                      double* out);
 
 /* ---- background E-field (pwl on a scattered (r, z) triangulation) ---- */
 

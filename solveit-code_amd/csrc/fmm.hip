@@ -1284,51 +1284,59 @@ __device__ __forceinline__ int find_cell(const Level& lv, const GridView& gv, in
 }
 
 // Work of a uniform leaf level l, counted over every stride-th box b of the
-// level: out[0] += n_b x (particles in the near cells of b) = P2P pairs;
-// out[1] += far-band boxes of b = occupied children of the parent's near
-// boxes minus the occupied near boxes of b = M2L box pairs.
+// level (one wave per box at a time, its lanes splitting the cell lookups;
+// the waves of the grid stride over the sampled boxes): out[0] += n_b x
+// (particles in the near cells of b) = P2P pairs; out[1] += far-band boxes of
+// b = occupied children of the parent's near boxes minus the occupied near
+// boxes of b = M2L box pairs.  One pair of atomics per workgroup.
+constexpr int kLevelCostBlocks = 512;
 __global__ __launch_bounds__(256) void k_fmm_level_cost(Level lv, GridView g, Level par, GridView gp,
                                                         int ring, int stride,
                                                         unsigned long long* __restrict__ out) {
-    const long b = ((long)blockIdx.x * blockDim.x + threadIdx.x) * stride;
-    unsigned long long pp = 0, fb = 0;
-    if (b < lv.nb) {
+    __shared__ long red[2][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int side = 2 * ring + 1, ncell = side * side * side;
+    long nsum_all = 0, far = 0;
+    for (long s = (long)blockIdx.x * 4 + wv; s * stride < lv.nb; s += (long)gridDim.x * 4) {
+        const long b = s * stride;
         const uint64_t key = lv.key[b];
         const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
         long nsum = 0;
-        int occ = 0;
-        for (int dz = -ring; dz <= ring; dz++)
-            for (int dy = -ring; dy <= ring; dy++)
-                for (int dx = -ring; dx <= ring; dx++) {
-                    const int c = find_cell(lv, g, ix + dx, iy + dy, iz + dz);
-                    if (c >= 0) {
-                        nsum += lv.end[c] - lv.start[c];
-                        occ++;
-                    }
-                }
-        long nch = 0;
-        for (int dz = -ring; dz <= ring; dz++)
-            for (int dy = -ring; dy <= ring; dy++)
-                for (int dx = -ring; dx <= ring; dx++) {
-                    const int c = find_cell(par, gp, (ix >> 1) + dx, (iy >> 1) + dy, (iz >> 1) + dz);
-                    if (c >= 0) nch += (c + 1 < par.nb ? par.cfirst[c + 1] : lv.nb) - par.cfirst[c];
-                }
-        pp = (unsigned long long)(lv.end[b] - lv.start[b]) * (unsigned long long)nsum;
-        fb = (unsigned long long)(nch - occ);
+        for (int t = lane; t < ncell; t += 64) {
+            const int dx = t % side - ring, dy = (t / side) % side - ring, dz = t / (side * side) - ring;
+            const int c = find_cell(lv, g, ix + dx, iy + dy, iz + dz);
+            if (c >= 0) {
+                nsum += lv.end[c] - lv.start[c];
+                far -= 1;
+            }
+            const int q = find_cell(par, gp, (ix >> 1) + dx, (iy >> 1) + dy, (iz >> 1) + dz);
+            if (q >= 0) far += (q + 1 < par.nb ? par.cfirst[q + 1] : lv.nb) - par.cfirst[q];
+        }
+        nsum_all += nsum * (lv.end[b] - lv.start[b]);
     }
     for (int off = 32; off > 0; off >>= 1) {
-        pp += __shfl_xor(pp, off);
-        fb += __shfl_xor(fb, off);
+        nsum_all += __shfl_xor(nsum_all, off);
+        far += __shfl_xor(far, off);
     }
-    if ((threadIdx.x & 63) == 0 && (pp || fb)) {
-        atomicAdd(out, pp);
-        atomicAdd(out + 1, fb);
+    if (lane == 0) {
+        red[0][wv] = nsum_all;
+        red[1][wv] = far;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        const long f = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        if (a || f) {
+            atomicAdd(out, (unsigned long long)a);
+            atomicAdd(out + 1, (unsigned long long)f);
+        }
     }
 }
 
 // Leaf work units: a leaf's targets in runs of kLeafUnit, one wave each, so
 // that the dense leaves of a plume core spread over many waves.
 constexpr int kLeafUnit = 64;
+constexpr int kLeafChunk = 16;
 
 __global__ void k_fmm_unit_count(const int* __restrict__ start, const int* __restrict__ end, int nb,
                                  int* __restrict__ cnt) {
@@ -1362,8 +1370,14 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
     __shared__ float Lsh[kMaxNc];
     __shared__ int rf[128], rl[128], rpre[129];
     __shared__ double red[3][64];
-    if ((int)blockIdx.x >= *nunits) return;
-    const int2 unit = units[blockIdx.x];
+    // Workgroups are dealt round-robin to the 8 XCDs: renumber so that each
+    // XCD runs chunks of kLeafChunk consecutive (Morton-ordered) units, whose
+    // near fields overlap in that XCD's L2, with the chunks dealt round-robin
+    // so that a dense region spreads over every XCD.
+    const int w = blockIdx.x, j = w >> 3;
+    const int u = (8 * (j / kLeafChunk) + (w & 7)) * kLeafChunk + j % kLeafChunk;
+    if (u >= *nunits) return;
+    const int2 unit = units[u];
     const int b = unit.x, lane = threadIdx.x;
     const uint64_t key = lv.key[b];
     double c[3], h;
@@ -1767,15 +1781,17 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     }
     if (choose) {
         // P2P pairs and M2L far-band box pairs of the candidate leaf levels
-        // Lrule - 2 .. Lrule + 1, counted on up to 65536 boxes per level.
+        // Lrule - 2 .. Lrule + 1, counted on every stride-th box (at most
+        // 16384 boxes per level; the sums are scaled to the whole level).
         const int L0 = std::max(2, Lrule - 2);
         SHP_WS(lcost, unsigned long long, h, WS_FMM_LCOST, 2 * (Lb + 1));
         hipLaunchKernelGGL(k_fmm_u64fill, dim3(1), dim3(64), 0, st, lcost, 2 * (Lb + 1));
         std::vector<int> stride(Lb + 1, 1);
         for (int l = L0; l <= Lb; l++) {
-            stride[l] = std::max(1, fdiv_up(nbox[l], 1 << 16));
+            stride[l] = std::max(1, fdiv_up(nbox[l], 1 << 14));
             const int ns = fdiv_up(nbox[l], stride[l]);
-            hipLaunchKernelGGL(k_fmm_level_cost, dim3(fdiv_up(ns, 256)), dim3(256), 0, st, lv[l], gview(l),
+            hipLaunchKernelGGL(k_fmm_level_cost, dim3(std::min(kLevelCostBlocks, fdiv_up(ns, 4))), dim3(256), 0,
+                               st, lv[l], gview(l),
                                lv[l - 1], gview(l - 1), ring, stride[l], lcost + 2 * l);
         }
         unsigned long long hcost[2 * (kMaxLevel + 1)];
@@ -1783,17 +1799,24 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
                                      hipMemcpyDeviceToHost, st));
         SHP_HIP_CHECK(hipStreamSynchronize(st));
         double best = 0.0, m2l = 0.0;
+        h->fmm_est_n = 0;
         for (int l = L0; l <= Lb; l++) {
             const double sc = (double)nbox[l] / fdiv_up(nbox[l], stride[l]);
             m2l += fmm_m2l_cost_ms(nc, nbox[l], sc * (double)hcost[2 * l + 1]);
             const double c = m2l + fmm_leaf_cost_ms(nbox[l], sc * (double)hcost[2 * l]);
+            double* row = h->fmm_est + 4 * h->fmm_est_n++;
+            row[0] = l;
+            row[1] = sc * (double)hcost[2 * l];
+            row[2] = sc * (double)hcost[2 * l + 1];
+            row[3] = c;
             if (l == L0 || c < best) {
                 best = c;
                 L = l;
             }
         }
-    } else if (L <= 0) {
-        L = Lrule;
+    } else {
+        h->fmm_est_n = 0;
+        if (L <= 0) L = Lrule;
     }
     last_leaf_level(h) = L;
     nbox.resize(L + 1);
@@ -1818,6 +1841,23 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ax, n);
     hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ay, n);
     hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, az, n);
+    // leaf work units (at most nbox + na / kLeafUnit of them)
+    const int umax = nbox[L] + na / kLeafUnit + 1;
+    SHP_WS(ucnt, int, h, WS_FMM_UCNT, 2 * (size_t)nbox[L] + 1);
+    SHP_WS(units, int2, h, WS_FMM_UNITS, (size_t)umax);
+    int* const nunits = ucnt + 2 * nbox[L];
+    {
+        const int nbl = nbox[L];
+        int* uoffs = ucnt + nbl;
+        hipLaunchKernelGGL(k_fmm_unit_count, dim3(fdiv_up(nbl, 256)), dim3(256), 0, st, lv[L].start, lv[L].end,
+                           nbl, ucnt);
+        size_t tsz = 0;
+        SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tsz, ucnt, uoffs, nbl, st));
+        SHP_WS(cubu, char, h, WS_CUB, tsz);
+        SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cubu, tsz, ucnt, uoffs, nbl, st));
+        hipLaunchKernelGGL(k_fmm_unit_fill, dim3(fdiv_up(nbl, 256)), dim3(256), 0, st, ucnt, uoffs, nbl, units,
+                           nunits);
+    }
     // upward pass
     SHP_HIP_CHECK(hipEventRecord(h->ev0, st));
     SHP_HIP_CHECK(hipEventRecord(h->fev[0], st));
@@ -1935,26 +1975,10 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
         hipLaunchKernelGGL(k_fmm_l2l, dim3(nbox[l]), dim3(kShiftLanes), 9 * nc * sizeof(float), st,
                            lv[l], lv[l + 1], nc, tb->mi, tb->l2l_term, tb->l2l_nt);
     SHP_HIP_CHECK(hipEventRecord(h->fev[4], st));
-    {
-        // leaf work units (at most nbox + na / kLeafUnit of them)
-        const int nbl = nbox[L];
-        const int umax = nbl + na / kLeafUnit + 1;
-        SHP_WS(ucnt, int, h, WS_FMM_UCNT, 2 * (size_t)nbl + 1);
-        SHP_WS(units, int2, h, WS_FMM_UNITS, (size_t)umax);
-        int* uoffs = ucnt + nbl;
-        int* nunits = ucnt + 2 * nbl;
-        hipLaunchKernelGGL(k_fmm_unit_count, dim3(fdiv_up(nbl, 256)), dim3(256), 0, st, lv[L].start, lv[L].end,
-                           nbl, ucnt);
-        size_t tsz = 0;
-        SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tsz, ucnt, uoffs, nbl, st));
-        SHP_WS(cubu, char, h, WS_CUB, tsz);
-        SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cubu, tsz, ucnt, uoffs, nbl, st));
-        hipLaunchKernelGGL(k_fmm_unit_fill, dim3(fdiv_up(nbl, 256)), dim3(256), 0, st, ucnt, uoffs, nbl, units,
-                           nunits);
-        hipLaunchKernelGGL(k_fmm_leaf, dim3(umax), dim3(64), 0, st, lv[L], L, ring, leafgrid, root, sp, perm,
-                           pt->status, pt->species, nc, order, k_const_host(), units, nunits, h->fmm_stats,
-                           ax, ay, az);
-    }
+    hipLaunchKernelGGL(k_fmm_leaf, dim3(8 * fdiv_up(umax + 8 * kLeafChunk, 8)), dim3(64), 0, st, lv[L], L, ring,
+                       leafgrid, root, sp, perm,
+                       pt->status, pt->species, nc, order, k_const_host(), units, nunits, h->fmm_stats, ax, ay,
+                       az);
     SHP_HIP_CHECK(hipGetLastError());
     SHP_HIP_CHECK(hipEventRecord(h->fev[5], st));
     SHP_HIP_CHECK(hipEventRecord(h->ev1, st));
@@ -1975,6 +1999,13 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
 }
 
 int shp_fmm_last_leaf_level(shp_handle h) { return h ? last_leaf_level(h) : -1; }
+
+int shp_fmm_level_estimates(shp_handle h, double* out, int cap) {
+    SHP_REQUIRE(h && (out || cap == 0), "shp_fmm_level_estimates: null argument");
+    const int n = std::min(cap, h->fmm_est_n);
+    for (int k = 0; k < 4 * n; k++) out[k] = h->fmm_est[k];
+    return h->fmm_est_n;
+}
 
 int shp_fmm_stats(shp_handle h, double* out) {
     SHP_REQUIRE(h && out, "shp_fmm_stats: null argument");

@@ -18,6 +18,7 @@
 // conditions (particles.cpp:34-40, 73-177, 199-340).  It also writes the
 // status masked by the next step's active range for the next force call.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <functional>
 #include <vector>
@@ -389,6 +390,84 @@ static UpdateParams make_params(const shp_step_opts* o) {
     return P;
 }
 
+// Accuracy metric of Integrator::compareAcceleration
+// (integration_schemes.cpp:374-470) in FP64.  Per ion with status 1:
+// rel = |a - a_ref|^2 / |a_ref|^2, 0 when both are 0.  Each block folds its
+// grid-strided ions in index order per lane, then a fixed tree; the maximum
+// keeps the smallest index among equal values (the reference's first index).
+constexpr int kCmpBlocks = 1024;
+__global__ __launch_bounds__(256) void k_cmp_partial(int n, const int* __restrict__ st,
+                                                     const double* __restrict__ ax,
+                                                     const double* __restrict__ ay,
+                                                     const double* __restrict__ az,
+                                                     const double* __restrict__ rx,
+                                                     const double* __restrict__ ry,
+                                                     const double* __restrict__ rz,
+                                                     double* __restrict__ psum, double* __restrict__ pmax,
+                                                     int* __restrict__ pidx, int* __restrict__ pcnt) {
+    __shared__ double ss[256], sm[256];
+    __shared__ int si[256], sc[256];
+    double sum = 0.0, mx = 0.0;
+    int idx = INT_MAX, cnt = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        if (st[i] != 1) continue;
+        cnt++;
+        const double dx = ax[i] - rx[i], dy = ay[i] - ry[i], dz = az[i] - rz[i];
+        const double d = dx * dx + dy * dy + dz * dz;
+        const double nr = rx[i] * rx[i] + ry[i] * ry[i] + rz[i] * rz[i];
+        const double rel = (d == 0.0 && nr == 0.0) ? 0.0 : d / nr;
+        sum += rel;
+        if (rel > mx) {
+            mx = rel;
+            idx = i;
+        }
+    }
+    const int t = threadIdx.x;
+    ss[t] = sum;
+    sm[t] = mx;
+    si[t] = idx;
+    sc[t] = cnt;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            ss[t] += ss[t + w];
+            sc[t] += sc[t + w];
+            if (sm[t + w] > sm[t] || (sm[t + w] == sm[t] && si[t + w] < si[t])) {
+                sm[t] = sm[t + w];
+                si[t] = si[t + w];
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        psum[blockIdx.x] = ss[0];
+        pmax[blockIdx.x] = sm[0];
+        pidx[blockIdx.x] = si[0];
+        pcnt[blockIdx.x] = sc[0];
+    }
+}
+
+__global__ void k_cmp_final(int nblk, const double* __restrict__ psum, const double* __restrict__ pmax,
+                            const int* __restrict__ pidx, const int* __restrict__ pcnt,
+                            double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double sum = 0.0, mx = 0.0;
+    int idx = INT_MAX;
+    long cnt = 0;
+    for (int b = 0; b < nblk; b++) {
+        sum += psum[b];
+        cnt += pcnt[b];
+        if (pmax[b] > mx || (pmax[b] == mx && pidx[b] < idx)) {
+            mx = pmax[b];
+            idx = pidx[b];
+        }
+    }
+    out[0] = (double)cnt;
+    out[1] = sqrt(sum / (double)cnt);  // NaN without valid ions, as in the reference
+    out[2] = sqrt(mx);
+    out[3] = idx == INT_MAX ? 0.0 : (double)idx;
+}
+
 }  // namespace shp
 
 namespace shp {
@@ -632,6 +711,30 @@ int shp_field_energy_dev(shp_handle h, int n, const double* x0, const double* y0
     hipLaunchKernelGGL(k_field_energy, dim3(cdiv_l(n, 256)), dim3(256), 0, h->stream,
                        view_of(h->mesh), n, x0, y0, z0, x, y, z, q, nsteps_arr, nsteps, fe);
     SHP_HIP_CHECK(hipGetLastError());
+    return SHP_OK;
+}
+
+int shp_compare_accel(shp_handle h, int n, const int* status, const double* ax, const double* ay,
+                      const double* az, const double* rx, const double* ry, const double* rz, double* out) {
+    SHP_REQUIRE(h && out, "shp_compare_accel: null argument");
+    SHP_REQUIRE(n >= 0, "shp_compare_accel: n %d < 0", n);
+    SHP_REQUIRE(n == 0 || (status && ax && ay && az && rx && ry && rz), "shp_compare_accel: null array");
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    hipStream_t st = h->stream;
+    const int nblk = std::max(1, std::min(kCmpBlocks, (n + 255) / 256));
+    // partials: sums and maxima (doubles), then indices and counts (ints), then the result
+    SHP_WS(part, double, h, WS_CMP, 3 * (size_t)nblk + 4);
+    double* psum = part;
+    double* pmax = part + nblk;
+    int* pidx = (int*)(part + 2 * nblk);
+    int* pcnt = pidx + nblk;
+    double* dout = part + 3 * nblk;
+    hipLaunchKernelGGL(k_cmp_partial, dim3(nblk), dim3(256), 0, st, n, status, ax, ay, az, rx, ry, rz, psum,
+                       pmax, pidx, pcnt);
+    hipLaunchKernelGGL(k_cmp_final, dim3(1), dim3(64), 0, st, nblk, psum, pmax, pidx, pcnt, dout);
+    SHP_HIP_CHECK(hipGetLastError());
+    SHP_HIP_CHECK(hipMemcpyAsync(out, dout, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    SHP_HIP_CHECK(hipStreamSynchronize(st));
     return SHP_OK;
 }
 

@@ -89,6 +89,7 @@ enum WsSlot {
     WS_FMM_LCOST,   // FMM leaf-level work estimates
     WS_FMM_UCNT,    // FMM leaf work-unit counts and offsets
     WS_FMM_UNITS,   // FMM leaf work units (leaf, target run)
+    WS_CMP,         // accuracy metric: per-block partials
     WS_NSLOTS
 };
 
@@ -131,6 +132,10 @@ struct shp_context {
     double fmm_last[shp::kFmmStats] = {};
     double fmm_acc[shp::kFmmStats] = {};
     long fmm_acc_n = 0;
+    // leaf-level choice of the last FMM call: rows (level, P2P pairs, far-band
+    // box pairs, estimated ms), fmm_est_n of them
+    int fmm_est_n = 0;
+    double fmm_est[4 * 24] = {};
     bool timed = false, timed_update = false;
     void* ws[shp::WS_NSLOTS] = {};
     size_t ws_bytes[shp::WS_NSLOTS] = {};

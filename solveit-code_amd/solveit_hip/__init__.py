@@ -150,6 +150,8 @@ def _load():
                                     c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
         "shp_fmm_stats": (c_int, [c_void_p, dp]),
         "shp_fmm_timing": (c_int, [c_void_p, dp, c_int]),
+        "shp_fmm_level_estimates": (c_int, [c_void_p, dp, c_int]),
+        "shp_compare_accel": (c_int, [c_void_p, c_int] + [c_void_p] * 7 + [dp]),
         "shp_energy_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), c_void_p, c_void_p,
                                    c_void_p, dp]),
         "shp_field_energy_dev": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -601,7 +603,34 @@ def _engine_fmm_timing(self, reset=False):
     return d
 
 
+def _engine_fmm_level_estimates(self):
+    """The last FMM call's leaf-level candidates: list of dicts (level,
+    p2p_pairs, far_pairs, est_ms); empty when the level was not chosen."""
+    out = (ctypes.c_double * (4 * 24))()
+    n = lib.shp_fmm_level_estimates(self.h, out, 24)
+    _check(min(n, 0), "shp_fmm_level_estimates")
+    return [dict(level=int(out[4 * k]), p2p_pairs=out[4 * k + 1], far_pairs=out[4 * k + 2],
+                 est_ms=out[4 * k + 3]) for k in range(min(n, 24))]
+
+
+def _engine_compare_accel(self, status, a, ref):
+    """Accuracy of a = (ax, ay, az) against ref (device float64 tensors) over
+    the ions whose int32 status is 1, as the reference's compareAcceleration
+    defines it: dict(n, l2, max, idx)."""
+    out = (ctypes.c_double * 4)()
+    n = status.shape[0]
+    for t in tuple(a) + tuple(ref):
+        assert t.shape[0] == n and t.is_contiguous()
+    _torch_ready(status.device)
+    _check(lib.shp_compare_accel(self.h, n, status.data_ptr(), a[0].data_ptr(), a[1].data_ptr(),
+                                 a[2].data_ptr(), ref[0].data_ptr(), ref[1].data_ptr(),
+                                 ref[2].data_ptr(), out), "shp_compare_accel")
+    return dict(n=int(out[0]), l2=out[1], max=out[2], idx=int(out[3]))
+
+
 Engine.fmm_dev = _engine_fmm_dev
+Engine.fmm_level_estimates = _engine_fmm_level_estimates
+Engine.compare_accel = _engine_compare_accel
 Engine.fmm_leaf_level = _engine_fmm_leaf_level
 Engine.fmm_stats = _engine_fmm_stats
 Engine.fmm_timing = _engine_fmm_timing

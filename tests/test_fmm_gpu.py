@@ -152,3 +152,33 @@ def test_fmm_symmetry_vs_fp64_direct_images(eng, sym):
     l2, mx, _ = oracle.rel_l2(got, ref.cpu().numpy(), d["status"] == 1)
     print(f"  {sym}: fmm p=10 vs fp64 direct over images rel L2 {l2:.3e} max {mx:.3e}")
     assert l2 <= 1e-5
+
+
+def test_leaf_level_choice_counts(eng):
+    """The leaf-level choice counts, on every box of each candidate level (no
+    sampling below 16384 boxes), the near-field particle pairs and far-band
+    box pairs; for the chosen level they equal the pairs the leaf and M2L
+    kernels then evaluate, and the chosen level has the least estimate."""
+    import solveit_hip
+    import torch
+    d = synth.cone(20000, seed=26)
+    st = solveit_hip.DeviceState.from_numpy(d)
+    torch.cuda.synchronize()
+    print("case fmm level choice", flush=True)
+    for leaf_size in (200, 60):
+        eng.fmm_dev(st, order=10, leaf_level=0, leaf_size=leaf_size, ring=2)
+        rows = eng.fmm_level_estimates()
+        s = eng.fmm_stats()
+        L = eng.fmm_leaf_level()
+        print(f"  leaf_size {leaf_size}: chose {L}; " +
+              ", ".join(f"L{r['level']}: p2p {r['p2p_pairs']:.3e} far {r['far_pairs']:.3e} "
+                        f"{r['est_ms']:.3f} ms" for r in rows))
+        assert rows and s["leaf_level"] == L
+        best = min(rows, key=lambda r: r["est_ms"])
+        assert best["level"] == L
+        chosen = [r for r in rows if r["level"] == L][0]
+        assert chosen["p2p_pairs"] == s["p2p_pairs"]
+        if rows[0]["level"] == 2:  # every M2L level is a candidate
+            assert sum(r["far_pairs"] for r in rows if r["level"] <= L) == s["m2l_pairs"]
+    eng.fmm_dev(st, order=10, leaf_level=5, ring=2)
+    assert eng.fmm_level_estimates() == []
