@@ -84,6 +84,292 @@ def cpu_baseline(d, n_targets, threads=None):
                       f"N x N step"}
 
 
+def _timed(fn, reps=2):
+    """Wall ms of fn() (synchronized), the last of `reps` calls, and its result."""
+    import torch
+    out = None
+    ms = 0.0
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+    return ms, out
+
+
+def _metric(eng, st, a, ref, sub=None):
+    """compareAcceleration's rel L2 / max (GPU, FP64), on all ions or a subsample."""
+    import torch
+    if sub is None:
+        m = eng.compare_accel(st.status, tuple(a), tuple(ref))
+    else:
+        m = eng.compare_accel(st.status[sub].contiguous(), tuple(x[sub].contiguous() for x in a),
+                              tuple(x[sub].contiguous() for x in ref))
+    return {"rel_l2": m["l2"], "max_rel": m["max"], "n": m["n"]}
+
+
+def _energy_total(eng, st, x0, y0, z0, nsteps_field):
+    """KE + PE + field work from (x0, y0, z0) (J)."""
+    ke, pe = eng.energy(st)
+    fe = 0.0
+    if nsteps_field:
+        f = eng.field_energy(x0, y0, z0, st.x, st.y, st.z, st.q, nsteps=nsteps_field)
+        fe = float(f[(st.status == 1) | (st.status == 2)].sum().item())
+    return ke + pe + fe
+
+
+def run_config(k, args):
+    """One JSON line per BASELINE config (1-5) measured on this GPU; see the
+    docstring of each branch for what is timed."""
+    import numpy as np
+    import torch
+    import solveit_hip
+    from solveit_hip import synth
+    eng = solveit_hip.Engine(0)
+    P = solveit_hip
+    res = {"config": k}
+
+    def log(msg):
+        print(f"[config {k}] {msg}", file=sys.stderr, flush=True)
+
+    if k == 1:
+        # FP64 direct, N=4096 hemisphere, 1000 DKD steps at dt=1e-12 s with the
+        # synthetic tip field: particle-steps/s and relative energy drift.
+        d = synth.hemisphere(4096)
+        eng.efield_setup(synth.tip_field_mesh())
+        st = P.DeviceState.from_numpy(d)
+        x0, y0, z0 = st.x.clone(), st.y.clone(), st.z.clone()
+        e0 = _energy_total(eng, st, x0, y0, z0, 0)
+        opts = P.make_step_opts(method=P.SHP_METHOD_DIRECT, precision=P.SHP_PREC_FP64, use_efield=True)
+        nsteps = 1000
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.leapfrog_run(st, opts, 0.0, 1e-12, nsteps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        e1 = _energy_total(eng, st, x0, y0, z0, 64)
+        res.update(workload="hemisphere 20 nm N=4096, FP64 direct + pwl tip field, 1000 DKD steps dt=1e-12 s",
+                   particle_steps_per_s=4096 * nsteps / wall, ms_per_step=wall * 1e3 / nsteps,
+                   energy_drift_rel=(e1 - e0) / abs(e0))
+    elif k == 2:
+        # FP32C direct at N=65536 and 262144 (cone): one force evaluation (kernel
+        # time, pair rate, fraction of the FP32 peak, rel L2 vs the GPU FP64
+        # path), then 100 DKD steps with FP32C and with FP64 forces from the
+        # same state: energy drift of each and the final position difference.
+        out = []
+        for n in (65536, 262144):
+            d = synth.cone(n)
+            st = P.DeviceState.from_numpy(d)
+            nsrc = int(((st.status == 1) | (st.status == 2)).sum().item())
+            nt = int((st.status == 1).sum().item())
+            ms32, a32 = _timed(lambda: eng.direct_dev(st, precision=P.SHP_PREC_FP32C))
+            k32 = eng.last_kernel_ms()
+            ms64, a64 = _timed(lambda: eng.direct_dev(st, precision=P.SHP_PREC_FP64))
+            k64 = eng.last_kernel_ms()
+            acc = _metric(eng, st, a32, a64)
+            pairs = float(nt) * nsrc
+            row = {"n": n, "fp32c_kernel_ms": k32, "fp64_kernel_ms": k64,
+                   "fp32c_pairs_per_s": pairs / (k32 * 1e-3),
+                   "fp32c_tflops_20flop": FLOP_PER_PAIR * pairs / (k32 * 1e-3) / 1e12,
+                   "fp32c_frac_of_157": FLOP_PER_PAIR * pairs / (k32 * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                   "fp64_tflops_20flop": FLOP_PER_PAIR * pairs / (k64 * 1e-3) / 1e12,
+                   "rel_l2_fp32c_vs_fp64": acc["rel_l2"], "max_rel": acc["max_rel"]}
+            drift = {}
+            fin = {}
+            for name, prec in (("fp32c", P.SHP_PREC_FP32C), ("fp64", P.SHP_PREC_FP64)):
+                s2 = P.DeviceState.from_numpy(d)
+                x0, y0, z0 = s2.x.clone(), s2.y.clone(), s2.z.clone()
+                e0 = _energy_total(eng, s2, x0, y0, z0, 0)
+                opts = P.make_step_opts(method=P.SHP_METHOD_DIRECT, precision=prec)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                eng.leapfrog_run(s2, opts, 0.0, 1e-12, 100)
+                torch.cuda.synchronize()
+                row[f"{name}_100_steps_ms"] = (time.perf_counter() - t0) * 1e3
+                drift[name] = (_energy_total(eng, s2, x0, y0, z0, 0) - e0) / abs(e0)
+                fin[name] = torch.stack([s2.x, s2.y, s2.z], 1)
+            row["energy_drift_rel"] = drift
+            row["final_pos_rel_diff_fp32c_vs_fp64"] = float(
+                (fin["fp32c"] - fin["fp64"]).norm() / (fin["fp64"] - torch.stack([
+                    torch.as_tensor(d["x"]), torch.as_tensor(d["y"]), torch.as_tensor(d["z"])],
+                    1).to(fin["fp64"].device)).norm())
+            out.append(row)
+            log(json.dumps(row))
+        res.update(workload="cone plume, FP32C direct vs GPU FP64 direct", runs=out)
+    elif k == 3:
+        # N=1M cone: Barnes-Hut theta 0.3/0.5/0.7 and FMM p=4/6/8/10 (leaf
+        # size 64): ms per force evaluation and rel L2 vs FP64 direct on the
+        # seeded 8192-target subsample.
+        n = 1048576
+        d = synth.cone(n)
+        st = P.DeviceState.from_numpy(d)
+        ms64, a64 = _timed(lambda: eng.direct_dev(st, precision=P.SHP_PREC_FP64), reps=1)
+        sub = torch.as_tensor(np.sort(np.random.default_rng(8192).choice(n, 8192, replace=False)),
+                              device="cuda:0")
+        rows = [{"method": "fp64_direct", "ms": ms64}]
+        for th in (0.3, 0.5, 0.7):
+            ms, a = _timed(lambda: eng.bh_dev(st, theta=th))
+            rows.append({"method": "bh", "theta": th, "ms": ms, **_metric(eng, st, a, a64, sub)})
+            log(json.dumps(rows[-1]))
+        for p in (4, 6, 8, 10):
+            ms, a = _timed(lambda: eng.fmm_dev(st, order=p, leaf_size=64, ring=2))
+            rows.append({"method": "fmm", "order": p, "ms": ms, "leaf_level": eng.fmm_leaf_level(),
+                         **_metric(eng, st, a, a64, sub)})
+            log(json.dumps(rows[-1]))
+        res.update(workload="cone plume N=1048576 (one force evaluation each)", runs=rows)
+    elif k == 4:
+        # N=262144 per cell/sector: translational FMM (3x3x3 shell of 2 um
+        # cells, uniform fill + wrapped cone) and rotational FMM (6 sectors,
+        # sector-folded cone), vs FP64 direct over the same images.
+        n = 262144
+        rows = []
+        for kind in ("translational", "rotational"):
+            if kind == "translational":
+                d = synth.cell_fill(n)
+                sym = P.make_symmetry("translational", lattice=synth.full_lattice(1, (1, 1, 1)),
+                                      domain=(2e-6, 2e-6, 2e-6))
+                nimg = 27
+            else:
+                d = synth.cone(n)
+                phi = np.arctan2(d["y"], d["x"])
+                kk = np.floor(np.mod(phi, 2 * np.pi) / (np.pi / 3))
+                c, s_ = np.cos(-kk * np.pi / 3), np.sin(-kk * np.pi / 3)
+                d["x"], d["y"] = c * d["x"] - s_ * d["y"], s_ * d["x"] + c * d["y"]
+                sym = P.make_symmetry("rotational", nsec=6)
+                nimg = 6
+            st = P.DeviceState.from_numpy(d)
+            ms64, a64 = _timed(lambda: eng.direct_dev(st, precision=P.SHP_PREC_FP64, symmetry=sym), reps=1)
+            ms32, a32 = _timed(lambda: eng.direct_dev(st, precision=P.SHP_PREC_FP32C, symmetry=sym), reps=1)
+            msf, af = _timed(lambda: eng.fmm_dev(st, order=10, leaf_size=20, ring=2, symmetry=sym))
+            rows.append({"kind": kind, "n": n, "images": nimg, "fmm_ms": msf, "fp64_direct_ms": ms64,
+                         "fp32c_direct_ms": ms32, "fmm_speedup_vs_fp64": ms64 / msf,
+                         "fmm": _metric(eng, st, af, a64), "fp32c": _metric(eng, st, a32, a64)})
+            log(json.dumps(rows[-1]))
+        res.update(workload="FMM with symmetry images, N=262144 per cell/sector", runs=rows)
+    elif k == 5:
+        res.update(run_config5(eng, args, log))
+    eng.close()
+    print(json.dumps(res), flush=True)
+
+
+def run_config5(eng, args, log):
+    """64 cone plumes, N lognormal (median 32768, sigma ln 4 / 2) clipped to
+    [8192, 131072], in one batch: K=200 graph-captured DKD steps (FP32C
+    forces, FP64 state) with checkpoints every 20; the checkpointed discrete
+    adjoint of J = mean sin^2 of the velocity divergence angle with respect to
+    charges and initial velocities; central differences of the FP64 rollout
+    (h=1e-6) on `--fd-dirs` seeded directions."""
+    import numpy as np
+    import torch
+    import solveit_hip as P
+    from solveit_hip import synth
+    rng = np.random.default_rng(20261208)
+    sizes = np.clip(np.round(np.exp(rng.normal(np.log(32768), np.log(4) / 2, 64))), 8192, 131072).astype(int)
+    parts = [synth.cone(int(m), seed=1000 + i) for i, m in enumerate(sizes)]
+    keys = ("x", "y", "z", "vx", "vy", "vz", "q", "status", "species", "t_inj")
+    allb = {key: np.concatenate([p[key] for p in parts]) for key in keys}
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n = int(off[-1])
+    pairs_step = float(np.sum(sizes.astype(float) ** 2))
+    log(f"64 instances, N total {n}, min {sizes.min()} max {sizes.max()}, pairs/step {pairs_step:.3e}")
+    dev = "cuda:0"
+    base = {key: torch.as_tensor(allb[key], device=dev) for key in ("x", "y", "z")}
+    status = torch.as_tensor(allb["status"], dtype=torch.int32, device=dev)
+    species = torch.as_tensor(allb["species"], dtype=torch.int32, device=dev)
+    q0 = torch.as_tensor(allb["q"], device=dev)
+    v0 = [torch.as_tensor(allb[key], device=dev) for key in ("vx", "vy", "vz")]
+    dt, K, C = 1e-12, 200, 20
+    plan = eng.batch(off, P.SHP_PREC_FP32C)
+    plan64 = eng.batch(off, P.SHP_PREC_FP64)
+    opts = P.make_step_opts(method=P.SHP_METHOD_DIRECT, precision=P.SHP_PREC_FP32C)
+    opts64 = P.make_step_opts(method=P.SHP_METHOD_DIRECT, precision=P.SHP_PREC_FP64)
+
+    def state(q, v):
+        return P.DeviceState(base["x"].clone(), base["y"].clone(), base["z"].clone(), q.clone(), status,
+                             species, v[0].clone(), v[1].clone(), v[2].clone(), torch.zeros_like(q))
+
+    def copy(st):
+        return P.DeviceState(st.x.clone(), st.y.clone(), st.z.clone(), st.q, st.status, st.species,
+                             st.vx.clone(), st.vy.clone(), st.vz.clone(), st.t_inj)
+
+    mask = status == 1
+
+    def objective(st):
+        s2 = (st.vx ** 2 + st.vy ** 2) / (st.vx ** 2 + st.vy ** 2 + st.vz ** 2)
+        return s2[mask].mean()
+
+    tune_ms, chosen = plan.tune(state(q0, v0))
+    log(f"tuner ms {['%.2f' % m for m in tune_ms]} chosen {chosen}")
+    # forward with checkpoints
+    st = state(q0, v0)
+    ckpt = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s0 in range(0, K, C):
+        ckpt.append(copy(st))
+        plan.leapfrog(st, opts, s0 * dt, dt, C, use_graph=True)
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - t0
+    J = float(objective(st))
+    log(f"forward {fwd_s:.2f} s, J {J:.6e}")
+    # backward: dJ/dv at the end, then the checkpointed reverse sweep
+    vx, vy, vz = st.vx, st.vy, st.vz
+    r2 = vx ** 2 + vy ** 2 + vz ** 2
+    m = mask.double() / mask.sum()
+    gvx = m * (2 * vx / r2 - 2 * vx * (vx ** 2 + vy ** 2) / r2 ** 2)
+    gvy = m * (2 * vy / r2 - 2 * vy * (vx ** 2 + vy ** 2) / r2 ** 2)
+    gvz = m * (-2 * vz * (vx ** 2 + vy ** 2) / r2 ** 2)
+    xb = torch.zeros(3, n, dtype=torch.float64, device=dev)
+    vb = torch.stack([gvx, gvy, gvz]).contiguous()
+    qb = torch.zeros(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for seg in range(len(ckpt) - 1, -1, -1):
+        s = copy(ckpt[seg])
+        starts = []
+        for j in range(C):
+            starts.append(copy(s))
+            plan.leapfrog(s, opts, (seg * C + j) * dt, dt, 1, use_graph=False)
+        for s in reversed(starts):
+            xb, vb = plan.step_vjp(s, dt, xb, vb, qb)
+        log(f"backward segment {seg} done")
+    torch.cuda.synchronize()
+    bwd_s = time.perf_counter() - t0
+    grads = [qb, vb[0], vb[1], vb[2]]
+
+    def J64(q, v):
+        s = state(q, v)
+        for s0 in range(0, K, C):
+            plan64.leapfrog(s, opts64, s0 * dt, dt, C, use_graph=True)
+        return float(objective(s))
+
+    gen = torch.Generator(device="cpu").manual_seed(2026)
+    h = 1e-6
+    worst, fd_rows = 0.0, []
+    t0 = time.perf_counter()
+    for i in range(args.fd_dirs):
+        dirs = [torch.randn(n, generator=gen, dtype=torch.float64).to(dev) for _ in range(4)]
+        dq, dv = dirs[0] * 0.01, [x * 10.0 for x in dirs[1:]]
+        ana = float(sum((g * x).sum() for g, x in zip(grads, [dq] + dv)))
+        jp = J64(q0 + h * dq, [a + h * b for a, b in zip(v0, dv)])
+        jm = J64(q0 - h * dq, [a - h * b for a, b in zip(v0, dv)])
+        fd = (jp - jm) / (2 * h)
+        worst = max(worst, abs(ana - fd) / abs(fd))
+        fd_rows.append({"adjoint": ana, "fd": fd})
+        log(f"direction {i}: adjoint {ana:.8e} fd {fd:.8e}")
+    fd_s = time.perf_counter() - t0
+    plan.close()
+    plan64.close()
+    return {"workload": "64 cone plumes N lognormal in [8192, 131072], K=200 steps, checkpoints every 20",
+            "n_total": n, "sizes_min_max": [int(sizes.min()), int(sizes.max())],
+            "pairs_per_step": pairs_step, "tuner_ms": tune_ms, "tuner_choice": chosen,
+            "forward_s": fwd_s, "forward_particle_steps_per_s": n * K / fwd_s,
+            "forward_fp32c_tflops_20flop": FLOP_PER_PAIR * pairs_step * K / fwd_s / 1e12,
+            "backward_s": bwd_s, "objective": J, "fd_dirs": args.fd_dirs, "fd_s": fd_s,
+            "fd_worst_rel_diff": worst, "fd": fd_rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +382,9 @@ def main():
     ap.add_argument("--cpu-targets", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--config", type=int, default=0,
+                    help="measure BASELINE config 1-5 instead of the headline step (one JSON line)")
+    ap.add_argument("--fd-dirs", type=int, default=32, help="config 5: finite-difference directions")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy first (reference kernel for --pmc byte counts)")
     args = ap.parse_args()
@@ -110,6 +399,9 @@ def main():
     import solveit_hip
     from solveit_hip import synth
 
+    if args.config:
+        run_config(args.config, args)
+        return
     if args.copy_calibration:
         copy_calibration()
     n = args.n

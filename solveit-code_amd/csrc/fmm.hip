@@ -606,41 +606,47 @@ __global__ __launch_bounds__(kShiftLanes) void k_fmm_m2m(Level par, Level ch, in
 
 // ------------------------------------------------------------- T tables ----
 
-// T[level][offset][nc] for integer offsets D in [-W, W]^3 (R = 2D).
-__global__ void k_fmm_ttable(int p, int nc, const int4* __restrict__ mi, int W, int L0, int L,
-                             const double* __restrict__ root, double eps2, float* __restrict__ T) {
+// T[level][offset][nc] for integer offsets D in [-W, W]^3 (R = 2D).  One
+// wave per (level, offset): the coefficients of degree n depend only on those
+// of degrees n-1 and n-2, so the lanes compute one degree at a time in LDS.
+__global__ __launch_bounds__(64) void k_fmm_ttable(int p, int nc, const int4* __restrict__ mi, int W, int L0,
+                                                   int L, const double* __restrict__ root, double eps2,
+                                                   float* __restrict__ T) {
+    __shared__ double Td[kMaxNc];
     const int nside = 2 * W + 1, noff = nside * nside * nside;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= noff * (L - L0 + 1)) return;
+    const int t = blockIdx.x;
+    if (t >= noff * (L - L0 + 1)) return;  // uniform over the workgroup
     const int l = L0 + t / noff, o = t % noff;
     const int dx = o % nside - W, dy = (o / nside) % nside - W, dz = o / (nside * nside) - W;
     const double h = 0.5 * root[3] / (double)(1 << l);
     const double R[3] = {2.0 * dx, 2.0 * dy, 2.0 * dz};
     const double s = R[0] * R[0] + R[1] * R[1] + R[2] * R[2] + eps2 / (h * h);
-    double Td[kMaxNc];
-    Td[0] = 1.0 / sqrt(s);
+    if (threadIdx.x == 0) Td[0] = 1.0 / sqrt(s);
+    __syncthreads();
     // mi is ordered by degree, and within a degree by (a desc, b desc); index of
     // (a,b,c) with n=a+b+c is n(n+1)(n+2)/6 + (n-a)(n-a+1)/2 + (n-a-b).
-    for (int k = 1; k < nc; k++) {
-        const int4 g = mi[k];
-        const int n = g.w;
-        double acc = 0.0;
-        const int gg[3] = {g.x, g.y, g.z};
-        for (int i = 0; i < 3; i++) {
-            for (int dec = 1; dec <= 2; dec++) {
-                if (gg[i] < dec) continue;
-                int hh[3] = {gg[0], gg[1], gg[2]};
-                hh[i] -= dec;
-                const int m = hh[0] + hh[1] + hh[2];
-                const int id = m * (m + 1) * (m + 2) / 6 + (m - hh[0]) * (m - hh[0] + 1) / 2 +
-                               (m - hh[0] - hh[1]);
-                acc += dec == 1 ? (2 * n - 1) * R[i] * Td[id] : (n - 1) * Td[id];
+    for (int n = 1; n <= p; n++) {
+        for (int k = dstart(n) + (int)threadIdx.x; k < dstart(n + 1); k += 64) {
+            const int4 g = mi[k];
+            const int gg[3] = {g.x, g.y, g.z};
+            double acc = 0.0;
+            for (int i = 0; i < 3; i++) {
+                for (int dec = 1; dec <= 2; dec++) {
+                    if (gg[i] < dec) continue;
+                    int hh[3] = {gg[0], gg[1], gg[2]};
+                    hh[i] -= dec;
+                    const int m = hh[0] + hh[1] + hh[2];
+                    const int id = m * (m + 1) * (m + 2) / 6 + (m - hh[0]) * (m - hh[0] + 1) / 2 +
+                                   (m - hh[0] - hh[1]);
+                    acc += dec == 1 ? (2 * n - 1) * R[i] * Td[id] : (n - 1) * Td[id];
+                }
             }
+            Td[k] = -acc / (n * s);
         }
-        Td[k] = -acc / (n * s);
+        __syncthreads();
     }
     float* out = T + (size_t)t * nc;
-    for (int k = 0; k < nc; k++) out[k] = (float)Td[k];
+    for (int k = threadIdx.x; k < nc; k += 64) out[k] = (float)Td[k];
 }
 
 // ------------------------------------------------------------------- M2L ----
@@ -1337,6 +1343,7 @@ __global__ __launch_bounds__(256) void k_fmm_level_cost(Level lv, GridView g, Le
 // that the dense leaves of a plume core spread over many waves.
 constexpr int kLeafUnit = 64;
 constexpr int kLeafChunk = 16;
+constexpr int kLeafTile = 256;  // sources staged in LDS per barrier pair
 
 __global__ void k_fmm_unit_count(const int* __restrict__ start, const int* __restrict__ end, int nb,
                                  int* __restrict__ cnt) {
@@ -1366,7 +1373,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
     const int* __restrict__ species, int nc, int p, double K, const int2* __restrict__ units,
     const int* __restrict__ nunits, unsigned long long* __restrict__ stats, double* __restrict__ ax,
     double* __restrict__ ay, double* __restrict__ az) {
-    __shared__ float4 tile[64];
+    __shared__ float4 tile[kLeafTile + 64];  // staged sources, then 64 zero-charge entries
     __shared__ float Lsh[kMaxNc];
     __shared__ int rf[128], rl[128], rpre[129];
     __shared__ double red[3][64];
@@ -1379,6 +1386,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
     if (u >= *nunits) return;
     const int2 unit = units[u];
     const int b = unit.x, lane = threadIdx.x;
+    tile[kLeafTile + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const uint64_t key = lv.key[b];
     double c[3], h;
     box_center(root, L, key, c, h);
@@ -1474,26 +1482,45 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
         // far part in units of 1/h^2 relative to the P2P sum expressed in u units:
         // P2P below runs in u = (x - c)/h, so sum q u/|u|^3 = h^2 sum q d/|d|^3.
         float hx = -gx, hy = -gy, hz = -gz, lx = 0.0f, ly = 0.0f, lz = 0.0f;
-        for (int jb = 0; jb < nsrc; jb += 64) {
+        for (int jb = 0; jb < nsrc; jb += kLeafTile) {
             __syncthreads();
-            const int j = jb + lane;
-            if (j < nsrc) {
-                int lo = 0, hi = nrun - 1;  // run holding source j: last r with rpre[r] <= j
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (rpre[mid] <= j) lo = mid;
+            // each lane stages kLeafTile / 64 consecutive sources: one search
+            // for the run holding the first (the last r with rpre[r] <= j),
+            // then a walk forward over the runs
+            const int j0 = jb + (kLeafTile / 64) * lane;
+            int r = 0;
+            if (j0 < nsrc) {
+                int hi = nrun - 1;
+                while (r < hi) {
+                    const int mid = (r + hi + 1) >> 1;
+                    if (rpre[mid] <= j0) r = mid;
                     else hi = mid - 1;
                 }
-                const double4 rs = sp[rf[lo] + (j - rpre[lo])];
-                tile[lane] = make_float4((float)((rs.x - c[0]) * ih), (float)((rs.y - c[1]) * ih),
-                                         (float)((rs.z - c[2]) * ih), (float)rs.w);
+            }
+#pragma unroll
+            for (int k = 0; k < kLeafTile / 64; k++) {
+                const int j = j0 + k;
+                float4 sv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (j < nsrc) {
+                    while (rpre[r + 1] <= j) r++;
+                    const double4 rs = sp[rf[r] + (j - rpre[r])];
+                    sv = make_float4((float)((rs.x - c[0]) * ih), (float)((rs.y - c[1]) * ih),
+                                     (float)((rs.z - c[2]) * ih), (float)rs.w);
+                }
+                tile[(kLeafTile / 64) * lane + k] = sv;
             }
             __syncthreads();
-            const int cnt = min(64, nsrc - jb);
+            // group g takes the contiguous chunk [g nit, (g + 1) nit) of the
+            // tile; the trip count is wave-uniform and entries past the staged
+            // sources have zero charge
+            const int cnt = min(kLeafTile, nsrc - jb);
+            const int nit = (cnt + G - 1) / G;
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
             if (active) {
-                for (int jj = g; jj < cnt; jj += G) {
-                    const float4 q = tile[jj];
+                const float4* tg = tile + g * nit;
+#pragma unroll 8
+                for (int it = 0; it < nit; it++) {
+                    const float4 q = tg[it];
                     const float dx = ux - q.x, dy = uy - q.y, dz = uz - q.z;
                     const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2f)));
                     const float inv = __builtin_amdgcn_rsqf(r2);
@@ -1871,7 +1898,7 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     const int W = 2 * ring + 1;
     const int nside = 2 * W + 1, noff = nside * nside * nside;
     SHP_WS(ttab, float, h, WS_FMM_T, (size_t)noff * (L + 1) * nc);
-    hipLaunchKernelGGL(k_fmm_ttable, dim3(fdiv_up(noff * (L + 1), 64)), dim3(64), 0, st, order, nc,
+    hipLaunchKernelGGL(k_fmm_ttable, dim3(noff * (L + 1)), dim3(64), 0, st, order, nc,
                        tb->mi, W, 0, L, root, kSoftening, ttab);
     // Unrolled M2L (ring 2, orders with compiled pair lists) needs a dense box
     // grid per level over the occupied cells; very large grids use the
