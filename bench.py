@@ -255,8 +255,8 @@ def run_config(k, args):
 
 def run_config5(eng, args, log):
     """64 cone plumes, N lognormal (median 32768, sigma ln 4 / 2) clipped to
-    [8192, 131072], in one batch: K=200 graph-captured DKD steps (FP32C
-    forces, FP64 state) with checkpoints every 20; the checkpointed discrete
+    [8192, 131072], in one batch: K=200 graph-captured DKD steps of --adj-dt
+    (FP32C forces, FP64 state) with checkpoints every 20; the checkpointed discrete
     adjoint of J = mean sin^2 of the velocity divergence angle with respect to
     charges and initial velocities; central differences of the FP64 rollout
     (h=1e-6) on `--fd-dirs` seeded directions."""
@@ -279,7 +279,7 @@ def run_config5(eng, args, log):
     species = torch.as_tensor(allb["species"], dtype=torch.int32, device=dev)
     q0 = torch.as_tensor(allb["q"], device=dev)
     v0 = [torch.as_tensor(allb[key], device=dev) for key in ("vx", "vy", "vz")]
-    dt, K, C = 1e-12, 200, 20
+    dt, K, C = args.adj_dt, 200, 20
     plan = eng.batch(off, P.SHP_PREC_FP32C)
     plan64 = eng.batch(off, P.SHP_PREC_FP64)
     opts = P.make_step_opts(method=P.SHP_METHOD_DIRECT, precision=P.SHP_PREC_FP32C)
@@ -362,6 +362,7 @@ def run_config5(eng, args, log):
     plan.close()
     plan64.close()
     return {"workload": "64 cone plumes N lognormal in [8192, 131072], K=200 steps, checkpoints every 20",
+            "dt": dt,
             "n_total": n, "sizes_min_max": [int(sizes.min()), int(sizes.max())],
             "pairs_per_step": pairs_step, "tuner_ms": tune_ms, "tuner_choice": chosen,
             "forward_s": fwd_s, "forward_particle_steps_per_s": n * K / fwd_s,
@@ -385,6 +386,8 @@ def main():
     ap.add_argument("--config", type=int, default=0,
                     help="measure BASELINE config 1-5 instead of the headline step (one JSON line)")
     ap.add_argument("--fd-dirs", type=int, default=32, help="config 5: finite-difference directions")
+    ap.add_argument("--adj-dt", type=float, default=1e-14,
+                    help="config 5: time step of the differentiated rollout (s)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy first (reference kernel for --pmc byte counts)")
     args = ap.parse_args()

@@ -9,14 +9,6 @@ constexpr int kThreads = 256;
 constexpr int kTile = 256;
 constexpr int kF32TargetsPerLane = 2;
 
-__device__ __forceinline__ double rsqrt_f64(double x) {
-    // v_rsq_f64 estimate refined by one third-order Newton step:
-    // y <- y (1 + e/2 + 3e^2/8), e = 1 - x y^2.
-    double y = __builtin_amdgcn_rsq(x);
-    double e = fma(-x * y, y, 1.0);
-    return fma(y * e, fma(e, 0.375, 0.5), y);
-}
-
 // ---------------------------------------------------------------- staging ----
 
 __global__ void k_flag_sources_f64(const int* __restrict__ status, int j0, int n, char* flags) {

@@ -1114,13 +1114,13 @@ static std::vector<int> m2l_chunk_bounds(int p3, int p4, int p5, int S, int s4, 
 // far-band offset costs 0.022 ns when its source cell is empty and 0.27 ns
 // more when it holds a box (the pass is bound by the source expansion loads,
 // so the cost scales with the coefficient count), and the upward and downward
-// passes 7 ns; a near-field particle pair costs 0.45 ps and a leaf box 11 ns
-// of leaf-kernel set-up (near-field run list, L2P; within 13%).
+// passes 7 ns; a near-field particle pair costs 0.37 ps and a leaf box 11 ns
+// of leaf-kernel set-up (near-field run list, L2P; within 15%).
 static double fmm_m2l_cost_ms(int nc, int nbox, double far_pairs) {
     const double s = nc / 286.0;
     return s * (2.16e-8 * 875.0 * nbox + 2.7e-7 * far_pairs) + 7.0e-6 * nbox;
 }
-static double fmm_leaf_cost_ms(int nbox, double p2p_pairs) { return 4.5e-10 * p2p_pairs + 1.12e-5 * nbox; }
+static double fmm_leaf_cost_ms(int nbox, double p2p_pairs) { return 3.7e-10 * p2p_pairs + 1.11e-5 * nbox; }
 
 static void m2l_orders(int p, int ring, int& p3, int& p4, int& p5) {
     p3 = p4 = p5 = 0;
@@ -1342,13 +1342,44 @@ __global__ __launch_bounds__(256) void k_fmm_level_cost(Level lv, GridView g, Le
 // Leaf work units: a leaf's targets in runs of kLeafUnit, one wave each, so
 // that the dense leaves of a plume core spread over many waves.
 constexpr int kLeafUnit = 64;
-constexpr int kLeafChunk = 16;
 constexpr int kLeafTile = 256;  // sources staged in LDS per barrier pair
 
-__global__ void k_fmm_unit_count(const int* __restrict__ start, const int* __restrict__ end, int nb,
-                                 int* __restrict__ cnt) {
+// Per leaf: the number of work units and the number of near-field sources.
+__global__ void k_fmm_unit_count(Level lv, GridView gv, int L, int ring, int* __restrict__ cnt,
+                                 int* __restrict__ nsrc) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nb) cnt[b] = max(1, (end[b] - start[b] + kLeafUnit - 1) / kLeafUnit);
+    if (b >= lv.nb) return;
+    cnt[b] = max(1, (lv.end[b] - lv.start[b] + kLeafUnit - 1) / kLeafUnit);
+    const uint64_t key = lv.key[b];
+    const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
+    const int cmax = (1 << L) - 1;
+    int ns = 0;
+    for (int dz = -ring; dz <= ring; dz++)
+        for (int dy = -ring; dy <= ring; dy++)
+            for (int dx = -ring; dx <= ring; dx++) {
+                const int qx = ix + dx, qy = iy + dy, qz = iz + dz;
+                if (qx < 0 || qy < 0 || qz < 0 || qx > cmax || qy > cmax || qz > cmax) continue;
+                const int c = find_cell(lv, gv, qx, qy, qz);
+                if (c >= 0) ns += lv.end[c] - lv.start[c];
+            }
+    nsrc[b] = ns;
+}
+
+// Sort keys of the units: work = targets x near-field sources (at least 1);
+// slots past the unit count get 0 and sort last.
+__global__ void k_fmm_unit_keys(const int2* __restrict__ units, const int* __restrict__ nunits, int umax,
+                                Level lv, const int* __restrict__ nsrc, unsigned* __restrict__ key,
+                                int* __restrict__ val) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= umax) return;
+    unsigned w = 0;
+    if (u < *nunits) {
+        const int2 un = units[u];
+        const int nt = min(kLeafUnit, lv.end[un.x] - lv.start[un.x] - kLeafUnit * un.y);
+        w = (unsigned)nt * (unsigned)max(1, nsrc[un.x]);
+    }
+    key[u] = w;
+    val[u] = u;
 }
 
 __global__ void k_fmm_unit_fill(const int* __restrict__ cnt, const int* __restrict__ offs, int nb,
@@ -1357,6 +1388,94 @@ __global__ void k_fmm_unit_fill(const int* __restrict__ cnt, const int* __restri
     if (b >= nb) return;
     for (int k = 0; k < cnt[b]; k++) units[offs[b] + k] = make_int2(b, k);
     if (b == nb - 1) *nunits = offs[b] + cnt[b];
+}
+
+// L2P share of lane group g of G: the gradient of sum_b L_b u^b over the
+// multi-indices with b_x = g mod G (u in units of the leaf half-width).
+__device__ __forceinline__ void leaf_l2p(const float* __restrict__ Lsh, int p, int g, int G, float ux,
+                                         float uy, float uz, float& gx, float& gy, float& gz) {
+    gx = gy = gz = 0.0f;
+    for (int a = g; a <= p; a += G) {
+        float xa = 1.0f, xam1 = 0.0f;
+        for (int k = 0; k < a; k++) {
+            xam1 = xa;
+            xa *= ux;
+        }
+        float yb = 1.0f, ybm1 = 0.0f;
+        for (int bb = 0; a + bb <= p; bb++) {
+            float zc = 1.0f, zcm1 = 0.0f;
+            for (int cc = 0; a + bb + cc <= p; cc++) {
+                const float Lk = Lsh[cidx(a, bb, cc)];
+                gx = fmaf(Lk * (float)a, xam1 * yb * zc, gx);
+                gy = fmaf(Lk * (float)bb, xa * ybm1 * zc, gy);
+                gz = fmaf(Lk * (float)cc, xa * yb * zcm1, gz);
+                zcm1 = zc;
+                zc *= uz;
+            }
+            ybm1 = yb;
+            yb *= uy;
+        }
+    }
+}
+
+// One softened pair in leaf units: s += q.w (u - q.xyz) / (|u - q.xyz|^2 + eps2)^(3/2).
+__device__ __forceinline__ void leaf_pair(float ux, float uy, float uz, const float4 q, float eps2f,
+                                          float& sx, float& sy, float& sz) {
+    const float dx = ux - q.x, dy = uy - q.y, dz = uz - q.z;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2f)));
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float w = (q.w * inv) * (inv * inv);
+    sx = fmaf(dx, w, sx);
+    sy = fmaf(dy, w, sy);
+    sz = fmaf(dz, w, sz);
+}
+
+// Stages the near-field sources [jb, jb + kLeafTile) relative to the leaf
+// centre in units of the half-width (zero charge past nsrc).  Each lane takes
+// kLeafTile / 64 consecutive sources: one search for the run holding the
+// first (the last r with rpre[r] <= j), then a walk forward over the runs.
+__device__ __forceinline__ void leaf_stage(float4* tile, int jb, int lane, int nsrc, int nrun,
+                                           const int* rpre, const int* rf, const double4* __restrict__ sp,
+                                           const double c[3], double ih) {
+    __syncthreads();
+    const int j0 = jb + (kLeafTile / 64) * lane;
+    int r = 0;
+    if (j0 < nsrc) {
+        int hi = nrun - 1;
+        while (r < hi) {
+            const int mid = (r + hi + 1) >> 1;
+            if (rpre[mid] <= j0) r = mid;
+            else hi = mid - 1;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kLeafTile / 64; k++) {
+        const int j = j0 + k;
+        float4 sv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (j < nsrc) {
+            while (rpre[r + 1] <= j) r++;
+            const double4 rs = sp[rf[r] + (j - rpre[r])];
+            sv = make_float4((float)((rs.x - c[0]) * ih), (float)((rs.y - c[1]) * ih),
+                             (float)((rs.z - c[2]) * ih), (float)rs.w);
+        }
+        tile[(kLeafTile / 64) * lane + k] = sv;
+    }
+    __syncthreads();
+}
+
+// Acceleration of sorted target i from its sum in leaf units (status 1 only).
+__device__ __forceinline__ void leaf_write(int i, double qi, double sx, double sy, double sz,
+                                           const int* __restrict__ perm, const int* __restrict__ status,
+                                           const int* __restrict__ species, double K, double h,
+                                           double* __restrict__ ax, double* __restrict__ ay,
+                                           double* __restrict__ az) {
+    const int oi = perm[i];
+    if (status[oi] == 1) {
+        const double kc = K * qi / species_mass(species[oi]) / (h * h);
+        ax[oi] = sx * kc;
+        ay[oi] = sy * kc;
+        az[oi] = sz * kc;
+    }
 }
 
 // One wave per leaf work unit.  The near-field sources (particles of the
@@ -1371,20 +1490,17 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
     Level lv, int L, int ring, GridView gv, const double* __restrict__ root,
     const double4* __restrict__ sp, const int* __restrict__ perm, const int* __restrict__ status,
     const int* __restrict__ species, int nc, int p, double K, const int2* __restrict__ units,
-    const int* __restrict__ nunits, unsigned long long* __restrict__ stats, double* __restrict__ ax,
+    const int* __restrict__ uorder, const int* __restrict__ nunits, unsigned long long* __restrict__ stats, double* __restrict__ ax,
     double* __restrict__ ay, double* __restrict__ az) {
     __shared__ float4 tile[kLeafTile + 64];  // staged sources, then 64 zero-charge entries
     __shared__ float Lsh[kMaxNc];
     __shared__ int rf[128], rl[128], rpre[129];
     __shared__ double red[3][64];
-    // Workgroups are dealt round-robin to the 8 XCDs: renumber so that each
-    // XCD runs chunks of kLeafChunk consecutive (Morton-ordered) units, whose
-    // near fields overlap in that XCD's L2, with the chunks dealt round-robin
-    // so that a dense region spreads over every XCD.
-    const int w = blockIdx.x, j = w >> 3;
-    const int u = (8 * (j / kLeafChunk) + (w & 7)) * kLeafChunk + j % kLeafChunk;
-    if (u >= *nunits) return;
-    const int2 unit = units[u];
+    // Units run longest first (uorder sorts them by targets x near-field
+    // sources, descending), so the dense units of a plume core do not finish
+    // last.
+    if ((int)blockIdx.x >= *nunits) return;
+    const int2 unit = units[uorder[blockIdx.x]];
     const int b = unit.x, lane = threadIdx.x;
     tile[kLeafTile + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const uint64_t key = lv.key[b];
@@ -1440,10 +1556,10 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
     const int nt = s1 - s0;
     if (lane == 0) atomicAdd(stats + 1, (unsigned long long)nsrc * (unsigned long long)nt);
     for (int tb = s0; tb < s1;) {
+        const int rem = s1 - tb;
         // T targets x G source groups per block: 64 targets while more than 53
         // remain, then 32 while more than 32 remain (two source groups), then
         // all that remain (64 / T groups), so every block keeps most lanes busy
-        const int rem = s1 - tb;
         const int T = rem >= 54 ? 64 : (rem > 32 ? 32 : rem), G = 64 / T;
         const int g = lane / T, tl = lane - g * T;
         const bool active = g < G;
@@ -1456,60 +1572,12 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
                     uz = (float)((r.z - c[2]) * ih);
         // L2P share of this group: gradient of sum L_b u^b over b_x = g mod G
         float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-        if (active) {
-            for (int a = g; a <= p; a += G) {
-                float xa = 1.0f, xam1 = 0.0f;
-                for (int k = 0; k < a; k++) {
-                    xam1 = xa;
-                    xa *= ux;
-                }
-                float yb = 1.0f, ybm1 = 0.0f;
-                for (int bb = 0; a + bb <= p; bb++) {
-                    float zc = 1.0f, zcm1 = 0.0f;
-                    for (int cc = 0; a + bb + cc <= p; cc++) {
-                        const float Lk = Lsh[cidx(a, bb, cc)];
-                        gx = fmaf(Lk * (float)a, xam1 * yb * zc, gx);
-                        gy = fmaf(Lk * (float)bb, xa * ybm1 * zc, gy);
-                        gz = fmaf(Lk * (float)cc, xa * yb * zcm1, gz);
-                        zcm1 = zc;
-                        zc *= uz;
-                    }
-                    ybm1 = yb;
-                    yb *= uy;
-                }
-            }
-        }
+        if (active) leaf_l2p(Lsh, p, g, G, ux, uy, uz, gx, gy, gz);
         // far part in units of 1/h^2 relative to the P2P sum expressed in u units:
         // P2P below runs in u = (x - c)/h, so sum q u/|u|^3 = h^2 sum q d/|d|^3.
         float hx = -gx, hy = -gy, hz = -gz, lx = 0.0f, ly = 0.0f, lz = 0.0f;
         for (int jb = 0; jb < nsrc; jb += kLeafTile) {
-            __syncthreads();
-            // each lane stages kLeafTile / 64 consecutive sources: one search
-            // for the run holding the first (the last r with rpre[r] <= j),
-            // then a walk forward over the runs
-            const int j0 = jb + (kLeafTile / 64) * lane;
-            int r = 0;
-            if (j0 < nsrc) {
-                int hi = nrun - 1;
-                while (r < hi) {
-                    const int mid = (r + hi + 1) >> 1;
-                    if (rpre[mid] <= j0) r = mid;
-                    else hi = mid - 1;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kLeafTile / 64; k++) {
-                const int j = j0 + k;
-                float4 sv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (j < nsrc) {
-                    while (rpre[r + 1] <= j) r++;
-                    const double4 rs = sp[rf[r] + (j - rpre[r])];
-                    sv = make_float4((float)((rs.x - c[0]) * ih), (float)((rs.y - c[1]) * ih),
-                                     (float)((rs.z - c[2]) * ih), (float)rs.w);
-                }
-                tile[(kLeafTile / 64) * lane + k] = sv;
-            }
-            __syncthreads();
+            leaf_stage(tile, jb, lane, nsrc, nrun, rpre, rf, sp, c, ih);
             // group g takes the contiguous chunk [g nit, (g + 1) nit) of the
             // tile; the trip count is wave-uniform and entries past the staged
             // sources have zero charge
@@ -1519,16 +1587,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
             if (active) {
                 const float4* tg = tile + g * nit;
 #pragma unroll 8
-                for (int it = 0; it < nit; it++) {
-                    const float4 q = tg[it];
-                    const float dx = ux - q.x, dy = uy - q.y, dz = uz - q.z;
-                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2f)));
-                    const float inv = __builtin_amdgcn_rsqf(r2);
-                    const float w = (q.w * inv) * (inv * inv);
-                    sx = fmaf(dx, w, sx);
-                    sy = fmaf(dy, w, sy);
-                    sz = fmaf(dz, w, sz);
-                }
+                for (int it = 0; it < nit; it++) leaf_pair(ux, uy, uz, tg[it], eps2f, sx, sy, sz);
             }
             fmm_two_sum(hx, lx, sx);
             fmm_two_sum(hy, ly, sy);
@@ -1545,13 +1604,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
                 syd += red[1][gg * T + tl];
                 szd += red[2][gg * T + tl];
             }
-            const int oi = perm[i];
-            if (status[oi] == 1) {
-                const double kc = K * r.w / species_mass(species[oi]) / (h * h);
-                ax[oi] = sxd * kc;
-                ay[oi] = syd * kc;
-                az[oi] = szd * kc;
-            }
+            leaf_write(i, r.w, sxd, syd, szd, perm, status, species, K, h, ax, ay, az);
         }
     }
 }
@@ -1868,22 +1921,35 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ax, n);
     hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ay, n);
     hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, az, n);
-    // leaf work units (at most nbox + na / kLeafUnit of them)
+    // leaf work units (at most nbox + na / kLeafUnit of them), longest first
     const int umax = nbox[L] + na / kLeafUnit + 1;
-    SHP_WS(ucnt, int, h, WS_FMM_UCNT, 2 * (size_t)nbox[L] + 1);
+    SHP_WS(ucnt, int, h, WS_FMM_UCNT, 3 * (size_t)nbox[L] + 1);
     SHP_WS(units, int2, h, WS_FMM_UNITS, (size_t)umax);
-    int* const nunits = ucnt + 2 * nbox[L];
+    SHP_WS(usort, int, h, WS_FMM_USORT, 4 * (size_t)umax);
+    int* const nunits = ucnt + 3 * nbox[L];
+    int* const uorder = usort + 3 * (size_t)umax;
     {
         const int nbl = nbox[L];
         int* uoffs = ucnt + nbl;
-        hipLaunchKernelGGL(k_fmm_unit_count, dim3(fdiv_up(nbl, 256)), dim3(256), 0, st, lv[L].start, lv[L].end,
-                           nbl, ucnt);
+        int* nsrcb = ucnt + 2 * nbl;
+        hipLaunchKernelGGL(k_fmm_unit_count, dim3(fdiv_up(nbl, 256)), dim3(256), 0, st, lv[L], gview(L), L, ring,
+                           ucnt, nsrcb);
         size_t tsz = 0;
         SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tsz, ucnt, uoffs, nbl, st));
         SHP_WS(cubu, char, h, WS_CUB, tsz);
         SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cubu, tsz, ucnt, uoffs, nbl, st));
         hipLaunchKernelGGL(k_fmm_unit_fill, dim3(fdiv_up(nbl, 256)), dim3(256), 0, st, ucnt, uoffs, nbl, units,
                            nunits);
+        unsigned* ukey = (unsigned*)usort;
+        int* uval = usort + 2 * (size_t)umax;
+        hipLaunchKernelGGL(k_fmm_unit_keys, dim3(fdiv_up(umax, 256)), dim3(256), 0, st, units, nunits, umax, lv[L],
+                           nsrcb, ukey, uval);
+        size_t ssz = 0;
+        SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, ssz, ukey, ukey + umax, uval, uorder,
+                                                                   umax, 0, 32, st));
+        SHP_WS(cubs, char, h, WS_CUB, ssz);
+        SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(cubs, ssz, ukey, ukey + umax, uval, uorder,
+                                                                   umax, 0, 32, st));
     }
     // upward pass
     SHP_HIP_CHECK(hipEventRecord(h->ev0, st));
@@ -2002,10 +2068,9 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
         hipLaunchKernelGGL(k_fmm_l2l, dim3(nbox[l]), dim3(kShiftLanes), 9 * nc * sizeof(float), st,
                            lv[l], lv[l + 1], nc, tb->mi, tb->l2l_term, tb->l2l_nt);
     SHP_HIP_CHECK(hipEventRecord(h->fev[4], st));
-    hipLaunchKernelGGL(k_fmm_leaf, dim3(8 * fdiv_up(umax + 8 * kLeafChunk, 8)), dim3(64), 0, st, lv[L], L, ring,
-                       leafgrid, root, sp, perm,
-                       pt->status, pt->species, nc, order, k_const_host(), units, nunits, h->fmm_stats, ax, ay,
-                       az);
+    hipLaunchKernelGGL(k_fmm_leaf, dim3(umax), dim3(64), 0, st, lv[L], L, ring, leafgrid, root, sp, perm,
+                       pt->status, pt->species, nc, order, k_const_host(), units, uorder, nunits, h->fmm_stats,
+                       ax, ay, az);
     SHP_HIP_CHECK(hipGetLastError());
     SHP_HIP_CHECK(hipEventRecord(h->fev[5], st));
     SHP_HIP_CHECK(hipEventRecord(h->ev1, st));

@@ -193,10 +193,11 @@ __global__ __launch_bounds__(kEnergyThreads) void k_energy_pe(
                                                    : make_double4(0.0, 0.0, 0.0, 0.0);
         __syncthreads();
         const int cnt = min(kEnergyThreads, n - j0);
+#pragma unroll 4
         for (int k = 0; k < cnt; k++) {
             const double4 p = tile[k];
             const double dx = xi - p.x, dy = yi - p.y, dz = zi - p.z;
-            const double w = p.w / sqrt(dx * dx + dy * dy + dz * dz + eps2);
+            const double w = p.w * rsqrt_f64(fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2))));
             a += j0 + k == i ? 0.0 : w;
         }
     }

@@ -89,6 +89,7 @@ enum WsSlot {
     WS_FMM_LCOST,   // FMM leaf-level work estimates
     WS_FMM_UCNT,    // FMM leaf work-unit counts and offsets
     WS_FMM_UNITS,   // FMM leaf work units (leaf, target run)
+    WS_FMM_USORT,   // FMM leaf work units: sort keys, values and the longest-first order
     WS_CMP,         // accuracy metric: per-block partials
     WS_NSLOTS
 };
@@ -113,6 +114,16 @@ struct FieldMesh {
     double root[3] = {0, 0, 0};  // x0, y0, side
     double r0 = 0, E0 = 0;
 };
+}  // namespace shp
+
+namespace shp {
+__device__ __forceinline__ double rsqrt_f64(double x) {
+    // v_rsq_f64 estimate refined by one third-order Newton step:
+    // y <- y (1 + e/2 + 3e^2/8), e = 1 - x y^2.
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
 }  // namespace shp
 
 struct shp_context {
