@@ -1344,6 +1344,22 @@ __global__ __launch_bounds__(256) void k_fmm_level_cost(Level lv, GridView g, Le
 constexpr int kLeafUnit = 64;
 constexpr int kLeafTile = 256;  // sources staged in LDS per barrier pair
 
+// Sorted ion i relative to the centre of its own leaf in units of the leaf
+// half-width h (FP32), with its charge.  A source j of leaf S seen from leaf T
+// is then spf[j] + 2 (cell(S) - cell(T)): the shift is a small integer, exact
+// in FP32, so staging needs no FP64 arithmetic.
+__global__ void k_fmm_leaf_rel(int na, int L, const uint64_t* __restrict__ keys, const double* __restrict__ root,
+                               const double4* __restrict__ sp, float4* __restrict__ spf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= na) return;
+    double c[3], h;
+    box_center(root, L, keys[i] >> (3 * (kMaxLevel - L)), c, h);
+    const double ih = 1.0 / h;
+    const double4 r = sp[i];
+    spf[i] = make_float4((float)((r.x - c[0]) * ih), (float)((r.y - c[1]) * ih), (float)((r.z - c[2]) * ih),
+                         (float)r.w);
+}
+
 // Per leaf: the number of work units and the number of near-field sources.
 __global__ void k_fmm_unit_count(Level lv, GridView gv, int L, int ring, int* __restrict__ cnt,
                                  int* __restrict__ nsrc) {
@@ -1431,12 +1447,15 @@ __device__ __forceinline__ void leaf_pair(float ux, float uy, float uz, const fl
 }
 
 // Stages the near-field sources [jb, jb + kLeafTile) relative to the leaf
-// centre in units of the half-width (zero charge past nsrc).  Each lane takes
+// centre in units of the half-width (zero charge past nsrc): each source's
+// offset from its own leaf centre plus twice the cell offset of its leaf
+// (run t of the candidate list).  Each lane takes
 // kLeafTile / 64 consecutive sources: one search for the run holding the
 // first (the last r with rpre[r] <= j), then a walk forward over the runs.
 __device__ __forceinline__ void leaf_stage(float4* tile, int jb, int lane, int nsrc, int nrun,
-                                           const int* rpre, const int* rf, const double4* __restrict__ sp,
-                                           const double c[3], double ih) {
+                                           const int* rpre, const int* rf, const unsigned char* rt, int ring,
+                                           const float4* __restrict__ spf) {
+    const int nside = 2 * ring + 1;
     __syncthreads();
     const int j0 = jb + (kLeafTile / 64) * lane;
     int r = 0;
@@ -1454,9 +1473,10 @@ __device__ __forceinline__ void leaf_stage(float4* tile, int jb, int lane, int n
         float4 sv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (j < nsrc) {
             while (rpre[r + 1] <= j) r++;
-            const double4 rs = sp[rf[r] + (j - rpre[r])];
-            sv = make_float4((float)((rs.x - c[0]) * ih), (float)((rs.y - c[1]) * ih),
-                             (float)((rs.z - c[2]) * ih), (float)rs.w);
+            const float4 rs = spf[rf[r] + (j - rpre[r])];
+            const int t = rt[r];  // candidate cell of the run: offset (t % n, t / n % n, t / n^2) - ring
+            sv = make_float4(rs.x + (float)(2 * (t % nside - ring)), rs.y + (float)(2 * ((t / nside) % nside - ring)),
+                             rs.z + (float)(2 * (t / (nside * nside) - ring)), rs.w);
         }
         tile[(kLeafTile / 64) * lane + k] = sv;
     }
@@ -1489,13 +1509,14 @@ __device__ __forceinline__ void leaf_write(int i, double qi, double sx, double s
 __global__ __launch_bounds__(64) void k_fmm_leaf(
     Level lv, int L, int ring, GridView gv, const double* __restrict__ root,
     const double4* __restrict__ sp, const int* __restrict__ perm, const int* __restrict__ status,
-    const int* __restrict__ species, int nc, int p, double K, const int2* __restrict__ units,
-    const int* __restrict__ uorder, const int* __restrict__ nunits, unsigned long long* __restrict__ stats, double* __restrict__ ax,
+    const int* __restrict__ species, const float4* __restrict__ spf, int nc, int p, double K,
+    const int2* __restrict__ units, const int* __restrict__ uorder, const int* __restrict__ nunits, unsigned long long* __restrict__ stats, double* __restrict__ ax,
     double* __restrict__ ay, double* __restrict__ az) {
     __shared__ float4 tile[kLeafTile + 64];  // staged sources, then 64 zero-charge entries
     __shared__ float Lsh[kMaxNc];
     __shared__ int rf[128], rl[128], rpre[129];
-    __shared__ double red[3][64];
+    __shared__ unsigned char rt[128];
+    double(*red)[64] = reinterpret_cast<double(*)[64]>(tile);  // group partials, after a block's tiles
     // Units run longest first (uorder sorts them by targets x near-field
     // sources, descending), so the dense units of a plume core do not finish
     // last.
@@ -1534,6 +1555,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
             const int pos = nrun + __popcll(m & ((1ull << lane) - 1ull));
             rf[pos] = f;
             rl[pos] = e - f;
+            rt[pos] = (unsigned char)t;
         }
         nrun += __popcll(m);
     }
@@ -1567,9 +1589,8 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
         const int tend = min(tb + T, s1);
         tb += T;
         const bool valid = active && i < tend;
-        const double4 r = sp[valid ? i : s0];
-        const float ux = (float)((r.x - c[0]) * ih), uy = (float)((r.y - c[1]) * ih),
-                    uz = (float)((r.z - c[2]) * ih);
+        const float4 rv = spf[valid ? i : s0];
+        const float ux = rv.x, uy = rv.y, uz = rv.z;
         // L2P share of this group: gradient of sum L_b u^b over b_x = g mod G
         float gx = 0.0f, gy = 0.0f, gz = 0.0f;
         if (active) leaf_l2p(Lsh, p, g, G, ux, uy, uz, gx, gy, gz);
@@ -1577,7 +1598,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
         // P2P below runs in u = (x - c)/h, so sum q u/|u|^3 = h^2 sum q d/|d|^3.
         float hx = -gx, hy = -gy, hz = -gz, lx = 0.0f, ly = 0.0f, lz = 0.0f;
         for (int jb = 0; jb < nsrc; jb += kLeafTile) {
-            leaf_stage(tile, jb, lane, nsrc, nrun, rpre, rf, sp, c, ih);
+            leaf_stage(tile, jb, lane, nsrc, nrun, rpre, rf, rt, ring, spf);
             // group g takes the contiguous chunk [g nit, (g + 1) nit) of the
             // tile; the trip count is wave-uniform and entries past the staged
             // sources have zero charge
@@ -1593,6 +1614,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
             fmm_two_sum(hy, ly, sy);
             fmm_two_sum(hz, lz, sz);
         }
+        __syncthreads();  // the last tile is read: its space holds the group partials
         red[0][lane] = (double)hx + (double)lx;
         red[1][lane] = (double)hy + (double)ly;
         red[2][lane] = (double)hz + (double)lz;
@@ -1604,7 +1626,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
                 syd += red[1][gg * T + tl];
                 szd += red[2][gg * T + tl];
             }
-            leaf_write(i, r.w, sxd, syd, szd, perm, status, species, K, h, ax, ay, az);
+            leaf_write(i, sp[i].w, sxd, syd, szd, perm, status, species, K, h, ax, ay, az);
         }
     }
 }
@@ -1951,6 +1973,8 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
         SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(cubs, ssz, ukey, ukey + umax, uval, uorder,
                                                                    umax, 0, 32, st));
     }
+    SHP_WS(spf, float4, h, WS_FMM_SPF, na);
+    hipLaunchKernelGGL(k_fmm_leaf_rel, dim3(fdiv_up(na, 256)), dim3(256), 0, st, na, L, keys2, root, sp, spf);
     // upward pass
     SHP_HIP_CHECK(hipEventRecord(h->ev0, st));
     SHP_HIP_CHECK(hipEventRecord(h->fev[0], st));
@@ -2069,7 +2093,8 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
                            lv[l], lv[l + 1], nc, tb->mi, tb->l2l_term, tb->l2l_nt);
     SHP_HIP_CHECK(hipEventRecord(h->fev[4], st));
     hipLaunchKernelGGL(k_fmm_leaf, dim3(umax), dim3(64), 0, st, lv[L], L, ring, leafgrid, root, sp, perm,
-                       pt->status, pt->species, nc, order, k_const_host(), units, uorder, nunits, h->fmm_stats,
+                       pt->status, pt->species, spf, nc, order, k_const_host(), units, uorder, nunits,
+                       h->fmm_stats,
                        ax, ay, az);
     SHP_HIP_CHECK(hipGetLastError());
     SHP_HIP_CHECK(hipEventRecord(h->fev[5], st));
