@@ -1114,13 +1114,13 @@ static std::vector<int> m2l_chunk_bounds(int p3, int p4, int p5, int S, int s4, 
 // far-band offset costs 0.022 ns when its source cell is empty and 0.27 ns
 // more when it holds a box (the pass is bound by the source expansion loads,
 // so the cost scales with the coefficient count), and the upward and downward
-// passes 7 ns; a near-field particle pair costs 0.37 ps and a leaf box 11 ns
-// of leaf-kernel set-up (near-field run list, L2P; within 15%).
+// passes 7 ns; a near-field particle pair costs 0.34 ps and a leaf box 11 ns
+// of leaf-kernel set-up (near-field run list, L2P; within 11%).
 static double fmm_m2l_cost_ms(int nc, int nbox, double far_pairs) {
     const double s = nc / 286.0;
     return s * (2.16e-8 * 875.0 * nbox + 2.7e-7 * far_pairs) + 7.0e-6 * nbox;
 }
-static double fmm_leaf_cost_ms(int nbox, double p2p_pairs) { return 3.7e-10 * p2p_pairs + 1.11e-5 * nbox; }
+static double fmm_leaf_cost_ms(int nbox, double p2p_pairs) { return 3.4e-10 * p2p_pairs + 1.13e-5 * nbox; }
 
 static void m2l_orders(int p, int ring, int& p3, int& p4, int& p5) {
     p3 = p4 = p5 = 0;
