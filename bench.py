@@ -152,6 +152,12 @@ def run_config(k, args):
         res.update(workload="hemisphere 20 nm N=4096, FP64 direct + pwl tip field, 1000 DKD steps dt=1e-12 s",
                    particle_steps_per_s=4096 * nsteps / wall, ms_per_step=wall * 1e3 / nsteps,
                    energy_drift_rel=(e1 - e0) / abs(e0))
+        import oracle
+        t0 = time.perf_counter()
+        oracle.leapfrog_run_f64(d, 0.0, 1e-12, 20, mesh=synth.tip_field_mesh())
+        dtc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": 4096 * 20 / dtc, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle FP64 DKD leapfrog with the pwl field, 20 steps ({dtc:.1f} s)"}
     elif k == 2:
         # FP32C direct at N=65536 and 262144 (cone): one force evaluation (kernel
         # time, pair rate, fraction of the FP32 peak, rel L2 vs the GPU FP64
@@ -190,6 +196,13 @@ def run_config(k, args):
                 drift[name] = (_energy_total(eng, s2, x0, y0, z0, 0) - e0) / abs(e0)
                 fin[name] = torch.stack([s2.x, s2.y, s2.z], 1)
             row["energy_drift_rel"] = drift
+            import oracle
+            t0 = time.perf_counter()
+            oracle.direct_f64(d, i0=0, i1=1024)
+            dtc = time.perf_counter() - t0
+            row["cpu_baseline"] = {"pairs_per_s": 1024 * nsrc / dtc, "cores": 1, "kind": "port",
+                                   "particle_steps_per_s": 1024 * nsrc / dtc / n,
+                                   "sample": f"oracle FP64 direct sum, 1024 targets ({dtc:.1f} s)"}
             row["final_pos_rel_diff_fp32c_vs_fp64"] = float(
                 (fin["fp32c"] - fin["fp64"]).norm() / (fin["fp64"] - torch.stack([
                     torch.as_tensor(d["x"]), torch.as_tensor(d["y"]), torch.as_tensor(d["z"])],
@@ -217,6 +230,30 @@ def run_config(k, args):
             rows.append({"method": "fmm", "order": p, "ms": ms, "leaf_level": eng.fmm_leaf_level(),
                          **_metric(eng, st, a, a64, sub)})
             log(json.dumps(rows[-1]))
+        # CPU legs (one thread, bounded): the oracle's Barnes-Hut (reference
+        # arithmetic) on a whole 65536-ion cone of the same shape, with the GPU
+        # on the same input beside it, and the oracle's FP64 FMM (the engine's
+        # algorithm) on a 131072-ion cone.
+        import oracle
+        dbh = synth.cone(65536)
+        t0 = time.perf_counter()
+        oracle.bh_ref(*synth.to_reference_aos(dbh), theta=0.5)
+        tb = time.perf_counter() - t0
+        stb = P.DeviceState.from_numpy(dbh)
+        msb, _ = _timed(lambda: eng.bh_dev(stb, theta=0.5))
+        dsm = synth.cone(131072)
+        oracle.fmm_set_shell_orders([0, 0, 0, 10, 8, 7])
+        t0 = time.perf_counter()
+        oracle.fmm_f64(dsm, 10, 6, ring=2)
+        tf = time.perf_counter() - t0
+        oracle.fmm_set_shell_orders(None)
+        msf, _ = _timed(lambda: eng.fmm_dev(P.DeviceState.from_numpy(dsm), order=10, leaf_level=6, ring=2))
+        res["cpu_baseline"] = {
+            "bh_theta0.5_s_65536": tb, "gpu_bh_theta0.5_ms_65536": msb,
+            "fmm_p10_s_131072": tf, "gpu_fmm_p10_ms_131072": msf,
+            "cores": 1, "kind": "port",
+            "sample": "oracle Barnes-Hut (reference arithmetic) on a whole 65536-ion cone; oracle FP64 FMM "
+                      "(p 10/8/7, ring 2, leaf level 6) on a 131072-ion cone; the GPU times on the same inputs"}
         res.update(workload="cone plume N=1048576 (one force evaluation each)", runs=rows)
     elif k == 4:
         # N=262144 per cell/sector: translational FMM (3x3x3 shell of 2 um
@@ -246,6 +283,13 @@ def run_config(k, args):
                          "fp32c_direct_ms": ms32, "fmm_speedup_vs_fp64": ms64 / msf,
                          "fmm": _metric(eng, st, af, a64), "fp32c": _metric(eng, st, a32, a64)})
             log(json.dumps(rows[-1]))
+        import oracle
+        t0 = time.perf_counter()
+        oracle.direct_f64(d, i0=0, i1=128, sym="rotational", nsec=6)
+        dtc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"rotational_fp64_direct_s_per_eval_est": dtc / 128 * n, "cores": 1, "kind": "port",
+                               "sample": f"oracle FP64 direct sum over the 6 sectors, 128 targets ({dtc:.1f} s), "
+                                         "scaled to 262144 targets"}
         res.update(workload="FMM with symmetry images, N=262144 per cell/sector", runs=rows)
     elif k == 5:
         res.update(run_config5(eng, args, log))
@@ -361,6 +405,13 @@ def run_config5(eng, args, log):
     fd_s = time.perf_counter() - t0
     plan.close()
     plan64.close()
+    # CPU leg: the oracle's FP64 discrete adjoint (rollout + reverse sweep) of
+    # one 4096-ion cone over 20 steps
+    import oracle
+    d1 = synth.cone(4096, seed=1000)
+    t0 = time.perf_counter()
+    oracle.rollout_grad_f64(d1, dt, 20, lambda x, v: (np.zeros_like(x), np.ones_like(v)))
+    cpu_s = time.perf_counter() - t0
     return {"workload": "64 cone plumes N lognormal in [8192, 131072], K=200 steps, checkpoints every 20",
             "dt": dt,
             "n_total": n, "sizes_min_max": [int(sizes.min()), int(sizes.max())],
@@ -368,7 +419,10 @@ def run_config5(eng, args, log):
             "forward_s": fwd_s, "forward_particle_steps_per_s": n * K / fwd_s,
             "forward_fp32c_tflops_20flop": FLOP_PER_PAIR * pairs_step * K / fwd_s / 1e12,
             "backward_s": bwd_s, "objective": J, "fd_dirs": args.fd_dirs, "fd_s": fd_s,
-            "fd_worst_rel_diff": worst, "fd": fd_rows}
+            "fd_worst_rel_diff": worst, "fd": fd_rows,
+            "cpu_baseline": {"adjoint_particle_steps_per_s": 4096 * 20 / cpu_s, "cores": 1, "kind": "port",
+                             "sample": f"oracle FP64 rollout + discrete adjoint, one 4096-ion cone, 20 steps "
+                                       f"({cpu_s:.1f} s)"}}
 
 
 def main():
