@@ -493,6 +493,7 @@ struct Level {
     const int* cfirst;  // nullptr at the leaf level
     float* M;
     float* Lc;
+    int ld;  // row stride of M and Lc (coefficients padded to whole 128-byte lines)
 };
 
 __device__ __forceinline__ void box_center(const double* __restrict__ root, int l, uint64_t key,
@@ -546,8 +547,8 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_p2m(
             for (int j = 0; j < cnt; j++)
                 acc1 = fmaf(qs[j], pw[j][0][A1.x] * pw[j][1][A1.y] * pw[j][2][A1.z], acc1);
     }
-    if (a0 < nc) lv.M[(size_t)b * nc + a0] = acc0;
-    if (a1 < nc) lv.M[(size_t)b * nc + a1] = acc1;
+    if (a0 < nc) lv.M[(size_t)b * lv.ld + a0] = acc0;
+    if (a1 < nc) lv.M[(size_t)b * lv.ld + a1] = acc1;
 }
 
 // ------------------------------------------------------------------- M2M ----
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(kShiftLanes) void k_fmm_m2m(Level par, Level ch, in
     __syncthreads();
     for (int c = c0; c < c1; c++) {
         const int o = (int)(ch.key[c] & 7);
-        const float* Mc = ch.M + (size_t)c * nc;
+        const float* Mc = ch.M + (size_t)c * ch.ld;
         for (int k = t; k < nc; k += kShiftLanes) W[o * nc + k] = Mc[k];
     }
     __syncthreads();
@@ -592,7 +593,7 @@ __global__ __launch_bounds__(kShiftLanes) void k_fmm_m2m(Level par, Level ch, in
         for (int o = 0; o < 8; o++) W[o * nc + k] = u[o];
     }
     __syncthreads();
-    float* Mp = par.M + (size_t)b * nc;
+    float* Mp = par.M + (size_t)b * par.ld;
     float acc = 0.0f;
     for (int j = 0; j < nt; j++) {
         const int2 e = term[j * kShiftLanes + t];
@@ -735,7 +736,7 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_m2l(
         for (int t = threadIdx.x; t < kM2LBatch * nc; t += blockDim.x) {
             const int s = t / nc, k = t % nc;
             if (s < nb) {
-                Ms[s][k] = tl.M[(size_t)src[base + s] * nc + k];
+                Ms[s][k] = tl.M[(size_t)src[base + s] * tl.ld + k];
                 Ts[s][k] = Ttab[(size_t)srcoff[base + s] * nc + k];
             } else {
                 Ms[s][k] = 0.0f;
@@ -765,7 +766,7 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_m2l(
         const int2 bt = beta_tasks[be];
         float s = 0.0f;
         for (int t = bt.x; t < bt.y; t++) s += part[t];
-        tl.Lc[(size_t)b * nc + be] += s;
+        tl.Lc[(size_t)b * tl.ld + be] += s;
     }
 }
 
@@ -905,7 +906,7 @@ struct M2LLevel {
 // per pass.
 constexpr int kM2LMaxLev = 20;
 struct M2LPlan {
-    int nlev, ntile, nchunk, nc;
+    int nlev, ntile, nchunk, nc, ld;  // ld: row stride of the expansions
     int nol, s4, s5;                  // offsets per octant list; first entries of shells 4 and 5
     const int4* tiles;                // {first sorted box, count, level index, octant}
     const int* sorted;                // box index within its level, grouped by (level, octant)
@@ -1007,7 +1008,7 @@ __global__ __launch_bounds__(64, BLO == 0 ? 4 : (BLO == 7 ? 3 : 1)) void k_fmm_m
                 s = lv.grid[gx + lv.dim[0] * (gy + lv.dim[1] * gz)];
         }
         used[shell - 3] += s >= 0;
-        const float* Ms = s >= 0 ? lv.Mhat + (size_t)s * nc : zeros;
+        const float* Ms = s >= 0 ? lv.Mhat + (size_t)s * pl.ld : zeros;
         const float* D = lv.D + (size_t)o * nc;
         if (shell == 3) {
             if constexpr (w3) m2l_apply<P3, BLO, BHI>(acc, Ms, D);
@@ -1019,12 +1020,12 @@ __global__ __launch_bounds__(64, BLO == 0 ? 4 : (BLO == 7 ? 3 : 1)) void k_fmm_m
     }
     if (valid) {
         if (pl.nchunk == 1) {
-            float* Lb = lv.Lc + (size_t)b * nc + dstart(BLO);
+            float* Lb = lv.Lc + (size_t)b * pl.ld + dstart(BLO);
             const float* sc = lscale + dstart(BLO);
 #pragma unroll
             for (int k = 0; k < NB; k++) Lb[k] += acc[k] * sc[k];
         } else {
-            float* Pb = part + chunk * part_stride + (size_t)(lv.gbase + b) * nc + dstart(BLO);
+            float* Pb = part + chunk * part_stride + (size_t)(lv.gbase + b) * pl.ld + dstart(BLO);
 #pragma unroll
             for (int k = 0; k < NB; k++) Pb[k] = acc[k];
         }
@@ -1042,14 +1043,17 @@ __global__ __launch_bounds__(64, BLO == 0 ? 4 : (BLO == 7 ? 3 : 1)) void k_fmm_m
     }
 }
 
-// L[box][k] += lscale[k] * sum over chunks of the partial sums (fixed order).
-__global__ void k_fmm_m2l_reduce(const float* __restrict__ part, size_t stride, int nchunk, int nc,
+// L[box][k] += lscale[k] * sum over chunks of the partial sums (fixed order);
+// rows of stride ld, the padding past nc left alone.
+__global__ void k_fmm_m2l_reduce(const float* __restrict__ part, size_t stride, int nchunk, int nc, int ld,
                                  const float* __restrict__ lscale, float* __restrict__ Lc, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int k = (int)(i % ld);
+    if (k >= nc) return;
     float s = 0.0f;
     for (int c = 0; c < nchunk; c++) s += part[c * stride + i];
-    Lc[i] += s * lscale[i % nc];
+    Lc[i] += s * lscale[k];
 }
 
 // Offsets of shells 3..5 (ring 2) sorted by shell; chunk bounds per pass such
@@ -1186,9 +1190,12 @@ static int m2l_bounds(FmmDevTables* tb, int p3, int p4, int p5, int S, const int
     return SHP_OK;
 }
 
-__global__ void k_fmm_scale(float* __restrict__ v, const float* __restrict__ sc, int nc, size_t n) {
+// v[row][k] *= sc[k] over rows of stride ld (padding past nc left alone).
+__global__ void k_fmm_scale(float* __restrict__ v, const float* __restrict__ sc, int nc, int ld, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) v[i] *= sc[i % nc];
+    if (i >= n) return;
+    const int k = (int)(i % ld);
+    if (k < nc) v[i] *= sc[k];
 }
 
 __global__ void k_fmm_grid_scatter(const uint64_t* __restrict__ key, int nb, int lo0, int lo1, int lo2,
@@ -1218,7 +1225,7 @@ __global__ __launch_bounds__(kShiftLanes) void k_fmm_l2l(Level par, Level ch, in
     const int b = blockIdx.x, t = threadIdx.x;
     const int c0 = par.cfirst[b];
     const int c1 = b + 1 < par.nb ? par.cfirst[b + 1] : ch.nb;
-    const float* Lsrc = par.Lc + (size_t)b * nc;
+    const float* Lsrc = par.Lc + (size_t)b * par.ld;
     for (int k = t; k < nc; k += kShiftLanes) Lp[k] = Lsrc[k];
     for (int i = t; i < 8 * nc; i += kShiftLanes) Q[i] = 0.0f;
     __syncthreads();
@@ -1255,7 +1262,7 @@ __global__ __launch_bounds__(kShiftLanes) void k_fmm_l2l(Level par, Level ch, in
     __syncthreads();
     for (int c = c0; c < c1; c++) {
         const int o = (int)(ch.key[c] & 7);
-        float* Lc = ch.Lc + (size_t)c * nc;
+        float* Lc = ch.Lc + (size_t)c * ch.ld;
         for (int k = t; k < nc; k += kShiftLanes) Lc[k] += Q[o * nc + k];
     }
 }
@@ -1392,7 +1399,7 @@ __global__ void k_fmm_unit_keys(const int2* __restrict__ units, const int* __res
     if (u < *nunits) {
         const int2 un = units[u];
         const int nt = min(kLeafUnit, lv.end[un.x] - lv.start[un.x] - kLeafUnit * un.y);
-        w = (unsigned)nt * (unsigned)max(1, nsrc[un.x]);
+        w = (unsigned)nt * (unsigned)min(max(1, nsrc[un.x]), 1 << 25);  // < 2^31
     }
     key[u] = w;
     val[u] = u;
@@ -1529,7 +1536,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
     box_center(root, L, key, c, h);
     const double ih = 1.0 / h;
     const float eps2f = (float)(kSoftening / (h * h));
-    for (int k = lane; k < nc; k += 64) Lsh[k] = lv.Lc[(size_t)b * nc + k];
+    for (int k = lane; k < nc; k += 64) Lsh[k] = lv.Lc[(size_t)b * lv.ld + k];
     const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
     const int nside = 2 * ring + 1, ncand = nside * nside * nside;
     const int cmax = (1 << L) - 1;
@@ -1927,18 +1934,20 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     // expansions M, L per box of levels 0..L
     size_t tot = 0;
     for (int l = 0; l <= L; l++) tot += nbox[l];
-    SHP_WS(bexp, float, h, WS_FMM_EXP, 2 * tot * (size_t)nc);
+    const int ld = (nc + 31) & ~31;  // expansion rows padded to whole 128-byte lines
+    SHP_WS(bexp, float, h, WS_FMM_EXP, 2 * tot * (size_t)ld);
     {
         size_t off = 0;
         for (int l = 0; l <= L; l++) {
-            lv[l].M = bexp + off * nc;
-            lv[l].Lc = bexp + (tot + off) * nc;
+            lv[l].M = bexp + off * ld;
+            lv[l].Lc = bexp + (tot + off) * ld;
+            lv[l].ld = ld;
             off += nbox[l];
         }
     }
     SHP_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_fmm_zero, dim3(fdiv_up(tot * nc, 256)), dim3(256), 0, st, bexp + tot * nc,
-                       tot * nc);
+    hipLaunchKernelGGL(k_fmm_zero, dim3(fdiv_up(tot * ld, 256)), dim3(256), 0, st, bexp + tot * ld,
+                       tot * ld);
     hipLaunchKernelGGL(k_fmm_u64fill, dim3(1), dim3(64), 0, st, h->fmm_stats, 8);
     hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ax, n);
     hipLaunchKernelGGL(k_fmm_fill, dim3(fdiv_up(n, 256)), dim3(256), 0, st, ay, n);
@@ -1996,13 +2005,14 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
     const GridView leafgrid = gview(L);  // leaf-level cell grid for the near-field lookups
     SHP_HIP_CHECK(hipEventRecord(h->fev[2], st));
     if (unrolled) {
-        hipLaunchKernelGGL(k_fmm_scale, dim3(fdiv_up((long)tot * nc, 256)), dim3(256), 0, st, bexp, tb->mscale,
-                           nc, tot * (size_t)nc);
+        hipLaunchKernelGGL(k_fmm_scale, dim3(fdiv_up((long)tot * ld, 256)), dim3(256), 0, st, bexp, tb->mscale,
+                           nc, ld, tot * (size_t)ld);
         hipLaunchKernelGGL(k_fmm_scale, dim3(fdiv_up((long)noff * (L + 1) * nc, 256)), dim3(256), 0, st, ttab,
-                           tb->dscale, nc, (size_t)noff * (L + 1) * nc);
+                           tb->dscale, nc, nc, (size_t)noff * (L + 1) * nc);
         if (L >= 2) {
             M2LPlan pl{};
             pl.nc = nc;
+            pl.ld = ld;
             size_t gbase = 0;
             for (int l = 0; l < 2; l++) gbase += nbox[l];
             const size_t first = gbase;
@@ -2055,7 +2065,7 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
             // sums of the offset chunks take S x (boxes of levels >= 2) x nc floats.
             const size_t nrow = gbase - first;
             int S = std::max(1, std::min(32, fdiv_up(8 * 4 * h->num_cus, std::max(1, pl.ntile))));
-            while (S > 1 && (size_t)S * nrow * nc * sizeof(float) > ((size_t)4 << 30)) S--;
+            while (S > 1 && (size_t)S * nrow * ld * sizeof(float) > ((size_t)4 << 30)) S--;
             pl.nchunk = S;
             const int* cb = nullptr;
             SHP_REQUIRE(m2l_bounds(tb, p3, p4, p5, S, &cb) == SHP_OK, "shp_fmm_dev: M2L chunk table");
@@ -2063,7 +2073,7 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
             pl.s4 = tb->olist_s4;
             pl.s5 = tb->olist_s5;
             float* part = nullptr;
-            const size_t stride = nrow * nc;
+            const size_t stride = nrow * ld;
             if (S > 1) {
                 SHP_WS(pw, float, h, WS_FMM_PART, (size_t)S * stride);
                 part = pw;
@@ -2071,7 +2081,7 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
             launch_m2l_tiled(pl, p3, tb, cb, part, stride, h->fmm_stats, st, h->fev[6], h->fev[7]);
             if (S > 1)
                 hipLaunchKernelGGL(k_fmm_m2l_reduce, dim3(fdiv_up((long)stride, 256)), dim3(256), 0, st,
-                                   part, stride, S, nc, tb->lscale, lv[2].Lc, stride);
+                                   part, stride, S, nc, ld, tb->lscale, lv[2].Lc, stride);
         }
         if (L < 2) {
             SHP_HIP_CHECK(hipEventRecord(h->fev[6], st));
