@@ -1303,9 +1303,20 @@ __device__ __forceinline__ int find_cell(const Level& lv, const GridView& gv, in
 // b = occupied children of the parent's near boxes minus the occupied near
 // boxes of b = M2L box pairs.  One pair of atomics per workgroup.
 constexpr int kLevelCostBlocks = 512;
-__global__ __launch_bounds__(256) void k_fmm_level_cost(Level lv, GridView g, Level par, GridView gp,
-                                                        int ring, int stride,
-                                                        unsigned long long* __restrict__ out) {
+constexpr int kLevelCostMax = 4;  // candidate levels counted by one launch (blockIdx.y)
+struct LevelCostArgs {
+    Level lv[kLevelCostMax], par[kLevelCostMax];
+    GridView g[kLevelCostMax], gp[kLevelCostMax];
+    int stride[kLevelCostMax];
+    unsigned long long* out[kLevelCostMax];
+};
+__global__ __launch_bounds__(256) void k_fmm_level_cost(const LevelCostArgs A, int ring) {
+    const Level& lv = A.lv[blockIdx.y];
+    const Level& par = A.par[blockIdx.y];
+    const GridView& g = A.g[blockIdx.y];
+    const GridView& gp = A.gp[blockIdx.y];
+    const int stride = A.stride[blockIdx.y];
+    unsigned long long* out = A.out[blockIdx.y];
     __shared__ long red[2][4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int side = 2 * ring + 1, ncell = side * side * side;
@@ -1896,13 +1907,22 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
         SHP_WS(lcost, unsigned long long, h, WS_FMM_LCOST, 2 * (Lb + 1));
         hipLaunchKernelGGL(k_fmm_u64fill, dim3(1), dim3(64), 0, st, lcost, 2 * (Lb + 1));
         std::vector<int> stride(Lb + 1, 1);
+        LevelCostArgs A{};
+        int ns_max = 1;
         for (int l = L0; l <= Lb; l++) {
+            const int k = l - L0;
             stride[l] = std::max(1, fdiv_up(nbox[l], 1 << 14));
-            const int ns = fdiv_up(nbox[l], stride[l]);
-            hipLaunchKernelGGL(k_fmm_level_cost, dim3(std::min(kLevelCostBlocks, fdiv_up(ns, 4))), dim3(256), 0,
-                               st, lv[l], gview(l),
-                               lv[l - 1], gview(l - 1), ring, stride[l], lcost + 2 * l);
+            ns_max = std::max(ns_max, fdiv_up(nbox[l], stride[l]));
+            A.lv[k] = lv[l];
+            A.par[k] = lv[l - 1];
+            A.g[k] = gview(l);
+            A.gp[k] = gview(l - 1);
+            A.stride[k] = stride[l];
+            A.out[k] = lcost + 2 * l;
         }
+        SHP_REQUIRE(Lb - L0 + 1 <= kLevelCostMax, "shp_fmm_dev: %d candidate levels", Lb - L0 + 1);
+        hipLaunchKernelGGL(k_fmm_level_cost, dim3(std::min(kLevelCostBlocks, fdiv_up(ns_max, 4)), Lb - L0 + 1),
+                           dim3(256), 0, st, A, ring);
         unsigned long long hcost[2 * (kMaxLevel + 1)];
         SHP_HIP_CHECK(hipMemcpyAsync(hcost, lcost, sizeof(unsigned long long) * 2 * (Lb + 1),
                                      hipMemcpyDeviceToHost, st));
