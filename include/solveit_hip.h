@@ -114,7 +114,7 @@ double shp_last_kernel_ms(shp_handle h);
  * M2L, L2L, L2P of fmm_cpukernel.cpp:103-1317).  order: expansion order p
  * (1..12); leaf_level: uniform leaf level of the octree (0 = choose it: let
  * Lr be the smallest level whose mean leaf occupancy is <= leaf_size; for
- * orders 4, 8 and 10 with ring 2 the levels max(2, Lr-2) .. Lr+1 are
+ * orders 4, 6, 8 and 10 with ring 2 the levels max(2, Lr-2) .. Lr+1 are
  * compared by an estimate of the pass times from exact counts of near-field
  * particle pairs and far-band box pairs (a function of the ion positions
  * only), otherwise Lr is used); ring: near-field width in leaf cells (1 or 2;
@@ -135,6 +135,14 @@ int shp_fmm_dev(shp_handle h, const shp_particles* p, int order, int leaf_level,
  * the reference's image sums).  sym NULL or SHP_SYM_NONE = shp_fmm_dev. */
 int shp_fmm_sym_dev(shp_handle h, const shp_particles* p, const shp_symmetry* sym, int order,
                     int leaf_level, int leaf_size, int ring, double* ax, double* ay, double* az);
+
+/* M2L expansion orders the engine uses on the Chebyshev shells 3, 4 and 5 of
+ * the far band for an FMM of this order and near-field width (sym_type
+ * SHP_SYM_ROTATIONAL: the rotational variant of shp_fmm_sym_dev, which keeps
+ * the full order on every shell); out3 = {0, 0, 0} when the order has no
+ * compiled term list and the table-driven M2L at the single order is used.
+ * A pure function (no device work).  No counterpart in the reference. */
+int shp_fmm_shell_orders(int order, int ring, int sym_type, int* out3);
 
 /* Leaf level used by the last shp_fmm_dev call on this handle. */
 int shp_fmm_last_leaf_level(shp_handle h);

@@ -224,7 +224,7 @@ struct FmmDevTables {
     short* olist = nullptr;
     int olist_n = 0, olist_s4 = 0, olist_s5 = 0;
     int* cbound = nullptr;
-    int cbound_key = -1;  // (p3, chunks) the bounds were made for
+    int cbound_key = -1;  // (shell orders, chunks) the bounds were made for
 };
 
 static std::map<shp_handle, FmmDevTables> g_tables;
@@ -1126,13 +1126,29 @@ static double fmm_m2l_cost_ms(int nc, int nbox, double far_pairs) {
 }
 static double fmm_leaf_cost_ms(int nbox, double p2p_pairs) { return 3.4e-10 * p2p_pairs + 1.13e-5 * nbox; }
 
-static void m2l_orders(int p, int ring, int& p3, int& p4, int& p5) {
+// M2L orders on the Chebyshev shells 3, 4, 5 of the far band (near field of
+// two cells), fitted to the FP64 restatement on cones: p = 10 keeps its error
+// with (10, 8, 7); lower orders lose more per dropped degree, so p = 8 uses
+// (8, 7, 6), p = 6 uses (6, 5, 4) and p = 4 keeps 4 throughout.  `full`
+// (rotational images, whose fields cancel near the axis) keeps p on every
+// shell.  Zeros: no compiled term list (the table-driven M2L is used).
+static void m2l_orders(int p, int ring, bool full, int& p3, int& p4, int& p5) {
     p3 = p4 = p5 = 0;
     if (ring != 2) return;
-    if (p == 4 || p == 8 || p == 10) {
-        p3 = p;
-        p4 = p - 2;
-        p5 = p - 3;
+    if (p == 10) {
+        p3 = 10;
+        p4 = full ? 10 : 8;
+        p5 = full ? 10 : 7;
+    } else if (p == 8) {
+        p3 = 8;
+        p4 = 7;
+        p5 = 6;
+    } else if (p == 6) {
+        p3 = 6;
+        p4 = 5;
+        p5 = 4;
+    } else if (p == 4) {
+        p3 = p4 = p5 = 4;
     }
 }
 
@@ -1158,15 +1174,19 @@ static void launch_m2l_passes(const M2LPlan& pl, const short* olist, const int* 
                            lscale, zeros, part, stride, stats);
 }
 
-static void launch_m2l_tiled(const M2LPlan& pl, int p3, const FmmDevTables* tb, const int* cbound,
+static void launch_m2l_tiled(const M2LPlan& pl, int p3, int p4, const FmmDevTables* tb, const int* cbound,
                              float* part, size_t stride, unsigned long long* stats, hipStream_t st,
                              hipEvent_t e0, hipEvent_t e1) {
-    if (p3 == 10)
+    if (p3 == 10 && p4 == 10)
+        launch_m2l_passes<10, 10, 10>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st, e0, e1);
+    else if (p3 == 10)
         launch_m2l_passes<10, 8, 7>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st, e0, e1);
     else if (p3 == 8)
-        launch_m2l_passes<8, 6, 5>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st, e0, e1);
+        launch_m2l_passes<8, 7, 6>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st, e0, e1);
+    else if (p3 == 6)
+        launch_m2l_passes<6, 5, 4>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st, e0, e1);
     else if (p3 == 4)
-        launch_m2l_passes<4, 2, 1>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st, e0, e1);
+        launch_m2l_passes<4, 4, 4>(pl, tb->olist, cbound, tb->lscale, tb->zeros, part, stride, stats, st, e0, e1);
 }
 
 // Offset list and chunk bounds of the tiled M2L, made once per (order, chunks).
@@ -1180,7 +1200,7 @@ static int m2l_bounds(FmmDevTables* tb, int p3, int p4, int p5, int S, const int
         tb->olist_s4 = s4;
         tb->olist_s5 = s5;
     }
-    const int key = p3 * 64 + S;
+    const int key = ((p3 * 16 + p4) * 16 + p5) * 128 + S;
     if (tb->cbound_key != key) {
         const std::vector<int> cb = m2l_chunk_bounds(p3, p4, p5, S, tb->olist_s4, tb->olist_s5, tb->olist_n);
         if (upload(&tb->cbound, cb) != SHP_OK) return SHP_ERR_HIP;
@@ -1728,8 +1748,8 @@ using namespace shp;
 
 extern "C" {
 
-int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level, int leaf_size,
-                int ring, double* ax, double* ay, double* az) {
+static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_level, int leaf_size, int ring,
+                    bool full, double* ax, double* ay, double* az) {
     SHP_REQUIRE(h && pt && ax && ay && az, "shp_fmm_dev: null argument");
     SHP_REQUIRE(order >= 1 && order <= kMaxP, "shp_fmm_dev: order %d outside [1, %d]", order, kMaxP);
     SHP_REQUIRE(ring >= 1 && ring <= 2, "shp_fmm_dev: ring %d outside [1, 2]", ring);
@@ -1798,7 +1818,7 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
         return SHP_OK;
     }
     int p3 = 0, p4 = 0, p5 = 0;
-    m2l_orders(order, ring, p3, p4, p5);
+    m2l_orders(order, ring, full, p3, p4, p5);
     // Dense cell -> box grids per level over the occupied cells (unrolled M2L,
     // near-field lookups, leaf-level choice); very large grids fall back to the
     // table-driven M2L and key searches.
@@ -2098,7 +2118,7 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
                 SHP_WS(pw, float, h, WS_FMM_PART, (size_t)S * stride);
                 part = pw;
             }
-            launch_m2l_tiled(pl, p3, tb, cb, part, stride, h->fmm_stats, st, h->fev[6], h->fev[7]);
+            launch_m2l_tiled(pl, p3, p4, tb, cb, part, stride, h->fmm_stats, st, h->fev[6], h->fev[7]);
             if (S > 1)
                 hipLaunchKernelGGL(k_fmm_m2l_reduce, dim3(fdiv_up((long)stride, 256)), dim3(256), 0, st,
                                    part, stride, S, nc, ld, tb->lscale, lv[2].Lc, stride);
@@ -2142,6 +2162,17 @@ int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level
         h->fmm_shell_madds[k] = unrolled ? madds_of(ps[k]) : (double)tb->npairs;
         h->fmm_shell_madds0[k] = unrolled ? madds0_of(ps[k]) : (double)tb->npairs;
     }
+    return SHP_OK;
+}
+
+int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level, int leaf_size,
+                int ring, double* ax, double* ay, double* az) {
+    return fmm_eval(h, pt, order, leaf_level, leaf_size, ring, false, ax, ay, az);
+}
+
+int shp_fmm_shell_orders(int order, int ring, int sym_type, int* out3) {
+    SHP_REQUIRE(out3, "shp_fmm_shell_orders: null argument");
+    m2l_orders(order, ring, sym_type == SHP_SYM_ROTATIONAL, out3[0], out3[1], out3[2]);
     return SHP_OK;
 }
 
@@ -2235,7 +2266,8 @@ int shp_fmm_sym_dev(shp_handle h, const shp_particles* pt, const shp_symmetry* s
                        pt->y, pt->z, pt->q, pt->status, pt->species, X2, Y2, Z2, Q2, st2, sp2);
     SHP_HIP_CHECK(hipGetLastError());
     shp_particles all{ntot, X2, Y2, Z2, Q2, st2, sp2};
-    int rc = shp_fmm_dev(h, &all, order, leaf_level, leaf_size, ring, A2, A2 + ntot, A2 + 2 * (size_t)ntot);
+    int rc = fmm_eval(h, &all, order, leaf_level, leaf_size, ring, symt == SHP_SYM_ROTATIONAL, A2, A2 + ntot,
+                      A2 + 2 * (size_t)ntot);
     if (rc) return rc;
     SHP_HIP_CHECK(hipMemcpyAsync(ax, A2, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
     SHP_HIP_CHECK(hipMemcpyAsync(ay, A2 + ntot, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));

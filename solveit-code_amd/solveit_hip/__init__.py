@@ -151,6 +151,7 @@ def _load():
         "shp_fmm_stats": (c_int, [c_void_p, dp]),
         "shp_fmm_timing": (c_int, [c_void_p, dp, c_int]),
         "shp_fmm_level_estimates": (c_int, [c_void_p, dp, c_int]),
+        "shp_fmm_shell_orders": (c_int, [c_int, c_int, c_int, ctypes.POINTER(c_int)]),
         "shp_compare_accel": (c_int, [c_void_p, c_int] + [c_void_p] * 7 + [dp]),
         "shp_energy_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), c_void_p, c_void_p,
                                    c_void_p, dp]),
@@ -209,6 +210,16 @@ def _torch_ready(device):
     wrapper below synchronizes the engine's stream before returning tensors."""
     import torch
     torch.cuda.current_stream(device).synchronize()
+
+
+def fmm_shell_orders(order, ring=2, rotational=False):
+    """M2L orders on the far-band shells 3, 4, 5 for an FMM of this order
+    (0, 0, 0: the table-driven M2L at the single order)."""
+    out = (ctypes.c_int * 3)()
+    _check(lib.shp_fmm_shell_orders(int(order), int(ring),
+                                    SHP_SYM_ROTATIONAL if rotational else SHP_SYM_NONE, out),
+           "shp_fmm_shell_orders")
+    return tuple(out)
 
 
 def make_symmetry(kind="none", nsec=4, lattice=None, domain=(0.0, 0.0, 0.0)):

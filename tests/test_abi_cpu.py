@@ -54,3 +54,16 @@ def test_error_path_without_device_is_reported():
     with pytest.raises(solveit_hip.ShpError):
         solveit_hip.Engine(device=4096)
     assert solveit_hip.lib.shp_last_error()
+
+
+def test_fmm_shell_orders_rule():
+    """The M2L orders per far-band shell (3, 4, 5) the engine reports: a pure
+    host function, so it is checked here without a GPU."""
+    import solveit_hip
+    assert solveit_hip.fmm_shell_orders(10) == (10, 8, 7)
+    assert solveit_hip.fmm_shell_orders(10, rotational=True) == (10, 10, 10)
+    assert solveit_hip.fmm_shell_orders(8) == (8, 7, 6)
+    assert solveit_hip.fmm_shell_orders(6) == (6, 5, 4)
+    assert solveit_hip.fmm_shell_orders(4) == (4, 4, 4)
+    assert solveit_hip.fmm_shell_orders(12) == (0, 0, 0)
+    assert solveit_hip.fmm_shell_orders(10, ring=1) == (0, 0, 0)

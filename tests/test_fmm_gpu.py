@@ -35,11 +35,12 @@ def _fmm(eng, d, **kw):
 
 
 def _oracle_fmm(d, p, L, ring):
-    """Oracle FMM with the engine's M2L order per shell: with a near field of
-    two cells and p in {4, 8, 10} the engine truncates shells 3/4/5 of the box
-    offset at orders p, p-2, p-3."""
-    if ring == 2 and p in (4, 8, 10):
-        oracle.fmm_set_shell_orders([0, 0, 0, p, p - 2, p - 3])
+    """Oracle FMM with the engine's M2L order per far-band shell (the engine
+    reports them through shp_fmm_shell_orders)."""
+    import solveit_hip
+    orders = solveit_hip.fmm_shell_orders(p, ring)
+    if orders[0]:
+        oracle.fmm_set_shell_orders([0, 0, 0] + list(orders))
     try:
         return oracle.fmm_f64(d, p, L, ring=ring)
     finally:

@@ -87,3 +87,23 @@ def test_shell_orders():
     e_base, e_red, e_crude = (oracle.rel_l2(a, ref)[0] for a in (base, red, crude))
     assert e_base <= e_red < 1e-5
     assert e_crude > 3 * e_red
+
+
+def test_reduced_shell_orders_of_lower_orders():
+    """The engine's shell orders below p = 10 lose little against the uniform
+    order, where (p, p-2, p-3) would lose much more (8192-ion cone, leaf
+    level 5, near field of two cells)."""
+    d = synth.cone(8192, seed=5)
+    ref = oracle.direct_f64(d)
+    try:
+        for p, used, cut in ((8, [8, 7, 6], [8, 6, 5]), (6, [6, 5, 4], [6, 4, 3]), (4, [4, 4, 4], [4, 2, 1])):
+            oracle.fmm_set_shell_orders(None)
+            e_full = oracle.rel_l2(oracle.fmm_f64(d, p, 5, ring=2)[0], ref)[0]
+            oracle.fmm_set_shell_orders([0, 0, 0] + used)
+            e_used = oracle.rel_l2(oracle.fmm_f64(d, p, 5, ring=2)[0], ref)[0]
+            oracle.fmm_set_shell_orders([0, 0, 0] + cut)
+            e_cut = oracle.rel_l2(oracle.fmm_f64(d, p, 5, ring=2)[0], ref)[0]
+            assert e_used <= 1.6 * e_full, (p, e_used, e_full)
+            assert e_cut > 1.5 * e_used, (p, e_cut, e_used)
+    finally:
+        oracle.fmm_set_shell_orders(None)
