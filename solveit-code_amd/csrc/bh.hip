@@ -64,6 +64,23 @@ struct BhNodes {
     float2* du;   // delta, sum |q|
 };
 
+// What the walk reads of a node, in one 32-byte record: the centre of charge
+// and net charge, then (first child, successor, and for a cell delta and side
+// as float bits, for a leaf the ion count and first ion).
+struct BhWalk {
+    float4* cq;
+    int4* aux;
+};
+
+__global__ void k_bh_pack(BhNodes t, BhWalk w, const int* __restrict__ misc) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= misc[0]) return;
+    const int f = t.first[node];
+    w.cq[node] = t.cq[node];
+    w.aux[node] = f < 0 ? make_int4(f, t.next[node], t.count[node], t.start[node])
+                        : make_int4(f, t.next[node], __float_as_int(t.du[node].x), __float_as_int(t.geo[node].w));
+}
+
 // ---------------------------------------------------------------- inputs ----
 
 __global__ void k_bh_from_f64(const double* __restrict__ X, const double* __restrict__ Y,
@@ -341,7 +358,7 @@ __device__ __forceinline__ int bh_qrf(int n) {
 
 template <int SYM>
 __global__ __launch_bounds__(kBhThreads) void k_bh_walk(
-    BhNodes t, const float4* __restrict__ SP, const int* __restrict__ SI, int ns,
+    BhWalk t, const float4* __restrict__ SP, const int* __restrict__ SI, int ns,
     const int* __restrict__ st, const int* __restrict__ sp, float theta, const int* __restrict__ lat,
     int nimg, float dx0, float dy0, float dz0, double K, int i0, double* __restrict__ ax_out,
     double* __restrict__ ay_out, double* __restrict__ az_out) {
@@ -374,12 +391,13 @@ __global__ __launch_bounds__(kBhThreads) void k_bh_walk(
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         int node = 0;
         while (node >= 0) {
-            const int f = t.first[node];
+            const float4 q = t.cq[node];
+            const int4 x = t.aux[node];
+            const int f = x.x;
             if (f < 0) {
-                const int c = t.count[node], j0 = t.start[node];
+                const int c = x.z, j0 = x.w;
                 if (c == 1) {
                     if (!(self && j0 == s)) {
-                        const float4 q = t.cq[node];
                         float rx, ry, rz;
                         const float d = bh_dist(px, py, pz, q.x, q.y, q.z, rx, ry, rz);
                         bh_add<SYM>(q.w / (d * d * d), rx, ry, rz, cb, sb, ax, ay, az);
@@ -393,14 +411,13 @@ __global__ __launch_bounds__(kBhThreads) void k_bh_walk(
                         bh_add<SYM>(q.w / (d * d * d), rx, ry, rz, cb, sb, ax, ay, az);
                     }
                 }
-                node = t.next[node];
+                node = x.y;
             } else {
-                const float4 q = t.cq[node];
                 float rx, ry, rz;
                 const float d = bh_dist(px, py, pz, q.x, q.y, q.z, rx, ry, rz);
-                if (theta * (d - t.du[node].x) > t.geo[node].w) {
+                if (theta * (d - __int_as_float(x.z)) > __int_as_float(x.w)) {
                     bh_add<SYM>(q.w / (d * d * d), rx, ry, rz, cb, sb, ax, ay, az);
-                    node = t.next[node];
+                    node = x.y;
                 } else {
                     node = f;
                 }
@@ -514,6 +531,11 @@ int bh_eval(shp_handle h, const float4* P, const int* st, const int* sp, int i0,
             const long bound = std::min<long>(lev < 8 ? (1L << (3 * lev)) : (long)ns, (long)ns);
             hipLaunchKernelGGL(k_bh_props, dim3(bdiv(bound, 256)), dim3(256), 0, stream, t, lev, SP, misc);
         }
+        SHP_WS(wrec, char, h, WS_BH_WALK, cap * (sizeof(float4) + sizeof(int4)));
+        BhWalk w;
+        w.cq = (float4*)wrec;
+        w.aux = (int4*)(w.cq + cap);
+        hipLaunchKernelGGL(k_bh_pack, dim3(bdiv((long)cap, 256)), dim3(256), 0, stream, t, w, misc);
         const int* lat = nullptr;
         int nimg = 1;
         float dom[3] = {0.0f, 0.0f, 0.0f};
@@ -528,13 +550,13 @@ int bh_eval(shp_handle h, const float4* P, const int* st, const int* sp, int i0,
         SHP_HIP_CHECK(hipEventRecord(h->ev0, stream));
         const dim3 g(bdiv(ns, kBhThreads)), b(kBhThreads);
         if (symt == SHP_SYM_NONE)
-            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_NONE>, g, b, 0, stream, t, SP, SI, ns, st, sp, theta, lat,
+            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_NONE>, g, b, 0, stream, w, SP, SI, ns, st, sp, theta, lat,
                                nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az);
         else if (symt == SHP_SYM_ROTATIONAL)
-            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_ROTATIONAL>, g, b, 0, stream, t, SP, SI, ns, st, sp, theta,
+            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_ROTATIONAL>, g, b, 0, stream, w, SP, SI, ns, st, sp, theta,
                                lat, nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az);
         else
-            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_TRANSLATIONAL>, g, b, 0, stream, t, SP, SI, ns, st, sp,
+            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_TRANSLATIONAL>, g, b, 0, stream, w, SP, SI, ns, st, sp,
                                theta, lat, nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az);
         SHP_HIP_CHECK(hipGetLastError());
         SHP_HIP_CHECK(hipEventRecord(h->ev1, stream));

@@ -80,6 +80,7 @@ enum WsSlot {
     WS_BH_SP,       // Barnes-Hut ions in key order
     WS_BH_SI,       // their original indices
     WS_BH_NODES,    // Barnes-Hut tree nodes
+    WS_BH_WALK,     // Barnes-Hut walk records (one 32-byte record per node)
     WS_ADJ_U,       // force VJP: scaled cotangents
     WS_ADJ_TMP,     // leapfrog VJP: half-step positions and cotangents
     WS_BATCH_TUNE,  // autotuner scratch outputs
