@@ -209,7 +209,10 @@ __global__ __launch_bounds__(WG) void k_batch_direct(
                     dhz[k] += az;
                 }
             } else {
-                const float e2 = (float)eps2;
+                // in a VGPR: an FP32 FMA that reads an SGPR issues at half the
+                // rate of one whose operands are all VGPRs
+                float e2 = (float)eps2;
+                asm volatile("" : "+v"(e2));
                 float qx[TPL], qy[TPL], qz[TPL], sx[TPL], sy[TPL], sz[TPL];
 #pragma unroll
                 for (int k = 0; k < TPL; k++) {
