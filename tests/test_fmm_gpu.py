@@ -96,6 +96,31 @@ def test_fmm_edge_cases(eng):
     assert np.all(got == 0)
 
 
+def test_fmm_coincident_ions(eng):
+    """Clusters of ions at one point each (identical Morton keys at every
+    level) next to a scattered plume, automatic leaf level: coincident pairs
+    contribute nothing (zero displacement, softened distance), so the FMM
+    still matches the FP64 direct sum; a plume that is one point throughout
+    gives zero forces at every automatic depth."""
+    d = synth.cone(4000, seed=31)
+    for k, (a, b) in enumerate(((0, 600), (600, 700), (700, 701))):
+        j = 1000 + 7 * k
+        for c in ("x", "y", "z"):
+            d[c][a:b] = d[c][j]
+    print("case fmm coincident", flush=True)
+    got, L = _fmm(eng, d, order=10, leaf_level=0, leaf_size=20, ring=2)
+    ref = oracle.direct_f64(d)
+    l2, mx, _ = oracle.rel_l2(got, ref)
+    print(f"  leaf level {L}: rel L2 {l2:.3e} max {mx:.3e}")
+    assert l2 <= 1e-5
+    one = synth.cone(500, seed=32)
+    for c in ("x", "y", "z"):
+        one[c][:] = one[c][0]
+    got, L = _fmm(eng, one, order=10, leaf_level=0, leaf_size=20, ring=2)
+    print(f"  single point: leaf level {L}")
+    assert np.all(got == 0)
+
+
 def test_fmm_full_size_vs_fp64_direct(eng):
     import solveit_hip
     import torch
