@@ -47,6 +47,7 @@ constexpr int kMaxLevel = 21;
 constexpr int kFmmThreads = 256;
 constexpr int kM2LBatch = 4;
 constexpr int kShiftLanes = 64;  // one wave per parent box in M2M / L2L
+constexpr int kShiftUnroll = 16;  // shift terms loaded per batch (the lists are padded to a multiple)
 
 // Index of the coefficient (a, b, c) in the degree-major order (within a
 // degree: a descending, then b descending) and first index of degree n.
@@ -165,7 +166,10 @@ static void build_tables(FmmHostTables& t, int p) {
             }
             load[ln] += outs[i].size();
         }
+        // a whole number of kShiftUnroll-term batches per lane; the padding
+        // terms have coefficient 0 and no `last` flag
         nt = (int)*std::max_element(load.begin(), load.end());
+        nt = (nt + kShiftUnroll - 1) / kShiftUnroll * kShiftUnroll;
         table.assign((size_t)nt * kShiftLanes, make_int2(0, 0));
         for (int ln = 0; ln < kShiftLanes; ln++)
             for (size_t j = 0; j < lane[ln].size(); j++) table[j * kShiftLanes + ln] = lane[ln][j];
@@ -595,12 +599,19 @@ __global__ __launch_bounds__(kShiftLanes) void k_fmm_m2m(Level par, Level ch, in
     __syncthreads();
     float* Mp = par.M + (size_t)b * par.ld;
     float acc = 0.0f;
-    for (int j = 0; j < nt; j++) {
-        const int2 e = term[j * kShiftLanes + t];
-        acc = fmaf(__int_as_float(e.y), W[e.x & 0xfff], acc);
-        if (e.x >> 24) {
-            Mp[(e.x >> 12) & 0xfff] = acc;
-            acc = 0.0f;
+    // the term loads of a batch are issued together, ahead of its chain of
+    // dependent multiply-adds
+    for (int j0 = 0; j0 < nt; j0 += kShiftUnroll) {
+        int2 e[kShiftUnroll];
+#pragma unroll
+        for (int u = 0; u < kShiftUnroll; u++) e[u] = term[(j0 + u) * kShiftLanes + t];
+#pragma unroll
+        for (int u = 0; u < kShiftUnroll; u++) {
+            acc = fmaf(__int_as_float(e[u].y), W[e[u].x & 0xfff], acc);
+            if (e[u].x >> 24) {
+                Mp[(e[u].x >> 12) & 0xfff] = acc;
+                acc = 0.0f;
+            }
         }
     }
 }
@@ -1250,12 +1261,17 @@ __global__ __launch_bounds__(kShiftLanes) void k_fmm_l2l(Level par, Level ch, in
     for (int i = t; i < 8 * nc; i += kShiftLanes) Q[i] = 0.0f;
     __syncthreads();
     float acc = 0.0f;
-    for (int j = 0; j < nt; j++) {
-        const int2 e = term[j * kShiftLanes + t];
-        acc = fmaf(__int_as_float(e.y), Lp[e.x & 0xfff], acc);
-        if (e.x >> 24) {
-            Q[(e.x >> 12) & 0xfff] = acc;
-            acc = 0.0f;
+    for (int j0 = 0; j0 < nt; j0 += kShiftUnroll) {
+        int2 e[kShiftUnroll];
+#pragma unroll
+        for (int u = 0; u < kShiftUnroll; u++) e[u] = term[(j0 + u) * kShiftLanes + t];
+#pragma unroll
+        for (int u = 0; u < kShiftUnroll; u++) {
+            acc = fmaf(__int_as_float(e[u].y), Lp[e[u].x & 0xfff], acc);
+            if (e[u].x >> 24) {
+                Q[(e[u].x >> 12) & 0xfff] = acc;
+                acc = 0.0f;
+            }
         }
     }
     __syncthreads();
