@@ -421,17 +421,31 @@ __global__ __launch_bounds__(256) void k_fmm_level_hist(const uint64_t* __restri
     __shared__ int bins[kMaxLevel + 1];
     if (threadIdx.x <= kMaxLevel) bins[threadIdx.x] = 0;
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const uint64_t k = keys[i];
-        if (k != ~0ULL) {
-            if (i + 1 == n || keys[i + 1] == ~0ULL) counts[0] = i + 1;
-            int lmin = 0;
-            if (i > 0) {
-                const uint64_t x = k ^ keys[i - 1];
-                lmin = x == 0 ? kMaxLevel + 1 : kMaxLevel - (63 - __clzll((long long)x)) / 3;
+    // grid-stride over the keys with a bounded grid: the per-block bins meet
+    // in a few hundred global atomics (same-address atomics serialise in L2)
+    const int n256 = (n + 255) & ~255;  // whole waves take every trip
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n256; i += gridDim.x * blockDim.x) {
+        int lmin = kMaxLevel + 1;  // nothing to count
+        if (i < n) {
+            const uint64_t k = keys[i];
+            if (k != ~0ULL) {
+                if (i + 1 == n || keys[i + 1] == ~0ULL) counts[0] = i + 1;
+                lmin = 0;
+                if (i > 0) {
+                    const uint64_t x = k ^ keys[i - 1];
+                    lmin = x == 0 ? kMaxLevel + 1 : kMaxLevel - (63 - __clzll((long long)x)) / 3;
+                }
             }
-            if (lmin <= kMaxLevel) atomicAdd(&bins[lmin], 1);
+        }
+        // one LDS atomic per distinct level in the wave (most keys of a wave
+        // share one), instead of one per key on the same few bins
+        unsigned long long rem = __ballot(lmin <= kMaxLevel);
+        while (rem) {
+            const int src = __ffsll((long long)rem) - 1;
+            const int lb = __shfl(lmin, src);
+            const unsigned long long m = __ballot(lmin == lb);
+            if ((int)(threadIdx.x & 63) == src) atomicAdd(&bins[lb], __popcll(m));
+            rem &= ~m;
         }
     }
     __syncthreads();
@@ -1055,16 +1069,30 @@ __global__ __launch_bounds__(64, BLO == 0 ? 4 : (BLO == 7 ? 3 : 1)) void k_fmm_m
 }
 
 // L[box][k] += lscale[k] * sum over chunks of the partial sums (fixed order);
-// rows of stride ld, the padding past nc left alone.
-__global__ void k_fmm_m2l_reduce(const float* __restrict__ part, size_t stride, int nchunk, int nc, int ld,
-                                 const float* __restrict__ lscale, float* __restrict__ Lc, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int k = (int)(i % ld);
+// rows of stride ld (a multiple of 4), the padding past nc left alone.  Each
+// lane takes four consecutive coefficients of one row.
+__global__ __launch_bounds__(256) void k_fmm_m2l_reduce(const float* __restrict__ part, size_t stride, int nchunk,
+                                                        int nc, int ld, const float* __restrict__ lscale,
+                                                        float* __restrict__ Lc, size_t n4) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n4) return;
+    const size_t e = 4 * q;
+    const int k = (int)(e - e / (size_t)ld * (size_t)ld);
     if (k >= nc) return;
-    float s = 0.0f;
-    for (int c = 0; c < nchunk; c++) s += part[c * stride + i];
-    Lc[i] += s * lscale[k];
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int c = 0; c < nchunk; c++) {
+        const float4 v = *reinterpret_cast<const float4*>(part + c * stride + e);
+        s.x += v.x;
+        s.y += v.y;
+        s.z += v.z;
+        s.w += v.w;
+    }
+    float4 l = *reinterpret_cast<float4*>(Lc + e);
+    l.x += s.x * lscale[k];
+    if (k + 1 < nc) l.y += s.y * lscale[k + 1];
+    if (k + 2 < nc) l.z += s.z * lscale[k + 2];
+    if (k + 3 < nc) l.w += s.w * lscale[k + 3];
+    *reinterpret_cast<float4*>(Lc + e) = l;
 }
 
 // Offsets of shells 3..5 (ring 2) sorted by shell; chunk bounds per pass such
@@ -1221,12 +1249,11 @@ static int m2l_bounds(FmmDevTables* tb, int p3, int p4, int p5, int S, const int
     return SHP_OK;
 }
 
-// v[row][k] *= sc[k] over rows of stride ld (padding past nc left alone).
-__global__ void k_fmm_scale(float* __restrict__ v, const float* __restrict__ sc, int nc, int ld, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int k = (int)(i % ld);
-    if (k < nc) v[i] *= sc[k];
+// v[row][k] *= sc[k] over rows of stride ld (padding past nc left alone):
+// one workgroup per row, one lane per coefficient.
+__global__ void k_fmm_scale(float* __restrict__ v, const float* __restrict__ sc, int nc, int ld) {
+    const int k = threadIdx.x;
+    if (k < nc) v[(size_t)blockIdx.x * ld + k] *= sc[k];
 }
 
 __global__ void k_fmm_grid_scatter(const uint64_t* __restrict__ key, int nb, int lo0, int lo1, int lo2,
@@ -1415,24 +1442,27 @@ __global__ void k_fmm_leaf_rel(int na, int L, const uint64_t* __restrict__ keys,
 }
 
 // Per leaf: the number of work units and the number of near-field sources.
-__global__ void k_fmm_unit_count(Level lv, GridView gv, int L, int ring, int* __restrict__ cnt,
-                                 int* __restrict__ nsrc) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= lv.nb) return;
-    cnt[b] = max(1, (lv.end[b] - lv.start[b] + kLeafUnit - 1) / kLeafUnit);
+__global__ __launch_bounds__(256) void k_fmm_unit_count(Level lv, GridView gv, int L, int ring,
+                                                        int* __restrict__ cnt, int* __restrict__ nsrc) {
+    // one wave per leaf box, its lanes splitting the near-field cell lookups
+    const int b = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (b >= lv.nb) return;  // wave-uniform
     const uint64_t key = lv.key[b];
     const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
     const int cmax = (1 << L) - 1;
+    const int nside = 2 * ring + 1, ncand = nside * nside * nside;
     int ns = 0;
-    for (int dz = -ring; dz <= ring; dz++)
-        for (int dy = -ring; dy <= ring; dy++)
-            for (int dx = -ring; dx <= ring; dx++) {
-                const int qx = ix + dx, qy = iy + dy, qz = iz + dz;
-                if (qx < 0 || qy < 0 || qz < 0 || qx > cmax || qy > cmax || qz > cmax) continue;
-                const int c = find_cell(lv, gv, qx, qy, qz);
-                if (c >= 0) ns += lv.end[c] - lv.start[c];
-            }
-    nsrc[b] = ns;
+    for (int t = lane; t < ncand; t += 64) {
+        const int qx = ix + t % nside - ring, qy = iy + (t / nside) % nside - ring, qz = iz + t / (nside * nside) - ring;
+        if (qx < 0 || qy < 0 || qz < 0 || qx > cmax || qy > cmax || qz > cmax) continue;
+        const int c = find_cell(lv, gv, qx, qy, qz);
+        if (c >= 0) ns += lv.end[c] - lv.start[c];
+    }
+    for (int off = 32; off > 0; off >>= 1) ns += __shfl_xor(ns, off);
+    if (lane == 0) {
+        cnt[b] = max(1, (lv.end[b] - lv.start[b] + kLeafUnit - 1) / kLeafUnit);
+        nsrc[b] = ns;
+    }
 }
 
 // Sort keys of the units: work = targets x near-field sources (at least 1);
@@ -1805,7 +1835,7 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
     // box counts per level (the host needs them for launch sizes)
     SHP_WS(cnt, int, h, WS_COUNTS, 32);
     hipLaunchKernelGGL(k_fmm_ifill, dim3(1), dim3(32), 0, st, cnt, 32, 0);
-    hipLaunchKernelGGL(k_fmm_level_hist, dim3(fdiv_up(n, 256)), dim3(256), 0, st, keys2, n, cnt);
+    hipLaunchKernelGGL(k_fmm_level_hist, dim3(std::min(fdiv_up(n, 256), 512)), dim3(256), 0, st, keys2, n, cnt);
     int hc[32];
     double hroot[10];
     SHP_HIP_CHECK(hipMemcpyAsync(hc, cnt, sizeof(int) * 32, hipMemcpyDeviceToHost, st));
@@ -2019,7 +2049,7 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
         const int nbl = nbox[L];
         int* uoffs = ucnt + nbl;
         int* nsrcb = ucnt + 2 * nbl;
-        hipLaunchKernelGGL(k_fmm_unit_count, dim3(fdiv_up(nbl, 256)), dim3(256), 0, st, lv[L], gview(L), L, ring,
+        hipLaunchKernelGGL(k_fmm_unit_count, dim3(fdiv_up(nbl, 4)), dim3(256), 0, st, lv[L], gview(L), L, ring,
                            ucnt, nsrcb);
         size_t tsz = 0;
         SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tsz, ucnt, uoffs, nbl, st));
@@ -2061,10 +2091,9 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
     const GridView leafgrid = gview(L);  // leaf-level cell grid for the near-field lookups
     SHP_HIP_CHECK(hipEventRecord(h->fev[2], st));
     if (unrolled) {
-        hipLaunchKernelGGL(k_fmm_scale, dim3(fdiv_up((long)tot * ld, 256)), dim3(256), 0, st, bexp, tb->mscale,
-                           nc, ld, tot * (size_t)ld);
-        hipLaunchKernelGGL(k_fmm_scale, dim3(fdiv_up((long)noff * (L + 1) * nc, 256)), dim3(256), 0, st, ttab,
-                           tb->dscale, nc, nc, (size_t)noff * (L + 1) * nc);
+        const dim3 rowb(64 * fdiv_up(nc, 64));  // one lane per coefficient (nc <= 455)
+        hipLaunchKernelGGL(k_fmm_scale, dim3((unsigned)tot), rowb, 0, st, bexp, tb->mscale, nc, ld);
+        hipLaunchKernelGGL(k_fmm_scale, dim3((unsigned)(noff * (L + 1))), rowb, 0, st, ttab, tb->dscale, nc, nc);
         if (L >= 2) {
             M2LPlan pl{};
             pl.nc = nc;
@@ -2136,8 +2165,8 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
             }
             launch_m2l_tiled(pl, p3, p4, tb, cb, part, stride, h->fmm_stats, st, h->fev[6], h->fev[7]);
             if (S > 1)
-                hipLaunchKernelGGL(k_fmm_m2l_reduce, dim3(fdiv_up((long)stride, 256)), dim3(256), 0, st,
-                                   part, stride, S, nc, ld, tb->lscale, lv[2].Lc, stride);
+                hipLaunchKernelGGL(k_fmm_m2l_reduce, dim3(fdiv_up((long)(stride / 4), 256)), dim3(256), 0, st, part,
+                                   stride, S, nc, ld, tb->lscale, lv[2].Lc, stride / 4);
         }
         if (L < 2) {
             SHP_HIP_CHECK(hipEventRecord(h->fev[6], st));
