@@ -491,6 +491,26 @@ __global__ void k_fmm_box_info(const uint64_t* __restrict__ keys, const int* __r
     }
 }
 
+// Level-l boxes from the level-(l+1) boxes: a child box starts a parent box
+// when its parent key differs from the previous child's.
+__global__ void k_fmm_parent_flags(const uint64_t* __restrict__ ckey, int ncb, char* __restrict__ f) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ncb) return;
+    f[b] = (b == 0 || (ckey[b - 1] >> 3) != (ckey[b] >> 3)) ? 1 : 0;
+}
+
+// Per parent box k with first child cfirst[k]: key, particle start and end.
+__global__ void k_fmm_parent_info(const int* __restrict__ cfirst, int nb, int na, const uint64_t* __restrict__ ckey,
+                                  const int* __restrict__ cstart, uint64_t* __restrict__ bkey,
+                                  int* __restrict__ start, int* __restrict__ bend) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    const int c = cfirst[k];
+    bkey[k] = ckey[c] >> 3;
+    start[k] = cstart[c];
+    bend[k] = k + 1 < nb ? cstart[cfirst[k + 1]] : na;
+}
+
 __device__ __forceinline__ int find_key(const uint64_t* __restrict__ bkey, int nb, uint64_t k) {
     int lo = 0, hi = nb - 1;
     while (lo <= hi) {
@@ -1925,20 +1945,36 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
             off += nbox[l];
         }
     }
+    // The leaf-most level from the sorted particle keys; each coarser level
+    // from the boxes of the level below (a select over boxes, not particles),
+    // which also gives the first-child index directly.
     for (int l = Lb; l >= 0; l--) {
-        const int sh = 3 * (kMaxLevel - l);
-        hipLaunchKernelGGL(k_fmm_flags, dim3(fdiv_up(na, 256)), dim3(256), 0, st, keys2, na, sh, flags);
         hipcub::CountingInputIterator<int> it(0);
         size_t t2 = 0;
-        SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, t2, it, flags, (int*)lv[l].start, cnt + 24,
-                                                    na, st));
-        SHP_WS(cub2, char, h, WS_CUB, t2);
-        SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(cub2, t2, it, flags, (int*)lv[l].start, cnt + 24,
-                                                    na, st));
-        hipLaunchKernelGGL(k_fmm_box_info, dim3(fdiv_up(nbox[l], 256)), dim3(256), 0, st, keys2,
-                           lv[l].start, nbox[l], na, sh, (uint64_t*)lv[l].key, (int*)lv[l].end,
-                           l < Lb ? lv[l + 1].start : nullptr, l < Lb ? nbox[l + 1] : 0,
-                           (int*)lv[l].cfirst);
+        if (l == Lb) {
+            const int sh = 3 * (kMaxLevel - l);
+            hipLaunchKernelGGL(k_fmm_flags, dim3(fdiv_up(na, 256)), dim3(256), 0, st, keys2, na, sh, flags);
+            SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, t2, it, flags, (int*)lv[l].start, cnt + 24,
+                                                        na, st));
+            SHP_WS(cub2, char, h, WS_CUB, t2);
+            SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(cub2, t2, it, flags, (int*)lv[l].start, cnt + 24,
+                                                        na, st));
+            hipLaunchKernelGGL(k_fmm_box_info, dim3(fdiv_up(nbox[l], 256)), dim3(256), 0, st, keys2,
+                               lv[l].start, nbox[l], na, sh, (uint64_t*)lv[l].key, (int*)lv[l].end,
+                               nullptr, 0, nullptr);
+        } else {
+            const int ncb = nbox[l + 1];
+            hipLaunchKernelGGL(k_fmm_parent_flags, dim3(fdiv_up(ncb, 256)), dim3(256), 0, st, lv[l + 1].key, ncb,
+                               flags);
+            SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, t2, it, flags, (int*)lv[l].cfirst, cnt + 24,
+                                                        ncb, st));
+            SHP_WS(cub2, char, h, WS_CUB, t2);
+            SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(cub2, t2, it, flags, (int*)lv[l].cfirst, cnt + 24,
+                                                        ncb, st));
+            hipLaunchKernelGGL(k_fmm_parent_info, dim3(fdiv_up(nbox[l], 256)), dim3(256), 0, st, lv[l].cfirst,
+                               nbox[l], na, lv[l + 1].key, lv[l + 1].start, (uint64_t*)lv[l].key, (int*)lv[l].start,
+                               (int*)lv[l].end);
+        }
     }
     SHP_HIP_CHECK(hipGetLastError());
     std::vector<int*> gl(Lb + 1, nullptr);
