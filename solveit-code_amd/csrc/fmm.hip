@@ -511,6 +511,54 @@ __global__ void k_fmm_parent_info(const int* __restrict__ cfirst, int nb, int na
     bend[k] = k + 1 < nb ? cstart[cfirst[k + 1]] : na;
 }
 
+// The levels lhi, lhi - 1, ..., 0 built one after another from the level
+// below by a single workgroup (these levels hold few boxes; one launch
+// instead of a device-wide select per level).
+constexpr int kUpperMaxChildren = 16384;  // children of the top level built this way, at most
+struct UpperLevels {
+    int lhi, na;
+    uint64_t* key[kMaxLevel + 2];
+    int* start[kMaxLevel + 2];
+    int* end[kMaxLevel + 2];
+    int* cfirst[kMaxLevel + 2];
+    int nb[kMaxLevel + 2];
+};
+
+__global__ __launch_bounds__(1024) void k_fmm_upper_levels(const UpperLevels U) {
+    __shared__ int wsum[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int l = U.lhi; l >= 0; l--) {
+        const int ncb = U.nb[l + 1];
+        const uint64_t* ck = U.key[l + 1];
+        const int* cs = U.start[l + 1];
+        int carry = 0;  // parent boxes found so far (uniform)
+        for (int b0 = 0; b0 < ncb; b0 += 1024) {
+            const int b = b0 + (int)threadIdx.x;
+            const bool f = b < ncb && (b == 0 || (ck[b - 1] >> 3) != (ck[b] >> 3));
+            const unsigned long long m = __ballot(f);
+            if (lane == 0) wsum[w] = __popcll(m);
+            __syncthreads();
+            int before = 0, tot = 0;
+            for (int k = 0; k < 16; k++) {
+                const int v = wsum[k];
+                before += k < w ? v : 0;
+                tot += v;
+            }
+            if (f) {
+                const int k = carry + before + __popcll(m & ((1ull << lane) - 1ull));
+                U.cfirst[l][k] = b;
+                U.key[l][k] = ck[b] >> 3;
+                U.start[l][k] = cs[b];
+                if (k > 0) U.end[l][k - 1] = cs[b];
+            }
+            carry += tot;
+            __syncthreads();  // wsum is reused
+        }
+        if (threadIdx.x == 0 && carry > 0) U.end[l][carry - 1] = U.na;
+        __syncthreads();  // level l is complete before level l - 1 reads it
+    }
+}
+
 __device__ __forceinline__ int find_key(const uint64_t* __restrict__ bkey, int nb, uint64_t k) {
     int lo = 0, hi = nb - 1;
     while (lo <= hi) {
@@ -1948,9 +1996,14 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
     // The leaf-most level from the sorted particle keys; each coarser level
     // from the boxes of the level below (a select over boxes, not particles),
     // which also gives the first-child index directly.
+    int lup = -1;  // the top levels lup .. 0 are built by one single-workgroup launch
     for (int l = Lb; l >= 0; l--) {
         hipcub::CountingInputIterator<int> it(0);
         size_t t2 = 0;
+        if (l < Lb && nbox[l + 1] <= kUpperMaxChildren) {
+            lup = l;
+            break;
+        }
         if (l == Lb) {
             const int sh = 3 * (kMaxLevel - l);
             hipLaunchKernelGGL(k_fmm_flags, dim3(fdiv_up(na, 256)), dim3(256), 0, st, keys2, na, sh, flags);
@@ -1975,6 +2028,19 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
                                nbox[l], na, lv[l + 1].key, lv[l + 1].start, (uint64_t*)lv[l].key, (int*)lv[l].start,
                                (int*)lv[l].end);
         }
+    }
+    if (lup >= 0) {
+        UpperLevels U{};
+        U.lhi = lup;
+        U.na = na;
+        for (int l = 0; l <= lup + 1; l++) {
+            U.key[l] = (uint64_t*)lv[l].key;
+            U.start[l] = (int*)lv[l].start;
+            U.end[l] = (int*)lv[l].end;
+            U.cfirst[l] = (int*)lv[l].cfirst;
+            U.nb[l] = nbox[l];
+        }
+        hipLaunchKernelGGL(k_fmm_upper_levels, dim3(1), dim3(1024), 0, st, U);
     }
     SHP_HIP_CHECK(hipGetLastError());
     std::vector<int*> gl(Lb + 1, nullptr);
