@@ -2070,7 +2070,7 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
     if (choose) {
         // P2P pairs and M2L far-band box pairs of the candidate leaf levels
         // Lrule - 2 .. Lrule + 1, counted on every stride-th box (at most
-        // 16384 boxes per level; the sums are scaled to the whole level).
+        // 4096 boxes per level; the sums are scaled to the whole level).
         const int L0 = std::max(2, Lrule - 2);
         SHP_WS(lcost, unsigned long long, h, WS_FMM_LCOST, 2 * (Lb + 1));
         hipLaunchKernelGGL(k_fmm_u64fill, dim3(1), dim3(64), 0, st, lcost, 2 * (Lb + 1));
@@ -2079,7 +2079,7 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
         int ns_max = 1;
         for (int l = L0; l <= Lb; l++) {
             const int k = l - L0;
-            stride[l] = std::max(1, fdiv_up(nbox[l], 1 << 14));
+            stride[l] = std::max(1, fdiv_up(nbox[l], 1 << 12));
             ns_max = std::max(ns_max, fdiv_up(nbox[l], stride[l]));
             A.lv[k] = lv[l];
             A.par[k] = lv[l - 1];

@@ -182,7 +182,7 @@ def test_fmm_symmetry_vs_fp64_direct_images(eng, sym):
 
 def test_leaf_level_choice_counts(eng):
     """The leaf-level choice counts, on every box of each candidate level (no
-    sampling below 16384 boxes), the near-field particle pairs and far-band
+    sampling below 4096 boxes), the near-field particle pairs and far-band
     box pairs; for the chosen level they equal the pairs the leaf and M2L
     kernels then evaluate, and the chosen level has the least estimate."""
     import solveit_hip
