@@ -1601,13 +1601,13 @@ __device__ __forceinline__ void leaf_pair(float ux, float uy, float uz, const fl
 // Stages the near-field sources [jb, jb + kLeafTile) relative to the leaf
 // centre in units of the half-width (zero charge past nsrc): each source's
 // offset from its own leaf centre plus twice the cell offset of its leaf
-// (run t of the candidate list).  Each lane takes
+// (packed per run in rl; rf holds each run's start minus its prefix sum).
+// Each lane takes
 // kLeafTile / 64 consecutive sources: one search for the run holding the
 // first (the last r with rpre[r] <= j), then a walk forward over the runs.
 __device__ __forceinline__ void leaf_stage(float4* tile, int jb, int lane, int nsrc, int nrun,
-                                           const int* rpre, const int* rf, const unsigned char* rt, int ring,
+                                           const int* rpre, const int* rf, const int* rl,
                                            const float4* __restrict__ spf) {
-    const int nside = 2 * ring + 1;
     __syncthreads();
     const int j0 = jb + (kLeafTile / 64) * lane;
     int r = 0;
@@ -1625,10 +1625,10 @@ __device__ __forceinline__ void leaf_stage(float4* tile, int jb, int lane, int n
         float4 sv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (j < nsrc) {
             while (rpre[r + 1] <= j) r++;
-            const float4 rs = spf[rf[r] + (j - rpre[r])];
-            const int t = rt[r];  // candidate cell of the run: offset (t % n, t / n % n, t / n^2) - ring
-            sv = make_float4(rs.x + (float)(2 * (t % nside - ring)), rs.y + (float)(2 * ((t / nside) % nside - ring)),
-                             rs.z + (float)(2 * (t / (nside * nside) - ring)), rs.w);
+            const float4 rs = spf[rf[r] + j];
+            const int o = rl[r];  // doubled cell offset of the run, three signed bytes
+            sv = make_float4(rs.x + (float)(signed char)(o & 0xff), rs.y + (float)(signed char)((o >> 8) & 0xff),
+                             rs.z + (float)(signed char)((o >> 16) & 0xff), rs.w);
         }
         tile[(kLeafTile / 64) * lane + k] = sv;
     }
@@ -1725,6 +1725,16 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
     }
     if (lane == 0) rpre[0] = 0;
     __syncthreads();
+    // per run, for the staging: rf[r] - rpre[r] (so source j of run r is
+    // spf[rf[r] + j]) and, in place of its length, the cell offset doubled as
+    // three signed bytes
+    for (int r = lane; r < nrun; r += 64) {
+        const int t = rt[r];
+        const int ox = 2 * (t % nside - ring), oy = 2 * ((t / nside) % nside - ring), oz = 2 * (t / (nside * nside) - ring);
+        rl[r] = (ox & 0xff) | ((oy & 0xff) << 8) | ((oz & 0xff) << 16);
+        rf[r] -= rpre[r];
+    }
+    __syncthreads();
     const int nsrc = carry;
     const int s0 = lv.start[b] + kLeafUnit * unit.y, s1 = min(s0 + kLeafUnit, lv.end[b]);
     const int nt = s1 - s0;
@@ -1750,7 +1760,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
         // P2P below runs in u = (x - c)/h, so sum q u/|u|^3 = h^2 sum q d/|d|^3.
         float hx = -gx, hy = -gy, hz = -gz, lx = 0.0f, ly = 0.0f, lz = 0.0f;
         for (int jb = 0; jb < nsrc; jb += kLeafTile) {
-            leaf_stage(tile, jb, lane, nsrc, nrun, rpre, rf, rt, ring, spf);
+            leaf_stage(tile, jb, lane, nsrc, nrun, rpre, rf, rl, spf);
             // group g takes the contiguous chunk [g nit, (g + 1) nit) of the
             // tile; the trip count is wave-uniform and entries past the staged
             // sources have zero charge
