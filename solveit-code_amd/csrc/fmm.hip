@@ -1603,20 +1603,25 @@ __device__ __forceinline__ void leaf_pair(float ux, float uy, float uz, const fl
 // offset from its own leaf centre plus twice the cell offset of its leaf
 // (packed per run in rl; rf holds each run's start minus its prefix sum).
 // Each lane takes
-// kLeafTile / 64 consecutive sources: one search for the run holding the
-// first (the last r with rpre[r] <= j), then a walk forward over the runs.
+// kLeafTile / 64 consecutive sources: a search, from its run of the previous
+// tile, for the run holding the first (the last r with rpre[r] <= j), then a
+// walk forward over the runs.
 __device__ __forceinline__ void leaf_stage(float4* tile, int jb, int lane, int nsrc, int nrun,
                                            const int* rpre, const int* rf, const int* rl,
-                                           const float4* __restrict__ spf) {
+                                           const float4* __restrict__ spf, int& r) {
     __syncthreads();
     const int j0 = jb + (kLeafTile / 64) * lane;
-    int r = 0;
     if (j0 < nsrc) {
-        int hi = nrun - 1;
-        while (r < hi) {
-            const int mid = (r + hi + 1) >> 1;
-            if (rpre[mid] <= j0) r = mid;
-            else hi = mid - 1;
+        // r (the lane's run from the previous tile, rpre[r] <= j0) moves to the
+        // last run with rpre[r] <= j0: doubling steps, then halving ones
+        int step = 1;
+        while (r + step < nrun && rpre[r + step] <= j0) {
+            r += step;
+            step <<= 1;
+        }
+        while (step > 1) {
+            step >>= 1;
+            if (r + step < nrun && rpre[r + step] <= j0) r += step;
         }
     }
 #pragma unroll
@@ -1759,8 +1764,9 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
         // far part in units of 1/h^2 relative to the P2P sum expressed in u units:
         // P2P below runs in u = (x - c)/h, so sum q u/|u|^3 = h^2 sum q d/|d|^3.
         float hx = -gx, hy = -gy, hz = -gz, lx = 0.0f, ly = 0.0f, lz = 0.0f;
+        int run = 0;  // this lane's run in the staging, carried from tile to tile
         for (int jb = 0; jb < nsrc; jb += kLeafTile) {
-            leaf_stage(tile, jb, lane, nsrc, nrun, rpre, rf, rl, spf);
+            leaf_stage(tile, jb, lane, nsrc, nrun, rpre, rf, rl, spf, run);
             // group g takes the contiguous chunk [g nit, (g + 1) nit) of the
             // tile; the trip count is wave-uniform and entries past the staged
             // sources have zero charge
