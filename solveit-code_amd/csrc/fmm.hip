@@ -1560,6 +1560,13 @@ __global__ void k_fmm_unit_fill(const int* __restrict__ cnt, const int* __restri
 
 // L2P share of lane group g of G: the gradient of sum_b L_b u^b over the
 // multi-indices with b_x = g mod G (u in units of the leaf half-width).
+// Lsh holds the local coefficients in the order of the loops below: by a,
+// then b, then c (coefficient (a, b, c) at l2p_pos(p, a, b, c)).
+__device__ __forceinline__ int l2p_pos(int p, int a, int b, int c) {
+    const int q = p - a;
+    return dstart(p + 1) - dstart(q + 1) + b * (q + 1) - b * (b - 1) / 2 + c;
+}
+
 __device__ __forceinline__ void leaf_l2p(const float* __restrict__ Lsh, int p, int g, int G, float ux,
                                          float uy, float uz, float& gx, float& gy, float& gz) {
     gx = gy = gz = 0.0f;
@@ -1569,19 +1576,23 @@ __device__ __forceinline__ void leaf_l2p(const float* __restrict__ Lsh, int p, i
             xam1 = xa;
             xa *= ux;
         }
-        float yb = 1.0f, ybm1 = 0.0f;
+        const float fa = (float)a;
+        int k = dstart(p + 1) - dstart(p - a + 1);  // first coefficient of this a
+        float yb = 1.0f, ybm1 = 0.0f, fb = 0.0f;
         for (int bb = 0; a + bb <= p; bb++) {
-            float zc = 1.0f, zcm1 = 0.0f;
+            float zc = 1.0f, zcm1 = 0.0f, fc = 0.0f;
             for (int cc = 0; a + bb + cc <= p; cc++) {
-                const float Lk = Lsh[cidx(a, bb, cc)];
-                gx = fmaf(Lk * (float)a, xam1 * yb * zc, gx);
-                gy = fmaf(Lk * (float)bb, xa * ybm1 * zc, gy);
-                gz = fmaf(Lk * (float)cc, xa * yb * zcm1, gz);
+                const float Lk = Lsh[k++];
+                gx = fmaf(Lk * fa, xam1 * yb * zc, gx);
+                gy = fmaf(Lk * fb, xa * ybm1 * zc, gy);
+                gz = fmaf(Lk * fc, xa * yb * zcm1, gz);
                 zcm1 = zc;
                 zc *= uz;
+                fc += 1.0f;
             }
             ybm1 = yb;
             yb *= uy;
+            fb += 1.0f;
         }
     }
 }
@@ -1666,7 +1677,8 @@ __device__ __forceinline__ void leaf_write(int i, double qi, double sx, double s
 __global__ __launch_bounds__(64) void k_fmm_leaf(
     Level lv, int L, int ring, GridView gv, const double* __restrict__ root,
     const double4* __restrict__ sp, const int* __restrict__ perm, const int* __restrict__ status,
-    const int* __restrict__ species, const float4* __restrict__ spf, int nc, int p, double K,
+    const int* __restrict__ species, const float4* __restrict__ spf, const int4* __restrict__ mi, int nc, int p,
+    double K,
     const int2* __restrict__ units, const int* __restrict__ uorder, const int* __restrict__ nunits, unsigned long long* __restrict__ stats, double* __restrict__ ax,
     double* __restrict__ ay, double* __restrict__ az) {
     __shared__ float4 tile[kLeafTile + 64];  // staged sources, then 64 zero-charge entries
@@ -1686,7 +1698,10 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
     box_center(root, L, key, c, h);
     const double ih = 1.0 / h;
     const float eps2f = (float)(kSoftening / (h * h));
-    for (int k = lane; k < nc; k += 64) Lsh[k] = lv.Lc[(size_t)b * lv.ld + k];
+    for (int k = lane; k < nc; k += 64) {
+        const int4 A = mi[k];
+        Lsh[l2p_pos(p, A.x, A.y, A.z)] = lv.Lc[(size_t)b * lv.ld + k];
+    }
     const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
     const int nside = 2 * ring + 1, ncand = nside * nside * nside;
     const int cmax = (1 << L) - 1;
@@ -2306,7 +2321,7 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
                            lv[l], lv[l + 1], nc, tb->mi, tb->l2l_term, tb->l2l_nt);
     SHP_HIP_CHECK(hipEventRecord(h->fev[4], st));
     hipLaunchKernelGGL(k_fmm_leaf, dim3(umax), dim3(64), 0, st, lv[L], L, ring, leafgrid, root, sp, perm,
-                       pt->status, pt->species, spf, nc, order, k_const_host(), units, uorder, nunits,
+                       pt->status, pt->species, spf, tb->mi, nc, order, k_const_host(), units, uorder, nunits,
                        h->fmm_stats,
                        ax, ay, az);
     SHP_HIP_CHECK(hipGetLastError());
