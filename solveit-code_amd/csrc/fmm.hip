@@ -1780,13 +1780,14 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
         // P2P below runs in u = (x - c)/h, so sum q u/|u|^3 = h^2 sum q d/|d|^3.
         float hx = -gx, hy = -gy, hz = -gz, lx = 0.0f, ly = 0.0f, lz = 0.0f;
         int run = 0;  // this lane's run in the staging, carried from tile to tile
+        const int nit_full = (kLeafTile + G - 1) / G;  // trip count of a full tile
         for (int jb = 0; jb < nsrc; jb += kLeafTile) {
             leaf_stage(tile, jb, lane, nsrc, nrun, rpre, rf, rl, spf, run);
             // group g takes the contiguous chunk [g nit, (g + 1) nit) of the
             // tile; the trip count is wave-uniform and entries past the staged
             // sources have zero charge
             const int cnt = min(kLeafTile, nsrc - jb);
-            const int nit = (cnt + G - 1) / G;
+            const int nit = cnt == kLeafTile ? nit_full : (cnt + G - 1) / G;
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
             if (active) {
                 const float4* tg = tile + g * nit;
