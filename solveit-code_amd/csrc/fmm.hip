@@ -469,26 +469,15 @@ __global__ void k_fmm_flags(const uint64_t* __restrict__ keys, int na, int sh, c
     f[i] = (i == 0 || (keys[i - 1] >> sh) != (keys[i] >> sh)) ? 1 : 0;
 }
 
-// Per box: key at its level, particle range end, and (for non-leaf levels) the
-// first child index at level l+1.
+// Per box of the leaf-most level built: key at its level and particle range end.
 __global__ void k_fmm_box_info(const uint64_t* __restrict__ keys, const int* __restrict__ start,
                                int nb, int na, int sh, uint64_t* __restrict__ bkey,
-                               int* __restrict__ bend, const int* __restrict__ cstart, int ncb,
-                               int* __restrict__ cfirst) {
+                               int* __restrict__ bend) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
     const int s = start[b];
     bkey[b] = keys[s] >> sh;
     bend[b] = b + 1 < nb ? start[b + 1] : na;
-    if (cstart) {
-        int lo = 0, hi = ncb;  // first child with cstart >= s
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (cstart[mid] < s) lo = mid + 1;
-            else hi = mid;
-        }
-        cfirst[b] = lo;
-    }
 }
 
 // Level-l boxes from the level-(l+1) boxes: a child box starts a parent box
@@ -2045,8 +2034,7 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
             SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(cub2, t2, it, flags, (int*)lv[l].start, cnt + 24,
                                                         na, st));
             hipLaunchKernelGGL(k_fmm_box_info, dim3(fdiv_up(nbox[l], 256)), dim3(256), 0, st, keys2,
-                               lv[l].start, nbox[l], na, sh, (uint64_t*)lv[l].key, (int*)lv[l].end,
-                               nullptr, 0, nullptr);
+                               lv[l].start, nbox[l], na, sh, (uint64_t*)lv[l].key, (int*)lv[l].end);
         } else {
             const int ncb = nbox[l + 1];
             hipLaunchKernelGGL(k_fmm_parent_flags, dim3(fdiv_up(ncb, 256)), dim3(256), 0, st, lv[l + 1].key, ncb,
