@@ -175,15 +175,16 @@ int shp_fmm_level_estimates(shp_handle h, double* out, int cap);
 /*
  * Barnes-Hut evaluation, reference (FP32) arithmetic.  Replaces
  * BarnesHutForceCalculation::updateAcceleration (barnes_hut_force_calculation.cpp:574-715:
- * generateTree :258-331, computeCellProperties :337-390, computeForceFromTree
- * :417-450, barnesHutKernelCalculation :539-569) and its rotational
- * (barnes_hut_force_calculation_rotational.cpp:28-135, four quadrants: sym->nsec
- * must be 4) and translational (barnes_hut_force_calculation_translational.cpp:190-280,
- * lattice images in the given order) variants.  One ion per leaf (cells of
- * depth 21 hold buckets), monopole at the centre of charge, accepted iff
- * theta (d - delta) > side.  The root is one cube around the sources'
- * bounding box (largest extent x (1 + 2^-20)), where the reference lays a grid
- * of root cells.  Sources: status 1 or 2; targets: status 1.
+ * computeWorldLimits :456-522, generateTree :258-331, computeCellProperties
+ * :337-390, computeForceFromTree :417-450, barnesHutKernelCalculation :539-569),
+ * DomainBoundsFunction::computeDomainBounds (domain_bounds_functions.cpp:121-316)
+ * and its rotational (barnes_hut_force_calculation_rotational.cpp:28-135, four
+ * quadrants: sym->nsec must be 4) and translational
+ * (barnes_hut_force_calculation_translational.cpp:190-280, lattice images in
+ * the given order) variants.  The reference's grid of root cells chained as
+ * siblings, one ion per leaf (ions that agree to 42 levels below a root cell
+ * share a bucket leaf), monopole at the centre of charge, accepted iff
+ * theta (d - delta) > repLength.  Sources: status 1 or 2; targets: status 1.
  */
 int shp_bh_dev(shp_handle h, const shp_particles* p, float theta, const shp_symmetry* sym,
                double* ax, double* ay, double* az);
@@ -192,6 +193,31 @@ int shp_bh_dev(shp_handle h, const shp_particles* p, float theta, const shp_symm
  * accumulates, as double (0 for non-targets). */
 int shp_bh_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4, int n,
                 int i0, int i1, float theta, const shp_symmetry* sym, double* out3);
+
+/* Root-layout options of the handle's Barnes-Hut evaluations, the setters of
+ * BarnesHutForceCalculation (force_calculation_methods.h:222-329):
+ * setDomainToggle (0 cubic cells, the default; 1 rectangular prism),
+ * setBoxDimVals (force_box_dims = 1 and the cell counts), setSubcellBlockSize
+ * (2 only) and setOutlierToggle (0 only; other values return
+ * SHP_ERR_UNSUPPORTED). */
+typedef struct shp_bh_options {
+    int domain_toggle;
+    int force_box_dims;
+    int box_dims[3];
+    int subcell_block;
+    int outlier_toggle;
+} shp_bh_options;
+int shp_bh_set_options(shp_handle h, const shp_bh_options* o);
+
+/* Summary of the last Barnes-Hut evaluation (waits for it): out[0] root cells
+ * of the grid, [1] tree nodes (non-empty cells), [2] deepest level below a
+ * root cell, [3] bucket leaves, [4] accepted cells, [5] ion terms (both
+ * summed over targets and images), [6] sources placed in the last root cell
+ * because no cell matched, [7..9] cells per axis, [10..12] location of the
+ * first root cell, [13..15] root cell dimensions.  Has no counterpart in the
+ * reference (its timing record is BarnesHutForceCalculation's timeInfoFile). */
+#define SHP_BH_NSTATS 16
+int shp_bh_stats(shp_handle h, double* out);
 
 /* ---- energies (FP64) ---- */
 

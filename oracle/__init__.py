@@ -475,16 +475,23 @@ def fmm_set_shell_orders(orders=None):
     lib.orc_fmm_set_shell_orders(a.ctypes.data, a.shape[0])
 
 
-lib.orc_bh_ref.restype = _ci
-lib.orc_bh_ref.argtypes = [_vp, _vp, _vp, _ci, _ci, _ci, ctypes.c_float, _ci, _ci, _vp, _vp, _vp]
-lib.orc_bh_root.argtypes = [_vp, _vp, _ci, _ci, _vp]
+lib.orc_bh_ref2.restype = _ci
+lib.orc_bh_ref2.argtypes = [_vp, _vp, _vp, _ci, _ci, _ci, ctypes.c_float, _ci, _ci, _vp, _vp, _ci,
+                            _vp, _vp, _vp]
+lib.orc_bh_layout.restype = _ci
+lib.orc_bh_layout.argtypes = [_vp, _vp, _ci, _ci, _ci, _vp, _vp, _vp, _vp]
+
+BH_INFO_KEYS = ("cells", "root_cells", "max_depth", "buckets", "accepted_cells", "ion_terms",
+                "unmatched_root", "nonempty_cells")
 
 
 def bh_ref(pos4, vel4, acc4, i0=0, i1=None, theta=0.5, sym="none", lattice=None,
-           domain=(0.0, 0.0, 0.0)):
-    """Barnes-Hut restatement (reference float arithmetic, single root cube) on
-    AoS float32 arrays (n, 4).  Returns (out (i1-i0, 3) float64, number of
-    cells including empty ones)."""
+           domain=(0.0, 0.0, 0.0), prism=False, box_dims=None, info=False):
+    """Barnes-Hut restatement (the reference's root-cell grid, tree and float
+    arithmetic) on AoS float32 arrays (n, 4).  prism: rectangular-prism root
+    cells instead of cubic ones (setDomainToggle); box_dims: forced root cell
+    counts (setBoxDimVals).  Returns (out (i1-i0, 3) float64, number of cells
+    including empty ones), plus a dict of tree and walk counts when info."""
     pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
     vel4 = np.ascontiguousarray(vel4, dtype=np.float32)
     acc4 = np.ascontiguousarray(acc4, dtype=np.float32)
@@ -493,20 +500,31 @@ def bh_ref(pos4, vel4, acc4, i0=0, i1=None, theta=0.5, sym="none", lattice=None,
     lat = _i32(np.zeros((1, 3)) if lattice is None else lattice)
     dom = _f64(domain)
     out = np.zeros((max(i1 - i0, 0), 3))
-    ncell = lib.orc_bh_ref(pos4.ctypes.data, vel4.ctypes.data, acc4.ctypes.data, n, i0, i1,
-                           float(theta), _SYM[sym], lat.shape[0], lat.ctypes.data, dom.ctypes.data,
-                           out.ctypes.data)
+    forced = None if box_dims is None else _i32(box_dims)
+    inf = np.zeros(len(BH_INFO_KEYS), dtype=np.int64)
+    ncell = lib.orc_bh_ref2(pos4.ctypes.data, vel4.ctypes.data, acc4.ctypes.data, n, i0, i1,
+                            float(theta), _SYM[sym], lat.shape[0], lat.ctypes.data, dom.ctypes.data,
+                            1 if prism else 0, None if forced is None else forced.ctypes.data,
+                            out.ctypes.data, inf.ctypes.data)
+    if info:
+        return out, ncell, dict(zip(BH_INFO_KEYS, (int(v) for v in inf)))
     return out, ncell
 
 
-def bh_root(pos4, acc4, i0=0, i1=None):
-    """Float root cube (x0, y0, z0, side) of the Barnes-Hut tree."""
+def bh_layout(pos4, acc4, i0=0, i1=None, prism=False, box_dims=None):
+    """Root-cell grid of the Barnes-Hut tree: (roots (3,) float32, dims (3,)
+    float32, cells per axis (3,) int, number of sources)."""
     pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
     acc4 = np.ascontiguousarray(acc4, dtype=np.float32)
     i1 = pos4.shape[0] if i1 is None else i1
-    r = np.zeros(4, dtype=np.float32)
-    lib.orc_bh_root(pos4.ctypes.data, acc4.ctypes.data, i0, i1, r.ctypes.data)
-    return r
+    roots = np.zeros(3, dtype=np.float32)
+    dims = np.zeros(3, dtype=np.float32)
+    nd = np.zeros(3, dtype=np.int32)
+    forced = None if box_dims is None else _i32(box_dims)
+    ns = lib.orc_bh_layout(pos4.ctypes.data, acc4.ctypes.data, i0, i1, 1 if prism else 0,
+                           None if forced is None else forced.ctypes.data, roots.ctypes.data,
+                           dims.ctypes.data, nd.ctypes.data)
+    return roots, dims, nd, ns
 
 
 lib.orc_energy_f64.argtypes = [_ci] + [_vp] * 10

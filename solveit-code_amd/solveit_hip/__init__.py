@@ -56,6 +56,21 @@ class FieldMesh(ctypes.Structure):
     ]
 
 
+class BhOptions(ctypes.Structure):
+    _fields_ = [
+        ("domain_toggle", ctypes.c_int),
+        ("force_box_dims", ctypes.c_int),
+        ("box_dims", ctypes.c_int * 3),
+        ("subcell_block", ctypes.c_int),
+        ("outlier_toggle", ctypes.c_int),
+    ]
+
+
+BH_STAT_KEYS = ("root_cells", "nodes", "max_depth", "buckets", "accepted_cells", "ion_terms",
+                "unmatched_root", "nx", "ny", "nz", "root_x", "root_y", "root_z", "dim_x", "dim_y",
+                "dim_z")
+
+
 class State(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int),
@@ -179,6 +194,8 @@ def _load():
                                ctypes.POINTER(Symmetry), c_void_p, c_void_p, c_void_p]),
         "shp_bh_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, ctypes.c_float,
                                 ctypes.POINTER(Symmetry), dp]),
+        "shp_bh_set_options": (c_int, [c_void_p, ctypes.POINTER(BhOptions)]),
+        "shp_bh_stats": (c_int, [c_void_p, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -336,6 +353,24 @@ class Engine:
                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         _check(rc, "shp_bh_host")
         return out
+
+    def bh_set_options(self, prism=False, box_dims=None, subcell_block=2, outlier_toggle=0):
+        """Root-layout options of later Barnes-Hut calls (BarnesHutForceCalculation's
+        setDomainToggle / setBoxDimVals / setSubcellBlockSize / setOutlierToggle)."""
+        o = BhOptions()
+        o.domain_toggle = 1 if prism else 0
+        o.force_box_dims = 0 if box_dims is None else 1
+        for c in range(3):
+            o.box_dims[c] = 1 if box_dims is None else int(box_dims[c])
+        o.subcell_block = int(subcell_block)
+        o.outlier_toggle = int(outlier_toggle)
+        _check(lib.shp_bh_set_options(self.h, ctypes.byref(o)), "shp_bh_set_options")
+
+    def bh_stats(self):
+        """Tree and walk counts of the last Barnes-Hut evaluation (dict)."""
+        out = (ctypes.c_double * 16)()
+        _check(lib.shp_bh_stats(self.h, out), "shp_bh_stats")
+        return dict(zip(BH_STAT_KEYS, list(out)))
 
     def bh_dev(self, state, theta=0.5, symmetry=None, out=None):
         """Barnes-Hut on a DeviceState; returns (ax, ay, az) float64 device tensors."""

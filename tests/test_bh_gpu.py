@@ -1,11 +1,16 @@
 """Parity of the HIP Barnes-Hut with the oracle's restatement of the reference
-algorithm (same root cube, same float arithmetic), through the C ABI.
+algorithm (the reference's grid of root cells, its octree and its float
+arithmetic), through the C ABI.
 
-Bar: the float accelerations agree to within 2 float ulps per component
-(the only differences allowed are the last-bit rounding of K q/m, which the
-reference forms with a long double division), for the plain, rotational and
-translational variants; theta -> 0 reproduces the direct sum; at N = 1M the
-accuracy against the GPU FP64 direct sum is reported on a subsample."""
+Bar: the root-cell grid is equal; the tree's counts (non-empty cells, deepest
+level, buckets) and the walk's counts (accepted cells, ion terms) are equal,
+which holds only if every cell and every acceptance decision is the
+reference algorithm's; the float accelerations agree to within 2 float ulps
+per component (the only difference allowed is the last-bit rounding of
+K q / m, which the reference forms with a long double division), for the
+plain, rotational and translational variants and the root-layout options;
+theta -> 0 reproduces the direct sum; at N = 1M the accuracy against the GPU
+FP64 direct sum is reported on a subsample."""
 import numpy as np
 import pytest
 
@@ -55,11 +60,75 @@ def test_bh_host_matches_oracle(eng, sym, theta):
     s = solveit_hip.make_symmetry(sym, nsec=4, lattice=kw.get("lattice"),
                                   domain=kw.get("domain", (0.0, 0.0, 0.0)))
     got = eng.bh_host(pos4, vel4, acc4, theta=theta, symmetry=s)
-    ref, _ = oracle.bh_ref(pos4, vel4, acc4, theta=theta, sym=sym, **kw)
+    ref, _, info = oracle.bh_ref(pos4, vel4, acc4, theta=theta, sym=sym, info=True, **kw)
     u = _ulps(got, ref)
     print(f"  max ulps {u.max():.1f}, exact {np.mean(u == 0):.4f}")
     assert np.all(got[d["status"] != 1] == 0)
     assert u.max() <= 2
+    _same_tree(eng.bh_stats(), info, pos4, acc4)
+
+
+def _same_tree(st, info, pos4, acc4, **lay):
+    roots, dims, nd, _ = oracle.bh_layout(pos4, acc4, **lay)
+    print(f"  grid {tuple(nd)}, nodes {st['nodes']:.0f}, depth {st['max_depth']:.0f}, "
+          f"accepted {st['accepted_cells']:.0f}, ion terms {st['ion_terms']:.0f}")
+    assert (st["nx"], st["ny"], st["nz"]) == tuple(float(v) for v in nd)
+    np.testing.assert_array_equal(np.float32([st["root_x"], st["root_y"], st["root_z"]]), roots)
+    np.testing.assert_array_equal(np.float32([st["dim_x"], st["dim_y"], st["dim_z"]]), dims)
+    assert st["root_cells"] == info["root_cells"]
+    assert st["nodes"] == info["nonempty_cells"]
+    assert st["max_depth"] == info["max_depth"]
+    assert st["buckets"] == info["buckets"]
+    assert st["unmatched_root"] == info["unmatched_root"]
+    assert st["accepted_cells"] == info["accepted_cells"]
+    assert st["ion_terms"] == info["ion_terms"]
+
+
+@pytest.mark.parametrize("opt", ["prism", "forced"])
+def test_bh_layout_options_match_oracle(eng, opt):
+    """setDomainToggle(rectangular prism) and setBoxDimVals on the cone."""
+    d = synth.cone(5000, seed=46)
+    pos4, vel4, acc4 = synth.to_reference_aos(d)
+    lay = dict(prism=True) if opt == "prism" else dict(box_dims=(2, 3, 12))
+    print(f"case bh layout option {opt}", flush=True)
+    try:
+        eng.bh_set_options(**lay)
+        got = eng.bh_host(pos4, vel4, acc4, theta=0.5)
+        st = eng.bh_stats()
+    finally:
+        eng.bh_set_options()
+    ref, _, info = oracle.bh_ref(pos4, vel4, acc4, theta=0.5, info=True, **lay)
+    assert _ulps(got, ref).max() <= 2
+    _same_tree(st, info, pos4, acc4, **lay)
+
+
+def test_bh_unsupported_options_are_reported(eng):
+    import solveit_hip
+    for kw in (dict(subcell_block=3), dict(outlier_toggle=1)):
+        with pytest.raises(solveit_hip.ShpError):
+            eng.bh_set_options(**kw)
+
+
+def test_bh_deep_pairs_and_coincident_ions(eng):
+    """Two ions 1e-15 m apart near the origin of a 1 um cloud need about 30
+    levels below the root cell (the second word of the keys); two coincident
+    ions share a bucket at level 42, in the oracle and on the GPU alike."""
+    rng = np.random.default_rng(47)
+    n = 600
+    x = rng.uniform(0, 1e-6, size=(n, 3))
+    x[0] = [1e-15, 3e-7, 3e-7]
+    x[1] = [2e-15, 3e-7, 3e-7]
+    x[2] = x[3] = [5e-7, 5e-7, 5e-7]
+    q = np.where(rng.random(n) < 0.2, -1.0, 1.0)
+    d = dict(x=x[:, 0], y=x[:, 1], z=x[:, 2], vx=np.zeros(n), vy=np.zeros(n), vz=np.zeros(n), q=q,
+             status=np.ones(n, np.int32), species=np.zeros(n, np.int32), t_inj=np.zeros(n))
+    pos4, vel4, acc4 = synth.to_reference_aos(d)
+    print("case bh deep pairs", flush=True)
+    got = eng.bh_host(pos4, vel4, acc4, theta=0.5)
+    ref, _, info = oracle.bh_ref(pos4, vel4, acc4, theta=0.5, info=True)
+    assert info["max_depth"] > 21 and info["buckets"] == 1
+    assert _ulps(got, ref).max() <= 2
+    _same_tree(eng.bh_stats(), info, pos4, acc4)
 
 
 def test_bh_dev_equals_host(eng):
@@ -103,9 +172,11 @@ def test_bh_subrange_matches_oracle(eng):
     d = synth.cone(4000, seed=45)
     pos4, vel4, acc4 = synth.to_reference_aos(d)
     print("case bh subrange", flush=True)
+    acc4[500, 3] = 0  # the first ion of the range still seeds the bounds
     got = eng.bh_host(pos4, vel4, acc4, i0=500, i1=3500, theta=0.5)
-    ref, _ = oracle.bh_ref(pos4, vel4, acc4, i0=500, i1=3500, theta=0.5)
+    ref, _, info = oracle.bh_ref(pos4, vel4, acc4, i0=500, i1=3500, theta=0.5, info=True)
     assert _ulps(got, ref).max() <= 2
+    _same_tree(eng.bh_stats(), info, pos4, acc4, i0=500, i1=3500)
 
 
 def test_bh_full_size_accuracy(eng):

@@ -94,10 +94,108 @@ def test_symmetry_theta_zero_is_direct(sym):
     assert l2 < 5e-5
 
 
-def test_root_contains_sources():
-    d = synth.cone(500, seed=35)
+S = 2.0 ** -20  # 9.5e-7 m: grid arithmetic in units of S stays exact in float
+
+
+def _box_cloud(n, ext, seed, corner_status=1):
+    """n ions in [0, ext] (units of S) with two ions on opposite corners, so
+    the bounding box is exactly [0, ext]."""
+    rng = np.random.default_rng(seed)
+    ext = np.asarray(ext, np.float64)
+    x = rng.uniform(0.05, 0.95, size=(n, 3)) * ext * S
+    x[0] = 0.0
+    x[1] = ext * S
+    q = np.where(rng.random(n) < 0.1, -1.0, 1.0)
+    st = np.ones(n)
+    st[:2] = corner_status
+    return _aos(x, q, status=st)
+
+
+@pytest.mark.parametrize("n,nd,size", [
+    # 1 x 1 x 4 box: cells of side 1 give 4 cells; >= 100 ions per cell needs n >= 397
+    (400, (1, 1, 4), 1.0),
+    # too few ions: side doubled to 2 (ratio 0.5 rounds up to 1 cell, 4 / 2 = 2 cells)
+    (300, (1, 1, 2), 2.0),
+    # doubled again to 4: one cell of the longest extent
+    (150, (1, 1, 1), 4.0),
+])
+def test_layout_hand_cases(n, nd, size):
+    """computeDomainBounds + computeWorldLimits on a 1 x 1 x 4 box (units of S),
+    derived by hand: the cell counts per axis, the cell side (size x (1 +
+    0.01 / n_long), n_long the count along the long axis) and the grid centred
+    on the box."""
+    pos4, vel4, acc4 = _box_cloud(n, (1, 1, 4), seed=n)
+    roots, dims, got_nd, ns = oracle.bh_layout(pos4, acc4)
+    assert ns == n
+    assert tuple(got_nd) == nd
+    side = np.float32(size * S) * (np.float32(1) + np.float32(0.01) / np.float32(nd[2]))
+    np.testing.assert_array_equal(dims, np.full(3, side, np.float32))
+    ext = np.array([1, 1, 4], np.float32) * np.float32(S)
+    expect_roots = (ext + np.float32(0) - side * np.asarray(nd, np.float32)) / np.float32(2)
+    np.testing.assert_array_equal(roots, expect_roots.astype(np.float32))
+
+
+def test_layout_forced_counts_merge():
+    """setBoxDimVals(2, 2, 4) on the 1 x 1 x 4 box: side max(1/2, 1/2, 4/4) = 1,
+    scaled by 1 + 0.01 / 4; computeWorldLimits then merges 2 x 2 x 2 cells
+    because the smallest count is 2, giving 1 x 1 x 2 cells of twice the side."""
+    pos4, vel4, acc4 = _box_cloud(400, (1, 1, 4), seed=7)
+    roots, dims, nd, _ = oracle.bh_layout(pos4, acc4, box_dims=(2, 2, 4))
+    assert tuple(nd) == (1, 1, 2)
+    side = np.float32(S) * (np.float32(1) + np.float32(0.01) / np.float32(4))
+    np.testing.assert_array_equal(dims, np.full(3, np.float32(2) * side, np.float32))
+
+
+def test_layout_prism_and_unmatched_root():
+    """Rectangular-prism root: one cell equal to the bounding box.  The ion on
+    the far corner gives (pos - roots) / dims = 1 on every axis, which matches
+    no cell; the reference then carries it to the last cell of the chain."""
+    pos4, vel4, acc4 = _box_cloud(50, (1, 1, 4), seed=8)
+    roots, dims, nd, _ = oracle.bh_layout(pos4, acc4, prism=True)
+    assert tuple(nd) == (1, 1, 1)
+    np.testing.assert_array_equal(roots, np.zeros(3, np.float32))
+    np.testing.assert_array_equal(dims, np.array([1, 1, 4], np.float32) * np.float32(S))
+    out, _, info = oracle.bh_ref(pos4, vel4, acc4, theta=0.5, prism=True, info=True)
+    assert info["root_cells"] == 1 and info["unmatched_root"] == 1
+    ref = oracle.direct_ref(pos4, vel4, acc4)
+    out0, _ = oracle.bh_ref(pos4, vel4, acc4, theta=0.0, prism=True)
+    assert oracle.rel_l2(out0, ref)[0] < 5e-5
+
+
+def test_layout_seeded_by_first_ion_of_range():
+    """The bounds start from the first ion of the range whatever its status
+    (domain_bounds_functions.cpp:139-150): an excluded far ion at index 0
+    widens the grid as a source there would, but adds no force."""
+    pos4, vel4, acc4 = _box_cloud(400, (1, 1, 4), seed=9)
+    far = np.array([0.5, 0.5, 12.0]) * S
+    pos4 = np.vstack([np.r_[far, 1.0][None].astype(np.float32), pos4])
+    vel4 = np.vstack([vel4[:1], vel4])
+    acc4 = np.vstack([acc4[:1], acc4])
+    acc4[0, 3] = 0  # out: neither target nor source
+    lay_out = oracle.bh_layout(pos4, acc4)
+    acc_src = acc4.copy()
+    acc_src[0, 3] = 2
+    lay_src = oracle.bh_layout(pos4, acc_src)
+    lay_none = oracle.bh_layout(pos4, acc4, i0=1)
+    for a, b in zip(lay_out[:3], lay_src[:3]):
+        np.testing.assert_array_equal(a, b)
+    top = lambda lay: lay[0][2] + lay[2][2] * lay[1][2]
+    assert top(lay_out) >= np.float32(far[2]) > top(lay_none)
+    out, _ = oracle.bh_ref(pos4, vel4, acc4, theta=0.0)
+    ref = oracle.direct_ref(pos4, vel4, acc4)
+    assert oracle.rel_l2(out, ref, acc4[:, 3] == 1)[0] < 5e-5
+
+
+def test_cone_root_grid_covers_sources():
+    """The cone plume is about 10 um long and 2-3 um wide: the reference lays
+    several cubic root cells along it, and every source lies inside the grid."""
+    d = synth.cone(6000, seed=42)
     pos4, vel4, acc4 = synth.to_reference_aos(d)
-    r = oracle.bh_root(pos4, acc4)
+    roots, dims, nd, ns = oracle.bh_layout(pos4, acc4)
+    assert nd.prod() > 1 and nd[2] == nd.max()
     m = (acc4[:, 3] == 1) | (acc4[:, 3] == 2)
-    assert r[3] > 0
-    assert np.all(pos4[m, :3] >= r[:3] - 1e-12) and np.all(pos4[m, :3] <= r[:3] + r[3] * (1 + 1e-6))
+    p = pos4[m, :3]
+    assert np.all(p >= roots) and np.all(p <= roots + nd * dims)
+    _, _, info = oracle.bh_ref(pos4, vel4, acc4, theta=0.5, info=True)
+    assert info["root_cells"] == nd.prod() and info["buckets"] == 0
+    assert info["max_depth"] < 42
