@@ -101,6 +101,21 @@ int shp_direct_host(shp_handle h, const float* pos4, const float* vel4, const fl
 int shp_direct_dev(shp_handle h, const shp_particles* p, int precision,
                    const shp_symmetry* sym, double* ax, double* ay, double* az);
 
+/* The same as shp_direct_dev, allocating nothing: every device buffer it
+ * needs (staged sources, target order, split partials, sort and selection
+ * storage) is carved from a per-device scratch array of
+ * SHP_NOALLOC_SCRATCH_BYTES that the library holds from shp_create until the
+ * process ends.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  n is at most SHP_NOALLOC_MAX_N and
+ * a symmetry has at most 1024 images; larger inputs return SHP_ERR_INVALID.
+ * Calls that overlap on one GPU share its scratch and race: serialise them.
+ * The results equal shp_direct_dev's bit for bit. */
+#define SHP_NOALLOC_MAX_N 262144
+#define SHP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int shp_direct_dev_noalloc(shp_handle h, const shp_particles* p, int precision,
+                           const shp_symmetry* sym, double* ax, double* ay, double* az);
+
 /* Kernel time of the last shp_direct_dev/shp_direct_host force kernel (ms),
  * measured with HIP events on the handle's stream (for shp_fmm_dev: the span
  * from P2M to the last leaf kernel). */

@@ -74,8 +74,13 @@ int shp_create(int device, shp_handle* out) {
     if (e == hipSuccess) e = hipMalloc((void**)&h->fmm_stats, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) {
         shp::set_error("shp_create: %s", hipGetErrorString(e));
-        delete h;
+        shp_destroy(h);
         return SHP_ERR_HIP;
+    }
+    const int rc = shp::direct_scratch_setup(device);  // per-device scratch of shp_direct_dev_noalloc
+    if (rc != SHP_OK) {
+        shp_destroy(h);
+        return rc;
     }
     *out = h;
     return SHP_OK;

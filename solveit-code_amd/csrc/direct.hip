@@ -19,6 +19,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "shp_internal.h"
@@ -28,12 +29,40 @@ namespace shp {
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+// Device buffers of one evaluation: the handle's grow-only workspace, or, for
+// the allocation-free entry point, consecutive 256-byte aligned pieces of the
+// per-device scratch array (base != nullptr).
+struct Scratch {
+    shp_handle h;
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    void* get(int slot, size_t bytes) {
+        if (!base) return h->get(slot, bytes);
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off + bytes > cap) {
+            set_error("shp_direct_dev_noalloc: the evaluation needs more than the %zu bytes of scratch", cap);
+            return nullptr;
+        }
+        used = off + bytes;
+        return base + off;
+    }
+};
+
+#define SC_GET(ptr, type, sc, slot, count)                                                \
+    type* ptr = static_cast<type*>((sc).get((slot), sizeof(type) * (size_t)(count)));     \
+    if (!ptr) return (sc).base ? SHP_ERR_INVALID : SHP_ERR_NOMEM
+
+// Most images the allocation-free entry point takes (rotational sectors or
+// lattice points).
+constexpr int kNoAllocMaxImages = 1024;
+
 // Per-image transforms: ROT (c, s), TRANS (ox, oy, oz).  For the REF contract
 // the translational offset is the float product lattice * (float)domain, as
 // direct_force_method_translational.cpp:324-327 computes it; rotations use
 // float cos/sin (exact integers for 4 sectors, auxiliary_functions.h:55-57).
-int build_images(shp_handle h, const shp_symmetry* sym, bool ref, const double4** out, int* nimg) {
-    std::vector<double4> im;
+template <typename Vec>
+static int build_images_into(shp_handle h, Scratch& sc, const shp_symmetry* sym, bool ref, Vec& im,
+                             const double4** out, int* nimg) {
     if (!sym || sym->type == SHP_SYM_NONE) {
         im.push_back(make_double4(1.0, 0.0, 0.0, 0.0));
     } else if (sym->type == SHP_SYM_ROTATIONAL) {
@@ -70,14 +99,44 @@ int build_images(shp_handle h, const shp_symmetry* sym, bool ref, const double4*
     } else {
         SHP_REQUIRE(false, "unknown symmetry type %d", sym->type);
     }
-    SHP_WS(d, double4, h, WS_IMAGES, im.size());
+    SC_GET(d, double4, sc, WS_IMAGES, im.size());
     SHP_HIP_CHECK(hipMemcpyAsync(d, im.data(), im.size() * sizeof(double4), hipMemcpyHostToDevice,
                                  h->stream));
-    // The host vector dies on return: wait for the copy to leave it.
+    // The host copy dies on return: wait for the copy to leave it.
     SHP_HIP_CHECK(hipStreamSynchronize(h->stream));
     *out = d;
     *nimg = (int)im.size();
     return SHP_OK;
+}
+
+// A fixed-capacity host list of images (no heap allocation), for the
+// allocation-free entry point.
+struct ImageList {
+    double4 v[kNoAllocMaxImages];
+    size_t n = 0;
+    void push_back(const double4& x) {
+        if (n < kNoAllocMaxImages) v[n] = x;
+        n++;
+    }
+    size_t size() const { return n; }
+    const double4* data() const { return v; }
+};
+
+int build_images(shp_handle h, const shp_symmetry* sym, bool ref, const double4** out, int* nimg,
+                 Scratch* scp = nullptr) {
+    Scratch ws{h};
+    Scratch& sc = scp ? *scp : ws;
+    if (sc.base) {
+        const int want = !sym || sym->type == SHP_SYM_NONE
+                             ? 1
+                             : (sym->type == SHP_SYM_ROTATIONAL ? sym->nsec : sym->nimg);
+        SHP_REQUIRE(want <= kNoAllocMaxImages, "shp_direct_dev_noalloc: %d images (at most %d)", want,
+                    kNoAllocMaxImages);
+        ImageList im;  // on the stack: no heap allocation
+        return build_images_into(h, sc, sym, ref, im, out, nimg);
+    }
+    std::vector<double4> im;
+    return build_images_into(h, sc, sym, ref, im, out, nimg);
 }
 
 static int choose_split(shp_handle h, int target_blocks, int nsrc_upper) {
@@ -88,20 +147,20 @@ static int choose_split(shp_handle h, int target_blocks, int nsrc_upper) {
 }
 
 // Compact [j0,j1) sources into device records; returns device count pointer.
-static int stage_sources_f64(shp_handle h, const double* X, const double* Y, const double* Z,
-                             const double* Q, const int* status, int j0, int j1,
+static int stage_sources_f64(shp_handle h, Scratch& sc, const double* X, const double* Y,
+                             const double* Z, const double* Q, const int* status, int j0, int j1,
                              const double4** src, const int** count) {
     const int n = j1 - j0;
-    SHP_WS(flags, char, h, WS_FLAGS, n);
-    SHP_WS(idx, int, h, WS_KEYS, n);
-    SHP_WS(cnt, int, h, WS_COUNTS, 4);
-    SHP_WS(rec, double4, h, WS_SRC, n);
+    SC_GET(flags, char, sc, WS_FLAGS, n);
+    SC_GET(idx, int, sc, WS_KEYS, n);
+    SC_GET(cnt, int, sc, WS_COUNTS, 4);
+    SC_GET(rec, double4, sc, WS_SRC, n);
     hipLaunchKernelGGL(k_flag_sources_f64, dim3(cdiv(n, 256)), dim3(256), 0, h->stream, status, j0,
                        n, flags);
     size_t tmp = 0;
     hipcub::CountingInputIterator<int> it(j0);
     SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, it, flags, idx, cnt, n, h->stream));
-    SHP_WS(cub, char, h, WS_CUB, tmp);
+    SC_GET(cub, char, sc, WS_CUB, tmp);
     SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(cub, tmp, it, flags, idx, cnt, n, h->stream));
     hipLaunchKernelGGL(k_gather_sources_f64, dim3(cdiv(n, 256)), dim3(256), 0, h->stream, idx, cnt,
                        X, Y, Z, Q, n, rec);
@@ -132,16 +191,16 @@ static int stage_sources_ref(shp_handle h, const float4* pos4, const float4* acc
 }
 
 // Target slots in Morton order (status-1 particles first, others last).
-static int morton_targets(shp_handle h, const double* X, const double* Y, const double* Z,
-                          const int* status, int i0, int i1, const int** tgt) {
+static int morton_targets(shp_handle h, Scratch& sc, const double* X, const double* Y,
+                          const double* Z, const int* status, int i0, int i1, const int** tgt) {
     const int n = i1 - i0;
     const int nb = std::min(cdiv(n, 256), 1024);
-    SHP_WS(bpart, double, h, WS_BBOX, nb * 6);
-    SHP_WS(keys, uint32_t, h, WS_KEYS, n);
-    SHP_WS(vals, int, h, WS_TGT, n);
-    SHP_WS(keys2, uint32_t, h, WS_KEYS2, n);
-    SHP_WS(vals2, int, h, WS_VALS2, n);
-    SHP_WS(box, double, h, WS_BOX, 8);
+    SC_GET(bpart, double, sc, WS_BBOX, nb * 6);
+    SC_GET(keys, uint32_t, sc, WS_KEYS, n);
+    SC_GET(vals, int, sc, WS_TGT, n);
+    SC_GET(keys2, uint32_t, sc, WS_KEYS2, n);
+    SC_GET(vals2, int, sc, WS_VALS2, n);
+    SC_GET(box, double, sc, WS_BOX, 8);
     hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(kThreads), 0, h->stream, X, Y, Z, status, i0,
                        n, bpart);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, h->stream, bpart, nb, box);
@@ -150,7 +209,7 @@ static int morton_targets(shp_handle h, const double* X, const double* Y, const 
     size_t tmp = 0;
     SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, keys2, vals, vals2, n, 0, 32,
                                                      h->stream));
-    SHP_WS(cub, char, h, WS_CUB, tmp);
+    SC_GET(cub, char, sc, WS_CUB, tmp);
     SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub, tmp, keys, keys2, vals, vals2, n, 0, 32,
                                                      h->stream));
     *tgt = vals2;
@@ -163,22 +222,25 @@ static int morton_targets(shp_handle h, const double* X, const double* Y, const 
 static int direct_f64_state(shp_handle h, int prec, const double* X, const double* Y,
                             const double* Z, const double* Q, const int* status, const int* species,
                             int i0, int i1, int j0, int j1, const shp_symmetry* sym, int mode,
-                            int nout, double* ax, double* ay, double* az, double* out3) {
+                            int nout, double* ax, double* ay, double* az, double* out3,
+                            Scratch* scp = nullptr) {
+    Scratch ws{h};
+    Scratch& sc = scp ? *scp : ws;
     const int nslots = i1 - i0;
     const double4* img;
     int nimg;
-    int rc = build_images(h, sym, false, &img, &nimg);
+    int rc = build_images(h, sym, false, &img, &nimg, &sc);
     if (rc) return rc;
     const double4* src;
     const int* nsrc;
-    rc = stage_sources_f64(h, X, Y, Z, Q, status, j0, j1, &src, &nsrc);
+    rc = stage_sources_f64(h, sc, X, Y, Z, Q, status, j0, j1, &src, &nsrc);
     if (rc) return rc;
     const int* tgt;
     if (prec == SHP_PREC_FP32C) {
-        rc = morton_targets(h, X, Y, Z, status, i0, i1, &tgt);
+        rc = morton_targets(h, sc, X, Y, Z, status, i0, i1, &tgt);
         if (rc) return rc;
     } else {
-        SHP_WS(ident, int, h, WS_TGT, nslots);
+        SC_GET(ident, int, sc, WS_TGT, nslots);
         hipLaunchKernelGGL(k_iota, dim3(cdiv(nslots, 256)), dim3(256), 0, h->stream, ident, i0,
                            nslots);
         tgt = ident;
@@ -186,7 +248,7 @@ static int direct_f64_state(shp_handle h, int prec, const double* X, const doubl
     const int per_block = prec == SHP_PREC_FP32C ? kThreads * kF32TargetsPerLane : kThreads;
     const int tb = cdiv(nslots, per_block);
     const int nsplit = choose_split(h, tb, j1 - j0);
-    SHP_WS(part, double, h, WS_PARTIAL, (size_t)nsplit * 3 * nslots);
+    SC_GET(part, double, sc, WS_PARTIAL, (size_t)nsplit * 3 * nslots);
     if (mode == 0) {
         hipLaunchKernelGGL(k_fill, dim3(cdiv(nout, 256)), dim3(256), 0, h->stream, ax, (size_t)nout, 0.0);
         hipLaunchKernelGGL(k_fill, dim3(cdiv(nout, 256)), dim3(256), 0, h->stream, ay, (size_t)nout, 0.0);
@@ -233,11 +295,66 @@ static int direct_f64_state(shp_handle h, int prec, const double* X, const doubl
     return SHP_OK;
 }
 
+// ------------------------------------------------ allocation-free variant ----
+
+// Scratch of shp_direct_dev_noalloc: one copy per GPU the library runs on,
+// held until the process ends.  Host code never uses this name as an address:
+// the per-device address comes from hipGetSymbolAddress at set-up.
+static __device__ __align__(256) char scratch[SHP_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[kMaxDevices] = {};
+
+// Looks up this device's copy of the scratch array (the device must be
+// current); called once per device from shp_create.
+int direct_scratch_setup(int device) {
+    SHP_REQUIRE(device >= 0 && device < kMaxDevices, "scratch set-up: device %d", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return SHP_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+        set_error("scratch set-up on device %d: hipGetSymbolAddress failed: %s", device, hipGetErrorString(e));
+        return SHP_ERR_HIP;
+    }
+    g_scratch_ptr[device] = (char*)ptr;
+    return SHP_OK;
+}
+
+static char* scratch_of(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 }  // namespace shp
 
 using namespace shp;
 
 extern "C" {
+
+int shp_direct_dev_noalloc(shp_handle h, const shp_particles* p, int precision, const shp_symmetry* sym,
+                           double* ax, double* ay, double* az) {
+    SHP_REQUIRE(h && p && ax && ay && az, "shp_direct_dev_noalloc: null argument");
+    SHP_REQUIRE(p->n >= 0 && p->n <= SHP_NOALLOC_MAX_N, "shp_direct_dev_noalloc: n=%d (at most %d)", p->n,
+                SHP_NOALLOC_MAX_N);
+    SHP_REQUIRE(precision == SHP_PREC_FP64 || precision == SHP_PREC_FP32C,
+                "shp_direct_dev_noalloc: precision %d not supported on FP64 state", precision);
+    if (p->n == 0) return SHP_OK;
+    SHP_REQUIRE(p->x && p->y && p->z && p->q && p->status && p->species,
+                "shp_direct_dev_noalloc: null particle array");
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    Scratch sc{h};
+    sc.base = scratch_of(h->device);
+    sc.cap = SHP_NOALLOC_SCRATCH_BYTES;
+    if (!sc.base) {
+        set_error("shp_direct_dev_noalloc: no scratch on device %d (handle not set up)", h->device);
+        return SHP_ERR_HIP;
+    }
+    return direct_f64_state(h, precision, p->x, p->y, p->z, p->q, p->status, p->species, 0, p->n, 0,
+                            p->n, sym, 0, p->n, ax, ay, az, nullptr, &sc);
+}
 
 int shp_direct_dev(shp_handle h, const shp_particles* p, int precision, const shp_symmetry* sym,
                    double* ax, double* ay, double* az) {

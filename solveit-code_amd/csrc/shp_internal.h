@@ -41,6 +41,7 @@ int lf_begin(shp_handle h, const shp_state* s, const shp_step_opts* o, double t0
 int lf_kick_drift(shp_handle h, const shp_state* s, const shp_step_opts* o, double t, const double* tdev,
                   double dt, int next, const double* ax, const double* ay, const double* az, int* eff);
 void free_mesh(shp_handle h);
+int direct_scratch_setup(int device);
 void fmm_release(shp_handle h);
 
 // Grow-only device workspace slots owned by a context.

@@ -195,6 +195,8 @@ def _load():
         "shp_bh_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, ctypes.c_float,
                                 ctypes.POINTER(Symmetry), dp]),
         "shp_bh_set_options": (c_int, [c_void_p, ctypes.POINTER(BhOptions)]),
+        "shp_direct_dev_noalloc": (c_int, [c_void_p, ctypes.POINTER(Particles), c_int,
+                                           ctypes.POINTER(Symmetry), c_void_p, c_void_p, c_void_p]),
         "shp_bh_stats": (c_int, [c_void_p, dp]),
     }
     for name, (res, args) in sig.items():
