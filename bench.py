@@ -10,9 +10,11 @@ kick/drift update.  Inputs are resident in HBM when the timed region starts.
 Also measured, on the same state: the force error of the FMM against the GPU
 FP64 direct sum on a seeded 8192-ion subsample (relative L2 as
 integration_schemes.cpp:411-455 defines it), one FP64 direct leapfrog step
-(the speed-up denominator), per-pass FMM kernel times, and a CPU baseline (the
-oracle's FP64 direct sum on a bounded target sample), and the relative drift of
-the total energy (kinetic + pair potential + field work) over the timed steps.
+(the speed-up denominator), per-pass FMM kernel times, the relative drift of
+the total energy (kinetic + pair potential + field work) over the timed steps,
+and a CPU baseline: the oracle's FP64 restatement of the same FMM on the host
+cores (bounded sample: every 16th leaf box's near field), beside the oracle's
+FP64 direct sum on a bounded target sample.
 Prints one JSON line.
 """
 import argparse
@@ -55,16 +57,45 @@ def copy_calibration(nbytes=256 << 20):
     del a, b
 
 
-def cpu_baseline(d, n_targets, threads=None):
-    """The oracle's FP64 direct sum (the reference's direct method restated,
-    one target per loop iteration) on the host cores: n_targets targets of the
-    same plume on one thread, then threads x n_targets targets split over
-    `threads` threads (ctypes releases the GIL), each scaled to
-    particle-steps/s of a full N x N direct step."""
+def host_cpu_info():
+    """nproc, the cores this process may run on, the box's CPU share used for
+    worker threads, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = nproc
+    # Worker threads: the cores available to this process, at most the host's
+    # CPU share per GPU (16 on these boxes; OMP_NUM_THREADS states it).
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
+    return {"nproc": nproc, "available": avail, "threads": max(1, min(avail, share)),
+            "model": model}
+
+
+def cpu_baseline(d, n_targets, fmm_args=None):
+    """CPU legs on the host cores, on bounded samples of the timed workload.
+
+    fmm (value, like for like): the oracle's FP64 restatement of the same FMM
+    (same order, shell orders, near ring and leaf level) on the plume state at
+    the start of the timed steps, OpenMP over the available cores: the tree and
+    far-field passes over every box, L2P + P2P on every k-th leaf box, scaled
+    to all leaf boxes.  direct: the oracle's FP64 direct sum (the reference's
+    direct method restated) on n_targets targets on 1 thread and on
+    threads x n_targets targets, scaled to one N x N step."""
     from concurrent.futures import ThreadPoolExecutor
     import oracle
+    info = host_cpu_info()
+    threads = info["threads"]
     n = d["x"].shape[0]
-    threads = threads or max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
     oracle.direct_f64(d, i0=0, i1=n_targets)
     dt1 = time.perf_counter() - t0
@@ -76,12 +107,35 @@ def cpu_baseline(d, n_targets, threads=None):
         list(ex.map(lambda b: oracle.direct_f64(d, i0=b[0], i1=b[1]), bounds))
     dtm = time.perf_counter() - t0
     ratem = nt * n / dtm
-    return {"value": ratem / n, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "value_1core": rate1 / n,
-            "sample": f"oracle FP64 direct sum over {n} sources of the same plume: {n_targets} "
-                      f"targets on 1 thread ({dt1:.1f} s, {rate1:.3e} pairs/s) and {nt} targets "
-                      f"on {threads} threads ({dtm:.1f} s, {ratem:.3e} pairs/s), scaled to one "
-                      f"N x N step"}
+    out = {"unit": "particle-steps/s", "cores": threads, "kind": "port",
+           "nproc": info["nproc"], "cores_available": info["available"], "cpu_model": info["model"],
+           "direct_value": ratem / n, "direct_value_1core": rate1 / n,
+           "direct_sample": f"oracle FP64 direct sum over {n} sources: {n_targets} targets on 1 thread "
+                            f"({dt1:.1f} s, {rate1:.3e} pairs/s) and {nt} targets on {threads} threads "
+                            f"({dtm:.1f} s, {ratem:.3e} pairs/s), scaled to one N x N step"}
+    if fmm_args is None:
+        out["value"] = out["direct_value"]
+        out["sample"] = out["direct_sample"]
+        return out
+    order, leaf_level, ring, shells, stride = fmm_args
+    oracle.set_threads(threads)
+    oracle.fmm_set_shell_orders(shells)
+    try:
+        _, st = oracle.fmm_f64(d, p=order, L=leaf_level, ring=ring, leaf_stride=stride)
+    finally:
+        oracle.fmm_set_shell_orders(None)
+        oracle.set_threads(0)
+    scale = st["leaves"] / max(1, st["leaves_evaluated"])
+    t_full = st["far_s"] + st["near_s"] * scale
+    out["value"] = float(n / t_full)
+    out["sample"] = (f"oracle FP64 FMM (p={order}, shell orders {shells[3:6]}, ring {ring}, leaf level "
+                     f"{leaf_level}) on the {n}-ion plume at the start of the timed steps, {threads} "
+                     f"OpenMP threads: tree + P2M/M2M/M2L/L2L over all boxes {st['far_s']:.1f} s, "
+                     f"L2P + P2P on every {stride}th of {st['leaves']} leaf boxes {st['near_s']:.1f} s "
+                     f"({st['p2p_pairs']:.3e} pairs), scaled to all leaf boxes: {t_full:.1f} s per step")
+    out["fmm_far_s"] = st["far_s"]
+    out["fmm_near_s_sampled"] = st["near_s"]
+    return out
 
 
 def _timed(fn, reps=2):
@@ -435,6 +489,8 @@ def main():
     ap.add_argument("--leaf-size", type=int, default=20)
     ap.add_argument("--dt", type=float, default=1e-12)
     ap.add_argument("--cpu-targets", type=int, default=1024)
+    ap.add_argument("--cpu-leaf-stride", type=int, default=16,
+                    help="CPU FMM leg: evaluate L2P + P2P on every k-th leaf box")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--config", type=int, default=0,
@@ -481,6 +537,8 @@ def main():
     ke0, pe0 = eng.energy(st)
     eng.sync()
     torch.cuda.synchronize()
+    cpu_state = None if args.no_cpu else st.to_numpy()  # plume state at the start of the timed steps
+    leaf0 = eng.fmm_leaf_level()
     eng.fmm_timing(reset=True)
     t0 = time.perf_counter()
     eng.leapfrog_run(st, opts, t, args.dt, args.steps, acc=acc)
@@ -587,7 +645,9 @@ def main():
         "roofline": roof,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(d, args.cpu_targets)
+        shells = [0, 0, 0] + list(solveit_hip.fmm_shell_orders(args.order, 2))
+        line["cpu_baseline"] = cpu_baseline(cpu_state, args.cpu_targets,
+                                            (args.order, leaf0, 2, shells, args.cpu_leaf_stride))
     eng.close()
     print(json.dumps(line), flush=True)
 

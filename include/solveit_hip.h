@@ -151,6 +151,21 @@ int shp_fmm_dev(shp_handle h, const shp_particles* p, int order, int leaf_level,
 int shp_fmm_sym_dev(shp_handle h, const shp_particles* p, const shp_symmetry* sym, int order,
                     int leaf_level, int leaf_size, int ring, double* ax, double* ay, double* az);
 
+/* The FMM on the reference's host AoS arrays, the drop-in for
+ * MultipoleMethodForceCalculation::updateAcceleration
+ * (multipole_method_force_calculation.cpp:148-182) and its rotational and
+ * translational siblings behind ForceCalculationMethod
+ * (force_calculation_methods.h:28-87): the ions [i0, i1) are the sources
+ * (status 1 or 2) and targets (status 1), as [firstParticle, lastParticle);
+ * order, leaf_level, leaf_size, ring and sym as for shp_fmm_sym_dev (sym NULL
+ * = no symmetry).  out3[3*(i-i0)+c] receives the acceleration (m/s^2) as
+ * double, zero rows for non-targets; the host arrays are not modified (the
+ * reference permutes and restores them, multipole_method_force_calculation.cpp:1086-1242).
+ * Waits for the result. */
+int shp_fmm_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4, int n,
+                 int i0, int i1, int order, int leaf_level, int leaf_size, int ring,
+                 const shp_symmetry* sym, double* out3);
+
 /* M2L expansion orders the engine uses on the Chebyshev shells 3, 4 and 5 of
  * the far band for an FMM of this order and near-field width (sym_type
  * SHP_SYM_ROTATIONAL: the rotational variant of shp_fmm_sym_dev, which keeps

@@ -430,16 +430,40 @@ lib.orc_fmm_key.restype = ctypes.c_uint64
 lib.orc_fmm_key.argtypes = [_cd, _cd, _cd, _vp, _ci]
 
 
-def fmm_f64(d, p, L, ring=1):
-    """FP64 FMM accelerations of the engine's algorithm; returns (acc[n,3], stats)."""
+lib.orc_set_threads.argtypes = [_ci]
+lib.orc_get_threads.restype = _ci
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (0: OpenMP's default)."""
+    lib.orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib.orc_get_threads()
+
+
+lib.orc_fmm_set_leaf_stride.argtypes = [_ci]
+
+
+def fmm_f64(d, p, L, ring=1, leaf_stride=1):
+    """FP64 FMM accelerations of the engine's algorithm; returns (acc[n,3], stats).
+    leaf_stride k > 1 evaluates L2P + P2P on every k-th leaf box only (the
+    other rows stay zero), for bounded timing samples."""
     x, y, z, q = _f64(d["x"]), _f64(d["y"]), _f64(d["z"]), _f64(d["q"])
     st, sp = _i32(d["status"]), _i32(d["species"])
     n = x.shape[0]
     out = np.zeros((n, 3))
-    stats = np.zeros(3)
-    lib.orc_fmm_f64(n, x.ctypes.data, y.ctypes.data, z.ctypes.data, q.ctypes.data,
-                    st.ctypes.data, sp.ctypes.data, p, L, ring, out.ctypes.data, stats.ctypes.data)
-    return out, {"leaves": int(stats[0]), "p2p_pairs": stats[1], "m2l": stats[2]}
+    stats = np.zeros(6)
+    lib.orc_fmm_set_leaf_stride(int(leaf_stride))
+    try:
+        lib.orc_fmm_f64(n, x.ctypes.data, y.ctypes.data, z.ctypes.data, q.ctypes.data,
+                        st.ctypes.data, sp.ctypes.data, p, L, ring, out.ctypes.data,
+                        stats.ctypes.data)
+    finally:
+        lib.orc_fmm_set_leaf_stride(1)
+    return out, {"leaves": int(stats[0]), "p2p_pairs": stats[1], "m2l": stats[2],
+                 "far_s": stats[3], "near_s": stats[4], "leaves_evaluated": int(stats[5])}
 
 
 def fmm_taylor(p, R, eps2):

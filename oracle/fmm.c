@@ -31,9 +31,22 @@
  *   |g| s T_g + (2|g|-1) sum_i R_i T_{g-e_i} + (|g|-1) sum_i T_{g-2e_i} = 0.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Threads of the box loops below (0: OpenMP's default).  Every loop runs
+ * over target boxes that own their outputs, so the result does not depend on
+ * the thread count. */
+static int g_threads = 0;
+void orc_set_threads(int n) { g_threads = n; }
+
+/* L2P + P2P on every k-th leaf box only (1: all), for bounded timing samples;
+ * the far-field passes always run over every box. */
+static int g_leaf_stride = 1;
+void orc_fmm_set_leaf_stride(int k) { g_leaf_stride = k > 1 ? k : 1; }
+int orc_get_threads(void) { return g_threads > 0 ? g_threads : omp_get_max_threads(); }
 
 void orc_direct_f64(int n, const double* x, const double* y, const double* z, const double* q,
                     const int* status, const int* species, int i0, int i1, int sym, int nsec,
@@ -208,11 +221,13 @@ static int cmp_ki(const void* a, const void* b) {
  * FMM accelerations for the active particles (status 1 targets, 1/2 sources).
  * Outputs out3 (double, zero for non-targets).  p: expansion order, L: leaf
  * level, ring: near-field width.  stats (may be 0) receives
- * {boxes at leaf level, P2P pairs, M2L interactions}.
+ * {boxes at leaf level, P2P pairs, M2L interactions, seconds of the tree and
+ * far-field passes (sort to L2L), seconds of L2P + P2P, leaf boxes evaluated}.
  */
 void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const double* q,
                  const int* status, const int* species, int p, int L, int ring, double* out3,
                  double* stats) {
+    const double t_start = omp_get_wtime();
     Basis B;
     basis_init(&B, p);
     const int nc = B.nc;
@@ -282,7 +297,9 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
         t.M[l] = calloc((size_t)t.nbox[l] * nc, sizeof(double));
         t.Lc[l] = calloc((size_t)t.nbox[l] * nc, sizeof(double));
     }
+    const int nt = orc_get_threads();
     /* P2M */
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
     for (int b = 0; b < t.nbox[L]; b++) {
         double c[3], h;
         box_center(&t, L, t.key[L][b], c, &h);
@@ -296,6 +313,7 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
     }
     /* M2M */
     for (int l = L - 1; l >= 0; l--) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
         for (int b = 0; b < t.nbox[l]; b++) {
             double cp[3], hp;
             box_center(&t, l, t.key[l][b], cp, &hp);
@@ -321,11 +339,34 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
             }
         }
     }
-    /* M2L */
+    /* M2L.  The weights C(a+b, a) (-1)^|a| and the index of T_{a+b} of every
+     * (b, a) term are tabulated once; each is the same product as written in
+     * the header, so the sums are unchanged. */
     double nm2l = 0, np2p = 0;
     const double eps2 = 6.4e-19;
-    double* T = malloc(sizeof(double) * nc);
+    double* W = malloc(sizeof(double) * (size_t)nc * nc);
+    int* TI = malloc(sizeof(int) * (size_t)nc * nc);
+    for (int bb = 0; bb < nc; bb++) {
+        int* Bm = B.mi[bb];
+        for (int a = 0; a < nc; a++) {
+            int* A = B.mi[a];
+            int na_ = A[0] + A[1] + A[2], nb_ = Bm[0] + Bm[1] + Bm[2];
+            double w = 0.0;
+            int ti = 0;
+            if (na_ + nb_ <= p) {
+                w = binom(A[0] + Bm[0], A[0]) * binom(A[1] + Bm[1], A[1]) * binom(A[2] + Bm[2], A[2]);
+                if (na_ & 1) w = -w;
+                ti = B.idx[A[0] + Bm[0]][A[1] + Bm[1]][A[2] + Bm[2]];
+            }
+            W[(size_t)bb * nc + a] = w;
+            TI[(size_t)bb * nc + a] = ti;
+        }
+    }
     for (int l = 1; l <= L; l++) {
+#pragma omp parallel num_threads(nt) reduction(+ : nm2l)
+        {
+        double* T = malloc(sizeof(double) * nc);
+#pragma omp for schedule(dynamic, 8)
         for (int b = 0; b < t.nbox[l]; b++) {
             uint64_t key = t.key[l][b];
             int64_t ix[3];
@@ -363,15 +404,14 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
                                 int* Bm = B.mi[bb];
                                 int nb_ = Bm[0] + Bm[1] + Bm[2];
                                 if (nb_ > pp) break;
+                                const double* Wb = W + (size_t)bb * nc;
+                                const int* Tb = TI + (size_t)bb * nc;
                                 double acc = 0.0;
                                 for (int a = 0; a < nc; a++) {
                                     int* A = B.mi[a];
                                     int na_ = A[0] + A[1] + A[2];
                                     if (na_ + nb_ > pp) break;
-                                    double w = binom(A[0] + Bm[0], A[0]) * binom(A[1] + Bm[1], A[1]) *
-                                               binom(A[2] + Bm[2], A[2]);
-                                    if (na_ & 1) w = -w;
-                                    acc += w * T[B.idx[A[0] + Bm[0]][A[1] + Bm[1]][A[2] + Bm[2]]] * Ms[a];
+                                    acc += Wb[a] * T[Tb[a]] * Ms[a];
                                 }
                                 Lb[bb] += acc;
                             }
@@ -379,10 +419,14 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
                         }
                     }
         }
+        free(T);
+        }
     }
-    free(T);
+    free(W);
+    free(TI);
     /* L2L */
     for (int l = 1; l < L; l++) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
         for (int b = 0; b < t.nbox[l]; b++) {
             double cp[3], hp;
             box_center(&t, l, t.key[l][b], cp, &hp);
@@ -408,9 +452,14 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
         }
     }
     /* L2P + P2P */
+    const double t_far = omp_get_wtime();
     double K = orc_k_const();
     for (int i = 0; i < 3 * n; i++) out3[i] = 0.0;
-    for (int b = 0; b < t.nbox[L]; b++) {
+    const int stride = g_leaf_stride;
+    const int nleaf_eval = (t.nbox[L] + stride - 1) / stride;
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nt) reduction(+ : np2p)
+    for (int bs = 0; bs < nleaf_eval; bs++) {
+        const int b = bs * stride;
         uint64_t key = t.key[L][b];
         double c[3], h;
         box_center(&t, L, key, c, &h);
@@ -458,7 +507,15 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
             out3[3 * i + 2] = az * kc;
         }
     }
-    if (stats) { stats[0] = t.nbox[L]; stats[1] = np2p; stats[2] = nm2l; }
+    const double t_end = omp_get_wtime();
+    if (stats) {
+        stats[0] = t.nbox[L];
+        stats[1] = np2p;
+        stats[2] = nm2l;
+        stats[3] = t_far - t_start;
+        stats[4] = t_end - t_far;
+        stats[5] = nleaf_eval;
+    }
     for (int l = 0; l <= L; l++) {
         free(t.key[l]); free(t.first[l]); free(t.count[l]); free(t.M[l]); free(t.Lc[l]);
     }

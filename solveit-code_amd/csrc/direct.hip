@@ -295,6 +295,18 @@ static int direct_f64_state(shp_handle h, int prec, const double* X, const doubl
     return SHP_OK;
 }
 
+// out3 rows (ax, ay, az) of the targets (status 1), zero rows for the others.
+__global__ void k_soa_to_rows(const double* __restrict__ ax, const double* __restrict__ ay,
+                              const double* __restrict__ az, const int* __restrict__ status, int m,
+                              double* __restrict__ out3) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const bool t = status[i] == 1;
+    out3[3 * (size_t)i] = t ? ax[i] : 0.0;
+    out3[3 * (size_t)i + 1] = t ? ay[i] : 0.0;
+    out3[3 * (size_t)i + 2] = t ? az[i] : 0.0;
+}
+
 // ------------------------------------------------ allocation-free variant ----
 
 // Scratch of shp_direct_dev_noalloc: one copy per GPU the library runs on,
@@ -443,6 +455,49 @@ int shp_direct_host(shp_handle h, const float* pos4, const float* vel4, const fl
     }
     SHP_HIP_CHECK(hipMemcpyAsync(out3, dout, sizeof(double) * 3 * (size_t)nt, hipMemcpyDeviceToHost,
                                  h->stream));
+    SHP_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return SHP_OK;
+}
+
+// FMM on the reference's host AoS arrays: the ions [i0, i1) are widened to
+// FP64 SoA on the device and evaluated by shp_fmm_sym_dev.
+int shp_fmm_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4, int n, int i0,
+                 int i1, int order, int leaf_level, int leaf_size, int ring, const shp_symmetry* sym,
+                 double* out3) {
+    SHP_REQUIRE(h && pos4 && vel4 && acc4 && out3, "shp_fmm_host: null argument");
+    SHP_REQUIRE(n >= 0 && 0 <= i0 && i0 <= i1 && i1 <= n, "shp_fmm_host: bad range n=%d [%d,%d)", n, i0, i1);
+    const int m = i1 - i0;
+    if (m == 0) return SHP_OK;
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    SHP_WS(dpos, float4, h, WS_HOST_IN, m);
+    SHP_WS(dvel, float4, h, WS_HOST_IN2, m);
+    SHP_WS(dacc, float4, h, WS_HOST_IN3, m);
+    SHP_WS(X, double, h, WS_WIDE_F64, 7 * (size_t)m);
+    SHP_WS(ints, int, h, WS_WIDE_I32, 2 * (size_t)m);
+    double *Y = X + m, *Z = Y + m, *Q = Z + m, *AX = Q + m, *AY = AX + m, *AZ = AY + m;
+    int *st = ints, *sp = ints + m;
+    SHP_WS(dout, double, h, WS_HOST_OUT, 3 * (size_t)m);
+    SHP_HIP_CHECK(hipMemcpyAsync(dpos, pos4 + 4 * (size_t)i0, sizeof(float4) * m, hipMemcpyHostToDevice,
+                                 h->stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(dvel, vel4 + 4 * (size_t)i0, sizeof(float4) * m, hipMemcpyHostToDevice,
+                                 h->stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(dacc, acc4 + 4 * (size_t)i0, sizeof(float4) * m, hipMemcpyHostToDevice,
+                                 h->stream));
+    hipLaunchKernelGGL(k_aos_to_f64, dim3(cdiv(m, 256)), dim3(256), 0, h->stream, dpos, dvel, dacc, m, X, Y,
+                       Z, Q, st, sp);
+    shp_particles p;
+    p.n = m;
+    p.x = X;
+    p.y = Y;
+    p.z = Z;
+    p.q = Q;
+    p.status = st;
+    p.species = sp;
+    const int rc = shp_fmm_sym_dev(h, &p, sym, order, leaf_level, leaf_size, ring, AX, AY, AZ);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_soa_to_rows, dim3(cdiv(m, 256)), dim3(256), 0, h->stream, AX, AY, AZ, st, m, dout);
+    SHP_HIP_CHECK(hipGetLastError());
+    SHP_HIP_CHECK(hipMemcpyAsync(out3, dout, sizeof(double) * 3 * (size_t)m, hipMemcpyDeviceToHost, h->stream));
     SHP_HIP_CHECK(hipStreamSynchronize(h->stream));
     return SHP_OK;
 }

@@ -195,6 +195,8 @@ def _load():
         "shp_bh_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, ctypes.c_float,
                                 ctypes.POINTER(Symmetry), dp]),
         "shp_bh_set_options": (c_int, [c_void_p, ctypes.POINTER(BhOptions)]),
+        "shp_fmm_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                 ctypes.POINTER(Symmetry), dp]),
         "shp_direct_dev_noalloc": (c_int, [c_void_p, ctypes.POINTER(Particles), c_int,
                                            ctypes.POINTER(Symmetry), c_void_p, c_void_p, c_void_p]),
         "shp_bh_stats": (c_int, [c_void_p, dp]),
@@ -354,6 +356,27 @@ class Engine:
                              acc4.ctypes.data_as(fp), n, i0, i1, float(theta), ctypes.byref(sym),
                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         _check(rc, "shp_bh_host")
+        return out
+
+    def fmm_host(self, pos4, vel4, acc4, i0=None, i1=None, order=10, leaf_level=0, leaf_size=20,
+                 ring=2, symmetry=None):
+        """FMM on host AoS float32 arrays of shape (n, 4); ions [i0, i1) are sources
+        and targets.  Returns the (i1-i0, 3) float64 acceleration (zero rows for
+        non-targets)."""
+        pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+        vel4 = np.ascontiguousarray(vel4, dtype=np.float32)
+        acc4 = np.ascontiguousarray(acc4, dtype=np.float32)
+        n = pos4.shape[0]
+        i0 = 0 if i0 is None else i0
+        i1 = n if i1 is None else i1
+        out = np.zeros((max(i1 - i0, 0), 3), dtype=np.float64)
+        fp = ctypes.POINTER(ctypes.c_float)
+        sym = symmetry if symmetry is not None else make_symmetry("none")
+        rc = lib.shp_fmm_host(self.h, pos4.ctypes.data_as(fp), vel4.ctypes.data_as(fp),
+                              acc4.ctypes.data_as(fp), n, i0, i1, int(order), int(leaf_level),
+                              int(leaf_size), int(ring), ctypes.byref(sym),
+                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        _check(rc, "shp_fmm_host")
         return out
 
     def bh_set_options(self, prism=False, box_dims=None, subcell_block=2, outlier_toggle=0):
