@@ -365,6 +365,20 @@ typedef struct shp_step_opts {
 int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double t0, double dt,
                      int nsteps, double* ax, double* ay, double* az);
 
+/* Resident state on the reference's host arrays, for an Integrator subclass
+ * (HipLeapfrogDKD) that keeps the plume on the GPU between steps:
+ *  - shp_state_upload_host copies position (x, y, z, q), velocity (vx, vy,
+ *    vz, species), acceleration (ax, ay, az, status) and timeInjected
+ *    (particles.h:99-104) for ions [0, n) into the handle's FP64 state;
+ *  - shp_state_run advances it by nsteps of shp_leapfrog_run from t0;
+ *  - shp_state_download_host writes positions, velocities, the last Coulomb
+ *    acceleration and the status back as float (q and species unchanged).
+ * The upload and download wait for their copies. */
+int shp_state_upload_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4,
+                          const float* t_inj, int n);
+int shp_state_run(shp_handle h, const shp_step_opts* o, double t0, double dt, int nsteps);
+int shp_state_download_host(shp_handle h, float* pos4, float* vel4, float* acc4);
+
 /* Kernel time (ms) of the last leapfrog update kernel. */
 double shp_last_update_ms(shp_handle h);
 

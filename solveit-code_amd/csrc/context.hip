@@ -99,6 +99,9 @@ int shp_destroy(shp_handle h) {
     for (int k = 0; k < 8; k++)
         if (h->fev[k]) (void)hipEventDestroy(h->fev[k]);
     if (h->fmm_stats) (void)hipFree(h->fmm_stats);
+    if (h->res_f64) (void)hipFree(h->res_f64);
+    if (h->res_i32) (void)hipFree(h->res_i32);
+    if (h->res_aos) (void)hipFree(h->res_aos);
     shp::free_mesh(h);
     shp::fmm_release(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);

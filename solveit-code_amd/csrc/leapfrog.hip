@@ -503,6 +503,47 @@ int lf_kick_drift(shp_handle h, const shp_state* s, const shp_step_opts* o, doub
     return SHP_OK;
 }
 
+
+// Resident state from the reference's float AoS arrays: FP64 positions,
+// velocities, charge, injection time; status from acceleration.w, species
+// from velocity.w; the acceleration starts at the array's xyz.
+__global__ void k_res_from_aos(const float4* __restrict__ pos4, const float4* __restrict__ vel4,
+                               const float4* __restrict__ acc4, const float* __restrict__ tinj, int n,
+                               double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                               double* q, double* ti, int* status, int* species, double* ax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos4[i], v = vel4[i], a = acc4[i];
+    x[i] = p.x;
+    y[i] = p.y;
+    z[i] = p.z;
+    q[i] = p.w;
+    vx[i] = v.x;
+    vy[i] = v.y;
+    vz[i] = v.z;
+    ti[i] = tinj[i];
+    status[i] = a.w == 1.0f ? 1 : (a.w == 2.0f ? 2 : (a.w == 0.0f ? 0 : -1));
+    species[i] = (int)v.w;
+    ax[i] = a.x;
+    ax[n + i] = a.y;
+    ax[2 * (size_t)n + i] = a.z;
+}
+
+// Back to float AoS: position (x, y, z, q), velocity (vx, vy, vz, species),
+// acceleration (the last Coulomb acceleration, status).
+__global__ void k_res_to_aos(const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, const double* __restrict__ vx,
+                             const double* __restrict__ vy, const double* __restrict__ vz,
+                             const double* __restrict__ ax, const double* __restrict__ ay,
+                             const double* __restrict__ az, const int* __restrict__ status, int n,
+                             float4* __restrict__ pos4, float4* __restrict__ vel4, float4* __restrict__ acc4) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pos4[i] = make_float4((float)x[i], (float)y[i], (float)z[i], pos4[i].w);
+    vel4[i] = make_float4((float)vx[i], (float)vy[i], (float)vz[i], vel4[i].w);
+    acc4[i] = make_float4((float)ax[i], (float)ay[i], (float)az[i], (float)status[i]);
+}
+
 }  // namespace shp
 
 using namespace shp;
@@ -736,6 +777,78 @@ int shp_compare_accel(shp_handle h, int n, const int* status, const double* ax, 
     SHP_HIP_CHECK(hipGetLastError());
     SHP_HIP_CHECK(hipMemcpyAsync(out, dout, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
     SHP_HIP_CHECK(hipStreamSynchronize(st));
+    return SHP_OK;
+}
+
+
+// ---------------------------------------------------------- resident state ----
+
+int shp_state_upload_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4,
+                          const float* t_inj, int n) {
+    SHP_REQUIRE(h && pos4 && vel4 && acc4 && t_inj, "shp_state_upload_host: null argument");
+    SHP_REQUIRE(n >= 0, "shp_state_upload_host: n=%d", n);
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    if (n > h->res_cap) {
+        SHP_HIP_CHECK(hipStreamSynchronize(h->stream));
+        if (h->res_f64) SHP_HIP_CHECK(hipFree(h->res_f64));
+        if (h->res_i32) SHP_HIP_CHECK(hipFree(h->res_i32));
+        if (h->res_aos) SHP_HIP_CHECK(hipFree(h->res_aos));
+        h->res_f64 = nullptr;
+        h->res_i32 = nullptr;
+        h->res_aos = nullptr;
+        h->res_cap = 0;
+        SHP_HIP_CHECK(hipMalloc((void**)&h->res_f64, sizeof(double) * 11 * (size_t)n));
+        SHP_HIP_CHECK(hipMalloc((void**)&h->res_i32, sizeof(int) * 2 * (size_t)n));
+        SHP_HIP_CHECK(hipMalloc((void**)&h->res_aos, sizeof(float4) * 3 * (size_t)n + sizeof(float) * (size_t)n));
+        h->res_cap = n;
+    }
+    h->res_n = n;
+    if (n == 0) return SHP_OK;
+    float4* dpos = h->res_aos;
+    float4* dvel = dpos + n;
+    float4* dacc = dvel + n;
+    float* dti = (float*)(dacc + n);
+    SHP_HIP_CHECK(hipMemcpyAsync(dpos, pos4, sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(dvel, vel4, sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(dacc, acc4, sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(dti, t_inj, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+    double* f = h->res_f64;
+    hipLaunchKernelGGL(k_res_from_aos, dim3(cdiv_l(n, 256)), dim3(256), 0, h->stream, dpos, dvel, dacc, dti, n,
+                       f, f + n, f + 2 * (size_t)n, f + 3 * (size_t)n, f + 4 * (size_t)n, f + 5 * (size_t)n,
+                       f + 6 * (size_t)n, f + 7 * (size_t)n, h->res_i32, h->res_i32 + n, f + 8 * (size_t)n);
+    SHP_HIP_CHECK(hipGetLastError());
+    SHP_HIP_CHECK(hipStreamSynchronize(h->stream));  // the host arrays may change after return
+    return SHP_OK;
+}
+
+int shp_state_run(shp_handle h, const shp_step_opts* o, double t0, double dt, int nsteps) {
+    SHP_REQUIRE(h && o, "shp_state_run: null argument");
+    SHP_REQUIRE(h->res_f64 || h->res_n == 0, "shp_state_run: no resident state (shp_state_upload_host)");
+    const int n = h->res_n;
+    if (n == 0 || nsteps == 0) return SHP_OK;
+    double* f = h->res_f64;
+    shp_state s{n, f, f + n, f + 2 * (size_t)n, f + 3 * (size_t)n, f + 4 * (size_t)n, f + 5 * (size_t)n,
+                f + 6 * (size_t)n, h->res_i32, h->res_i32 + n, f + 7 * (size_t)n};
+    return shp_leapfrog_run(h, &s, o, t0, dt, nsteps, f + 8 * (size_t)n, f + 9 * (size_t)n, f + 10 * (size_t)n);
+}
+
+int shp_state_download_host(shp_handle h, float* pos4, float* vel4, float* acc4) {
+    SHP_REQUIRE(h && pos4 && vel4 && acc4, "shp_state_download_host: null argument");
+    const int n = h->res_n;
+    if (n == 0) return SHP_OK;
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    float4* dpos = h->res_aos;
+    float4* dvel = dpos + n;
+    float4* dacc = dvel + n;
+    const double* f = h->res_f64;
+    hipLaunchKernelGGL(k_res_to_aos, dim3(cdiv_l(n, 256)), dim3(256), 0, h->stream, f, f + n, f + 2 * (size_t)n,
+                       f + 3 * (size_t)n, f + 4 * (size_t)n, f + 5 * (size_t)n, f + 8 * (size_t)n,
+                       f + 9 * (size_t)n, f + 10 * (size_t)n, h->res_i32, n, dpos, dvel, dacc);
+    SHP_HIP_CHECK(hipGetLastError());
+    SHP_HIP_CHECK(hipMemcpyAsync(pos4, dpos, sizeof(float4) * n, hipMemcpyDeviceToHost, h->stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(vel4, dvel, sizeof(float4) * n, hipMemcpyDeviceToHost, h->stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(acc4, dacc, sizeof(float4) * n, hipMemcpyDeviceToHost, h->stream));
+    SHP_HIP_CHECK(hipStreamSynchronize(h->stream));
     return SHP_OK;
 }
 

@@ -157,6 +157,12 @@ struct shp_context {
     bool bh_have_stats = false;
     unsigned long long* bh_ctr = nullptr;  // device counters in the workspace
     double bh_host_stats[11] = {};
+    // Resident state of shp_state_upload_host / shp_state_run (own buffers,
+    // apart from the workspace that the force evaluations use).
+    double* res_f64 = nullptr;  // x y z vx vy vz q t_inj ax ay az
+    int* res_i32 = nullptr;     // status, species
+    float4* res_aos = nullptr;  // host AoS staging: position, velocity, acceleration, t_inj
+    int res_n = 0, res_cap = 0;
     void* ws[shp::WS_NSLOTS] = {};
     size_t ws_bytes[shp::WS_NSLOTS] = {};
     int num_cus = 256;

@@ -195,6 +195,9 @@ def _load():
         "shp_bh_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, ctypes.c_float,
                                 ctypes.POINTER(Symmetry), dp]),
         "shp_bh_set_options": (c_int, [c_void_p, ctypes.POINTER(BhOptions)]),
+        "shp_state_upload_host": (c_int, [c_void_p, fp, fp, fp, fp, c_int]),
+        "shp_state_run": (c_int, [c_void_p, ctypes.POINTER(StepOpts), c_double, c_double, c_int]),
+        "shp_state_download_host": (c_int, [c_void_p, fp, fp, fp]),
         "shp_fmm_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                  ctypes.POINTER(Symmetry), dp]),
         "shp_direct_dev_noalloc": (c_int, [c_void_p, ctypes.POINTER(Particles), c_int,
@@ -378,6 +381,25 @@ class Engine:
                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         _check(rc, "shp_fmm_host")
         return out
+
+    def state_upload_host(self, pos4, vel4, acc4, t_inj):
+        """Copies the reference's host arrays into the engine's resident FP64 state."""
+        fp = ctypes.POINTER(ctypes.c_float)
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (pos4, vel4, acc4, t_inj)]
+        _check(lib.shp_state_upload_host(self.h, *(a.ctypes.data_as(fp) for a in arrs), arrs[0].shape[0]),
+               "shp_state_upload_host")
+
+    def state_run(self, opts, t0, dt, nsteps):
+        _check(lib.shp_state_run(self.h, ctypes.byref(opts), float(t0), float(dt), int(nsteps)),
+               "shp_state_run")
+
+    def state_download_host(self, pos4, vel4, acc4):
+        """Writes the resident state back into float32 (n, 4) arrays (in place)."""
+        fp = ctypes.POINTER(ctypes.c_float)
+        for a in (pos4, vel4, acc4):
+            assert a.dtype == np.float32 and a.flags.c_contiguous
+        _check(lib.shp_state_download_host(self.h, pos4.ctypes.data_as(fp), vel4.ctypes.data_as(fp),
+                                           acc4.ctypes.data_as(fp)), "shp_state_download_host")
 
     def bh_set_options(self, prism=False, box_dims=None, subcell_block=2, outlier_toggle=0):
         """Root-layout options of later Barnes-Hut calls (BarnesHutForceCalculation's

@@ -208,3 +208,54 @@ def test_leaf_level_choice_counts(eng):
             assert sum(r["far_pairs"] for r in rows if r["level"] <= L) == s["m2l_pairs"]
     eng.fmm_dev(st, order=10, leaf_level=5, ring=2)
     assert eng.fmm_level_estimates() == []
+
+
+def _float_state(d):
+    """The plume with float-representable positions and charges, so that the
+    host AoS entry (float positions, as the reference stores them) and the
+    FP64 oracle see the same ions."""
+    for k in ("x", "y", "z", "q"):
+        d[k] = d[k].astype(np.float32).astype(np.float64)
+    return d
+
+
+@pytest.mark.parametrize("p", [6, 10])
+def test_fmm_host_matches_oracle_fmm(eng, p):
+    """shp_fmm_host (the reference's vec4<float> AoS, [first, last) ranges)
+    against the oracle's FP64 FMM of the same tree: rel L2 <= 1e-5."""
+    d = _float_state(synth.cone(8192, seed=24))
+    d["status"][::23] = 2
+    d["status"][::29] = 0
+    pos4, vel4, acc4 = synth.to_reference_aos(d)
+    print(f"case fmm host p={p}", flush=True)
+    got = eng.fmm_host(pos4, vel4, acc4, order=p, leaf_level=5, ring=2)
+    ref, _ = _oracle_fmm(d, p, 5, 2)
+    l2, mx, _ = oracle.rel_l2(got, ref, d["status"] == 1)
+    print(f"  p={p}: host fmm vs oracle fmm rel L2 {l2:.3e} max {mx:.3e}")
+    assert l2 <= 1e-5
+    assert np.all(got[d["status"] != 1] == 0)
+    # the host arrays are left as they were
+    p2, v2, a2 = synth.to_reference_aos(d)
+    assert np.array_equal(pos4, p2) and np.array_equal(vel4, v2) and np.array_equal(acc4, a2)
+
+
+def test_fmm_host_subrange_and_symmetry(eng):
+    """[i0, i1) evaluates the sub-plume alone (the ions outside the range are
+    neither sources nor targets), and the symmetric variant equals
+    shp_fmm_sym_dev on the same ions."""
+    import solveit_hip
+    import torch
+    d = _float_state(synth.cone(6000, seed=25))
+    pos4, vel4, acc4 = synth.to_reference_aos(d)
+    print("case fmm host subrange", flush=True)
+    got = eng.fmm_host(pos4, vel4, acc4, i0=1000, i1=5000, order=8, leaf_level=4, ring=2)
+    sub = {k: v[1000:5000].copy() for k, v in d.items()}
+    ref, _ = _oracle_fmm(sub, 8, 4, 2)
+    assert oracle.rel_l2(got, ref)[0] <= 1e-5
+    print("case fmm host rotational", flush=True)
+    s = solveit_hip.make_symmetry("rotational", nsec=6)
+    got = eng.fmm_host(pos4, vel4, acc4, order=10, leaf_level=4, ring=2, symmetry=s)
+    st = solveit_hip.DeviceState.from_numpy(d)
+    torch.cuda.synchronize()
+    dev = torch.stack(eng.fmm_dev(st, order=10, leaf_level=4, ring=2, symmetry=s), 1).cpu().numpy()
+    np.testing.assert_array_equal(got, dev)

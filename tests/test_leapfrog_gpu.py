@@ -152,3 +152,41 @@ def test_leapfrog_fp32c_trajectory(eng, mesh):
           f"p99 {np.percentile(r, 99):.3e} max {np.max(r):.3e}")
     assert np.median(r) < 1e-7
     assert np.percentile(r, 99) < 1e-4
+
+
+def test_resident_state_on_host_arrays(eng, mesh):
+    """shp_state_upload_host / shp_state_run / shp_state_download_host (the
+    HipLeapfrogDKD adapter's path on the driver's float arrays) equal
+    shp_leapfrog_run on the same FP64 state, rounded to float, with injection
+    and boundary conditions."""
+    import solveit_hip
+    import torch
+    d = synth.cone(1500, seed=12)
+    dt = 1e-12
+    d["t_inj"] = np.sort(np.random.default_rng(2).uniform(0, 6 * dt, 1500))
+    for k in ("x", "y", "z", "vx", "vy", "vz", "q", "t_inj"):
+        d[k] = d[k].astype(np.float32).astype(np.float64)
+    pos4, vel4, acc4 = synth.to_reference_aos(d)
+    opts = solveit_hip.make_step_opts(method=solveit_hip.SHP_METHOD_DIRECT,
+                                      precision=solveit_hip.SHP_PREC_FP64, use_efield=True,
+                                      bc_space=(0, 0, 9.0e-6, 2e-7), bc_time=8 * dt)
+    print("case resident state", flush=True)
+    eng.state_upload_host(pos4, vel4, acc4, d["t_inj"])
+    for k in range(3):  # one step per call, as the adapter runs them
+        eng.state_run(opts, k * dt, dt, 1)
+    eng.state_download_host(pos4, vel4, acc4)
+    st = solveit_hip.DeviceState.from_numpy(d)
+    torch.cuda.synchronize()
+    acc = eng.leapfrog_run(st, opts, 0.0, dt, 1)
+    for k in range(1, 3):
+        acc = eng.leapfrog_run(st, opts, k * dt, dt, 1, acc=acc)
+    ref = st.to_numpy()
+    a = torch.stack(acc, 1).cpu().numpy()
+    np.testing.assert_array_equal(pos4[:, 0], ref["x"].astype(np.float32))
+    np.testing.assert_array_equal(pos4[:, 2], ref["z"].astype(np.float32))
+    np.testing.assert_array_equal(vel4[:, 1], ref["vy"].astype(np.float32))
+    np.testing.assert_array_equal(acc4[:, 3], ref["status"].astype(np.float32))
+    np.testing.assert_array_equal(acc4[:, :3], a.astype(np.float32))
+    np.testing.assert_array_equal(pos4[:, 3], d["q"].astype(np.float32))
+    np.testing.assert_array_equal(vel4[:, 3], d["species"].astype(np.float32))
+    assert np.any(ref["status"] == 0)
