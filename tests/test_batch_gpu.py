@@ -218,3 +218,103 @@ def test_batched_adjoint_directional_fd(eng, prec):
     assert worst < (1e-5 if prec == "fp64" else 1e-3)
     plan.close()
     plan64.close()
+
+
+def test_batch_forces_and_leapfrog_match_oracle(eng):
+    """Each instance of a batch against the oracle itself: the batched FP64
+    forces equal oracle.direct_f64 to 1e-12 (FP32C within 5e-6), and 12
+    graph-captured batched leapfrog steps equal oracle.leapfrog_run_f64 to
+    1e-10 per instance."""
+    import solveit_hip
+    import torch
+    sizes = [700, 1900, 257, 1300, 1]
+    parts, allb, off = _batch(sizes, seed=111)
+    st = solveit_hip.DeviceState.from_numpy(allb)
+    torch.cuda.synchronize()
+    print("case batch vs oracle forces", flush=True)
+    ref = [oracle.direct_f64(d) for d in parts]
+    for prec, tol in ((solveit_hip.SHP_PREC_FP64, 1e-12), (solveit_hip.SHP_PREC_FP32C, 5e-6)):
+        plan = eng.batch(off, prec)
+        got = torch.stack(plan.forces(st), 1).cpu().numpy()
+        plan.close()
+        for k, d in enumerate(parts):
+            g = got[off[k]:off[k + 1]]
+            m = d["status"] == 1
+            if prec == solveit_hip.SHP_PREC_FP64:
+                assert np.linalg.norm(g - ref[k]) <= tol * np.linalg.norm(ref[k]) + 1e-300, k
+            elif m.sum() > 1:
+                assert oracle.rel_l2(g, ref[k], m)[0] <= tol, k
+    print("case batch vs oracle leapfrog", flush=True)
+    dt, steps = 1e-12, 12
+    opts = solveit_hip.make_step_opts(method=solveit_hip.SHP_METHOD_DIRECT,
+                                      precision=solveit_hip.SHP_PREC_FP64)
+    plan = eng.batch(off, solveit_hip.SHP_PREC_FP64)
+    st = solveit_hip.DeviceState.from_numpy(allb)
+    torch.cuda.synchronize()
+    plan.leapfrog(st, opts, 0.0, dt, steps, use_graph=True)
+    got = st.to_numpy()
+    plan.close()
+    for k, d in enumerate(parts):
+        r, _ = oracle.leapfrog_run_f64(d, 0.0, dt, steps)
+        for c in ("x", "y", "z", "vx", "vy", "vz"):
+            g = got[c][off[k]:off[k + 1]]
+            scale = np.max(np.abs(r[c])) + 1e-300
+            assert np.max(np.abs(g - r[c])) <= 1e-10 * scale, (k, c)
+        np.testing.assert_array_equal(got["status"][off[k]:off[k + 1]], r["status"])
+
+
+def test_batched_adjoint_matches_oracle_config5_shape(eng):
+    """The batched checkpointed adjoint at the config-5 schedule (K = 200
+    leapfrog steps, a checkpoint every 20) on 8 plumes of uneven N drawn
+    lognormally in [256, 2048]: the gradient of the mean sin^2 of the
+    velocity divergence angle with respect to charges and initial velocities
+    equals, instance by instance, the oracle's FP64 discrete adjoint
+    (oracle.rollout_grad_f64) to 1e-8."""
+    from concurrent.futures import ThreadPoolExecutor
+    import solveit_hip
+    import torch
+    rng = np.random.default_rng(5)
+    sizes = [int(v) for v in np.clip(np.round(np.exp(rng.normal(np.log(700), 0.6, 8))), 256, 2048)]
+    parts, allb, off = _batch(sizes, seed=121)
+    for d in parts:
+        d["status"][d["status"] == 0] = 1
+    allb["status"][allb["status"] == 0] = 1
+    base = {k: _t(allb[k]) for k in ("x", "y", "z")}
+    base["status"] = _t(allb["status"], torch.int32)
+    base["species"] = _t(allb["species"], torch.int32)
+    dt, nsteps, K = 1e-14, 200, 20
+    plan = eng.batch(off, solveit_hip.SHP_PREC_FP64)
+    opts = solveit_hip.make_step_opts(method=solveit_hip.SHP_METHOD_DIRECT,
+                                      precision=solveit_hip.SHP_PREC_FP64)
+    mask = base["status"] == 1
+    Roll = _batch_rollout_fn()
+    q = _t(allb["q"]).requires_grad_(True)
+    v = [_t(allb[k]).requires_grad_(True) for k in ("vx", "vy", "vz")]
+    print(f"case batched adjoint vs oracle, sizes {sizes}", flush=True)
+    J = _sin2(*Roll.apply(q, *v, plan, base, opts, dt, nsteps, K), mask)
+    J.backward()
+    plan.close()
+    gq = q.grad.cpu().numpy()
+    gv = np.stack([t.grad.cpu().numpy() for t in v], 1)
+    M = int(mask.sum().item())
+
+    def oracle_grad(k):
+        d = parts[k]
+        kick = d["status"] == 1
+
+        def gfin(x, vel):
+            vx, vy, vz = vel[:, 0], vel[:, 1], vel[:, 2]
+            a = vx * vx + vy * vy
+            b = a + vz * vz
+            g = np.stack([2 * vx * vz * vz / b ** 2, 2 * vy * vz * vz / b ** 2, -2 * a * vz / b ** 2], 1)
+            return np.zeros_like(x), np.where(kick[:, None], g / M, 0.0)
+
+        return oracle.rollout_grad_f64(d, dt, nsteps, gfin)
+
+    with ThreadPoolExecutor(8) as ex:
+        res = list(ex.map(oracle_grad, range(len(parts))))
+    for k, (xb, vb, qb, _) in enumerate(res):
+        sl = slice(off[k], off[k + 1])
+        assert np.linalg.norm(gq[sl] - qb) <= 1e-8 * np.linalg.norm(qb), k
+        assert np.linalg.norm(gv[sl] - vb) <= 1e-8 * np.linalg.norm(vb), k
+    print(f"  {len(parts)} instances, {sum(sizes)} ions: adjoint equals the oracle's")
