@@ -365,6 +365,32 @@ typedef struct shp_step_opts {
 int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double t0, double dt,
                      int nsteps, double* ax, double* ay, double* az);
 
+/* LeapfrogDKDSubsteps (integration_schemes.cpp:899-1529, class at
+ * integration_schemes.h:104-161): multi-rate DKD with levels 0..max_level of
+ * time step dt_l = dt_{l-1} / 3.  levels[l] is the level's row of the
+ * reference's substepBoundaryConditions table after setSubstepTimes
+ * (:1375-1460): the boundary-condition type (BoundaryConditionOptions: 0 none,
+ * 1 space, 2 time, 3 space and time), the spatial limits (|x|, |y|, |z|, rho;
+ * 0 = none) and the time-since-injection limit.  level (device, n ints, in
+ * and out) is particleLevels: start every ion at max_level
+ * (initializeParticleLevels :1481-1489).  Per step all injected ions drift
+ * together in steps of dt_max / 2; each of the 3^max_level finest kicks sets
+ * the status of the injected ions from their levels (setParticleStatus
+ * :1193-1211), evaluates the force (o when the kick's top level is 0, sub
+ * otherwise, NULL = o) and kicks the status-1 ions by their own level's dt
+ * (variableTimestepKick :1216-1250); crossoverParticles (:1117-1127)
+ * re-levels ions by the table after each level's second drift.  Options,
+ * field, folding and boundary conditions as for shp_leapfrog_run; max_level
+ * at most 7.  Stream-ordered. */
+typedef struct shp_substep_level {
+    int bc_type;
+    double bc_space[4];
+    double bc_time;
+} shp_substep_level;
+int shp_leapfrog_substeps_run(shp_handle h, shp_state* s, const shp_step_opts* o, const shp_step_opts* sub,
+                              const shp_substep_level* levels, int max_level, int* level, double t0,
+                              double dt, int nsteps, double* ax, double* ay, double* az);
+
 /* Resident state on the reference's host arrays, for an Integrator subclass
  * (HipLeapfrogDKD) that keeps the plume on the GPU between steps:
  *  - shp_state_upload_host copies position (x, y, z, q), velocity (vx, vy,
@@ -378,6 +404,12 @@ int shp_state_upload_host(shp_handle h, const float* pos4, const float* vel4, co
                           const float* t_inj, int n);
 int shp_state_run(shp_handle h, const shp_step_opts* o, double t0, double dt, int nsteps);
 int shp_state_download_host(shp_handle h, float* pos4, float* vel4, float* acc4);
+/* shp_leapfrog_substeps_run on the resident state, with the ion levels kept
+ * in the handle between calls (reset_levels != 0 sets every ion to
+ * max_level first, as initializeParticleLevels does). */
+int shp_state_run_substeps(shp_handle h, const shp_step_opts* o, const shp_step_opts* sub,
+                           const shp_substep_level* levels, int max_level, int reset_levels, double t0,
+                           double dt, int nsteps);
 
 /* Kernel time (ms) of the last leapfrog update kernel. */
 double shp_last_update_ms(shp_handle h);

@@ -357,13 +357,7 @@ def background_accel_ref(mesh, pos4, vel4, acc4, start=-1):
     return acc, ef
 
 
-def leapfrog_run_f64(d, t0, dt, nsteps, mesh=None, sym="none", nsec=4, lattice=None,
-                     domain=(0.0, 0.0, 0.0), bc_space=(0, 0, 0, 0), bc_time=0.0):
-    """nsteps of DKD (FP64 direct force) on a copy of the synth dict; returns
-    (state dict, last Coulomb acceleration[n,3])."""
-    s = {k: (_f64(d[k]).copy() if k not in ("status", "species") else _i32(d[k]).copy())
-         for k in ("x", "y", "z", "vx", "vy", "vz", "q", "status", "species", "t_inj")}
-    n = s["x"].shape[0]
+def _lf_params(mesh, sym, nsec, lattice, domain, bc_space, bc_time):
     P = _LfParams()
     P.sym = _SYM[sym]
     P.nsec = nsec
@@ -385,12 +379,77 @@ def leapfrog_run_f64(d, t0, dt, nsteps, mesh=None, sym="none", nsec=4, lattice=N
     keep.append(lat)
     P.nimg = lat.shape[0]
     P.lattice = lat.ctypes.data
+    return P, keep
+
+
+_STATE_KEYS = ("x", "y", "z", "vx", "vy", "vz", "q", "status", "species", "t_inj")
+
+
+def _state_copy(d):
+    return {k: (_f64(d[k]).copy() if k not in ("status", "species") else _i32(d[k]).copy())
+            for k in _STATE_KEYS}
+
+
+def leapfrog_run_f64(d, t0, dt, nsteps, mesh=None, sym="none", nsec=4, lattice=None,
+                     domain=(0.0, 0.0, 0.0), bc_space=(0, 0, 0, 0), bc_time=0.0):
+    """nsteps of DKD (FP64 direct force) on a copy of the synth dict; returns
+    (state dict, last Coulomb acceleration[n,3])."""
+    s = _state_copy(d)
+    n = s["x"].shape[0]
+    P, keep = _lf_params(mesh, sym, nsec, lattice, domain, bc_space, bc_time)
     acc = np.zeros((n, 3))
-    lib.orc_leapfrog_run_f64(ctypes.byref(P), n, *[s[k].ctypes.data for k in
-                                                    ("x", "y", "z", "vx", "vy", "vz", "q",
-                                                     "status", "species", "t_inj")],
+    lib.orc_leapfrog_run_f64(ctypes.byref(P), n, *[s[k].ctypes.data for k in _STATE_KEYS],
                              float(t0), float(dt), int(nsteps), acc.ctypes.data)
     return s, acc
+
+
+class SubstepLevel(ctypes.Structure):
+    """One level of LeapfrogDKDSubsteps' boundary-condition table: bc_type
+    0 none, 1 space, 2 time, 3 space and time; bc_space |x|, |y|, |z|, rho
+    limits; bc_time limit on the time since injection (0: no limit)."""
+    _fields_ = [("bc_type", ctypes.c_int), ("bc_space", ctypes.c_double * 4), ("bc_time", ctypes.c_double)]
+
+
+def substep_table(levels):
+    """levels: list of (bc_type, (bx, by, bz, brho), bt) for levels 0..M."""
+    arr = (SubstepLevel * len(levels))()
+    for k, (typ, sp, tm) in enumerate(levels):
+        arr[k].bc_type = int(typ)
+        for c in range(4):
+            arr[k].bc_space[c] = float(sp[c])
+        arr[k].bc_time = float(tm)
+    return arr
+
+
+lib.orc_substep_level_of.restype = _ci
+lib.orc_substep_level_of.argtypes = [_vp, _ci, _ci, _ci, _cd, _cd, _cd, _cd, _cd]
+lib.orc_leapfrog_substeps_f64.argtypes = [ctypes.POINTER(_LfParams), _vp, _ci, _ci] + [_vp] * 11 + \
+    [_cd, _cd, _ci, _vp]
+
+
+def substep_level_of(levels, level, cur, x, y, z, tnow, tinj):
+    """checkConditions of LeapfrogDKDSubsteps for one ion: its new level."""
+    T = substep_table(levels)
+    return lib.orc_substep_level_of(ctypes.addressof(T), len(levels) - 1, int(level), int(cur), float(x),
+                                    float(y), float(z), float(tnow), float(tinj))
+
+
+def leapfrog_substeps_f64(d, t0, dt, nsteps, levels, level0=None, mesh=None, sym="none", nsec=4,
+                          lattice=None, domain=(0.0, 0.0, 0.0), bc_space=(0, 0, 0, 0), bc_time=0.0):
+    """nsteps of LeapfrogDKDSubsteps (FP64 direct force) on a copy of the synth
+    dict with the level table `levels` (levels 0..M); level0: the ions' levels
+    at the start (default M).  Returns (state dict, levels, last force)."""
+    s = _state_copy(d)
+    n = s["x"].shape[0]
+    M = len(levels) - 1
+    lev = _i32(np.full(n, M) if level0 is None else level0).copy()
+    P, keep = _lf_params(mesh, sym, nsec, lattice, domain, bc_space, bc_time)
+    T = substep_table(levels)
+    acc = np.zeros((n, 3))
+    lib.orc_leapfrog_substeps_f64(ctypes.byref(P), ctypes.addressof(T), M, n,
+                                  *[s[k].ctypes.data for k in _STATE_KEYS], lev.ctypes.data,
+                                  float(t0), float(dt), int(nsteps), acc.ctypes.data)
+    return s, lev, acc
 
 
 _REF_DIR = os.path.join(_HERE, "_ref")

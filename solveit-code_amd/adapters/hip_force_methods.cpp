@@ -164,27 +164,35 @@ void HipMultipoleMethodForceCalculation::setTimerName() { timerName = "Fast Mult
 
 // --------------------------------------------------------------- integrator ----
 
-HipLeapfrogDKD::HipLeapfrogDKD(ParticleSystem& particleSystem, ForceCalculationMethod& forceMethod,
-                               const shp_step_opts& opts, const shp_field_mesh* mesh, int device,
-                               const Grid* grid)
-    : Integrator{particleSystem, forceMethod}, handle{createHandle(device)}, opts(opts) {
-    energySaveName = "_LeapfrogDKD_HIP.bin";
-    this->opts.sym = symmetryOf(particleSystem, grid, opts.sym.nsec > 0 ? opts.sym.nsec : 4, lattice);
+// The run's symmetry, boundary conditions and background field in the engine's options.
+static shp_step_opts runOptions(ParticleSystem& particleSystem, const shp_step_opts& opts, const Grid* grid,
+                                std::vector<int>& lattice, shp_handle handle, const shp_field_mesh* mesh) {
+    shp_step_opts o = opts;
+    o.sym = symmetryOf(particleSystem, grid, opts.sym.nsec > 0 ? opts.sym.nsec : 4, lattice);
     const BoundaryConditionParameters& bc = particleSystem.params.boundaryConditions;
     const bool space = bc.boundaryConditionType == BoundaryConditionOptions::space ||
                        bc.boundaryConditionType == BoundaryConditionOptions::space_and_time;
     const bool time = bc.boundaryConditionType == BoundaryConditionOptions::time ||
                       bc.boundaryConditionType == BoundaryConditionOptions::space_and_time;
-    this->opts.bc_space[0] = space ? bc.spatialBoundaryConditions.x : 0.0;
-    this->opts.bc_space[1] = space ? bc.spatialBoundaryConditions.y : 0.0;
-    this->opts.bc_space[2] = space ? bc.spatialBoundaryConditions.z : 0.0;
-    this->opts.bc_space[3] = space ? bc.spatialBoundaryConditions.w : 0.0;
-    this->opts.bc_time = time ? bc.temporalBoundaryCondition : 0.0;
-    this->opts.use_efield = particleSystem.interpScheme.bgField == BackgroundEfield::singleEmitter;
-    if (this->opts.use_efield) {
-        if (mesh == nullptr) fail("HipLeapfrogDKD: the background field needs its mesh");
+    o.bc_space[0] = space ? bc.spatialBoundaryConditions.x : 0.0;
+    o.bc_space[1] = space ? bc.spatialBoundaryConditions.y : 0.0;
+    o.bc_space[2] = space ? bc.spatialBoundaryConditions.z : 0.0;
+    o.bc_space[3] = space ? bc.spatialBoundaryConditions.w : 0.0;
+    o.bc_time = time ? bc.temporalBoundaryCondition : 0.0;
+    o.use_efield = particleSystem.interpScheme.bgField == BackgroundEfield::singleEmitter;
+    if (o.use_efield) {
+        if (mesh == nullptr) fail("the background field needs its mesh");
         if (shp_efield_setup(handle, mesh) != SHP_OK) fail("shp_efield_setup");
     }
+    return o;
+}
+
+HipLeapfrogDKD::HipLeapfrogDKD(ParticleSystem& particleSystem, ForceCalculationMethod& forceMethod,
+                               const shp_step_opts& opts, const shp_field_mesh* mesh, int device,
+                               const Grid* grid)
+    : Integrator{particleSystem, forceMethod}, handle{createHandle(device)} {
+    energySaveName = "_LeapfrogDKD_HIP.bin";
+    this->opts = runOptions(particleSystem, opts, grid, lattice, handle, mesh);
 }
 
 HipLeapfrogDKD::~HipLeapfrogDKD() { shp_destroy(handle); }
@@ -209,6 +217,81 @@ void HipLeapfrogDKD::runSimulation() {
             SHP_OK)
             fail("shp_state_download_host");
         // ions injected by the end of the step (findMostRecentInjected, particles.cpp:1328-1359)
+        setLastChargedParticle(cp.findMostRecentInjected(timeStep * i) + 1);
+        currentTime = timeStep * i;
+        finishIter(i);
+    }
+    finishFullSim();
+}
+
+// ---------------------------------------------------------- multi-rate ----
+
+HipLeapfrogDKDSubsteps::HipLeapfrogDKDSubsteps(ParticleSystem& particleSystem, ForceCalculationMethod& forceMethod,
+                                               const shp_step_opts& opts, const shp_field_mesh* mesh, int device,
+                                               const Grid* grid, const shp_step_opts* substepOpts)
+    : LeapfrogDKDSubsteps{particleSystem, forceMethod}, handle{createHandle(device)} {
+    energySaveName = "_LeapfrogDKDSubsteps_HIP.bin";
+    this->opts = runOptions(particleSystem, opts, grid, lattice, handle, mesh);
+    subOpts = this->opts;
+    if (substepOpts != nullptr) {
+        subOpts.method = substepOpts->method;
+        subOpts.precision = substepOpts->precision;
+        subOpts.fmm_order = substepOpts->fmm_order;
+        subOpts.leaf_size = substepOpts->leaf_size;
+        subOpts.theta = substepOpts->theta;
+        subOpts.fmm_ring = substepOpts->fmm_ring;
+        subOpts.leaf_level = substepOpts->leaf_level;
+    }
+}
+
+HipLeapfrogDKDSubsteps::~HipLeapfrogDKDSubsteps() { shp_destroy(handle); }
+
+void HipLeapfrogDKDSubsteps::setSubstepTimes(float* substepTimes, BoundaryConditionParameters* bc, int size) {
+    LeapfrogDKDSubsteps::setSubstepTimes(substepTimes, bc, size);
+    float min_dt = substepTimes[0];
+    for (int i = 1; i < size; i++) min_dt = substepTimes[i] < min_dt ? substepTimes[i] : min_dt;
+    float temp_dt = particleSystem.params.timeStep;
+    levelsM = 0;
+    while (temp_dt > min_dt) {
+        temp_dt /= 3.0;
+        levelsM++;
+    }
+}
+
+void HipLeapfrogDKDSubsteps::runSimulation() {
+    intervalCount = 0;
+    prevTime = 0.0f;
+    const float timeStep = particleSystem.params.timeStep;
+    if (substepBoundaryConditions == nullptr) {
+        setDefaultSubstepTimes();
+        levelsM = 0;
+    }
+    std::vector<shp_substep_level> table((size_t)levelsM + 1);
+    for (int l = 0; l <= levelsM; l++) {
+        const BoundaryConditionParameters& b = substepBoundaryConditions[l].bc;
+        table[l].bc_type = (int)b.boundaryConditionType;
+        table[l].bc_space[0] = b.spatialBoundaryConditions.x;
+        table[l].bc_space[1] = b.spatialBoundaryConditions.y;
+        table[l].bc_space[2] = b.spatialBoundaryConditions.z;
+        table[l].bc_space[3] = b.spatialBoundaryConditions.w;
+        table[l].bc_time = b.temporalBoundaryCondition;
+    }
+    initializeSimulation();
+    ChargedParticles& cp = particleSystem.chargedParticles;
+    const int n = cp.maxNumParticles;
+    if (shp_state_upload_host(handle, &cp.position[0].x, &cp.velocity[0].x, &cp.acceleration[0].x,
+                              cp.timeInjected, n) != SHP_OK)
+        fail("shp_state_upload_host");
+    for (int i = 1; i <= particleSystem.params.numIters; i++) {
+        iterTimer.changeCount(i);
+        tempPosTime = timeStep * (i - 1);
+        tempVelTime = timeStep * (i - 1);
+        if (shp_state_run_substeps(handle, &opts, &subOpts, table.data(), levelsM, i == 1, (double)tempPosTime,
+                                   (double)timeStep, 1) != SHP_OK)
+            fail("shp_state_run_substeps");
+        if (shp_state_download_host(handle, &cp.position[0].x, &cp.velocity[0].x, &cp.acceleration[0].x) !=
+            SHP_OK)
+            fail("shp_state_download_host");
         setLastChargedParticle(cp.findMostRecentInjected(timeStep * i) + 1);
         currentTime = timeStep * i;
         finishIter(i);

@@ -138,4 +138,28 @@ private:
     std::vector<int> lattice;
 };
 
+// LeapfrogDKDSubsteps (integration_schemes.cpp:899-1529) with the plume
+// resident on the GPU.  The level table is the reference's own: the base
+// class's setSubstepTimes builds substepBoundaryConditions; the number of
+// levels is counted as setSubstepTimes counts it (:1379-1385).  Each
+// iteration runs on the device (shp_state_run_substeps) and writes the state
+// back into the driver's arrays before finishIter, as HipLeapfrogDKD does.
+// substepOpts: the force of the kicks below the top level (the reference's
+// substepMethod); nullptr = opts.
+class HipLeapfrogDKDSubsteps : public LeapfrogDKDSubsteps {
+public:
+    HipLeapfrogDKDSubsteps(ParticleSystem& particleSystem, ForceCalculationMethod& forceMethod,
+                           const shp_step_opts& opts, const shp_field_mesh* mesh = nullptr, int device = 0,
+                           const Grid* grid = nullptr, const shp_step_opts* substepOpts = nullptr);
+    ~HipLeapfrogDKDSubsteps();
+    void setSubstepTimes(float* substepTimes, BoundaryConditionParameters* bc, int size);
+    void runSimulation() override;
+
+private:
+    shp_handle handle = nullptr;
+    shp_step_opts opts, subOpts;
+    std::vector<int> lattice;
+    int levelsM = 0;
+};
+
 #endif  // HIP_FORCE_METHODS_H

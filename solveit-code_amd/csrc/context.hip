@@ -102,6 +102,7 @@ int shp_destroy(shp_handle h) {
     if (h->res_f64) (void)hipFree(h->res_f64);
     if (h->res_i32) (void)hipFree(h->res_i32);
     if (h->res_aos) (void)hipFree(h->res_aos);
+    if (h->res_level) (void)hipFree(h->res_level);
     shp::free_mesh(h);
     shp::fmm_release(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -544,6 +544,119 @@ __global__ void k_res_to_aos(const double* __restrict__ x, const double* __restr
     acc4[i] = make_float4((float)ax[i], (float)ay[i], (float)az[i], (float)status[i]);
 }
 
+
+// ------------------------------------------------------------ substeps ----
+// LeapfrogDKDSubsteps (integration_schemes.cpp:899-1529): levels 0..M with
+// dt_l = dt_{l-1} / 3, a level per ion; all ions drift together by dt_M / 2,
+// and each finest kick kicks the ions whose level is at least the kick's top
+// level by their own level's dt (variableTimestepKick :1216-1250).
+
+constexpr int kMaxSubLevels = 8;
+
+struct SubstepTable {
+    int M;
+    int type[kMaxSubLevels];        // BoundaryConditionOptions of each level
+    double bc[kMaxSubLevels][4];    // |x|, |y|, |z|, rho limits
+    double bt[kMaxSubLevels];       // time-since-injection limit
+    double dtl[kMaxSubLevels];      // dt of each level
+};
+
+// The finest level (from M down to `level`) whose limit admits v, extended
+// down over the levels sharing that limit; -1 when none does
+// (checkSpatialConditions :944-1066, checkTemporalConditions :1073-1110).
+__device__ __forceinline__ int ss_scan(const SubstepTable& T, int level, int axis, double v) {
+    int possible = -1;
+    double bound = 0.0;
+    for (int i = T.M; i >= level; i--) {
+        const double bc = axis < 4 ? T.bc[i][axis] : T.bt[i];
+        if (possible == -1) {
+            if (v < bc || bc == 0) {
+                possible = i;
+                bound = bc;
+            }
+        } else {
+            if (bound == bc) possible--;
+            else break;
+        }
+    }
+    return possible;
+}
+
+// checkConditions(index, level) (:919-937): the ion's new level.
+__device__ int ss_level_of(const SubstepTable& T, int level, int cur, double x, double y, double z,
+                           double tnow, double tinj) {
+    const int type = T.type[level + 1];
+    if (type == 0) return cur;
+    int sp = cur;
+    if (type == 1 || type == 3) {
+        sp = ss_scan(T, level, 0, fabs(x));
+        sp = max(sp, ss_scan(T, level, 1, fabs(y)));
+        sp = max(sp, ss_scan(T, level, 2, fabs(z)));
+        sp = max(sp, ss_scan(T, level, 3, sqrt(x * x + y * y)));
+        if (type == 1) return sp;
+    }
+    const int tp = ss_scan(T, level, 4, tnow - tinj);
+    return type == 2 ? tp : max(tp, sp);
+}
+
+// setParticleStatus (:1193-1211) over the injected ions, and the status the
+// force sees (0 for ions not injected yet).
+__global__ void k_ss_status(int n, const double* __restrict__ tinj, double tpos, const int* __restrict__ level,
+                            int top, int* __restrict__ status, int* __restrict__ eff) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool injected = tinj[i] <= tpos;
+    int st = status[i];
+    if (injected && st != 0 && st != -1) {
+        st = level[i] >= top ? 1 : 2;
+        status[i] = st;
+    }
+    eff[i] = injected ? st : 0;
+}
+
+// The kick of the status-1 ions: Coulomb plus field acceleration over the
+// ion's own level dt, clamped from its injection time (window ending at
+// tvel + dt_level).
+__global__ void k_ss_kick(UpdateParams P, MeshView M, SubstepTable T, int n, const double* __restrict__ X,
+                          const double* __restrict__ Y, const double* __restrict__ Z, double* __restrict__ VX,
+                          double* __restrict__ VY, double* __restrict__ VZ, const double* __restrict__ Q,
+                          const int* __restrict__ species, const double* __restrict__ tinj,
+                          const int* __restrict__ eff, const int* __restrict__ level,
+                          const double* __restrict__ AX, const double* __restrict__ AY,
+                          const double* __restrict__ AZ, double tvel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || eff[i] != 1) return;
+    double ax = AX[i], ay = AY[i], az = AZ[i];
+    if (P.use_efield) {
+        double ex, ey, ez;
+        efield_f64(M, X[i], Y[i], Z[i], ex, ey, ez);
+        const double f = P.e_over_amu * Q[i] / species_mass(species[i]);
+        ax += f * ex;
+        ay += f * ey;
+        az += f * ez;
+    }
+    const double dl = T.dtl[level[i]];
+    const double s = fmin(fmax(tvel + dl - tinj[i], 0.0), dl);
+    VX[i] += ax * s;
+    VY[i] += ay * s;
+    VZ[i] += az * s;
+}
+
+__global__ void k_ss_fill(int* __restrict__ p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// crossoverParticles(L) (:1117-1127) over the injected ions.
+__global__ void k_ss_crossover(SubstepTable T, int L, int n, const double* __restrict__ X,
+                               const double* __restrict__ Y, const double* __restrict__ Z,
+                               const double* __restrict__ tinj, double tpos, int* __restrict__ level) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || tinj[i] > tpos) return;
+    const int lv = level[i];
+    if (lv == L || lv == L + 1) level[i] = ss_level_of(T, L, lv, X[i], Y[i], Z[i], tpos, tinj[i]);
+}
+
 }  // namespace shp
 
 using namespace shp;
@@ -781,6 +894,133 @@ int shp_compare_accel(shp_handle h, int n, const int* status, const double* ax, 
 }
 
 
+
+// ------------------------------------------------------------ substeps ----
+
+namespace {
+struct SubstepRun {
+    shp_handle h;
+    shp_state* s;
+    const shp_step_opts* o;
+    const shp_step_opts* sub;
+    UpdateParams P, Psub;
+    MeshView M;
+    SubstepTable T;
+    int* level;
+    int* eff;
+    double *ax, *ay, *az;
+    double tpos, tvel;
+    dim3 grid, block;
+};
+
+int ss_drift(SubstepRun& R) {
+    const double hh = R.T.dtl[R.T.M] / 2;
+    shp_state* s = R.s;
+    hipLaunchKernelGGL(k_drift, R.grid, R.block, 0, R.h->stream, R.P, s->n, s->x, s->y, s->z, s->vx, s->vy, s->vz,
+                       s->status, s->t_inj, R.tpos, hh, R.eff);
+    SHP_HIP_CHECK(hipGetLastError());
+    R.tpos += hh;
+    return SHP_OK;
+}
+
+int ss_kick(SubstepRun& R, int top) {
+    shp_state* s = R.s;
+    const int n = s->n;
+    hipLaunchKernelGGL(k_ss_status, R.grid, R.block, 0, R.h->stream, n, s->t_inj, R.tpos, R.level, top, s->status,
+                       R.eff);
+    const shp_step_opts* o = top == 0 ? R.o : R.sub;
+    shp_particles p{n, s->x, s->y, s->z, s->q, R.eff, s->species};
+    const int rc = o->method == SHP_METHOD_FMM
+                       ? shp_fmm_sym_dev(R.h, &p, &o->sym, o->fmm_order, o->leaf_level, o->leaf_size,
+                                         o->fmm_ring > 0 ? o->fmm_ring : 2, R.ax, R.ay, R.az)
+                   : o->method == SHP_METHOD_BH ? shp_bh_dev(R.h, &p, (float)o->theta, &o->sym, R.ax, R.ay, R.az)
+                                                : shp_direct_dev(R.h, &p, o->precision, &o->sym, R.ax, R.ay, R.az);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_ss_kick, R.grid, R.block, 0, R.h->stream, R.P, R.M, R.T, n, s->x, s->y, s->z, s->vx, s->vy,
+                       s->vz, s->q, s->species, s->t_inj, R.eff, R.level, R.ax, R.ay, R.az, R.tvel);
+    SHP_HIP_CHECK(hipGetLastError());
+    R.tvel += R.T.dtl[R.T.M];
+    return SHP_OK;
+}
+
+int ss_K(SubstepRun& R, int L, int top);
+
+int ss_D(SubstepRun& R, int L, bool toggle) {
+    if (L == R.T.M) return ss_drift(R);
+    int rc = ss_D(R, L + 1, false);
+    if (!rc) rc = ss_K(R, L + 1, L + 1);
+    if (!rc) rc = ss_D(R, L + 1, true);
+    if (!rc && toggle) {
+        shp_state* s = R.s;
+        hipLaunchKernelGGL(k_ss_crossover, R.grid, R.block, 0, R.h->stream, R.T, L, s->n, s->x, s->y, s->z,
+                           s->t_inj, R.tpos, R.level);
+        SHP_HIP_CHECK(hipGetLastError());
+    }
+    return rc;
+}
+
+int ss_K(SubstepRun& R, int L, int top) {
+    if (L == R.T.M) return ss_kick(R, top);
+    int rc = ss_D(R, L + 1, false);
+    if (!rc) rc = ss_K(R, L + 1, top);
+    if (!rc) rc = ss_D(R, L + 1, true);
+    return rc;
+}
+}  // namespace
+
+int shp_leapfrog_substeps_run(shp_handle h, shp_state* s, const shp_step_opts* o, const shp_step_opts* sub,
+                              const shp_substep_level* levels, int max_level, int* level, double t0, double dt,
+                              int nsteps, double* ax, double* ay, double* az) {
+    SHP_REQUIRE(h && s && o && levels && level && ax && ay && az, "shp_leapfrog_substeps_run: null argument");
+    SHP_REQUIRE(max_level >= 0 && max_level < kMaxSubLevels, "shp_leapfrog_substeps_run: max_level %d (0..%d)",
+                max_level, kMaxSubLevels - 1);
+    SHP_REQUIRE(s->n >= 0 && nsteps >= 0 && dt >= 0, "shp_leapfrog_substeps_run: n=%d steps=%d dt=%g", s->n,
+                nsteps, dt);
+    SHP_REQUIRE(s->x && s->y && s->z && s->vx && s->vy && s->vz && s->q && s->status && s->species && s->t_inj,
+                "shp_leapfrog_substeps_run: null state array");
+    if (!sub) sub = o;
+    for (const shp_step_opts* q : {o, sub})
+        SHP_REQUIRE(q->method == SHP_METHOD_DIRECT || q->method == SHP_METHOD_FMM || q->method == SHP_METHOD_BH,
+                    "shp_leapfrog_substeps_run: method %d not available", q->method);
+    SHP_REQUIRE(!o->use_efield || h->mesh.ready, "shp_leapfrog_substeps_run: E-field requested but no mesh");
+    for (int l = 0; l <= max_level; l++)
+        SHP_REQUIRE(levels[l].bc_type >= 0 && levels[l].bc_type <= 3, "shp_leapfrog_substeps_run: level %d bc_type %d",
+                    l, levels[l].bc_type);
+    if (s->n == 0 || nsteps == 0) return SHP_OK;
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    SubstepRun R;
+    R.h = h;
+    R.s = s;
+    R.o = o;
+    R.sub = sub;
+    R.P = make_params(o);
+    R.M = h->mesh.ready ? view_of(h->mesh) : MeshView{};
+    R.T = SubstepTable{};
+    R.T.M = max_level;
+    for (int l = 0; l <= max_level; l++) {
+        R.T.type[l] = levels[l].bc_type;
+        for (int c = 0; c < 4; c++) R.T.bc[l][c] = levels[l].bc_space[c];
+        R.T.bt[l] = levels[l].bc_time;
+        R.T.dtl[l] = l == 0 ? dt : R.T.dtl[l - 1] / 3.0;
+    }
+    R.level = level;
+    SHP_WS(eff, int, h, WS_EFF_STATUS, s->n);
+    R.eff = eff;
+    R.ax = ax;
+    R.ay = ay;
+    R.az = az;
+    R.grid = dim3(cdiv_l(s->n, 256));
+    R.block = dim3(256);
+    for (int k = 0; k < nsteps; k++) {
+        R.tpos = R.tvel = t0 + k * dt;
+        int rc = ss_D(R, 0, false);
+        if (!rc) rc = ss_K(R, 0, 0);
+        if (!rc) rc = ss_D(R, 0, true);
+        if (rc) return rc;
+    }
+    return SHP_OK;
+}
+
 // ---------------------------------------------------------- resident state ----
 
 int shp_state_upload_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4,
@@ -830,6 +1070,33 @@ int shp_state_run(shp_handle h, const shp_step_opts* o, double t0, double dt, in
     shp_state s{n, f, f + n, f + 2 * (size_t)n, f + 3 * (size_t)n, f + 4 * (size_t)n, f + 5 * (size_t)n,
                 f + 6 * (size_t)n, h->res_i32, h->res_i32 + n, f + 7 * (size_t)n};
     return shp_leapfrog_run(h, &s, o, t0, dt, nsteps, f + 8 * (size_t)n, f + 9 * (size_t)n, f + 10 * (size_t)n);
+}
+
+int shp_state_run_substeps(shp_handle h, const shp_step_opts* o, const shp_step_opts* sub,
+                           const shp_substep_level* levels, int max_level, int reset_levels, double t0, double dt,
+                           int nsteps) {
+    SHP_REQUIRE(h && o && levels, "shp_state_run_substeps: null argument");
+    SHP_REQUIRE(h->res_f64 || h->res_n == 0, "shp_state_run_substeps: no resident state (shp_state_upload_host)");
+    SHP_REQUIRE(max_level >= 0 && max_level < kMaxSubLevels, "shp_state_run_substeps: max_level %d", max_level);
+    const int n = h->res_n;
+    if (n == 0) return SHP_OK;
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    if (n > h->res_level_cap) {
+        SHP_HIP_CHECK(hipStreamSynchronize(h->stream));
+        if (h->res_level) SHP_HIP_CHECK(hipFree(h->res_level));
+        h->res_level = nullptr;
+        h->res_level_cap = 0;
+        SHP_HIP_CHECK(hipMalloc((void**)&h->res_level, sizeof(int) * (size_t)n));
+        h->res_level_cap = n;
+        reset_levels = 1;
+    }
+    if (reset_levels)  // initializeParticleLevels: every ion at the finest level
+        hipLaunchKernelGGL(k_ss_fill, dim3(cdiv_l(n, 256)), dim3(256), 0, h->stream, h->res_level, n, max_level);
+    double* f = h->res_f64;
+    shp_state s{n, f, f + n, f + 2 * (size_t)n, f + 3 * (size_t)n, f + 4 * (size_t)n, f + 5 * (size_t)n,
+                f + 6 * (size_t)n, h->res_i32, h->res_i32 + n, f + 7 * (size_t)n};
+    return shp_leapfrog_substeps_run(h, &s, o, sub, levels, max_level, h->res_level, t0, dt, nsteps,
+                                     f + 8 * (size_t)n, f + 9 * (size_t)n, f + 10 * (size_t)n);
 }
 
 int shp_state_download_host(shp_handle h, float* pos4, float* vel4, float* acc4) {

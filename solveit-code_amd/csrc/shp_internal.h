@@ -163,6 +163,8 @@ struct shp_context {
     int* res_i32 = nullptr;     // status, species
     float4* res_aos = nullptr;  // host AoS staging: position, velocity, acceleration, t_inj
     int res_n = 0, res_cap = 0;
+    int* res_level = nullptr;   // ion levels of shp_state_run_substeps
+    int res_level_cap = 0;
     void* ws[shp::WS_NSLOTS] = {};
     size_t ws_bytes[shp::WS_NSLOTS] = {};
     int num_cus = 256;

@@ -71,6 +71,21 @@ BH_STAT_KEYS = ("root_cells", "nodes", "max_depth", "buckets", "accepted_cells",
                 "dim_z")
 
 
+class SubstepLevel(ctypes.Structure):
+    _fields_ = [("bc_type", ctypes.c_int), ("bc_space", ctypes.c_double * 4), ("bc_time", ctypes.c_double)]
+
+
+def make_substep_levels(levels):
+    """levels: list of (bc_type, (bx, by, bz, brho), bt), one per level 0..M."""
+    arr = (SubstepLevel * len(levels))()
+    for k, (typ, sp, tm) in enumerate(levels):
+        arr[k].bc_type = int(typ)
+        for c in range(4):
+            arr[k].bc_space[c] = float(sp[c])
+        arr[k].bc_time = float(tm)
+    return arr
+
+
 class State(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int),
@@ -196,7 +211,12 @@ def _load():
                                 ctypes.POINTER(Symmetry), dp]),
         "shp_bh_set_options": (c_int, [c_void_p, ctypes.POINTER(BhOptions)]),
         "shp_state_upload_host": (c_int, [c_void_p, fp, fp, fp, fp, c_int]),
+        "shp_leapfrog_substeps_run": (c_int, [c_void_p, ctypes.POINTER(State), ctypes.POINTER(StepOpts),
+                                              ctypes.POINTER(StepOpts), c_void_p, c_int, c_void_p, c_double,
+                                              c_double, c_int, c_void_p, c_void_p, c_void_p]),
         "shp_state_run": (c_int, [c_void_p, ctypes.POINTER(StepOpts), c_double, c_double, c_int]),
+        "shp_state_run_substeps": (c_int, [c_void_p, ctypes.POINTER(StepOpts), ctypes.POINTER(StepOpts), c_void_p,
+                                           c_int, c_int, c_double, c_double, c_int]),
         "shp_state_download_host": (c_int, [c_void_p, fp, fp, fp]),
         "shp_fmm_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                  ctypes.POINTER(Symmetry), dp]),
@@ -392,6 +412,12 @@ class Engine:
     def state_run(self, opts, t0, dt, nsteps):
         _check(lib.shp_state_run(self.h, ctypes.byref(opts), float(t0), float(dt), int(nsteps)),
                "shp_state_run")
+
+    def state_run_substeps(self, opts, levels, t0, dt, nsteps, reset=False, sub=None):
+        table = make_substep_levels(levels)
+        _check(lib.shp_state_run_substeps(self.h, ctypes.byref(opts), ctypes.byref(sub) if sub is not None else None,
+                                          ctypes.addressof(table), len(levels) - 1, 1 if reset else 0, float(t0),
+                                          float(dt), int(nsteps)), "shp_state_run_substeps")
 
     def state_download_host(self, pos4, vel4, acc4):
         """Writes the resident state back into float32 (n, 4) arrays (in place)."""
@@ -650,6 +676,25 @@ def _engine_leapfrog_run(self, state, opts, t0, dt, nsteps, acc=None):
     return acc
 
 
+def _engine_leapfrog_substeps(self, state, opts, levels, level, t0, dt, nsteps, sub=None, acc=None):
+    """nsteps of the multi-rate LeapfrogDKDSubsteps on a DeviceState; levels as for
+    make_substep_levels; level: int32 device tensor of the ions' levels (in/out)."""
+    import torch
+    n = state.n
+    if acc is None:
+        acc = tuple(torch.zeros(n, dtype=torch.float64, device=state.x.device) for _ in range(3))
+    s = state.as_state()
+    table = make_substep_levels(levels)
+    _torch_ready(state.x.device)
+    _check(lib.shp_leapfrog_substeps_run(self.h, ctypes.byref(s), ctypes.byref(opts),
+                                         ctypes.byref(sub) if sub is not None else None,
+                                         ctypes.addressof(table), len(levels) - 1, level.data_ptr(),
+                                         float(t0), float(dt), int(nsteps), acc[0].data_ptr(),
+                                         acc[1].data_ptr(), acc[2].data_ptr()), "shp_leapfrog_substeps_run")
+    self.sync()
+    return acc
+
+
 def _engine_fmm_dev(self, state, order=8, leaf_level=0, leaf_size=32, ring=2, out=None,
                     symmetry=None):
     """FMM accelerations on a DeviceState (with symmetry images when given);
@@ -725,6 +770,7 @@ Engine.fmm_dev = _engine_fmm_dev
 Engine.fmm_level_estimates = _engine_fmm_level_estimates
 Engine.compare_accel = _engine_compare_accel
 Engine.fmm_leaf_level = _engine_fmm_leaf_level
+Engine.leapfrog_substeps = _engine_leapfrog_substeps
 Engine.fmm_stats = _engine_fmm_stats
 Engine.fmm_timing = _engine_fmm_timing
 

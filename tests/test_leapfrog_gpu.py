@@ -190,3 +190,77 @@ def test_resident_state_on_host_arrays(eng, mesh):
     np.testing.assert_array_equal(pos4[:, 3], d["q"].astype(np.float32))
     np.testing.assert_array_equal(vel4[:, 3], d["species"].astype(np.float32))
     assert np.any(ref["status"] == 0)
+
+
+@pytest.mark.parametrize("kind", ["space", "space_and_time"])
+def test_leapfrog_substeps_match_oracle(eng, mesh, kind):
+    """LeapfrogDKDSubsteps on the GPU against the oracle's restatement: three
+    levels (dt, dt/3, dt/9) with |z| limits (and time-since-injection limits),
+    fast ions that cross the limits between steps, staggered injection, the
+    tip field and boundary conditions; FP64 direct force.  Positions and
+    velocities to 1e-10, statuses and levels exactly."""
+    import solveit_hip
+    import torch
+    d = synth.cone(900, seed=13)
+    dt = 1e-12
+    d["vz"] = d["vz"] * 1000.0  # cross the 1 um and 3 um limits within a few steps
+    d["t_inj"] = np.sort(np.random.default_rng(4).uniform(0, 4 * dt, 900))
+    if kind == "space":
+        levels = [(1, (0, 0, 0, 0), 0.0), (1, (0, 0, 3e-6, 0), 0.0), (1, (0, 0, 1e-6, 0), 0.0)]
+    else:
+        levels = [(3, (0, 0, 0, 0), 0.0), (3, (0, 0, 3e-6, 0), 4e-12), (3, (0, 0, 1e-6, 0), 1.5e-12)]
+    opts = solveit_hip.make_step_opts(method=solveit_hip.SHP_METHOD_DIRECT,
+                                      precision=solveit_hip.SHP_PREC_FP64, use_efield=True,
+                                      bc_space=(0, 0, 1.2e-5, 0), bc_time=0.0)
+    print(f"case substeps {kind}", flush=True)
+    eng.efield_setup(mesh)
+    st = solveit_hip.DeviceState.from_numpy(d)
+    level = torch.full((900,), 2, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+    acc = eng.leapfrog_substeps(st, opts, levels, level, 0.0, dt, 6)
+    got = st.to_numpy()
+    ref, rlev, racc = oracle.leapfrog_substeps_f64(d, 0.0, dt, 6, levels, mesh=mesh,
+                                                   bc_space=(0, 0, 1.2e-5, 0))
+    glev = level.cpu().numpy()
+    print(f"  levels: {np.bincount(glev + 1, minlength=4)} (-1, 0, 1, 2); statuses "
+          f"{np.bincount(got['status'] + 1, minlength=4)}")
+    np.testing.assert_array_equal(glev, rlev)
+    np.testing.assert_array_equal(got["status"], ref["status"])
+    assert len(set(glev.tolist())) >= 2  # ions on more than one level
+    _close(got, ref)
+    a = torch.stack(acc, 1).cpu().numpy()
+    assert np.linalg.norm(a - racc) <= 1e-10 * np.linalg.norm(racc)
+
+
+def test_resident_substeps_on_host_arrays(eng, mesh):
+    """shp_state_run_substeps (the HipLeapfrogDKDSubsteps adapter's path, one
+    step per call with the levels kept in the handle) equals
+    shp_leapfrog_substeps_run on the same FP64 state, rounded to float."""
+    import solveit_hip
+    import torch
+    d = synth.cone(800, seed=14)
+    dt = 1e-12
+    d["vz"] = d["vz"] * 1000.0
+    for k in ("x", "y", "z", "vx", "vy", "vz", "q", "t_inj"):
+        d[k] = d[k].astype(np.float32).astype(np.float64)
+    levels = [(1, (0, 0, 0, 0), 0.0), (1, (0, 0, 3e-6, 0), 0.0), (1, (0, 0, 1e-6, 0), 0.0)]
+    opts = solveit_hip.make_step_opts(method=solveit_hip.SHP_METHOD_DIRECT,
+                                      precision=solveit_hip.SHP_PREC_FP64, use_efield=True)
+    print("case resident substeps", flush=True)
+    eng.efield_setup(mesh)
+    pos4, vel4, acc4 = synth.to_reference_aos(d)
+    eng.state_upload_host(pos4, vel4, acc4, d["t_inj"])
+    for k in range(3):
+        eng.state_run_substeps(opts, levels, k * dt, dt, 1, reset=(k == 0))
+    eng.state_download_host(pos4, vel4, acc4)
+    st = solveit_hip.DeviceState.from_numpy(d)
+    level = torch.full((800,), 2, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+    acc = None
+    for k in range(3):
+        acc = eng.leapfrog_substeps(st, opts, levels, level, k * dt, dt, 1, acc=acc)
+    ref = st.to_numpy()
+    np.testing.assert_array_equal(pos4[:, 2], ref["z"].astype(np.float32))
+    np.testing.assert_array_equal(vel4[:, 0], ref["vx"].astype(np.float32))
+    np.testing.assert_array_equal(acc4[:, 3], ref["status"].astype(np.float32))
+    np.testing.assert_array_equal(acc4[:, :3], torch.stack(acc, 1).cpu().numpy().astype(np.float32))
