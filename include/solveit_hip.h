@@ -308,6 +308,18 @@ typedef struct shp_field_mesh {
  * (interpolation_scheme.cpp:152-191). */
 int shp_efield_setup(shp_handle h, const shp_field_mesh* mesh);
 
+/* Reads the reference's field files (InterpolationScheme::fillInterpolationScheme
+ * and fillTriangleScheme, interpolation_scheme.cpp:66-172; line counts as
+ * returnSize, auxiliary_functions.cpp:39-53): Fields_*.csv with x, y (units
+ * of r0), Er, Ez (units of E0) per line, Triangles_*.csv with three 0-based
+ * node ids and three neighbours per line, fields split on ',' and parsed with
+ * atof / atoi.  Fills *mesh (r0, E0 as given) with host arrays owned by the
+ * library; shp_field_mesh_free releases them.  A file that does not open is
+ * an error (the reference prints and exits).  No device work. */
+int shp_field_mesh_load_csv(const char* fields_path, const char* triangles_path, double r0, double E0,
+                            shp_field_mesh* mesh);
+void shp_field_mesh_free(shp_field_mesh* mesh);
+
 /* Field at n device points (FP64): E = E0 (Er cos(phi), Er sin(phi), Ez) with
  * (Er, Ez) interpolated at (rho/r0, z/r0), zero outside z/r0 in [0, 140].
  * Replaces InterpolationScheme::computeEfieldSingleEmitter

@@ -3,6 +3,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 namespace hip_adapter {
 
@@ -41,6 +42,14 @@ shp_handle createHandle(int device) {
     shp_handle h = nullptr;
     if (shp_create(device, &h) != SHP_OK) fail("shp_create");
     return h;
+}
+
+void loadFieldMesh(const ParticleSystem& ps, const char* folderName, const char* fieldsName,
+                   const char* trianglesName, shp_field_mesh* mesh) {
+    const std::string fields = std::string(folderName) + fieldsName;
+    const std::string triangles = std::string(folderName) + trianglesName;
+    if (shp_field_mesh_load_csv(fields.c_str(), triangles.c_str(), ps.params.r0, ps.params.E0, mesh) != SHP_OK)
+        fail("shp_field_mesh_load_csv");
 }
 
 static void requireSystem(const ParticleSystem* ps) {

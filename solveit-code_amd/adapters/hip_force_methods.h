@@ -29,6 +29,13 @@ void addInto(ChargedParticles& cp, int first, int last, const std::vector<double
 
 shp_handle createHandle(int device);
 
+// The run's field mesh from the files the driver hands to
+// ParticleSystem::initializeBackgroundEField (particles.cpp:452-463): folder
+// and file names joined as the reference joins them, r0 and E0 from
+// params.  Release it with shp_field_mesh_free.
+void loadFieldMesh(const ParticleSystem& ps, const char* folderName, const char* fieldsName,
+                   const char* trianglesName, shp_field_mesh* mesh);
+
 }  // namespace hip_adapter
 
 // Replaces DirectForceCalculation (direct_force_method.cpp:347-428): the

@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "solveit_hip", "libsolveit_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["context.hip", "direct.hip", "leapfrog.hip", "fmm.hip", "bh.hip", "adjoint.hip", "batch.hip"]
+SOURCES = ["context.hip", "direct.hip", "leapfrog.hip", "fmm.hip", "bh.hip", "adjoint.hip", "batch.hip",
+           "field_io.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

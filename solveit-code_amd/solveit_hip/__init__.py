@@ -210,6 +210,9 @@ def _load():
         "shp_bh_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, ctypes.c_float,
                                 ctypes.POINTER(Symmetry), dp]),
         "shp_bh_set_options": (c_int, [c_void_p, ctypes.POINTER(BhOptions)]),
+        "shp_field_mesh_load_csv": (c_int, [ctypes.c_char_p, ctypes.c_char_p, c_double, c_double,
+                                            ctypes.POINTER(FieldMesh)]),
+        "shp_field_mesh_free": (None, [ctypes.POINTER(FieldMesh)]),
         "shp_state_upload_host": (c_int, [c_void_p, fp, fp, fp, fp, c_int]),
         "shp_leapfrog_substeps_run": (c_int, [c_void_p, ctypes.POINTER(State), ctypes.POINTER(StepOpts),
                                               ctypes.POINTER(StepOpts), c_void_p, c_int, c_void_p, c_double,
@@ -264,6 +267,27 @@ def fmm_shell_orders(order, ring=2, rotational=False):
                                     SHP_SYM_ROTATIONAL if rotational else SHP_SYM_NONE, out),
            "shp_fmm_shell_orders")
     return tuple(out)
+
+
+def load_field_mesh_csv(fields_path, triangles_path, r0, E0):
+    """The reference's Fields_*.csv / Triangles_*.csv read by the library's loader;
+    returns the mesh dict (xy, er, ez, tri, nbr, r0, E0) as numpy copies."""
+    m = FieldMesh()
+    _check(lib.shp_field_mesh_load_csv(str(fields_path).encode(), str(triangles_path).encode(), float(r0),
+                                       float(E0), ctypes.byref(m)), "shp_field_mesh_load_csv")
+    try:
+        nd, nt = m.nnodes, m.ntri
+
+        def arr(ptr, ctype, count):
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(count,)).copy()
+
+        out = {"xy": arr(m.xy, ctypes.c_double, 2 * nd).reshape(nd, 2),
+               "er": arr(m.er, ctypes.c_double, nd), "ez": arr(m.ez, ctypes.c_double, nd),
+               "tri": arr(m.tri, ctypes.c_int, 3 * nt).reshape(nt, 3),
+               "nbr": arr(m.nbr, ctypes.c_int, 3 * nt).reshape(nt, 3), "r0": m.r0, "E0": m.E0}
+    finally:
+        lib.shp_field_mesh_free(ctypes.byref(m))
+    return out
 
 
 def make_symmetry(kind="none", nsec=4, lattice=None, domain=(0.0, 0.0, 0.0)):
