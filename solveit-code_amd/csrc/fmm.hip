@@ -412,6 +412,15 @@ __global__ void k_fmm_gather(const int* __restrict__ perm, int na, const double*
     sp[i] = make_double4(X[j], Y[j], Z[j], Q[j]);
 }
 
+// 1 for a sorted ion that is a target (status 1), for the prefix sums of
+// box_has_target; entry na is 0.
+__global__ void k_fmm_tflag(const int* __restrict__ perm, int na, const int* __restrict__ status,
+                            int* __restrict__ tf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > na) return;
+    tf[i] = i < na && status[perm[i]] == 1 ? 1 : 0;
+}
+
 // Number of active particles (keys != ~0 sort last) and, per level, the number
 // of particles that open a new box there: bin[lmin] counts particles whose key
 // first differs from the previous one at level lmin (so they open boxes at all
@@ -569,7 +578,17 @@ struct Level {
     float* M;
     float* Lc;
     int ld;  // row stride of M and Lc (coefficients padded to whole 128-byte lines)
+    // prefix sums over the sorted ions of "is a target" (status 1): box b holds
+    // a target iff tpre[end[b]] > tpre[start[b]]; nullptr: every box does
+    const int* tpre;
 };
+
+// Boxes without targets (source-only ions: symmetry images, or the ions of
+// other levels in a multi-rate step) still need their multipoles, but no
+// local expansion and no near field.
+__device__ __forceinline__ bool box_has_target(const Level& lv, int b) {
+    return !lv.tpre || lv.tpre[lv.end[b]] > lv.tpre[lv.start[b]];
+}
 
 __device__ __forceinline__ void box_center(const double* __restrict__ root, int l, uint64_t key,
                                            double c[3], double& h) {
@@ -754,6 +773,7 @@ __global__ __launch_bounds__(kFmmThreads) void k_fmm_m2l(
     __shared__ int pcnt[128];
     __shared__ int pfirst[128];
     const int b = blockIdx.x;
+    if (!box_has_target(tl, b)) return;  // uniform
     const uint64_t key = tl.key[b];
     const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
     const int px = ix >> 1, py = iy >> 1, pz = iz >> 1;
@@ -1001,21 +1021,28 @@ struct LevelOffs {
     int off[kM2LMaxLev + 1];  // level index li covers [off[li], off[li+1])
 };
 
-__global__ void k_fmm_oct_keys(const uint64_t* __restrict__ bkey, LevelOffs lo,
+// Boxes without targets get the key kNoGroup and sort after every group.
+constexpr int kNoGroup = 255;  // > 8 kM2LMaxLev, within the 8 sorted bits
+static_assert(8 * kM2LMaxLev < kNoGroup, "group keys must stay below the sentinel");
+
+__global__ void k_fmm_oct_keys(const uint64_t* __restrict__ bkey, LevelOffs lo, const int* __restrict__ bstart,
+                               const int* __restrict__ bend, const int* __restrict__ tpre,
                                unsigned short* __restrict__ okey, int* __restrict__ oval) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= lo.off[lo.nl]) return;
     int li = 0;
     while (li + 1 < lo.nl && lo.off[li + 1] <= i) li++;
-    okey[i] = (unsigned short)(li * 8 + (int)(bkey[i] & 7));
+    const bool tgt = !tpre || tpre[bend[i]] > tpre[bstart[i]];
+    okey[i] = (unsigned short)(tgt ? li * 8 + (int)(bkey[i] & 7) : kNoGroup);
     oval[i] = i - lo.off[li];
 }
 
-__global__ void k_fmm_seg_bounds(const unsigned short* __restrict__ sk, int n, int* __restrict__ start,
+__global__ void k_fmm_seg_bounds(const unsigned short* __restrict__ sk, int n, int nseg, int* __restrict__ start,
                                  int* __restrict__ end) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int k = sk[i];
+    if (k >= nseg) return;  // boxes without targets
     if (i == 0 || sk[i - 1] != k) start[k] = i;
     if (i == n - 1 || sk[i + 1] != k) end[k] = i + 1;
 }
@@ -1128,14 +1155,19 @@ __global__ __launch_bounds__(64, BLO == 0 ? 4 : (BLO == 7 ? 3 : 1)) void k_fmm_m
 // L[box][k] += lscale[k] * sum over chunks of the partial sums (fixed order);
 // rows of stride ld (a multiple of 4), the padding past nc left alone.  Each
 // lane takes four consecutive coefficients of one row.
+// Rows of boxes without targets were not written by the M2L and are skipped.
 __global__ __launch_bounds__(256) void k_fmm_m2l_reduce(const float* __restrict__ part, size_t stride, int nchunk,
                                                         int nc, int ld, const float* __restrict__ lscale,
-                                                        float* __restrict__ Lc, size_t n4) {
+                                                        float* __restrict__ Lc, size_t n4,
+                                                        const int* __restrict__ bstart, const int* __restrict__ bend,
+                                                        const int* __restrict__ tpre) {
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n4) return;
     const size_t e = 4 * q;
-    const int k = (int)(e - e / (size_t)ld * (size_t)ld);
+    const size_t row = e / (size_t)ld;
+    const int k = (int)(e - row * (size_t)ld);
     if (k >= nc) return;
+    if (tpre && tpre[bend[row]] <= tpre[bstart[row]]) return;
     float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (int c = 0; c < nchunk; c++) {
         const float4 v = *reinterpret_cast<const float4*>(part + c * stride + e);
@@ -1338,6 +1370,7 @@ __global__ __launch_bounds__(kShiftLanes) void k_fmm_l2l(Level par, Level ch, in
     float* Lp = sm;      // [nc]
     float* Q = sm + nc;  // [8][nc]: by parity class, then by child octant
     const int b = blockIdx.x, t = threadIdx.x;
+    if (!box_has_target(par, b)) return;  // uniform: no child holds a target
     const int c0 = par.cfirst[b];
     const int c1 = b + 1 < par.nb ? par.cfirst[b + 1] : ch.nb;
     const float* Lsrc = par.Lc + (size_t)b * par.ld;
@@ -1443,6 +1476,7 @@ __global__ __launch_bounds__(256) void k_fmm_level_cost(const LevelCostArgs A, i
     long nsum_all = 0, far = 0;
     for (long s = (long)blockIdx.x * 4 + wv; s * stride < lv.nb; s += (long)gridDim.x * 4) {
         const long b = s * stride;
+        if (!box_has_target(lv, (int)b)) continue;  // wave-uniform
         const uint64_t key = lv.key[b];
         const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
         long nsum = 0;
@@ -1504,6 +1538,13 @@ __global__ __launch_bounds__(256) void k_fmm_unit_count(Level lv, GridView gv, i
     // one wave per leaf box, its lanes splitting the near-field cell lookups
     const int b = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (b >= lv.nb) return;  // wave-uniform
+    if (!box_has_target(lv, b)) {  // no unit
+        if (lane == 0) {
+            cnt[b] = 0;
+            nsrc[b] = 0;
+        }
+        return;
+    }
     const uint64_t key = lv.key[b];
     const int ix = (int)compact21(key), iy = (int)compact21(key >> 1), iz = (int)compact21(key >> 2);
     const int cmax = (1 << L) - 1;
@@ -1993,6 +2034,17 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
     SHP_WS(sp, double4, h, WS_SRC, na);
     hipLaunchKernelGGL(k_fmm_gather, dim3(fdiv_up(na, 256)), dim3(256), 0, st, perm, na, pt->x, pt->y,
                        pt->z, pt->q, sp);
+    // target prefix sums over the sorted ions (boxes without targets skip the
+    // M2L, L2L and leaf work)
+    SHP_WS(tfl, int, h, WS_FMM_TPRE, 2 * ((size_t)na + 1));
+    int* const tpre = tfl + na + 1;
+    {
+        hipLaunchKernelGGL(k_fmm_tflag, dim3(fdiv_up(na + 1, 256)), dim3(256), 0, st, perm, na, pt->status, tfl);
+        size_t tsz = 0;
+        SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tsz, tfl, tpre, na + 1, st));
+        SHP_WS(cubp, char, h, WS_CUB, tsz);
+        SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cubp, tsz, tfl, tpre, na + 1, st));
+    }
     // level arrays in one workspace: key (u64), start, end, cfirst per box
     size_t totb = 0;
     for (int l = 0; l <= Lb; l++) totb += nbox[l];
@@ -2011,6 +2063,7 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
             v.cfirst = l < Lb ? bint + 2 * totb + off : nullptr;
             v.M = nullptr;
             v.Lc = nullptr;
+            v.tpre = tpre;
             off += nbox[l];
         }
     }
@@ -2253,8 +2306,10 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
             SHP_WS(oval, int, h, WS_FMM_OVAL, 2 * (size_t)nb2);
             SHP_WS(segb, int, h, WS_FMM_SEG, 2 * (size_t)nseg);
             SHP_WS(tiles, int4, h, WS_FMM_TILES, (size_t)ub);
+            const int* bst = bint + first;
+            const int* ben = bint + totb + first;
             hipLaunchKernelGGL(k_fmm_oct_keys, dim3(fdiv_up(nb2, 256)), dim3(256), 0, st, bkey + first, lofs,
-                               okey, oval);
+                               bst, ben, tpre, okey, oval);
             size_t tsz = 0;
             SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tsz, okey, okey + nb2, oval, oval + nb2,
                                                              nb2, 0, 8, st));
@@ -2263,7 +2318,7 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
                                                              nb2, 0, 8, st));
             hipLaunchKernelGGL(k_fmm_ifill, dim3(fdiv_up(2 * nseg, 256)), dim3(256), 0, st, segb, 2 * nseg, 0);
             hipLaunchKernelGGL(k_fmm_seg_bounds, dim3(fdiv_up(nb2, 256)), dim3(256), 0, st, okey + nb2, nb2,
-                               segb, segb + nseg);
+                               nseg, segb, segb + nseg);
             hipLaunchKernelGGL(k_fmm_tile_table, dim3(1), dim3(64), 0, st, segb, segb + nseg, nseg, ub, tiles);
             pl.tiles = tiles;
             pl.sorted = oval + nb2;
@@ -2288,7 +2343,7 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
             launch_m2l_tiled(pl, p3, p4, tb, cb, part, stride, h->fmm_stats, st, h->fev[6], h->fev[7]);
             if (S > 1)
                 hipLaunchKernelGGL(k_fmm_m2l_reduce, dim3(fdiv_up((long)(stride / 4), 256)), dim3(256), 0, st, part,
-                                   stride, S, nc, ld, tb->lscale, lv[2].Lc, stride / 4);
+                                   stride, S, nc, ld, tb->lscale, lv[2].Lc, stride / 4, bst, ben, tpre);
         }
         if (L < 2) {
             SHP_HIP_CHECK(hipEventRecord(h->fev[6], st));

@@ -93,6 +93,7 @@ enum WsSlot {
     WS_FMM_UNITS,   // FMM leaf work units (leaf, target run)
     WS_FMM_USORT,   // FMM leaf work units: sort keys, values and the longest-first order
     WS_FMM_SPF,     // FMM ions relative to their own leaf centre (FP32)
+    WS_FMM_TPRE,    // FMM target flags of the sorted ions and their prefix sums
     WS_CMP,         // accuracy metric: per-block partials
     WS_NSLOTS
 };
