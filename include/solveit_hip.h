@@ -21,6 +21,7 @@
 #define SOLVEIT_HIP_H
 
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -422,6 +423,45 @@ int shp_state_download_host(shp_handle h, float* pos4, float* vel4, float* acc4)
 int shp_state_run_substeps(shp_handle h, const shp_step_opts* o, const shp_step_opts* sub,
                            const shp_substep_level* levels, int max_level, int reset_levels, double t0,
                            double dt, int nsteps);
+
+/* ---- writers of the resident state (the reference's binary formats) ----
+ * While shp_state_run / shp_state_run_substeps drive the leapfrog, the engine
+ * keeps each ion's background field at its last kick (the reference's
+ * Efield2, particles.cpp:508-512) and queues the boundary-condition events
+ * that the reference writes from enforceBoundaryConditions
+ * (particles.cpp:73-166); an upload resets both. */
+
+/* SaveDataOptions (structures.h:244-247). */
+enum { SHP_SAVE_ORIGINAL = 0, SHP_SAVE_NEW_ORIGINAL = 1 };
+
+/* Integrator::saveData (integration_schemes.cpp:263-331): the ions of
+ * [first, last) with status 1 or 2, in index order, in the record layout of
+ * `format` (original: x y z vx vy vz, Efield1, Efield2 as floats, species,
+ * index, lastParticle = last as 32-bit integers; newOriginal: a header
+ * 0xffffffff, lastParticle, current_time, then index, species and the twelve
+ * floats per ion).  Efield1 is the Coulomb field AMU/e * a * m / q of the last
+ * force evaluation, rounded as the reference rounds it
+ * (integration_schemes.cpp:350-355).  The snapshot is packed on the GPU, copied
+ * to pinned host memory on a second stream and written to `out` by a host
+ * thread: the call returns once the pack is enqueued, and `out` must not be
+ * touched or closed before shp_state_flush returns.  Two snapshots can be in
+ * flight; a third call waits for the oldest. */
+int shp_state_save(shp_handle h, int format, int first, int last, float current_time, FILE* out);
+/* Integrator::saveFullCurrentConditions (integration_schemes.cpp:197-258):
+ * every resident ion, whatever its status, in the same layouts, with
+ * lastParticle = last_particle.  Asynchronous like shp_state_save. */
+int shp_state_save_full(shp_handle h, int format, int last_particle, float current_time, FILE* out);
+/* Waits until every snapshot is written; returns the first write failure. */
+int shp_state_flush(shp_handle h);
+/* ChargedParticles::writeToEventsFile (particles.cpp:666-684) for the events
+ * queued since the last call: time (the drift's end), x y z, vx vy vz as
+ * floats, index, species, event (0, outOfBounds) as 32-bit integers, in the
+ * order the reference writes them (by drift, spatial limits before the time
+ * limit, then index).  Synchronous; *nevents (may be null) gets the count. */
+int shp_state_events(shp_handle h, FILE* out, int* nevents);
+/* Efield1 and Efield2 of every resident ion into the reference's
+ * vec3<float> arrays (ChargedParticles::Efield1 / Efield2, particles.h:108-109). */
+int shp_state_fields_host(shp_handle h, float* efield1, float* efield2);
 
 /* Kernel time (ms) of the last leapfrog update kernel. */
 double shp_last_update_ms(shp_handle h);

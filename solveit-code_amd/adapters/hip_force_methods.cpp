@@ -196,6 +196,20 @@ static shp_step_opts runOptions(ParticleSystem& particleSystem, const shp_step_o
     return o;
 }
 
+// What the reference's own writers read besides the state: the Coulomb and
+// background fields of the step (Efield1 / Efield2, for saveData), and the
+// step's boundary-condition events, written to the run's events file as
+// writeToEventsFile writes them.
+static void downloadOutputs(ParticleSystem& particleSystem, shp_handle handle) {
+    ChargedParticles& cp = particleSystem.chargedParticles;
+    if (cp.Efield1 != nullptr && cp.Efield2 != nullptr &&
+        shp_state_fields_host(handle, &cp.Efield1[0].x, &cp.Efield2[0].x) != SHP_OK)
+        fail("shp_state_fields_host");
+    if (particleSystem.params.eventsFile != nullptr &&
+        shp_state_events(handle, particleSystem.params.eventsFile, nullptr) != SHP_OK)
+        fail("shp_state_events");
+}
+
 HipLeapfrogDKD::HipLeapfrogDKD(ParticleSystem& particleSystem, ForceCalculationMethod& forceMethod,
                                const shp_step_opts& opts, const shp_field_mesh* mesh, int device,
                                const Grid* grid)
@@ -225,6 +239,7 @@ void HipLeapfrogDKD::runSimulation() {
         if (shp_state_download_host(handle, &cp.position[0].x, &cp.velocity[0].x, &cp.acceleration[0].x) !=
             SHP_OK)
             fail("shp_state_download_host");
+        downloadOutputs(particleSystem, handle);
         // ions injected by the end of the step (findMostRecentInjected, particles.cpp:1328-1359)
         setLastChargedParticle(cp.findMostRecentInjected(timeStep * i) + 1);
         currentTime = timeStep * i;
@@ -301,6 +316,7 @@ void HipLeapfrogDKDSubsteps::runSimulation() {
         if (shp_state_download_host(handle, &cp.position[0].x, &cp.velocity[0].x, &cp.acceleration[0].x) !=
             SHP_OK)
             fail("shp_state_download_host");
+        downloadOutputs(particleSystem, handle);
         setLastChargedParticle(cp.findMostRecentInjected(timeStep * i) + 1);
         currentTime = timeStep * i;
         finishIter(i);

@@ -90,6 +90,7 @@ int shp_destroy(shp_handle h) {
     if (!h) return SHP_OK;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
+    shp::writer_release(h);
     for (int i = 0; i < shp::WS_NSLOTS; i++)
         if (h->ws[i]) (void)hipFree(h->ws[i]);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -103,6 +104,10 @@ int shp_destroy(shp_handle h) {
     if (h->res_i32) (void)hipFree(h->res_i32);
     if (h->res_aos) (void)hipFree(h->res_aos);
     if (h->res_level) (void)hipFree(h->res_level);
+    if (h->res_e2) (void)hipFree(h->res_e2);
+    if (h->res_ev) (void)hipFree(h->res_ev);
+    if (h->res_evkey) (void)hipFree(h->res_evkey);
+    if (h->res_evn) (void)hipFree(h->res_evn);
     shp::free_mesh(h);
     shp::fmm_release(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -245,11 +245,41 @@ struct UpdateParams {
     double bc_time;
     double e_over_amu;  // ELEM_CHARGE / AMU (particles.cpp:500)
     int use_efield;
+    // Outputs of the resident runs (null otherwise): the background field of
+    // each kick, [3][n], and the boundary-condition event queue.
+    float* e2;
+    unsigned int* ev;
+    unsigned long long* evkey;
+    int* evn;
+    int evcap;
+    long long seq;  // number of the kernel's first drift since the upload
 };
 
+// One boundary-condition event in writeToEventsFile's layout, queued with the
+// key (2 drift + pass) << 32 | index: the reference enforces the spatial
+// limits over all ions, then the temporal one (particles.cpp:73-92), each in
+// index order, after every drift.
+__device__ void bc_event(const UpdateParams& P, long long seq, int pass, double t, double x, double y,
+                         double z, double vx, double vy, double vz, int i, int sp) {
+    const int k = atomicAdd(P.evn, 1);
+    if (k >= P.evcap) return;  // cannot happen: a status-1 ion leaves once per upload
+    unsigned int* r = P.ev + (size_t)k * kEventWords;
+    r[0] = __float_as_uint((float)t);
+    r[1] = __float_as_uint((float)x);
+    r[2] = __float_as_uint((float)y);
+    r[3] = __float_as_uint((float)z);
+    r[4] = __float_as_uint((float)vx);
+    r[5] = __float_as_uint((float)vy);
+    r[6] = __float_as_uint((float)vz);
+    r[7] = (unsigned int)i;
+    r[8] = (unsigned int)sp;
+    r[9] = 0u;  // ParticleStatus::outOfBounds
+    P.evkey[k] = ((unsigned long long)(2 * seq + pass) << 32) | (unsigned int)i;
+}
+
 __device__ __forceinline__ void fold_and_bound(const UpdateParams& P, double& x, double& y,
-                                               double& z, double& vx, double& vy, int& st,
-                                               double t_now, double t_inj) {
+                                               double& z, double& vx, double& vy, double vz, int& st,
+                                               double t_now, double t_inj, int i, int sp, long long seq) {
     if (P.sym == SHP_SYM_ROTATIONAL) {
         if (P.nsec == 4) {
             // rotateToFirstQuadrant, particles.cpp:232-262
@@ -296,15 +326,19 @@ __device__ __forceinline__ void fold_and_bound(const UpdateParams& P, double& x,
         if (P.bc[1] != 0 && fabs(y) > P.bc[1]) out = true;
         if (P.bc[2] != 0 && fabs(z) > P.bc[2]) out = true;
         if (P.bc[3] != 0 && sqrt(x * x + y * y) > P.bc[3]) out = true;
-        if (P.bc_time != 0 && t_now - t_inj > P.bc_time) out = true;
-        if (out) st = 0;
+        const bool late = P.bc_time != 0 && t_now - t_inj > P.bc_time;
+        if (out || late) {
+            st = 0;
+            if (P.evn) bc_event(P, seq, out ? 0 : 1, t_now, x, y, z, vx, vy, vz, i, sp);
+        }
     }
 }
 
 // One half drift of ion state over [t_start, t_start + h].
 __device__ __forceinline__ void half_drift(const UpdateParams& P, double& x, double& y, double& z,
                                            double& vx, double& vy, double vz, int& st,
-                                           double t_inj, double t_start, double h) {
+                                           double t_inj, double t_start, double h, int i, int sp,
+                                           long long seq) {
     const double t_end = t_start + h;
     if (t_inj > t_end) return;  // not injected yet: outside [first, last)
     if (st == 1 || st == 2) {
@@ -313,21 +347,21 @@ __device__ __forceinline__ void half_drift(const UpdateParams& P, double& x, dou
         y += vy * s;
         z += vz * s;
     }
-    fold_and_bound(P, x, y, z, vx, vy, st, t_end, t_inj);
+    fold_and_bound(P, x, y, z, vx, vy, vz, st, t_end, t_inj, i, sp, seq);
 }
 
 // First half drift of a run; writes the active-masked status for the force.
 __global__ void k_drift(UpdateParams P, int n, double* __restrict__ X, double* __restrict__ Y,
                         double* __restrict__ Z, double* __restrict__ VX, double* __restrict__ VY,
                         const double* __restrict__ VZ, int* __restrict__ status,
-                        const double* __restrict__ tinj, double t_start, double h,
-                        int* __restrict__ eff) {
+                        const int* __restrict__ species, const double* __restrict__ tinj, double t_start,
+                        double h, int* __restrict__ eff) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double x = X[i], y = Y[i], z = Z[i], vx = VX[i], vy = VY[i];
     int st = status[i];
     const double ti = tinj[i];
-    half_drift(P, x, y, z, vx, vy, VZ[i], st, ti, t_start, h);
+    half_drift(P, x, y, z, vx, vy, VZ[i], st, ti, t_start, h, i, species[i], P.seq);
     X[i] = x; Y[i] = y; Z[i] = z; VX[i] = vx; VY[i] = vy;
     status[i] = st;
     eff[i] = ti <= t_start + h ? st : 0;
@@ -355,21 +389,27 @@ __global__ void k_kick_drift(UpdateParams P, MeshView M, int n, double* __restri
     const double h = 0.5 * dt;
     if (st == 1 && ti <= t + h) {
         double ax = AX[i], ay = AY[i], az = AZ[i];
+        double ex = 0.0, ey = 0.0, ez = 0.0;
         if (P.use_efield) {
-            double ex, ey, ez;
             efield_f64(M, x, y, z, ex, ey, ez);
             const double f = P.e_over_amu * Q[i] / species_mass(species[i]);
             ax += f * ex;
             ay += f * ey;
             az += f * ez;
         }
+        if (P.e2) {  // Efield2 of the kicked ion (particles.cpp:512)
+            P.e2[i] = (float)ex;
+            P.e2[n + i] = (float)ey;
+            P.e2[2 * (size_t)n + i] = (float)ez;
+        }
         const double s = fmin(fmax(t + dt - ti, 0.0), dt);
         vx += ax * s;
         vy += ay * s;
         vz += az * s;
     }
-    half_drift(P, x, y, z, vx, vy, vz, st, ti, t + h, h);
-    if (next) half_drift(P, x, y, z, vx, vy, vz, st, ti, t + dt, h);
+    const int sp = species[i];
+    half_drift(P, x, y, z, vx, vy, vz, st, ti, t + h, h, i, sp, P.seq);
+    if (next) half_drift(P, x, y, z, vx, vy, vz, st, ti, t + dt, h, i, sp, P.seq + 1);
     X[i] = x; Y[i] = y; Z[i] = z;
     VX[i] = vx; VY[i] = vy; VZ[i] = vz;
     status[i] = st;
@@ -389,6 +429,17 @@ static UpdateParams make_params(const shp_step_opts* o) {
     P.e_over_amu = (double)1.6021766e-19f / (double)1.66053904e-27f;
     P.use_efield = o->use_efield;
     return P;
+}
+
+// The resident run's outputs (field of each kick, event queue) while
+// shp_state_run* drives the leapfrog; P.seq is set per launch.
+static void attach_outputs(shp_handle h, UpdateParams& P) {
+    if (!h->res_events_on) return;
+    P.e2 = h->res_e2;
+    P.ev = h->res_ev;
+    P.evkey = h->res_evkey;
+    P.evn = h->res_evn;
+    P.evcap = h->res_cap;
 }
 
 // Accuracy metric of Integrator::compareAcceleration
@@ -482,7 +533,7 @@ int lf_begin(shp_handle h, const shp_state* s, const shp_step_opts* o, double t0
     const UpdateParams P = make_params(o);
     const int n = s->n;
     hipLaunchKernelGGL(k_drift, dim3(cdiv_l(n, 256)), dim3(256), 0, h->stream, P, n, s->x, s->y, s->z,
-                       s->vx, s->vy, s->vz, s->status, s->t_inj, t0, 0.5 * dt, eff);
+                       s->vx, s->vy, s->vz, s->status, s->species, s->t_inj, t0, 0.5 * dt, eff);
     SHP_HIP_CHECK(hipGetLastError());
     return SHP_OK;
 }
@@ -627,13 +678,18 @@ __global__ void k_ss_kick(UpdateParams P, MeshView M, SubstepTable T, int n, con
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || eff[i] != 1) return;
     double ax = AX[i], ay = AY[i], az = AZ[i];
+    double ex = 0.0, ey = 0.0, ez = 0.0;
     if (P.use_efield) {
-        double ex, ey, ez;
         efield_f64(M, X[i], Y[i], Z[i], ex, ey, ez);
         const double f = P.e_over_amu * Q[i] / species_mass(species[i]);
         ax += f * ex;
         ay += f * ey;
         az += f * ez;
+    }
+    if (P.e2) {  // Efield2 of the kicked ion (particles.cpp:512)
+        P.e2[i] = (float)ex;
+        P.e2[n + i] = (float)ey;
+        P.e2[2 * (size_t)n + i] = (float)ez;
     }
     const double dl = T.dtl[level[i]];
     const double s = fmin(fmax(tvel + dl - tinj[i], 0.0), dl);
@@ -796,11 +852,13 @@ int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double 
     SHP_HIP_CHECK(hipSetDevice(h->device));
     const int n = s->n;
     SHP_WS(eff, int, h, WS_EFF_STATUS, n);
-    const UpdateParams P = make_params(o);
+    UpdateParams P = make_params(o);
+    attach_outputs(h, P);
     const MeshView M = h->mesh.ready ? view_of(h->mesh) : MeshView{};
     const dim3 grid(cdiv_l(n, 256)), block(256);
+    P.seq = h->drift_seq++;
     hipLaunchKernelGGL(k_drift, grid, block, 0, h->stream, P, n, s->x, s->y, s->z, s->vx, s->vy,
-                       s->vz, s->status, s->t_inj, t0, 0.5 * dt, eff);
+                       s->vz, s->status, s->species, s->t_inj, t0, 0.5 * dt, eff);
     SHP_HIP_CHECK(hipGetLastError());
     for (int k = 0; k < nsteps; k++) {
         const double t = t0 + k * dt;
@@ -813,6 +871,8 @@ int shp_leapfrog_run(shp_handle h, shp_state* s, const shp_step_opts* o, double 
                      : shp_direct_dev(h, &p, o->precision, &o->sym, ax, ay, az);
         if (rc) return rc;
         SHP_HIP_CHECK(hipEventRecord(h->ev2, h->stream));
+        P.seq = h->drift_seq;
+        h->drift_seq += k + 1 < nsteps ? 2 : 1;
         hipLaunchKernelGGL(k_kick_drift, grid, block, 0, h->stream, P, M, n, s->x, s->y, s->z, s->vx,
                            s->vy, s->vz, s->q, s->status, s->species, s->t_inj, ax, ay, az, t, dt,
                            k + 1 < nsteps ? 1 : 0, eff, nullptr);
@@ -916,8 +976,9 @@ struct SubstepRun {
 int ss_drift(SubstepRun& R) {
     const double hh = R.T.dtl[R.T.M] / 2;
     shp_state* s = R.s;
+    R.P.seq = R.h->drift_seq++;
     hipLaunchKernelGGL(k_drift, R.grid, R.block, 0, R.h->stream, R.P, s->n, s->x, s->y, s->z, s->vx, s->vy, s->vz,
-                       s->status, s->t_inj, R.tpos, hh, R.eff);
+                       s->status, s->species, s->t_inj, R.tpos, hh, R.eff);
     SHP_HIP_CHECK(hipGetLastError());
     R.tpos += hh;
     return SHP_OK;
@@ -994,6 +1055,7 @@ int shp_leapfrog_substeps_run(shp_handle h, shp_state* s, const shp_step_opts* o
     R.o = o;
     R.sub = sub;
     R.P = make_params(o);
+    attach_outputs(h, R.P);
     R.M = h->mesh.ready ? view_of(h->mesh) : MeshView{};
     R.T = SubstepTable{};
     R.T.M = max_level;
@@ -1033,17 +1095,31 @@ int shp_state_upload_host(shp_handle h, const float* pos4, const float* vel4, co
         if (h->res_f64) SHP_HIP_CHECK(hipFree(h->res_f64));
         if (h->res_i32) SHP_HIP_CHECK(hipFree(h->res_i32));
         if (h->res_aos) SHP_HIP_CHECK(hipFree(h->res_aos));
+        if (h->res_e2) SHP_HIP_CHECK(hipFree(h->res_e2));
+        if (h->res_ev) SHP_HIP_CHECK(hipFree(h->res_ev));
+        if (h->res_evkey) SHP_HIP_CHECK(hipFree(h->res_evkey));
         h->res_f64 = nullptr;
         h->res_i32 = nullptr;
         h->res_aos = nullptr;
+        h->res_e2 = nullptr;
+        h->res_ev = nullptr;
+        h->res_evkey = nullptr;
         h->res_cap = 0;
         SHP_HIP_CHECK(hipMalloc((void**)&h->res_f64, sizeof(double) * 11 * (size_t)n));
         SHP_HIP_CHECK(hipMalloc((void**)&h->res_i32, sizeof(int) * 2 * (size_t)n));
         SHP_HIP_CHECK(hipMalloc((void**)&h->res_aos, sizeof(float4) * 3 * (size_t)n + sizeof(float) * (size_t)n));
+        SHP_HIP_CHECK(hipMalloc((void**)&h->res_e2, sizeof(float) * 3 * (size_t)n));
+        SHP_HIP_CHECK(hipMalloc((void**)&h->res_ev, sizeof(unsigned int) * kEventWords * (size_t)n));
+        SHP_HIP_CHECK(hipMalloc((void**)&h->res_evkey, sizeof(unsigned long long) * (size_t)n));
         h->res_cap = n;
     }
+    if (!h->res_evn) SHP_HIP_CHECK(hipMalloc((void**)&h->res_evn, sizeof(int)));
     h->res_n = n;
+    h->drift_seq = 0;
+    // no kick yet (Efield2 starts at zero, particles.cpp:1040-1041) and no event
+    hipLaunchKernelGGL(k_ss_fill, dim3(1), dim3(64), 0, h->stream, h->res_evn, 1, 0);
     if (n == 0) return SHP_OK;
+    hipLaunchKernelGGL(k_ss_fill, dim3(cdiv_l(3 * (long)n, 256)), dim3(256), 0, h->stream, (int*)h->res_e2, 3 * n, 0);
     float4* dpos = h->res_aos;
     float4* dvel = dpos + n;
     float4* dacc = dvel + n;
@@ -1061,6 +1137,14 @@ int shp_state_upload_host(shp_handle h, const float* pos4, const float* vel4, co
     return SHP_OK;
 }
 
+// Marks the leapfrog launches of a resident run, which record each kick's
+// field and the boundary-condition events.
+struct ResidentOutputs {
+    shp_handle h;
+    explicit ResidentOutputs(shp_handle hh) : h(hh) { h->res_events_on = true; }
+    ~ResidentOutputs() { h->res_events_on = false; }
+};
+
 int shp_state_run(shp_handle h, const shp_step_opts* o, double t0, double dt, int nsteps) {
     SHP_REQUIRE(h && o, "shp_state_run: null argument");
     SHP_REQUIRE(h->res_f64 || h->res_n == 0, "shp_state_run: no resident state (shp_state_upload_host)");
@@ -1069,6 +1153,7 @@ int shp_state_run(shp_handle h, const shp_step_opts* o, double t0, double dt, in
     double* f = h->res_f64;
     shp_state s{n, f, f + n, f + 2 * (size_t)n, f + 3 * (size_t)n, f + 4 * (size_t)n, f + 5 * (size_t)n,
                 f + 6 * (size_t)n, h->res_i32, h->res_i32 + n, f + 7 * (size_t)n};
+    ResidentOutputs on(h);
     return shp_leapfrog_run(h, &s, o, t0, dt, nsteps, f + 8 * (size_t)n, f + 9 * (size_t)n, f + 10 * (size_t)n);
 }
 
@@ -1095,6 +1180,7 @@ int shp_state_run_substeps(shp_handle h, const shp_step_opts* o, const shp_step_
     double* f = h->res_f64;
     shp_state s{n, f, f + n, f + 2 * (size_t)n, f + 3 * (size_t)n, f + 4 * (size_t)n, f + 5 * (size_t)n,
                 f + 6 * (size_t)n, h->res_i32, h->res_i32 + n, f + 7 * (size_t)n};
+    ResidentOutputs on(h);
     return shp_leapfrog_substeps_run(h, &s, o, sub, levels, max_level, h->res_level, t0, dt, nsteps,
                                      f + 8 * (size_t)n, f + 9 * (size_t)n, f + 10 * (size_t)n);
 }

@@ -44,6 +44,14 @@ void free_mesh(shp_handle h);
 int direct_scratch_setup(int device);
 void fmm_release(shp_handle h);
 
+// Writers of the resident state (writers.hip).  One boundary-condition event
+// is kEventWords 32-bit words in the layout of writeToEventsFile
+// (particles.cpp:666-684): time, x, y, z, vx, vy, vz (float), index, species,
+// event (uint32).
+constexpr int kEventWords = 10;
+struct StateWriter;              // snapshot staging and its host writer thread
+void writer_release(shp_handle h);  // waits for pending writes, frees the staging
+
 // Grow-only device workspace slots owned by a context.
 enum WsSlot {
     WS_SRC = 0,     // staged sources
@@ -166,6 +174,18 @@ struct shp_context {
     int res_n = 0, res_cap = 0;
     int* res_level = nullptr;   // ion levels of shp_state_run_substeps
     int res_level_cap = 0;
+    // Outputs of the resident runs: each ion's background field at its last
+    // kick ([3][res_cap] floats) and the queue of boundary-condition events
+    // (res_cap of them: an ion leaves once per upload), kept while
+    // res_events_on (inside shp_state_run*); drift_seq numbers the drifts
+    // since the upload, which orders the events as the reference writes them.
+    float* res_e2 = nullptr;
+    unsigned int* res_ev = nullptr;
+    unsigned long long* res_evkey = nullptr;
+    int* res_evn = nullptr;
+    bool res_events_on = false;
+    long long drift_seq = 0;
+    shp::StateWriter* writer = nullptr;
     void* ws[shp::WS_NSLOTS] = {};
     size_t ws_bytes[shp::WS_NSLOTS] = {};
     int num_cus = 256;

@@ -221,6 +221,11 @@ def _load():
         "shp_state_run_substeps": (c_int, [c_void_p, ctypes.POINTER(StepOpts), ctypes.POINTER(StepOpts), c_void_p,
                                            c_int, c_int, c_double, c_double, c_int]),
         "shp_state_download_host": (c_int, [c_void_p, fp, fp, fp]),
+        "shp_state_save": (c_int, [c_void_p, c_int, c_int, c_int, ctypes.c_float, c_void_p]),
+        "shp_state_save_full": (c_int, [c_void_p, c_int, c_int, ctypes.c_float, c_void_p]),
+        "shp_state_flush": (c_int, [c_void_p]),
+        "shp_state_events": (c_int, [c_void_p, c_void_p, ctypes.POINTER(c_int)]),
+        "shp_state_fields_host": (c_int, [c_void_p, fp, fp]),
         "shp_fmm_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                  ctypes.POINTER(Symmetry), dp]),
         "shp_direct_dev_noalloc": (c_int, [c_void_p, ctypes.POINTER(Particles), c_int,
@@ -267,6 +272,37 @@ def fmm_shell_orders(order, ring=2, rotational=False):
                                     SHP_SYM_ROTATIONAL if rotational else SHP_SYM_NONE, out),
            "shp_fmm_shell_orders")
     return tuple(out)
+
+
+SHP_SAVE_ORIGINAL = 0
+SHP_SAVE_NEW_ORIGINAL = 1
+
+_libc = ctypes.CDLL(None)
+_libc.fopen.restype = ctypes.c_void_p
+_libc.fopen.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+_libc.fclose.argtypes = [ctypes.c_void_p]
+_libc.fclose.restype = ctypes.c_int
+
+
+class CFile:
+    """A C stdio FILE* for the writer entry points (fopen / fclose of the C
+    library the engine links)."""
+
+    def __init__(self, path, mode="wb"):
+        self.ptr = _libc.fopen(str(path).encode(), mode.encode())
+        if not self.ptr:
+            raise OSError(f"fopen({path!r}, {mode!r}) failed")
+
+    def close(self):
+        if self.ptr:
+            _libc.fclose(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def load_field_mesh_csv(fields_path, triangles_path, r0, E0):
@@ -450,6 +486,34 @@ class Engine:
             assert a.dtype == np.float32 and a.flags.c_contiguous
         _check(lib.shp_state_download_host(self.h, pos4.ctypes.data_as(fp), vel4.ctypes.data_as(fp),
                                            acc4.ctypes.data_as(fp)), "shp_state_download_host")
+
+    def state_save(self, cfile, fmt, first, last, time):
+        """Integrator::saveData of the resident state into an open CFile (asynchronous)."""
+        _check(lib.shp_state_save(self.h, int(fmt), int(first), int(last), float(time), cfile.ptr),
+               "shp_state_save")
+
+    def state_save_full(self, cfile, fmt, last, time):
+        """Integrator::saveFullCurrentConditions of the resident state (asynchronous)."""
+        _check(lib.shp_state_save_full(self.h, int(fmt), int(last), float(time), cfile.ptr),
+               "shp_state_save_full")
+
+    def state_flush(self):
+        _check(lib.shp_state_flush(self.h), "shp_state_flush")
+
+    def state_events(self, cfile):
+        """Writes the queued boundary-condition events; returns their number."""
+        ne = ctypes.c_int(0)
+        _check(lib.shp_state_events(self.h, cfile.ptr, ctypes.byref(ne)), "shp_state_events")
+        return ne.value
+
+    def state_fields_host(self, n):
+        """Efield1 and Efield2 of the resident ions as float32 (n, 3) arrays."""
+        fp = ctypes.POINTER(ctypes.c_float)
+        e1 = np.zeros((n, 3), np.float32)
+        e2 = np.zeros((n, 3), np.float32)
+        _check(lib.shp_state_fields_host(self.h, e1.ctypes.data_as(fp), e2.ctypes.data_as(fp)),
+               "shp_state_fields_host")
+        return e1, e2
 
     def bh_set_options(self, prism=False, box_dims=None, subcell_block=2, outlier_toggle=0):
         """Root-layout options of later Barnes-Hut calls (BarnesHutForceCalculation's
