@@ -175,6 +175,28 @@ int shp_fmm_host(shp_handle h, const float* pos4, const float* vel4, const float
  * A pure function (no device work).  No counterpart in the reference. */
 int shp_fmm_shell_orders(int order, int ring, int sym_type, int* out3);
 
+/* FmmType (structures.h:283-288) of later FMM evaluations on this handle
+ * (shp_fmm_dev, shp_fmm_sym_dev, shp_fmm_host, the FMM leapfrog):
+ *  - SHP_FMM_FMM (the default): M2L, L2L and L2P, as above;
+ *  - SHP_FMM_TREECODE: no local expansions; every target evaluates the
+ *    multipoles of its far boxes at every level directly (M2P,
+ *    fmm_cpukernel.cpp:1319-1533), with the same near field;
+ *  - SHP_FMM_FMM_OUTLIERS / SHP_FMM_TREECODE_OUTLIERS: the ions that
+ *    DomainBoundsFunction::isOutlier flags (deviation from the median of at
+ *    least 3 x 1.4826 median absolute deviations on some axis,
+ *    domain_bounds_functions.cpp:16-93) are held out of the tree; they get the
+ *    tree's field by M2P from their far boxes plus the ions of their near
+ *    cells (m2outlier_p, p2outlier_p, fmm_cpukernel.cpp:2282-2776), and every
+ *    target gets the outliers' field by a direct FP64 sum (outlier_p2p,
+ *    :2230-2280).  Without symmetry only (SHP_ERR_UNSUPPORTED otherwise).
+ * M2P evaluates expansions of order <= 10; a larger order in these modes
+ * returns SHP_ERR_UNSUPPORTED. */
+enum { SHP_FMM_TREECODE = 0, SHP_FMM_TREECODE_OUTLIERS = 1, SHP_FMM_FMM = 2, SHP_FMM_FMM_OUTLIERS = 3 };
+int shp_fmm_set_type(shp_handle h, int fmm_type);
+/* The outlier flags (1 = outlier) of the outlier modes for device arrays
+ * (flags: n device ints), and their number (*nout, may be null). */
+int shp_fmm_outliers_dev(shp_handle h, const shp_particles* pt, int* flags, int* nout);
+
 /* Leaf level used by the last shp_fmm_dev call on this handle. */
 int shp_fmm_last_leaf_level(shp_handle h);
 

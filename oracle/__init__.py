@@ -505,24 +505,46 @@ def get_threads():
 lib.orc_fmm_set_leaf_stride.argtypes = [_ci]
 
 
-def fmm_f64(d, p, L, ring=1, leaf_stride=1):
+# FmmType (structures.h:283-288)
+FMM_TREECODE, FMM_TREECODE_OUTLIERS, FMM_FMM, FMM_FMM_OUTLIERS = 0, 1, 2, 3
+lib.orc_fmm_set_type.argtypes = [_ci]
+lib.orc_fmm_outliers.argtypes = [_ci, _vp, _vp, _vp, _vp, _vp]
+lib.orc_fmm_outliers.restype = _ci
+
+
+def fmm_f64(d, p, L, ring=1, leaf_stride=1, fmm_type=FMM_FMM):
     """FP64 FMM accelerations of the engine's algorithm; returns (acc[n,3], stats).
     leaf_stride k > 1 evaluates L2P + P2P on every k-th leaf box only (the
-    other rows stay zero), for bounded timing samples."""
+    other rows stay zero), for bounded timing samples.  fmm_type: FMM_FMM
+    (M2L/L2L/L2P), FMM_TREECODE (M2P from every level's far boxes), and their
+    *_OUTLIERS variants (outliers held out of the tree)."""
     x, y, z, q = _f64(d["x"]), _f64(d["y"]), _f64(d["z"]), _f64(d["q"])
     st, sp = _i32(d["status"]), _i32(d["species"])
     n = x.shape[0]
     out = np.zeros((n, 3))
-    stats = np.zeros(6)
+    stats = np.zeros(7)
     lib.orc_fmm_set_leaf_stride(int(leaf_stride))
+    lib.orc_fmm_set_type(int(fmm_type))
     try:
         lib.orc_fmm_f64(n, x.ctypes.data, y.ctypes.data, z.ctypes.data, q.ctypes.data,
                         st.ctypes.data, sp.ctypes.data, p, L, ring, out.ctypes.data,
                         stats.ctypes.data)
     finally:
         lib.orc_fmm_set_leaf_stride(1)
+        lib.orc_fmm_set_type(FMM_FMM)
     return out, {"leaves": int(stats[0]), "p2p_pairs": stats[1], "m2l": stats[2],
-                 "far_s": stats[3], "near_s": stats[4], "leaves_evaluated": int(stats[5])}
+                 "far_s": stats[3], "near_s": stats[4], "leaves_evaluated": int(stats[5]),
+                 "m2p": stats[6]}
+
+
+def fmm_outliers(d):
+    """Outlier flags (isOutlier as a defined statistic) and their number."""
+    x, y, z = _f64(d["x"]), _f64(d["y"]), _f64(d["z"])
+    st = _i32(d["status"])
+    flags = np.zeros(x.shape[0], np.int32)
+    k = lib.orc_fmm_outliers(x.shape[0], x.ctypes.data, y.ctypes.data, z.ctypes.data, st.ctypes.data,
+                             flags.ctypes.data)
+    return flags, k
 
 
 def fmm_taylor(p, R, eps2):

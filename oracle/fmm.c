@@ -85,7 +85,31 @@ static double binom(int n, int k) {
     return r;
 }
 
-/* T_g for |g| <= p at R (lengths already scaled), eps2 = eps^2 scaled. */
+/* T_g for |g| <= B->p at R (lengths already scaled), eps2 = eps^2 scaled. */
+static void taylor_b(const Basis* Bp, const double R[3], double eps2, double* T) {
+    const Basis* B = Bp;
+    double s = R[0] * R[0] + R[1] * R[1] + R[2] * R[2] + eps2;
+    T[0] = 1.0 / sqrt(s);
+    for (int k = 1; k < B->nc; k++) {
+        const int* g = B->mi[k];
+        int n = g[0] + g[1] + g[2];
+        double acc = 0.0;
+        for (int i = 0; i < 3; i++) {
+            if (g[i] >= 1) {
+                int h[3] = {g[0], g[1], g[2]};
+                h[i] -= 1;
+                acc += (2 * n - 1) * R[i] * T[B->idx[h[0]][h[1]][h[2]]];
+            }
+            if (g[i] >= 2) {
+                int h[3] = {g[0], g[1], g[2]};
+                h[i] -= 2;
+                acc += (n - 1) * T[B->idx[h[0]][h[1]][h[2]]];
+            }
+        }
+        T[k] = -acc / (n * s);
+    }
+}
+
 void orc_fmm_taylor(int p, const double R[3], double eps2, double* T) {
     Basis B;
     basis_init(&B, p);
@@ -131,6 +155,8 @@ typedef struct {
     double** M;                  /* multipoles [box][nc] */
     double** Lc;                 /* locals */
 } Tree;
+
+typedef struct { uint64_t k; int i; } KI;  /* leaf key and particle of the sorted sources */
 
 static uint64_t spread21(uint64_t v) {
     v &= 0x1fffff;
@@ -203,13 +229,146 @@ static void box_center(const Tree* t, int l, uint64_t key, double c[3], double* 
     *h = 0.5 * bs;
 }
 
+/* Field of box b of level l at X from its multipole expansion (M2P,
+ * fmm_cpukernel.cpp:1319-1533):  with u = (X - c)/h,
+ *   acc_k += -(1/h^2) sum_{|a|<=p} (-1)^|a| M_a (a_k + 1) T_{a+e_k}(u),
+ * B the basis of order p, B1 of order p + 1, T scratch of B1->nc. */
+static void m2p_box(const Tree* t, const Basis* B, const Basis* B1, int l, int b, const double X[3], double eps2,
+                    double* T, double acc[3]) {
+    double c[3], h;
+    box_center(t, l, t->key[l][b], c, &h);
+    double u[3] = {(X[0] - c[0]) / h, (X[1] - c[1]) / h, (X[2] - c[2]) / h};
+    taylor_b(B1, u, eps2 / (h * h), T);
+    const double* M = t->M[l] + (size_t)b * B->nc;
+    double g[3] = {0, 0, 0};
+    for (int a = 0; a < B->nc; a++) {
+        const int* A = B->mi[a];
+        const double sg = ((A[0] + A[1] + A[2]) & 1) ? -M[a] : M[a];
+        g[0] += sg * (A[0] + 1) * T[B1->idx[A[0] + 1][A[1]][A[2]]];
+        g[1] += sg * (A[1] + 1) * T[B1->idx[A[0]][A[1] + 1][A[2]]];
+        g[2] += sg * (A[2] + 1) * T[B1->idx[A[0]][A[1]][A[2] + 1]];
+    }
+    for (int k = 0; k < 3; k++) acc[k] -= g[k] / (h * h);
+}
+
+static int64_t floor_half(int64_t v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
+
+/* The tree's field at the point X: at each level l = 0..L the boxes whose
+ * parent is near X's cell one level up (the root at level 0) and that are not
+ * near X's own cell floor((X - x0) / w_l) (not clamped: X may lie outside the
+ * root cube) by M2P; with near_p2p, the ions of the leaf boxes near X's leaf
+ * cell directly.  Adds sum q d / r^3 into acc; returns the M2P count. */
+static double m2p_point(const Tree* t, const Basis* B, const Basis* B1, const KI* ki, const double* x,
+                        const double* y, const double* z, const double* q, const double X[3], double eps2,
+                        int near_p2p, double* T, double acc[3]) {
+    if (t->nbox[0] == 0) return 0;
+    double nm2p = 0;
+    const int ring = t->ring;
+    for (int l = 0; l <= t->L; l++) {
+        const double w = t->side / ldexp(1.0, l);
+        int64_t cc[3];
+        for (int d = 0; d < 3; d++) cc[d] = (int64_t)floor((X[d] - t->x0[d]) / w);
+        if (l == 0) {
+            int near = llabs(cc[0]) <= ring && llabs(cc[1]) <= ring && llabs(cc[2]) <= ring;
+            if (!near) {
+                m2p_box(t, B, B1, 0, 0, X, eps2, T, acc);
+                return nm2p + 1;
+            }
+            if (t->L == 0 && near_p2p) {
+                for (int s = t->first[0][0]; s < t->first[0][0] + t->count[0][0]; s++) {
+                    int j = ki[s].i;
+                    double dx = X[0] - x[j], dy = X[1] - y[j], dz = X[2] - z[j];
+                    double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz + eps2);
+                    double wq = q[j] * inv * inv * inv;
+                    acc[0] += dx * wq; acc[1] += dy * wq; acc[2] += dz * wq;
+                }
+            }
+            continue;
+        }
+        const int64_t qc[3] = {floor_half(cc[0]), floor_half(cc[1]), floor_half(cc[2])};
+        const int64_t ptop = ((int64_t)1 << (l - 1)) - 1;
+        for (int64_t dz = -ring; dz <= ring; dz++)
+            for (int64_t dy = -ring; dy <= ring; dy++)
+                for (int64_t dx = -ring; dx <= ring; dx++) {
+                    int64_t pc[3] = {qc[0] + dx, qc[1] + dy, qc[2] + dz};
+                    if (pc[0] < 0 || pc[1] < 0 || pc[2] < 0 || pc[0] > ptop || pc[1] > ptop || pc[2] > ptop) continue;
+                    for (int oc = 0; oc < 8; oc++) {
+                        int64_t bc[3] = {2 * pc[0] + (oc & 1), 2 * pc[1] + ((oc >> 1) & 1), 2 * pc[2] + ((oc >> 2) & 1)};
+                        int near = llabs(bc[0] - cc[0]) <= ring && llabs(bc[1] - cc[1]) <= ring &&
+                                   llabs(bc[2] - cc[2]) <= ring;
+                        if (near && (l < t->L || !near_p2p)) continue;
+                        uint64_t key = spread21(bc[0]) | spread21(bc[1]) << 1 | spread21(bc[2]) << 2;
+                        int b = find_box(t, l, key);
+                        if (b < 0) continue;
+                        if (!near) {
+                            m2p_box(t, B, B1, l, b, X, eps2, T, acc);
+                            nm2p += 1;
+                        } else {
+                            for (int s = t->first[l][b]; s < t->first[l][b] + t->count[l][b]; s++) {
+                                int j = ki[s].i;
+                                double ex = X[0] - x[j], ey = X[1] - y[j], ez = X[2] - z[j];
+                                double inv = 1.0 / sqrt(ex * ex + ey * ey + ez * ez + eps2);
+                                double wq = q[j] * inv * inv * inv;
+                                acc[0] += ex * wq; acc[1] += ey * wq; acc[2] += ez * wq;
+                            }
+                        }
+                    }
+                }
+    }
+    return nm2p;
+}
+
 /* Optional M2L order per Chebyshev shell of the box offset (0 = use p). */
 static int g_pshell[8];
 void orc_fmm_set_shell_orders(const int* ps, int n) {
     for (int k = 0; k < 8; k++) g_pshell[k] = (ps && k < n) ? ps[k] : 0;
 }
 
-typedef struct { uint64_t k; int i; } KI;
+/* FmmType (structures.h:283-288): 0 treecode, 1 treecode with outliers,
+ * 2 fmm (default), 3 fmm with outliers. */
+static int g_fmm_type = 2;
+void orc_fmm_set_type(int type) { g_fmm_type = type; }
+
+/* DomainBoundsFunction::isOutlier (domain_bounds_functions.cpp:16-93) as a
+ * defined statistic: per axis the median of the valid ions (status 1 or 2; the
+ * mean of the two middle values for an even count, findTrueMedian,
+ * outlier_detection.h:118-133) and their median absolute deviation times
+ * outlierFactor (1.4826 as a float); flag[i] = 1 for a valid ion whose
+ * deviation is at least 3 such units on some axis.  Returns the count. */
+static int cmp_dbl(const void* a, const void* b) {
+    double x = *(const double*)a, y = *(const double*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+static double median_sorted(const double* v, int m) { return m > 0 ? 0.5 * (v[(m - 1) / 2] + v[m / 2]) : 0.0; }
+
+int orc_fmm_outliers(int n, const double* x, const double* y, const double* z, const int* status, int* flag) {
+    const double* P[3] = {x, y, z};
+    double* v = malloc(sizeof(double) * (n > 0 ? n : 1));
+    double med[3], mad[3];
+    for (int c = 0; c < 3; c++) {
+        int m = 0;
+        for (int i = 0; i < n; i++)
+            if (status[i] == 1 || status[i] == 2) v[m++] = P[c][i];
+        qsort(v, m, sizeof(double), cmp_dbl);
+        med[c] = median_sorted(v, m);
+        m = 0;
+        for (int i = 0; i < n; i++)
+            if (status[i] == 1 || status[i] == 2) v[m++] = fabs(P[c][i] - med[c]);
+        qsort(v, m, sizeof(double), cmp_dbl);
+        mad[c] = (double)1.482602218505602f * median_sorted(v, m);
+    }
+    int nout = 0;
+    for (int i = 0; i < n; i++) {
+        int out = 0;
+        if (status[i] == 1 || status[i] == 2)
+            for (int c = 0; c < 3; c++) out |= fabs(P[c][i] - med[c]) / mad[c] >= 3.0;
+        flag[i] = out;
+        nout += out;
+    }
+    free(v);
+    return nout;
+}
+
 static int cmp_ki(const void* a, const void* b) {
     const KI* x = a;
     const KI* y = b;
@@ -222,14 +381,34 @@ static int cmp_ki(const void* a, const void* b) {
  * Outputs out3 (double, zero for non-targets).  p: expansion order, L: leaf
  * level, ring: near-field width.  stats (may be 0) receives
  * {boxes at leaf level, P2P pairs, M2L interactions, seconds of the tree and
- * far-field passes (sort to L2L), seconds of L2P + P2P, leaf boxes evaluated}.
+ * far-field passes (sort to L2L), seconds of L2P + P2P, leaf boxes evaluated,
+ * M2P evaluations}.
+ * orc_fmm_set_type selects FmmType (structures.h:283-288): the treecode types
+ * skip M2L and L2L and give every target the M2P field of its far boxes at
+ * every level (m2p_point); the outlier types hold the orc_fmm_outliers ions
+ * out of the tree, give them the tree's field by M2P plus their near leaf
+ * cells, and give every target the outliers' field directly.
  */
 void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const double* q,
-                 const int* status, const int* species, int p, int L, int ring, double* out3,
+                 const int* status_in, const int* species, int p, int L, int ring, double* out3,
                  double* stats) {
     const double t_start = omp_get_wtime();
-    Basis B;
+    const int treecode = g_fmm_type == 0 || g_fmm_type == 1;
+    const int outliers = g_fmm_type == 1 || g_fmm_type == 3;
+    /* outlier modes: the tree holds the other ions (outliers get status 0 in it) */
+    int* oflag = NULL;
+    int* core_st = NULL;
+    const int* status = status_in;
+    if (outliers) {
+        oflag = malloc(sizeof(int) * (n > 0 ? n : 1));
+        core_st = malloc(sizeof(int) * (n > 0 ? n : 1));
+        orc_fmm_outliers(n, x, y, z, status_in, oflag);
+        for (int i = 0; i < n; i++) core_st[i] = oflag[i] ? 0 : status_in[i];
+        status = core_st;
+    }
+    Basis B, B1;
     basis_init(&B, p);
+    basis_init(&B1, p + 1);
     const int nc = B.nc;
     double root[4];
     orc_fmm_root(n, x, y, z, status, root);
@@ -362,7 +541,7 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
             TI[(size_t)bb * nc + a] = ti;
         }
     }
-    for (int l = 1; l <= L; l++) {
+    for (int l = 1; l <= L && !treecode; l++) {
 #pragma omp parallel num_threads(nt) reduction(+ : nm2l)
         {
         double* T = malloc(sizeof(double) * nc);
@@ -425,7 +604,7 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
     free(W);
     free(TI);
     /* L2L */
-    for (int l = 1; l < L; l++) {
+    for (int l = 1; l < L && !treecode; l++) {
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
         for (int b = 0; b < t.nbox[l]; b++) {
             double cp[3], hp;
@@ -457,7 +636,11 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
     for (int i = 0; i < 3 * n; i++) out3[i] = 0.0;
     const int stride = g_leaf_stride;
     const int nleaf_eval = (t.nbox[L] + stride - 1) / stride;
-#pragma omp parallel for schedule(dynamic, 8) num_threads(nt) reduction(+ : np2p)
+    double nm2p = 0;
+#pragma omp parallel num_threads(nt) reduction(+ : np2p, nm2p)
+    {
+    double* T1 = malloc(sizeof(double) * B1.nc);
+#pragma omp for schedule(dynamic, 8)
     for (int bs = 0; bs < nleaf_eval; bs++) {
         const int b = bs * stride;
         uint64_t key = t.key[L][b];
@@ -501,11 +684,56 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
                             np2p += 1;
                         }
                     }
+            if (treecode) {
+                /* far field by M2P at every level (no local expansions) */
+                double X[3] = {x[i], y[i], z[i]}, f[3] = {0, 0, 0};
+                nm2p += m2p_point(&t, &B, &B1, ki, x, y, z, q, X, eps2, 0, T1, f);
+                ax += f[0]; ay += f[1]; az += f[2];
+            }
             double kc = (double)((long double)(K * q[i]) / orc_species_mass_ld(species[i]));
             out3[3 * i] = ax * kc;
             out3[3 * i + 1] = ay * kc;
             out3[3 * i + 2] = az * kc;
         }
+    }
+    free(T1);
+    }
+    if (outliers) {
+        /* outlier targets: the tree's field by M2P and their near leaf cells
+         * (m2outlier_p, p2outlier_p); every target: the outliers' field directly
+         * (outlier_p2p, fmm_cpukernel.cpp:2230-2280) */
+        int nout = 0;
+        for (int i = 0; i < n; i++) nout += oflag[i];
+        int* ol = malloc(sizeof(int) * (nout > 0 ? nout : 1));
+        for (int i = 0, k = 0; i < n; i++)
+            if (oflag[i]) ol[k++] = i;
+#pragma omp parallel num_threads(nt) reduction(+ : nm2p)
+        {
+        double* T1 = malloc(sizeof(double) * B1.nc);
+#pragma omp for schedule(dynamic, 16)
+        for (int i = 0; i < n; i++) {
+            if (status_in[i] != 1) continue;
+            double f[3] = {0, 0, 0};
+            if (oflag[i]) {
+                double X[3] = {x[i], y[i], z[i]};
+                nm2p += m2p_point(&t, &B, &B1, ki, x, y, z, q, X, eps2, 1, T1, f);
+            }
+            for (int k = 0; k < nout; k++) {
+                int j = ol[k];
+                if (j == i) continue;
+                double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+                double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz + eps2);
+                double wq = q[j] * inv * inv * inv;
+                f[0] += dx * wq; f[1] += dy * wq; f[2] += dz * wq;
+            }
+            double kc = (double)((long double)(K * q[i]) / orc_species_mass_ld(species[i]));
+            out3[3 * i] += f[0] * kc;
+            out3[3 * i + 1] += f[1] * kc;
+            out3[3 * i + 2] += f[2] * kc;
+        }
+        free(T1);
+        }
+        free(ol);
     }
     const double t_end = omp_get_wtime();
     if (stats) {
@@ -515,6 +743,7 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
         stats[3] = t_far - t_start;
         stats[4] = t_end - t_far;
         stats[5] = nleaf_eval;
+        stats[6] = nm2p;
     }
     for (int l = 0; l <= L; l++) {
         free(t.key[l]); free(t.first[l]); free(t.count[l]); free(t.M[l]); free(t.Lc[l]);
@@ -522,6 +751,9 @@ void orc_fmm_f64(int n, const double* x, const double* y, const double* z, const
     free(t.nbox); free(t.key); free(t.first); free(t.count); free(t.M); free(t.Lc);
     free(ki);
     free(B.mi);
+    free(B1.mi);
+    free(oflag);
+    free(core_st);
 }
 
 /* Interaction counts of the FMM tree definition above, without expansions:

@@ -156,6 +156,14 @@ HipMultipoleMethodForceCalculation::HipMultipoleMethodForceCalculation(ParticleS
 
 HipMultipoleMethodForceCalculation::~HipMultipoleMethodForceCalculation() { shp_destroy(handle); }
 
+void HipMultipoleMethodForceCalculation::setFmmType(FmmType fType) {
+    const int t = fType == FmmType::treecode           ? SHP_FMM_TREECODE
+                  : fType == FmmType::treecodeOutliers ? SHP_FMM_TREECODE_OUTLIERS
+                  : fType == FmmType::fmmOutliers      ? SHP_FMM_FMM_OUTLIERS
+                                                       : SHP_FMM_FMM;
+    if (shp_fmm_set_type(handle, t) != SHP_OK) fail("shp_fmm_set_type");
+}
+
 void HipMultipoleMethodForceCalculation::updateAcceleration() {
     requireSystem(particleSystem);
     ChargedParticles& cp = particleSystem->chargedParticles;

@@ -111,6 +111,9 @@ public:
     void setNumExpansions(int p) { order = p; }
     void setMaxLevel(int maxLevel) { leafLevel = maxLevel; }
     void setNumSectors(int n) { nsec = n; }
+    // The reference's FmmType (its constructor's fType, structures.h:283-288):
+    // treecode or FMM, with or without outliers held out of the tree.
+    void setFmmType(FmmType fType);
 
 protected:
     void allocateThreadMemory() override {}

@@ -1845,6 +1845,221 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
     }
 }
 
+// ------------------------------------------------------------------- M2P ----
+// Treecode and outlier evaluation (FmmType treecode / fmmOutliers,
+// structures.h:283-288; m2p and m2outlier_p, fmm_cpukernel.cpp:1319-1533,
+// 2429-2776): the multipole expansions of the far boxes evaluated at a point.
+// For a point X the far boxes of level l are those whose parent is near X's
+// cell one level up (at level 0: the root) and that are not near X's own
+// cell floor((X - x0) / w_l), not clamped, so X may lie outside the root cube.
+// With u = (X - c)/h for box centre c and half size h:
+//   E_k = -(1/h^2) sum_{|a|<=p} (-1)^|a| M_a (a_k + 1) T_{a+e_k}(u)
+// (M_a raw) = -(1/h^2) sum Mhat_a (a+e_k)! T_{a+e_k}(u) (Mhat_a = (-1)^|a|
+// M_a / a!, as the M2L stores them), T from the Taylor recurrence up to
+// degree p + 1.
+constexpr int kMaxPointLevels = 17;  // levels 0..16
+constexpr int kM2PMaxP = 10;         // three degree slices per lane fit 64 KB of LDS
+constexpr int kM2PList = 1024;       // candidate boxes of one level: (2 ring + 1)^3 parents x 8
+struct PointTree {
+    int L, ring, p, scaled;
+    const double* root;
+    Level lv[kMaxPointLevels];
+    GridView gv[kMaxPointLevels];
+    const double4* sp;  // sorted ions (FP64 position, charge): near field of outlier points
+};
+
+__constant__ float c_fact[kMaxP + 2] = {1.0f,     1.0f,        2.0f,        6.0f,         24.0f,
+                                        120.0f,   720.0f,      5040.0f,     40320.0f,     362880.0f,
+                                        3628800.0f, 39916800.0f, 479001600.0f, 6227020800.0f};
+
+// Field of one box's expansion at u (units of h; sum q d / r^3 convention,
+// still to be divided by h^2), from the lane's three degree slices of T in
+// LDS: entry e of the lane at S[e * 64] (S = shared base + lane); slice k
+// holds degree n with n % 3 == k, position (b + c)(b + c + 1)/2 + c within
+// the degree.
+__device__ void m2p_box(const float* __restrict__ M, int p, int scaled, float ux, float uy, float uz, float eps2,
+                        float* S, int S1, float& gx, float& gy, float& gz) {
+    auto T = [&](int deg, int pos) -> float& { return S[((deg % 3) * S1 + pos) * 64]; };
+    const float s = fmaf(ux, ux, fmaf(uy, uy, fmaf(uz, uz, eps2)));
+    const float is = 1.0f / s;
+    T(0, 0) = __builtin_amdgcn_rsqf(s);
+    for (int n = 1; n <= p + 1; n++) {
+        const float c1 = (float)(2 * n - 1), c2 = (float)(n - 1), cn = -is / (float)n;
+        int pos = 0;
+        for (int t = 0; t <= n; t++) {
+            const int a = n - t;
+            for (int c = 0; c <= t; c++, pos++) {
+                const int b = t - c;
+                float acc = 0.0f;
+                if (a >= 1) acc = fmaf(ux, T(n - 1, t * (t + 1) / 2 + c), acc);
+                if (b >= 1) acc = fmaf(uy, T(n - 1, (t - 1) * t / 2 + c), acc);
+                if (c >= 1) acc = fmaf(uz, T(n - 1, (t - 1) * t / 2 + c - 1), acc);
+                float acc2 = 0.0f;
+                if (n >= 2) {
+                    if (a >= 2) acc2 += T(n - 2, t * (t + 1) / 2 + c);
+                    if (b >= 2) acc2 += T(n - 2, (t - 2) * (t - 1) / 2 + c);
+                    if (c >= 2) acc2 += T(n - 2, (t - 2) * (t - 1) / 2 + c - 2);
+                }
+                T(n, pos) = cn * fmaf(c1, acc, c2 * acc2);
+            }
+        }
+        // terms of the coefficients of degree n - 1 (their T_{a+e_k} have degree n)
+        const int m = n - 1;
+        const float sg = (m & 1) ? -1.0f : 1.0f;
+        int q = 0;
+        for (int t = 0; t <= m; t++) {
+            const int a0 = m - t;
+            for (int c = 0; c <= t; c++, q++) {
+                const int a1 = t - c, a2 = c;
+                const float Ma = M[dstart(m) + q];
+                const float w = scaled ? c_fact[a0] * c_fact[a1] * c_fact[a2] : sg;
+                const float wm = w * Ma;
+                gx = fmaf(wm * (float)(a0 + 1), T(n, t * (t + 1) / 2 + c), gx);
+                gy = fmaf(wm * (float)(a1 + 1), T(n, (t + 1) * (t + 2) / 2 + c), gy);
+                gz = fmaf(wm * (float)(a2 + 1), T(n, (t + 1) * (t + 2) / 2 + c + 1), gz);
+            }
+        }
+    }
+}
+
+// One wave per point (an ion, by its index in the caller's arrays): adds
+// K q / m times the far field of the tree at the point (M2P, levels 0..L)
+// and, with near_p2p, the near field of the leaf boxes within `ring` cells of
+// its leaf cell (outliers, which are not in the tree).  Points with status
+// other than 1 and waves past *npts return at once.
+__global__ __launch_bounds__(64) void k_fmm_m2p(const PointTree P, const int* __restrict__ pts,
+                                                const int* __restrict__ npts, int near_p2p,
+                                                const double* __restrict__ X, const double* __restrict__ Y,
+                                                const double* __restrict__ Z, const double* __restrict__ Q,
+                                                const int* __restrict__ status, const int* __restrict__ species,
+                                                double K, double* __restrict__ ax, double* __restrict__ ay,
+                                                double* __restrict__ az) {
+    extern __shared__ float sm[];
+    __shared__ int list[kM2PList];
+    __shared__ int nlist;
+    const int lane = threadIdx.x;
+    if ((int)blockIdx.x >= *npts) return;
+    const int i = pts[blockIdx.x];
+    if (status && status[i] != 1) return;  // uniform
+    const double x = X[i], y = Y[i], z = Z[i];
+    const int S1 = (P.p + 2) * (P.p + 3) / 2;
+    float* S = sm + lane;
+    const int ring = P.ring, nside = 2 * ring + 1, ncand = nside * nside * nside * 8;
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    for (int l = 0; l <= P.L; l++) {
+        const double w = P.root[3] / (double)(1 << l);
+        const long long cx = (long long)floor((x - P.root[0]) / w), cy = (long long)floor((y - P.root[1]) / w),
+                        cz = (long long)floor((z - P.root[2]) / w);
+        const long long top = (1LL << l) - 1;
+        if (lane == 0) nlist = 0;
+        __syncthreads();
+        if (l == 0) {
+            const bool near = llabs(cx) <= ring && llabs(cy) <= ring && llabs(cz) <= ring;
+            if (lane == 0 && (!near || l == P.L)) {
+                list[0] = near ? -1 : 0;  // -1: near root leaf (P2P), else far root (M2P)
+                nlist = 1;
+            }
+        } else {
+            const long long qx = cx >> 1, qy = cy >> 1, qz = cz >> 1;  // the point's cell one level up (floor)
+            for (int t0 = 0; t0 < ncand; t0 += 64) {
+                const int t = t0 + lane;
+                int entry = 0;  // 0: none; b + 1: far box b; -(b + 1): near leaf box b
+                if (t < ncand) {
+                    const int po = t >> 3, oc = t & 7;
+                    const long long px = qx + po % nside - ring, py = qy + (po / nside) % nside - ring,
+                                    pz = qz + po / (nside * nside) - ring;
+                    const long long ptop = (1LL << (l - 1)) - 1;
+                    if (px >= 0 && py >= 0 && pz >= 0 && px <= ptop && py <= ptop && pz <= ptop) {
+                        const long long bx = 2 * px + (oc & 1), by = 2 * py + ((oc >> 1) & 1),
+                                        bz = 2 * pz + ((oc >> 2) & 1);
+                        const bool near = llabs(bx - cx) <= ring && llabs(by - cy) <= ring && llabs(bz - cz) <= ring;
+                        if ((!near || (l == P.L && near_p2p)) && bx <= top && by <= top && bz <= top) {
+                            const int b = find_cell(P.lv[l], P.gv[l], (int)bx, (int)by, (int)bz);
+                            if (b >= 0) entry = near ? -(b + 1) : b + 1;
+                        }
+                    }
+                }
+                const unsigned long long mk = __ballot(entry != 0);
+                if (entry != 0) list[nlist + __popcll(mk & ((1ull << lane) - 1ull))] = entry;
+                __syncthreads();
+                if (lane == 0) nlist += __popcll(mk);
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        const int nl = nlist;
+        const double h = 0.5 * w;
+        const float eps2 = (float)(kSoftening / (h * h));
+        for (int k = lane; k < nl; k += 64) {
+            const int e = list[k];
+            const int b = e > 0 ? e - 1 : (e == 0 ? 0 : -e - 1);
+            if (l == 0 && e == -1) continue;  // root is the near leaf: handled below
+            if (e > 0 || (l == 0 && e == 0)) {
+                double c[3], hb;
+                box_center(P.root, l, P.lv[l].key[b], c, hb);
+                float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                m2p_box(P.lv[l].M + (size_t)b * P.lv[l].ld, P.p, P.scaled, (float)((x - c[0]) / hb),
+                        (float)((y - c[1]) / hb), (float)((z - c[2]) / hb), eps2, S, S1, gx, gy, gz);
+                const double ih2 = 1.0 / (hb * hb);
+                ex -= (double)gx * ih2;
+                ey -= (double)gy * ih2;
+                ez -= (double)gz * ih2;
+            } else {
+                for (int j = P.lv[l].start[b]; j < P.lv[l].end[b]; j++) {
+                    const double4 sj = P.sp[j];
+                    const double dx = x - sj.x, dy = y - sj.y, dz = z - sj.z;
+                    const double r2 = dx * dx + dy * dy + dz * dz + kSoftening;
+                    const double inv = rsqrt_f64(r2);
+                    const double wq = sj.w * inv * inv * inv;
+                    ex += dx * wq;
+                    ey += dy * wq;
+                    ez += dz * wq;
+                }
+            }
+        }
+        if (l == 0 && nl == 1 && list[0] == 0) break;  // far from the whole tree: the root's expansion is all
+        if (l == 0 && nl == 1 && list[0] == -1 && near_p2p) {
+            // single-level tree (L = 0) with the point near the root: direct sum over the root's ions
+            for (int j = P.lv[0].start[0] + lane; j < P.lv[0].end[0]; j += 64) {
+                const double4 sj = P.sp[j];
+                const double dx = x - sj.x, dy = y - sj.y, dz = z - sj.z;
+                const double inv = rsqrt_f64(dx * dx + dy * dy + dz * dz + kSoftening);
+                const double wq = sj.w * inv * inv * inv;
+                ex += dx * wq;
+                ey += dy * wq;
+                ez += dz * wq;
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    double(*red)[64] = reinterpret_cast<double(*)[64]>(sm);  // the T slices are free now
+    red[0][lane] = ex;
+    red[1][lane] = ey;
+    red[2][lane] = ez;
+    __syncthreads();
+    if (lane == 0) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int k = 0; k < 64; k++) {
+            sx += red[0][k];
+            sy += red[1][k];
+            sz += red[2][k];
+        }
+        const double kc = K * Q[i] / species_mass(species[i]);
+        ax[i] += sx * kc;
+        ay[i] += sy * kc;
+        az[i] += sz * kc;
+    }
+}
+
+// The list of the sorted target ions by their original index (treecode:
+// every target gets the M2P far field).
+__global__ void k_fmm_target_list(int na, const int* __restrict__ tfl, const int* __restrict__ tpre,
+                                  const int* __restrict__ perm, int* __restrict__ list) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < na && tfl[i]) list[tpre[i]] = perm[i];
+}
+
 __global__ void k_fmm_fill(double* __restrict__ p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -1878,6 +2093,101 @@ __global__ void k_fmm_images(int n, int nblk, const double4* __restrict__ img,
     sp2[t] = sp[i];
     const int s = st[i];
     st2[t] = k == 0 ? s : ((s == 1 || s == 2) ? 2 : 0);
+}
+
+// ------------------------------------------------------------- outliers ----
+// DomainBoundsFunction::isOutlier (domain_bounds_functions.cpp:16-93): per
+// axis, the median of the valid ions (status 1 or 2) and their median
+// absolute deviation times outlierFactor (1.4826, as a float); an ion is an
+// outlier when its deviation is at least maxDeviation (3) such units on some
+// axis.  The reference's selection drops values on ties and takes the
+// deviation median over the whole range, valid or not (outlier_detection.h:
+// 61-82, domain_bounds_functions.cpp:62); here both medians are those of the
+// valid ions, read from sorted keys, in FP64.
+constexpr double kOutlierFactor = (double)1.482602218505602f;
+constexpr double kMaxDeviation = 3.0;
+
+// Sort keys of one axis: the coordinate (med == nullptr) or its deviation
+// from the median; invalid ions sort last (+inf).  Counts the valid ions.
+__global__ void k_outl_keys(int n, const double* __restrict__ v, const int* __restrict__ st,
+                            const double* __restrict__ med, double* __restrict__ key, int* __restrict__ nvalid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = st[i];
+    const bool valid = s == 1 || s == 2;
+    key[i] = valid ? (med ? fabs(v[i] - *med) : v[i]) : INFINITY;
+    if (nvalid && valid) atomicAdd(nvalid, 1);
+}
+
+// out = scale x the median of the nv smallest sorted keys (the mean of the two
+// middle ones for even nv, findTrueMedian, outlier_detection.h:118-133).
+__global__ void k_outl_median(const double* __restrict__ sorted, const int* __restrict__ nvalid, double scale,
+                              double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int m = *nvalid;
+    *out = m > 0 ? scale * (0.5 * (sorted[(m - 1) / 2] + sorted[m / 2])) : 0.0;
+}
+
+// stat = {median x, y, z, MAD x, y, z} (MAD already times outlierFactor).
+__global__ void k_outl_flags(int n, const double* __restrict__ X, const double* __restrict__ Y,
+                             const double* __restrict__ Z, const int* __restrict__ st,
+                             const double* __restrict__ stat, int* __restrict__ flag, int* __restrict__ pflag,
+                             int* __restrict__ core) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = st[i];
+    const bool valid = s == 1 || s == 2;
+    const double v[3] = {X[i], Y[i], Z[i]};
+    bool out = false;
+    for (int c = 0; c < 3; c++) out |= fabs(v[c] - stat[c]) / stat[3 + c] >= kMaxDeviation;
+    out = out && valid;
+    flag[i] = out;
+    pflag[i] = out && s == 1;  // outliers that are targets
+    core[i] = out ? 0 : s;
+}
+
+// Targets (status 1) += K q/m x the direct FP64 sum over the outlier sources
+// (outlier_p2p, fmm_cpukernel.cpp:2230-2280: every target against the
+// outliers), self excluded; the outliers staged in LDS 256 at a time.
+__global__ __launch_bounds__(256) void k_outl_sources(int n, const double* __restrict__ X,
+                                                      const double* __restrict__ Y, const double* __restrict__ Z,
+                                                      const double* __restrict__ Q, const int* __restrict__ st,
+                                                      const int* __restrict__ species, const int* __restrict__ olist,
+                                                      int nout, double K, double* __restrict__ ax,
+                                                      double* __restrict__ ay, double* __restrict__ az) {
+    __shared__ double4 src[256];
+    __shared__ int sid[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool tgt = i < n && st[i] == 1;
+    const double x = tgt ? X[i] : 0.0, y = tgt ? Y[i] : 0.0, z = tgt ? Z[i] : 0.0;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int j0 = 0; j0 < nout; j0 += 256) {
+        __syncthreads();
+        if (j0 + (int)threadIdx.x < nout) {
+            const int j = olist[j0 + threadIdx.x];
+            src[threadIdx.x] = make_double4(X[j], Y[j], Z[j], Q[j]);
+            sid[threadIdx.x] = j;
+        }
+        __syncthreads();
+        const int cnt = min(256, nout - j0);
+        if (tgt)
+            for (int k = 0; k < cnt; k++) {
+                if (sid[k] == i) continue;
+                const double4 s = src[k];
+                const double dx = x - s.x, dy = y - s.y, dz = z - s.z;
+                const double inv = rsqrt_f64(dx * dx + dy * dy + dz * dz + kSoftening);
+                const double w = s.w * inv * inv * inv;
+                sx += dx * w;
+                sy += dy * w;
+                sz += dz * w;
+            }
+    }
+    if (tgt && nout > 0) {
+        const double kc = K * Q[i] / species_mass(species[i]);
+        ax[i] += sx * kc;
+        ay[i] += sy * kc;
+        az[i] += sz * kc;
+    }
 }
 
 // Folds the measurements of the last FMM call into the handle: its events
@@ -1924,8 +2234,17 @@ using namespace shp;
 
 extern "C" {
 
+// Points outside the tree that get the tree's field through M2P plus the near
+// field of their leaf cell (the outliers of the *Outliers modes): original
+// indices, their number in device memory, and an upper bound of it.
+struct FmmPoints {
+    const int* idx;
+    const int* count;
+    int max;
+};
+
 static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_level, int leaf_size, int ring,
-                    bool full, double* ax, double* ay, double* az) {
+                    bool full, double* ax, double* ay, double* az, const FmmPoints* pts = nullptr) {
     SHP_REQUIRE(h && pt && ax && ay && az, "shp_fmm_dev: null argument");
     SHP_REQUIRE(order >= 1 && order <= kMaxP, "shp_fmm_dev: order %d outside [1, %d]", order, kMaxP);
     SHP_REQUIRE(ring >= 1 && ring <= 2, "shp_fmm_dev: ring %d outside [1, 2]", ring);
@@ -1937,6 +2256,13 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
     if (n <= 0) return SHP_OK;
     SHP_REQUIRE(pt->x && pt->y && pt->z && pt->q && pt->status && pt->species,
                 "shp_fmm_dev: null particle array");
+    // treecode: the far field of every target by M2P, no local expansions
+    const bool treecode = h->fmm_type == SHP_FMM_TREECODE || h->fmm_type == SHP_FMM_TREECODE_OUTLIERS;
+    if ((treecode || pts) && order > kM2PMaxP) {
+        set_error("shp_fmm_dev: treecode and outlier modes evaluate multipoles up to order %d (order %d)",
+                  kM2PMaxP, order);
+        return SHP_ERR_UNSUPPORTED;
+    }
     SHP_HIP_CHECK(hipSetDevice(h->device));
     FmmDevTables* tb;
     int rc = get_tables(h, order, &tb);
@@ -2258,14 +2584,19 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
     const int W = 2 * ring + 1;
     const int nside = 2 * W + 1, noff = nside * nside * nside;
     SHP_WS(ttab, float, h, WS_FMM_T, (size_t)noff * (L + 1) * nc);
-    hipLaunchKernelGGL(k_fmm_ttable, dim3(noff * (L + 1)), dim3(64), 0, st, order, nc,
-                       tb->mi, W, 0, L, root, kSoftening, ttab);
+    if (!treecode)
+        hipLaunchKernelGGL(k_fmm_ttable, dim3(noff * (L + 1)), dim3(64), 0, st, order, nc,
+                           tb->mi, W, 0, L, root, kSoftening, ttab);
     // Unrolled M2L (ring 2, orders with compiled pair lists) needs a dense box
     // grid per level over the occupied cells; very large grids use the
     // table-driven kernel instead.
     const GridView leafgrid = gview(L);  // leaf-level cell grid for the near-field lookups
     SHP_HIP_CHECK(hipEventRecord(h->fev[2], st));
-    if (unrolled) {
+    if (treecode) {
+        // no M2L: the leaf kernel's L2P adds zero, the M2P pass below the far field
+        SHP_HIP_CHECK(hipEventRecord(h->fev[6], st));
+        SHP_HIP_CHECK(hipEventRecord(h->fev[7], st));
+    } else if (unrolled) {
         const dim3 rowb(64 * fdiv_up(nc, 64));  // one lane per coefficient (nc <= 455)
         hipLaunchKernelGGL(k_fmm_scale, dim3((unsigned)tot), rowb, 0, st, bexp, tb->mscale, nc, ld);
         hipLaunchKernelGGL(k_fmm_scale, dim3((unsigned)(noff * (L + 1))), rowb, 0, st, ttab, tb->dscale, nc, nc);
@@ -2360,7 +2691,7 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
     }
     SHP_HIP_CHECK(hipGetLastError());
     SHP_HIP_CHECK(hipEventRecord(h->fev[3], st));
-    for (int l = 1; l < L; l++)
+    for (int l = 1; l < L && !treecode; l++)
         hipLaunchKernelGGL(k_fmm_l2l, dim3(nbox[l]), dim3(kShiftLanes), 9 * nc * sizeof(float), st,
                            lv[l], lv[l + 1], nc, tb->mi, tb->l2l_term, tb->l2l_nt);
     SHP_HIP_CHECK(hipEventRecord(h->fev[4], st));
@@ -2369,6 +2700,30 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
                        h->fmm_stats,
                        ax, ay, az);
     SHP_HIP_CHECK(hipGetLastError());
+    if (treecode || pts) {
+        PointTree P{};
+        P.L = L;
+        P.ring = ring;
+        P.p = order;
+        P.scaled = unrolled && !treecode;  // the unrolled M2L scales the multipoles in place
+        P.root = root;
+        P.sp = sp;
+        for (int l = 0; l <= L; l++) {
+            P.lv[l] = lv[l];
+            P.gv[l] = gview(l);
+        }
+        const size_t shm = sizeof(float) * 3 * (size_t)((order + 2) * (order + 3) / 2) * 64;
+        if (treecode) {
+            SHP_WS(tl, int, h, WS_FMM_PTS, (size_t)na);
+            hipLaunchKernelGGL(k_fmm_target_list, dim3(fdiv_up(na, 256)), dim3(256), 0, st, na, tfl, tpre, perm, tl);
+            hipLaunchKernelGGL(k_fmm_m2p, dim3(na), dim3(64), shm, st, P, tl, tpre + na, 0, pt->x, pt->y, pt->z,
+                               pt->q, (const int*)nullptr, pt->species, k_const_host(), ax, ay, az);
+        }
+        if (pts && pts->max > 0)
+            hipLaunchKernelGGL(k_fmm_m2p, dim3(pts->max), dim3(64), shm, st, P, pts->idx, pts->count, 1, pt->x,
+                               pt->y, pt->z, pt->q, (const int*)nullptr, pt->species, k_const_host(), ax, ay, az);
+        SHP_HIP_CHECK(hipGetLastError());
+    }
     SHP_HIP_CHECK(hipEventRecord(h->fev[5], st));
     SHP_HIP_CHECK(hipEventRecord(h->ev1, st));
     h->timed = true;
@@ -2387,9 +2742,112 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
     return SHP_OK;
 }
 
+// Outlier flags of the valid ions (status 1 or 2), the core status (outliers
+// set to 0), the outlier list and the list of outlier targets; the number of
+// outliers on the host.
+struct OutlierSplit {
+    int* flag;
+    int* core;
+    int* olist;   // every outlier (sources), count nout
+    int* plist;   // outliers with status 1 (targets), count *pcount
+    int* pcount;  // device
+    int nout;
+};
+
+static int fmm_outliers(shp_handle h, const shp_particles* pt, OutlierSplit* o) {
+    const int n = pt->n;
+    hipStream_t st = h->stream;
+    // doubles: keys in/out (2n), statistics (6); ints: flag, point flag, core,
+    // the two lists (5n), counters (4)
+    SHP_WS(buf, char, h, WS_FMM_OUTL, sizeof(double) * (2 * (size_t)n + 8) + sizeof(int) * (5 * (size_t)n + 8));
+    double* kin = (double*)buf;
+    double* kout = kin + n;
+    double* stat = kout + n;
+    int* flag = (int*)(stat + 8);
+    int* pflag = flag + n;
+    int* core = pflag + n;
+    int* olist = core + n;
+    int* plist = olist + n;
+    int* cnt = plist + n;  // nvalid, nout, npoints
+    hipLaunchKernelGGL(k_fmm_ifill, dim3(1), dim3(64), 0, st, cnt, 4, 0);
+    size_t tsz = 0;
+    SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tsz, kin, kout, n, 0, 64, st));
+    const double* V[3] = {pt->x, pt->y, pt->z};
+    for (int c = 0; c < 3; c++) {
+        SHP_WS(cub, char, h, WS_CUB, tsz);
+        hipLaunchKernelGGL(k_outl_keys, dim3(fdiv_up(n, 256)), dim3(256), 0, st, n, V[c], pt->status,
+                           (const double*)nullptr, kin, c == 0 ? cnt : (int*)nullptr);
+        SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(cub, tsz, kin, kout, n, 0, 64, st));
+        hipLaunchKernelGGL(k_outl_median, dim3(1), dim3(64), 0, st, kout, cnt, 1.0, stat + c);
+        hipLaunchKernelGGL(k_outl_keys, dim3(fdiv_up(n, 256)), dim3(256), 0, st, n, V[c], pt->status,
+                           (const double*)(stat + c), kin, (int*)nullptr);
+        SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(cub, tsz, kin, kout, n, 0, 64, st));
+        hipLaunchKernelGGL(k_outl_median, dim3(1), dim3(64), 0, st, kout, cnt, kOutlierFactor, stat + 3 + c);
+    }
+    hipLaunchKernelGGL(k_outl_flags, dim3(fdiv_up(n, 256)), dim3(256), 0, st, n, pt->x, pt->y, pt->z, pt->status,
+                       stat, flag, pflag, core);
+    hipcub::CountingInputIterator<int> it(0);
+    size_t t2 = 0;
+    SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, t2, it, flag, olist, cnt + 1, n, st));
+    SHP_WS(cub2, char, h, WS_CUB, t2);
+    SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(cub2, t2, it, flag, olist, cnt + 1, n, st));
+    SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(cub2, t2, it, pflag, plist, cnt + 2, n, st));
+    SHP_HIP_CHECK(hipGetLastError());
+    int nout = 0;
+    SHP_HIP_CHECK(hipMemcpyAsync(&nout, cnt + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    SHP_HIP_CHECK(hipStreamSynchronize(st));
+    *o = OutlierSplit{flag, core, olist, plist, cnt + 2, nout};
+    return SHP_OK;
+}
+
 int shp_fmm_dev(shp_handle h, const shp_particles* pt, int order, int leaf_level, int leaf_size,
                 int ring, double* ax, double* ay, double* az) {
-    return fmm_eval(h, pt, order, leaf_level, leaf_size, ring, false, ax, ay, az);
+    SHP_REQUIRE(h && pt && ax && ay && az, "shp_fmm_dev: null argument");
+    const bool outliers = h->fmm_type == SHP_FMM_TREECODE_OUTLIERS || h->fmm_type == SHP_FMM_FMM_OUTLIERS;
+    if (!outliers || pt->n <= 0) return fmm_eval(h, pt, order, leaf_level, leaf_size, ring, false, ax, ay, az);
+    SHP_REQUIRE(pt->x && pt->y && pt->z && pt->q && pt->status && pt->species, "shp_fmm_dev: null particle array");
+    if (order > kM2PMaxP) {
+        set_error("shp_fmm_dev: outlier modes evaluate multipoles up to order %d (order %d)", kM2PMaxP, order);
+        return SHP_ERR_UNSUPPORTED;
+    }
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    OutlierSplit o;
+    int rc = fmm_outliers(h, pt, &o);
+    if (rc) return rc;
+    // the tree holds the other ions; the outliers get its field through M2P
+    // and their near cells, and every target gets the outliers' field directly
+    shp_particles core = *pt;
+    core.status = o.core;
+    const FmmPoints P{o.plist, o.pcount, o.nout};
+    rc = fmm_eval(h, &core, order, leaf_level, leaf_size, ring, false, ax, ay, az, &P);
+    if (rc) return rc;
+    if (o.nout > 0)
+        hipLaunchKernelGGL(k_outl_sources, dim3(fdiv_up(pt->n, 256)), dim3(256), 0, h->stream, pt->n, pt->x, pt->y,
+                           pt->z, pt->q, pt->status, pt->species, o.olist, o.nout, k_const_host(), ax, ay, az);
+    SHP_HIP_CHECK(hipGetLastError());
+    return SHP_OK;
+}
+
+int shp_fmm_set_type(shp_handle h, int fmm_type) {
+    SHP_REQUIRE(h, "shp_fmm_set_type: null handle");
+    SHP_REQUIRE(fmm_type >= SHP_FMM_TREECODE && fmm_type <= SHP_FMM_FMM_OUTLIERS, "shp_fmm_set_type: type %d",
+                fmm_type);
+    h->fmm_type = fmm_type;
+    return SHP_OK;
+}
+
+int shp_fmm_outliers_dev(shp_handle h, const shp_particles* pt, int* flags, int* nout) {
+    SHP_REQUIRE(h && pt && flags, "shp_fmm_outliers_dev: null argument");
+    if (nout) *nout = 0;
+    if (pt->n <= 0) return SHP_OK;
+    SHP_REQUIRE(pt->x && pt->y && pt->z && pt->status, "shp_fmm_outliers_dev: null particle array");
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    OutlierSplit o;
+    const int rc = fmm_outliers(h, pt, &o);
+    if (rc) return rc;
+    SHP_HIP_CHECK(hipMemcpyAsync(flags, o.flag, sizeof(int) * pt->n, hipMemcpyDeviceToDevice, h->stream));
+    if (nout) *nout = o.nout;
+    return SHP_OK;
 }
 
 int shp_fmm_shell_orders(int order, int ring, int sym_type, int* out3) {
@@ -2436,6 +2894,10 @@ int shp_fmm_sym_dev(shp_handle h, const shp_particles* pt, const shp_symmetry* s
     SHP_REQUIRE(h && pt && ax && ay && az, "shp_fmm_sym_dev: null argument");
     const int symt = sym ? sym->type : SHP_SYM_NONE;
     if (symt == SHP_SYM_NONE) return shp_fmm_dev(h, pt, order, leaf_level, leaf_size, ring, ax, ay, az);
+    if (h->fmm_type == SHP_FMM_TREECODE_OUTLIERS || h->fmm_type == SHP_FMM_FMM_OUTLIERS) {
+        set_error("shp_fmm_sym_dev: the outlier modes have no symmetry images here");
+        return SHP_ERR_UNSUPPORTED;
+    }
     const int n = pt->n;
     if (n <= 0) return SHP_OK;
     SHP_REQUIRE(pt->x && pt->y && pt->z && pt->q && pt->status && pt->species,

@@ -102,6 +102,8 @@ enum WsSlot {
     WS_FMM_USORT,   // FMM leaf work units: sort keys, values and the longest-first order
     WS_FMM_SPF,     // FMM ions relative to their own leaf centre (FP32)
     WS_FMM_TPRE,    // FMM target flags of the sorted ions and their prefix sums
+    WS_FMM_PTS,     // FMM point targets of the M2P pass (treecode: every target)
+    WS_FMM_OUTL,    // FMM outlier detection: sort keys, flags, the outlier list
     WS_CMP,         // accuracy metric: per-block partials
     WS_NSLOTS
 };
@@ -158,6 +160,7 @@ struct shp_context {
     // leaf-level choice of the last FMM call: rows (level, P2P pairs, far-band
     // box pairs, estimated ms), fmm_est_n of them
     int fmm_est_n = 0;
+    int fmm_type = 2;  // FmmType of shp_fmm_set_type (SHP_FMM_*)
     double fmm_est[4 * 24] = {};
     bool timed = false, timed_update = false;
     // Barnes-Hut root-layout options (shp_bh_set_options; reference defaults)

@@ -221,6 +221,8 @@ def _load():
         "shp_state_run_substeps": (c_int, [c_void_p, ctypes.POINTER(StepOpts), ctypes.POINTER(StepOpts), c_void_p,
                                            c_int, c_int, c_double, c_double, c_int]),
         "shp_state_download_host": (c_int, [c_void_p, fp, fp, fp]),
+        "shp_fmm_set_type": (c_int, [c_void_p, c_int]),
+        "shp_fmm_outliers_dev": (c_int, [c_void_p, ctypes.POINTER(Particles), c_void_p, ctypes.POINTER(c_int)]),
         "shp_state_save": (c_int, [c_void_p, c_int, c_int, c_int, ctypes.c_float, c_void_p]),
         "shp_state_save_full": (c_int, [c_void_p, c_int, c_int, ctypes.c_float, c_void_p]),
         "shp_state_flush": (c_int, [c_void_p]),
@@ -273,6 +275,11 @@ def fmm_shell_orders(order, ring=2, rotational=False):
            "shp_fmm_shell_orders")
     return tuple(out)
 
+
+SHP_FMM_TREECODE = 0
+SHP_FMM_TREECODE_OUTLIERS = 1
+SHP_FMM_FMM = 2
+SHP_FMM_FMM_OUTLIERS = 3
 
 SHP_SAVE_ORIGINAL = 0
 SHP_SAVE_NEW_ORIGINAL = 1
@@ -805,6 +812,25 @@ def _engine_fmm_dev(self, state, order=8, leaf_level=0, leaf_size=32, ring=2, ou
     return out
 
 
+def _engine_fmm_set_type(self, fmm_type):
+    """FmmType of later FMM evaluations (SHP_FMM_TREECODE, _TREECODE_OUTLIERS,
+    _FMM (default), _FMM_OUTLIERS)."""
+    _check(lib.shp_fmm_set_type(self.h, int(fmm_type)), "shp_fmm_set_type")
+
+
+def _engine_fmm_outliers(self, state):
+    """Outlier flags (int32 numpy array) of the outlier modes, and their number."""
+    import torch
+    flags = torch.zeros(state.n, dtype=torch.int32, device=state.x.device)
+    p = state.as_struct()
+    _torch_ready(state.x.device)
+    nout = ctypes.c_int(0)
+    _check(lib.shp_fmm_outliers_dev(self.h, ctypes.byref(p), flags.data_ptr(), ctypes.byref(nout)),
+           "shp_fmm_outliers_dev")
+    self.sync()
+    return flags.cpu().numpy(), nout.value
+
+
 def _engine_fmm_leaf_level(self):
     return lib.shp_fmm_last_leaf_level(self.h)
 
@@ -858,6 +884,8 @@ Engine.fmm_dev = _engine_fmm_dev
 Engine.fmm_level_estimates = _engine_fmm_level_estimates
 Engine.compare_accel = _engine_compare_accel
 Engine.fmm_leaf_level = _engine_fmm_leaf_level
+Engine.fmm_set_type = _engine_fmm_set_type
+Engine.fmm_outliers = _engine_fmm_outliers
 Engine.leapfrog_substeps = _engine_leapfrog_substeps
 Engine.fmm_stats = _engine_fmm_stats
 Engine.fmm_timing = _engine_fmm_timing
