@@ -277,7 +277,15 @@ def run_config(k, args):
         rows = [{"method": "fp64_direct", "ms": ms64}]
         for th in (0.3, 0.5, 0.7):
             ms, a = _timed(lambda: eng.bh_dev(st, theta=th))
-            rows.append({"method": "bh", "theta": th, "ms": ms, **_metric(eng, st, a, a64, sub)})
+            walk = eng.last_kernel_ms()  # k_bh_walk alone
+            bs = eng.bh_stats()
+            inter = bs["accepted_cells"] + bs["ion_terms"]
+            visits = inter + bs["opened_cells"]
+            rows.append({"method": "bh", "theta": th, "ms": ms, "walk_ms": walk,
+                         "accepted_cells": bs["accepted_cells"], "ion_terms": bs["ion_terms"],
+                         "opened_cells": bs["opened_cells"], "max_depth": bs["max_depth"],
+                         "interactions_per_s": inter / (walk * 1e-3), "node_visits_per_s": visits / (walk * 1e-3),
+                         **_metric(eng, st, a, a64, sub)})
             log(json.dumps(rows[-1]))
         for p in (4, 6, 8, 10):
             ms, a = _timed(lambda: eng.fmm_dev(st, order=p, leaf_size=64, ring=2))

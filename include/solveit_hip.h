@@ -267,9 +267,10 @@ int shp_bh_set_options(shp_handle h, const shp_bh_options* o);
  * root cell, [3] bucket leaves, [4] accepted cells, [5] ion terms (both
  * summed over targets and images), [6] sources placed in the last root cell
  * because no cell matched, [7..9] cells per axis, [10..12] location of the
- * first root cell, [13..15] root cell dimensions.  Has no counterpart in the
- * reference (its timing record is BarnesHutForceCalculation's timeInfoFile). */
-#define SHP_BH_NSTATS 16
+ * first root cell, [13..15] root cell dimensions, [16] opened cells (summed
+ * over targets and images).  Has no counterpart in the reference (its timing
+ * record is BarnesHutForceCalculation's timeInfoFile). */
+#define SHP_BH_NSTATS 17
 int shp_bh_stats(shp_handle h, double* out);
 
 /* ---- energies (FP64) ---- */

@@ -587,7 +587,7 @@ lib.orc_bh_layout.restype = _ci
 lib.orc_bh_layout.argtypes = [_vp, _vp, _ci, _ci, _ci, _vp, _vp, _vp, _vp]
 
 BH_INFO_KEYS = ("cells", "root_cells", "max_depth", "buckets", "accepted_cells", "ion_terms",
-                "unmatched_root", "nonempty_cells")
+                "unmatched_root", "nonempty_cells", "opened_cells")
 
 
 def bh_ref(pos4, vel4, acc4, i0=0, i1=None, theta=0.5, sym="none", lattice=None,

@@ -385,7 +385,8 @@ static void bh_accum(float f, float rx, float ry, float rz, int cb, int sb, floa
 
 /* One walk for target position p (already transformed) starting at the first
  * root cell; (cb, sb) rotate the contributions back (1, 0 for none); the ion
- * `self` is skipped; counts[0] += accepted cells, counts[1] += ion terms. */
+ * `self` is skipped; counts[0] += accepted cells, counts[1] += ion terms,
+ * counts[2] += opened cells. */
 static void bh_walk(const bh_tree* t, const float* pos4, const float* p, int self, float theta,
                     int cb, int sb, float* acc, long long* counts) {
     int ci = 0;
@@ -420,6 +421,7 @@ static void bh_walk(const bh_tree* t, const float* pos4, const float* p, int sel
                 counts[0]++;
                 ci = c->next;
             } else {
+                counts[2]++;
                 ci = c->first_child;
             }
         }
@@ -488,13 +490,13 @@ static int bh_root_of(const bh_layout* L, const float* p) {
  * [0] cells of the tree including empty ones, [1] root cells, [2] deepest
  * level, [3] bucket leaves, [4] accepted cells, [5] ion terms, [6] ions
  * placed in the last root cell because no cell matched, [7] cells that hold
- * at least one ion.
+ * at least one ion, [8] opened cells (the walk descended into them).
  */
 int orc_bh_ref2(const float* pos4, const float* vel4, const float* acc4, int n, int i0, int i1,
                 float theta, int sym, int nimg, const int* lattice, const double* domain, int toggle,
                 const int* forced3, double* out3, long long* info) {
     (void)n;
-    long long inf[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long inf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = i0; i < i1; i++) out3[3 * (size_t)(i - i0)] = out3[3 * (size_t)(i - i0) + 1] =
                                         out3[3 * (size_t)(i - i0) + 2] = 0.0;
     if (i1 <= i0) {
@@ -562,7 +564,11 @@ int orc_bh_ref2(const float* pos4, const float* vel4, const float* acc4, int n, 
                 if (l[0] != 0 || l[1] != 0 || l[2] != 0) self = -1;
             }
             float acc[3] = {0.0f, 0.0f, 0.0f};
-            bh_walk(&t, pos4, tp, self, theta, cb, sb, acc, inf + 4);
+            long long wc[3] = {0, 0, 0};
+            bh_walk(&t, pos4, tp, self, theta, cb, sb, acc, wc);
+            inf[4] += wc[0];
+            inf[5] += wc[1];
+            inf[8] += wc[2];
             for (int c = 0; c < 3; c++) tot[c] += (float)((double)acc[c] * kc);
         }
         for (int c = 0; c < 3; c++) o[c] = (double)tot[c];

@@ -580,9 +580,48 @@ __device__ __forceinline__ int bh_qrf(int n) {
     return sg * oe2;
 }
 
-// One thread per source in key order; targets (status 1) walk the tree.
-// ctr[0] += accepted cells, ctr[1] += ion terms (one wave-reduced atomic
-// each per wave).
+// The walk of one target image: its (transformed) position, its sums, the
+// next node, and whether its own ion is skipped.
+struct BhPath {
+    float px, py, pz, ax, ay, az;
+    int node, s;
+    bool self;
+};
+
+// One step of a walk at the node whose record is (q, x): a bucket of
+// coincident ions (rare), else one distance for a single-ion leaf (its centre
+// of charge is the ion) or a cell, the reference's operations per ion in
+// either case.  Returns the next node.
+template <int SYM>
+__device__ __forceinline__ int bh_step(const float4* __restrict__ SP, float theta, int cb, int sb, const float4 q,
+                                       const int4 x, BhPath& w, unsigned& ncell, unsigned& nion, unsigned& nopen) {
+    const bool leaf = x.x < 0;
+    if (leaf && x.z != 1) {
+        for (int j = x.w; j < x.w + x.z; j++) {
+            if (w.self && j == w.s) continue;
+            const float4 qj = SP[j];
+            float rx, ry, rz;
+            const float d = bh_dist(w.px, w.py, w.pz, qj.x, qj.y, qj.z, rx, ry, rz);
+            bh_add<SYM>(qj.w / (d * d * d), rx, ry, rz, cb, sb, w.ax, w.ay, w.az);
+            nion++;
+        }
+        return x.y;
+    }
+    float rx, ry, rz;
+    const float d = bh_dist(w.px, w.py, w.pz, q.x, q.y, q.z, rx, ry, rz);
+    const bool take = leaf ? !(w.self && x.w == w.s) : theta * (d - __int_as_float(x.z)) > __int_as_float(x.w);
+    if (take) {
+        bh_add<SYM>(q.w / (d * d * d), rx, ry, rz, cb, sb, w.ax, w.ay, w.az);
+        if (leaf) nion++;
+        else ncell++;
+    }
+    if (!leaf && !take) nopen++;
+    return (leaf || take) ? x.y : x.x;
+}
+
+// One thread per source in key order; the targets (status 1) walk the tree.
+// ctr[0] += accepted cells, ctr[1] += ion terms, ctr[3] += opened cells (one
+// wave-reduced atomic each per wave).
 template <int SYM>
 __global__ __launch_bounds__(kBhThreads) void k_bh_walk(
     BhWalk t, const float4* __restrict__ SP, const int* __restrict__ SI, int ns,
@@ -592,84 +631,48 @@ __global__ __launch_bounds__(kBhThreads) void k_bh_walk(
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = s < ns ? SI[s] : 0;
     const bool active = s < ns && st[i] == 1;
-    unsigned ncell = 0, nion = 0;
+    unsigned ncell = 0, nion = 0, nopen = 0;
     if (active) {
         const float4 p = SP[s];
         const double kc = K * (double)p.w / species_mass(sp[i]);
         const int nrep = SYM == SHP_SYM_ROTATIONAL ? 4 : (SYM == SHP_SYM_TRANSLATIONAL ? nimg : 1);
         float tx = 0.0f, ty = 0.0f, tz = 0.0f;
         for (int k = 0; k < nrep; k++) {
-            float px = p.x, py = p.y, pz = p.z;
+            BhPath w{p.x, p.y, p.z, 0.0f, 0.0f, 0.0f, 0, s, true};
             int cb = 1, sb = 0;
-            bool self = true;
             if (SYM == SHP_SYM_ROTATIONAL) {
                 const int cf = bh_qrf(5 - k), sf = bh_qrf(4 - k);
                 cb = bh_qrf(k + 1);
                 sb = bh_qrf(k);
-                px = (float)cf * p.x - (float)sf * p.y;
-                py = (float)sf * p.x + (float)cf * p.y;
-                self = k == 0;
+                w.px = (float)cf * p.x - (float)sf * p.y;
+                w.py = (float)sf * p.x + (float)cf * p.y;
+                w.self = k == 0;
             } else if (SYM == SHP_SYM_TRANSLATIONAL) {
                 const int lx = lat[3 * k], ly = lat[3 * k + 1], lz = lat[3 * k + 2];
-                px = p.x - (float)lx * dx0;
-                py = p.y - (float)ly * dy0;
-                pz = p.z - (float)lz * dz0;
-                self = lx == 0 && ly == 0 && lz == 0;
+                w.px = p.x - (float)lx * dx0;
+                w.py = p.y - (float)ly * dy0;
+                w.pz = p.z - (float)lz * dz0;
+                w.self = lx == 0 && ly == 0 && lz == 0;
             }
-            float ax = 0.0f, ay = 0.0f, az = 0.0f;
-            int node = 0;
-            while (node >= 0) {
-                const float4 q = t.cq[node];
-                const int4 x = t.aux[node];
-                const int f = x.x;
-                if (f < 0) {
-                    const int c = x.z, j0 = x.w;
-                    if (c == 1) {
-                        if (!(self && j0 == s)) {
-                            float rx, ry, rz;
-                            const float d = bh_dist(px, py, pz, q.x, q.y, q.z, rx, ry, rz);
-                            bh_add<SYM>(q.w / (d * d * d), rx, ry, rz, cb, sb, ax, ay, az);
-                            nion++;
-                        }
-                    } else {
-                        for (int j = j0; j < j0 + c; j++) {
-                            if (self && j == s) continue;
-                            const float4 q = SP[j];
-                            float rx, ry, rz;
-                            const float d = bh_dist(px, py, pz, q.x, q.y, q.z, rx, ry, rz);
-                            bh_add<SYM>(q.w / (d * d * d), rx, ry, rz, cb, sb, ax, ay, az);
-                            nion++;
-                        }
-                    }
-                    node = x.y;
-                } else {
-                    float rx, ry, rz;
-                    const float d = bh_dist(px, py, pz, q.x, q.y, q.z, rx, ry, rz);
-                    if (theta * (d - __int_as_float(x.z)) > __int_as_float(x.w)) {
-                        bh_add<SYM>(q.w / (d * d * d), rx, ry, rz, cb, sb, ax, ay, az);
-                        ncell++;
-                        node = x.y;
-                    } else {
-                        node = f;
-                    }
-                }
-            }
-            tx += (float)((double)ax * kc);
-            ty += (float)((double)ay * kc);
-            tz += (float)((double)az * kc);
+            while (w.node >= 0) w.node = bh_step<SYM>(SP, theta, cb, sb, t.cq[w.node], t.aux[w.node], w, ncell, nion, nopen);
+            tx += (float)((double)w.ax * kc);
+            ty += (float)((double)w.ay * kc);
+            tz += (float)((double)w.az * kc);
         }
         ax_out[i - i0] = tx;
         ay_out[i - i0] = ty;
         az_out[i - i0] = tz;
     }
-    unsigned long long a = ncell, b = nion;
+    unsigned long long a = ncell, b = nion, o = nopen;
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_xor(a, off);
         b += __shfl_xor(b, off);
+        o += __shfl_xor(o, off);
     }
-    if ((threadIdx.x & 63) == 0 && (a | b)) {
+    if ((threadIdx.x & 63) == 0 && (a | b | o)) {
         atomicAdd(&ctr[0], a);
         atomicAdd(&ctr[1], b);
+        atomicAdd(&ctr[3], o);
     }
 }
 
@@ -694,8 +697,8 @@ __global__ void k_bh_interleave(const double* __restrict__ a, int n, double* __r
 
 // Counters of the last evaluation (device): [0] deepest level, [1] bucket
 // leaves, [2] accepted cells, [3] ion terms, [4] ions placed in the last root
-// cell because no cell matched.
-constexpr int kBhCtr = 5;
+// cell because no cell matched, [5] opened cells.
+constexpr int kBhCtr = 6;
 
 // Evaluates targets (status 1) in [i0, i1) from sources (status 1/2) in the
 // same range; outputs ax/ay/az indexed by i - i0 (zeroed first).
@@ -925,6 +928,7 @@ int shp_bh_stats(shp_handle h, double* out) {
     out[5] = (double)c[3];
     out[6] = (double)c[4];
     for (int k = 0; k < 9; k++) out[7 + k] = h->bh_host_stats[2 + k];
+    out[16] = (double)c[5];
     return SHP_OK;
 }
 

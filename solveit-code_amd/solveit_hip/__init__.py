@@ -68,7 +68,7 @@ class BhOptions(ctypes.Structure):
 
 BH_STAT_KEYS = ("root_cells", "nodes", "max_depth", "buckets", "accepted_cells", "ion_terms",
                 "unmatched_root", "nx", "ny", "nz", "root_x", "root_y", "root_z", "dim_x", "dim_y",
-                "dim_z")
+                "dim_z", "opened_cells")
 
 
 class SubstepLevel(ctypes.Structure):
@@ -536,7 +536,7 @@ class Engine:
 
     def bh_stats(self):
         """Tree and walk counts of the last Barnes-Hut evaluation (dict)."""
-        out = (ctypes.c_double * 16)()
+        out = (ctypes.c_double * len(BH_STAT_KEYS))()
         _check(lib.shp_bh_stats(self.h, out), "shp_bh_stats")
         return dict(zip(BH_STAT_KEYS, list(out)))
 

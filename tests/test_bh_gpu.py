@@ -82,6 +82,7 @@ def _same_tree(st, info, pos4, acc4, **lay):
     assert st["unmatched_root"] == info["unmatched_root"]
     assert st["accepted_cells"] == info["accepted_cells"]
     assert st["ion_terms"] == info["ion_terms"]
+    assert st["opened_cells"] == info["opened_cells"]
 
 
 @pytest.mark.parametrize("opt", ["prism", "forced"])
