@@ -106,9 +106,12 @@ struct BhNodes {
     float2* du;   // delta, sum |q|
 };
 
-// What the walk reads of a node, in one 32-byte record: the centre of charge
-// and net charge, then (first child, successor, and for a cell delta and
-// repLength as float bits, for a leaf the ion count and first ion).
+// What the walk reads of a node, in one 32-byte record, with the nodes in
+// depth-first preorder (the order of the reference's walk): the centre of
+// charge and net charge, then (1 for a cell / -1 for a leaf, the size of its
+// subtree, and for a cell delta and repLength as float bits, for a leaf the
+// ion count and first ion).  In preorder a cell's first child is the next
+// record and its successor lies `size` records on.
 struct BhWalk {
     float4* cq;
     int4* aux;
@@ -237,13 +240,43 @@ __host__ __device__ inline float3 bh_root_loc(const BhLayout& L, int r) {
                        L.roots[2] + (float)k * L.dims[2]);
 }
 
-__global__ void k_bh_pack(BhNodes t, BhWalk w, const int* __restrict__ misc) {
+// Subtree sizes, one level at a time from the deepest (children are
+// contiguous: first .. first + nch - 1).
+__global__ void k_bh_size(BhNodes t, int lev, int* __restrict__ size, const int* __restrict__ misc) {
+    const int i = misc[2 + lev] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= misc[2 + lev + 1]) return;
+    int sz = 1;
+    const int f = t.first[i];
+    if (f >= 0)
+        for (int c = f; c < f + t.nch[i]; c++) sz += size[c];
+    size[i] = sz;
+}
+
+// Preorder positions of the children of the level-lev cells, from the top:
+// the first child follows its parent, each next sibling follows the subtree
+// of the one before (the root cells get theirs from a scan of their sizes).
+__global__ void k_bh_pre(BhNodes t, int lev, const int* __restrict__ size, int* __restrict__ pre,
+                         const int* __restrict__ misc) {
+    const int i = misc[2 + lev] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= misc[2 + lev + 1]) return;
+    const int f = t.first[i];
+    if (f < 0) return;
+    int p = pre[i] + 1;
+    for (int c = f; c < f + t.nch[i]; c++) {
+        pre[c] = p;
+        p += size[c];
+    }
+}
+
+__global__ void k_bh_pack(BhNodes t, BhWalk w, const int* __restrict__ size, const int* __restrict__ pre,
+                          const int* __restrict__ misc) {
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= misc[0]) return;
-    const int f = t.first[node];
-    w.cq[node] = t.cq[node];
-    w.aux[node] = f < 0 ? make_int4(f, t.next[node], t.count[node], t.start[node])
-                        : make_int4(f, t.next[node], __float_as_int(t.du[node].x), __float_as_int(t.geo[node].w));
+    const int k = pre[node];
+    w.cq[k] = t.cq[node];
+    w.aux[k] = t.first[node] < 0
+                   ? make_int4(-1, 1, t.count[node], t.start[node])
+                   : make_int4(1, size[node], __float_as_int(t.du[node].x), __float_as_int(t.geo[node].w));
 }
 
 // ---------------------------------------------------------------- inputs ----
@@ -580,21 +613,23 @@ __device__ __forceinline__ int bh_qrf(int n) {
     return sg * oe2;
 }
 
-// The walk of one target image: its (transformed) position, its sums, the
-// next node, and whether its own ion is skipped.
+// The walk of one target image: its (transformed) position, its sums, its
+// next record, and whether its own ion is skipped.
 struct BhPath {
     float px, py, pz, ax, ay, az;
     int node, s;
     bool self;
 };
 
-// One step of a walk at the node whose record is (q, x): a bucket of
-// coincident ions (rare), else one distance for a single-ion leaf (its centre
-// of charge is the ion) or a cell, the reference's operations per ion in
-// either case.  Returns the next node.
+// One step of a walk at record k = (q, x): a bucket of coincident ions (rare),
+// else one distance for a single-ion leaf (its centre of charge is the ion)
+// or a cell, the reference's operations per ion in either case.  Returns the
+// next record: the following one (leaf, opened cell) or past the subtree
+// (accepted cell).
 template <int SYM>
-__device__ __forceinline__ int bh_step(const float4* __restrict__ SP, float theta, int cb, int sb, const float4 q,
-                                       const int4 x, BhPath& w, unsigned& ncell, unsigned& nion, unsigned& nopen) {
+__device__ __forceinline__ int bh_step(const float4* __restrict__ SP, float theta, int cb, int sb, int k,
+                                       const float4 q, const int4 x, BhPath& w, unsigned& ncell, unsigned& nion,
+                                       unsigned& nopen) {
     const bool leaf = x.x < 0;
     if (leaf && x.z != 1) {
         for (int j = x.w; j < x.w + x.z; j++) {
@@ -605,7 +640,7 @@ __device__ __forceinline__ int bh_step(const float4* __restrict__ SP, float thet
             bh_add<SYM>(qj.w / (d * d * d), rx, ry, rz, cb, sb, w.ax, w.ay, w.az);
             nion++;
         }
-        return x.y;
+        return k + 1;
     }
     float rx, ry, rz;
     const float d = bh_dist(w.px, w.py, w.pz, q.x, q.y, q.z, rx, ry, rz);
@@ -616,15 +651,25 @@ __device__ __forceinline__ int bh_step(const float4* __restrict__ SP, float thet
         else ncell++;
     }
     if (!leaf && !take) nopen++;
-    return (leaf || take) ? x.y : x.x;
+    return (!leaf && take) ? k + x.y : k + 1;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return v;
 }
 
 // One thread per source in key order; the targets (status 1) walk the tree.
+// The lanes of a wave walk together in preorder: each step visits the
+// smallest next record over the lanes, which only the lanes waiting for it
+// process; a lane that accepted a cell waits past its subtree.  Every lane
+// so visits its own nodes in its own order, as the reference's walk does, and
+// each record is one wave-uniform (scalar) load instead of 64 scattered ones.
 // ctr[0] += accepted cells, ctr[1] += ion terms, ctr[3] += opened cells (one
 // wave-reduced atomic each per wave).
 template <int SYM>
 __global__ __launch_bounds__(kBhThreads) void k_bh_walk(
-    BhWalk t, const float4* __restrict__ SP, const int* __restrict__ SI, int ns,
+    BhWalk t, int nrec, const float4* __restrict__ SP, const int* __restrict__ SI, int ns,
     const int* __restrict__ st, const int* __restrict__ sp, float theta, const int* __restrict__ lat,
     int nimg, float dx0, float dy0, float dz0, double K, int i0, double* __restrict__ ax_out,
     double* __restrict__ ay_out, double* __restrict__ az_out, unsigned long long* __restrict__ ctr) {
@@ -632,33 +677,56 @@ __global__ __launch_bounds__(kBhThreads) void k_bh_walk(
     const int i = s < ns ? SI[s] : 0;
     const bool active = s < ns && st[i] == 1;
     unsigned ncell = 0, nion = 0, nopen = 0;
-    if (active) {
-        const float4 p = SP[s];
-        const double kc = K * (double)p.w / species_mass(sp[i]);
-        const int nrep = SYM == SHP_SYM_ROTATIONAL ? 4 : (SYM == SHP_SYM_TRANSLATIONAL ? nimg : 1);
-        float tx = 0.0f, ty = 0.0f, tz = 0.0f;
-        for (int k = 0; k < nrep; k++) {
-            BhPath w{p.x, p.y, p.z, 0.0f, 0.0f, 0.0f, 0, s, true};
-            int cb = 1, sb = 0;
-            if (SYM == SHP_SYM_ROTATIONAL) {
-                const int cf = bh_qrf(5 - k), sf = bh_qrf(4 - k);
-                cb = bh_qrf(k + 1);
-                sb = bh_qrf(k);
-                w.px = (float)cf * p.x - (float)sf * p.y;
-                w.py = (float)sf * p.x + (float)cf * p.y;
-                w.self = k == 0;
-            } else if (SYM == SHP_SYM_TRANSLATIONAL) {
-                const int lx = lat[3 * k], ly = lat[3 * k + 1], lz = lat[3 * k + 2];
-                w.px = p.x - (float)lx * dx0;
-                w.py = p.y - (float)ly * dy0;
-                w.pz = p.z - (float)lz * dz0;
-                w.self = lx == 0 && ly == 0 && lz == 0;
-            }
-            while (w.node >= 0) w.node = bh_step<SYM>(SP, theta, cb, sb, t.cq[w.node], t.aux[w.node], w, ncell, nion, nopen);
-            tx += (float)((double)w.ax * kc);
-            ty += (float)((double)w.ay * kc);
-            tz += (float)((double)w.az * kc);
+    const float4 p = active ? SP[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const double kc = active ? K * (double)p.w / species_mass(sp[i]) : 0.0;
+    const int nrep = SYM == SHP_SYM_ROTATIONAL ? 4 : (SYM == SHP_SYM_TRANSLATIONAL ? nimg : 1);
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+    for (int k = 0; k < nrep; k++) {
+        BhPath w{p.x, p.y, p.z, 0.0f, 0.0f, 0.0f, active ? 0 : nrec, s, true};
+        int cb = 1, sb = 0;
+        if (SYM == SHP_SYM_ROTATIONAL) {
+            const int cf = bh_qrf(5 - k), sf = bh_qrf(4 - k);
+            cb = bh_qrf(k + 1);
+            sb = bh_qrf(k);
+            w.px = (float)cf * p.x - (float)sf * p.y;
+            w.py = (float)sf * p.x + (float)cf * p.y;
+            w.self = k == 0;
+        } else if (SYM == SHP_SYM_TRANSLATIONAL) {
+            const int lx = lat[3 * k], ly = lat[3 * k + 1], lz = lat[3 * k + 2];
+            w.px = p.x - (float)lx * dx0;
+            w.py = p.y - (float)ly * dy0;
+            w.pz = p.z - (float)lz * dz0;
+            w.self = lx == 0 && ly == 0 && lz == 0;
         }
+        // The next record is usually the following one (a leaf, an opened
+        // cell): it is loaded while the current one is processed.
+        int r = __builtin_amdgcn_readfirstlane(wave_min(w.node));
+        float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        int4 x = make_int4(0, 0, 0, 0);
+        if (r < nrec) {
+            q = t.cq[r];
+            x = t.aux[r];
+        }
+        while (r < nrec) {  // uniform: ends when every lane is past the last record
+            const int rn = r + 1 < nrec ? r + 1 : r;
+            const float4 qn = t.cq[rn];
+            const int4 xn = t.aux[rn];
+            if (w.node == r) w.node = bh_step<SYM>(SP, theta, cb, sb, r, q, x, w, ncell, nion, nopen);
+            const int r2 = __builtin_amdgcn_readfirstlane(wave_min(w.node));
+            if (r2 == r + 1) {
+                q = qn;
+                x = xn;
+            } else if (r2 < nrec) {
+                q = t.cq[r2];
+                x = t.aux[r2];
+            }
+            r = r2;
+        }
+        tx += (float)((double)w.ax * kc);
+        ty += (float)((double)w.ay * kc);
+        tz += (float)((double)w.az * kc);
+    }
+    if (active) {
         ax_out[i - i0] = tx;
         ay_out[i - i0] = ty;
         az_out[i - i0] = tz;
@@ -838,11 +906,25 @@ int bh_eval(shp_handle h, const float4* P, const int* st, const int* sp, int i0,
                                SP, misc);
         }
         hipLaunchKernelGGL(k_bh_tree_stats, dim3(bdiv(ns, 256)), dim3(256), 0, stream, t, misc, ctr);
-        SHP_WS(wrec, char, h, WS_BH_WALK, cap * (sizeof(float4) + sizeof(int4)));
+        // walk records in preorder: subtree sizes bottom-up, the root cells'
+        // positions by a scan of their sizes, the others top-down
+        SHP_WS(wrec, char, h, WS_BH_WALK, cap * (sizeof(float4) + sizeof(int4) + 2 * sizeof(int)));
         BhWalk w;
         w.cq = (float4*)wrec;
         w.aux = (int4*)(w.cq + cap);
-        hipLaunchKernelGGL(k_bh_pack, dim3(bdiv(nodes_total, 256)), dim3(256), 0, stream, t, w, misc);
+        int* size = (int*)(w.aux + cap);
+        int* pre = size + cap;
+        for (int lev = kBhDepth; lev >= 0; lev--)
+            hipLaunchKernelGGL(k_bh_size, dim3(bdiv(lev == 0 ? nr0 : ns, 256)), dim3(256), 0, stream, t, lev, size,
+                               misc);
+        size_t t4 = 0;
+        SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t4, size, pre, (int)nr0, stream));
+        SHP_WS(cub4, char, h, WS_CUB, t4);
+        SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cub4, t4, size, pre, (int)nr0, stream));
+        for (int lev = 0; lev < kBhDepth; lev++)
+            hipLaunchKernelGGL(k_bh_pre, dim3(bdiv(lev == 0 ? nr0 : ns, 256)), dim3(256), 0, stream, t, lev, size, pre,
+                               misc);
+        hipLaunchKernelGGL(k_bh_pack, dim3(bdiv(nodes_total, 256)), dim3(256), 0, stream, t, w, size, pre, misc);
         const int* lat = nullptr;
         int nimg = 1;
         float dom[3] = {0.0f, 0.0f, 0.0f};
@@ -857,13 +939,13 @@ int bh_eval(shp_handle h, const float4* P, const int* st, const int* sp, int i0,
         SHP_HIP_CHECK(hipEventRecord(h->ev0, stream));
         const dim3 g(bdiv(ns, kBhThreads)), b(kBhThreads);
         if (symt == SHP_SYM_NONE)
-            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_NONE>, g, b, 0, stream, w, SP, SI, ns, st, sp, theta, lat,
+            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_NONE>, g, b, 0, stream, w, nodes_total, SP, SI, ns, st, sp, theta, lat,
                                nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az, ctr + 2);
         else if (symt == SHP_SYM_ROTATIONAL)
-            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_ROTATIONAL>, g, b, 0, stream, w, SP, SI, ns, st, sp, theta,
+            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_ROTATIONAL>, g, b, 0, stream, w, nodes_total, SP, SI, ns, st, sp, theta,
                                lat, nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az, ctr + 2);
         else
-            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_TRANSLATIONAL>, g, b, 0, stream, w, SP, SI, ns, st, sp,
+            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_TRANSLATIONAL>, g, b, 0, stream, w, nodes_total, SP, SI, ns, st, sp,
                                theta, lat, nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az,
                                ctr + 2);
         SHP_HIP_CHECK(hipGetLastError());
