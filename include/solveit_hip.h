@@ -252,13 +252,16 @@ int shp_bh_host(shp_handle h, const float* pos4, const float* vel4, const float*
  * setDomainToggle (0 cubic cells, the default; 1 rectangular prism),
  * setBoxDimVals (force_box_dims = 1 and the cell counts), setSubcellBlockSize
  * (2 only) and setOutlierToggle (0 only; other values return
- * SHP_ERR_UNSUPPORTED). */
+ * SHP_ERR_UNSUPPORTED).  walk (no counterpart in the reference) picks the
+ * GPU walk: 0 by the number of targets (the default), 1 each lane on its
+ * own, 2 the lanes of a wave together; all give the same bits. */
 typedef struct shp_bh_options {
     int domain_toggle;
     int force_box_dims;
     int box_dims[3];
     int subcell_block;
     int outlier_toggle;
+    int walk;
 } shp_bh_options;
 int shp_bh_set_options(shp_handle h, const shp_bh_options* o);
 

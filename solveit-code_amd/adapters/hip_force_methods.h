@@ -87,7 +87,7 @@ private:
     void applyOptions();
     shp_handle handle = nullptr;
     float theta = BHConstant::THETA;
-    shp_bh_options options = {0, 0, {1, 1, 1}, 2, 0};
+    shp_bh_options options = {0, 0, {1, 1, 1}, 2, 0, 0};
     std::vector<double> out;
     std::vector<int> lattice;
 };

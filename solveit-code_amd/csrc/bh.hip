@@ -70,6 +70,7 @@ constexpr int kBhThreads = 256;
 constexpr int kBhMinAvgPPB = 100;      // FMMConstant::minAvgPPB (constants.h:80)
 constexpr float kBhDomainEps = 0.01f;  // DomainBoundsFunction::domainEps (domain_bounds_functions.h:39)
 constexpr int kBhMaxRoots = 1 << 24;
+constexpr long kBhCoherentMin = 1 << 18;  // sources from which the wave-coherent walk is used (walk = 0)
 
 inline int bdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -666,8 +667,11 @@ __device__ __forceinline__ int wave_min(int v) {
 // so visits its own nodes in its own order, as the reference's walk does, and
 // each record is one wave-uniform (scalar) load instead of 64 scattered ones.
 // ctr[0] += accepted cells, ctr[1] += ion terms, ctr[3] += opened cells (one
-// wave-reduced atomic each per wave).
-template <int SYM>
+// wave-reduced atomic each per wave).  With COHERENT false every lane walks
+// on its own (the same steps, in the same order, with its own record loads):
+// faster when there are too few waves to hide the coherent walk's chain of
+// wave-wide minimum, scalar load and step.
+template <int SYM, bool COHERENT>
 __global__ __launch_bounds__(kBhThreads) void k_bh_walk(
     BhWalk t, int nrec, const float4* __restrict__ SP, const int* __restrict__ SI, int ns,
     const int* __restrict__ st, const int* __restrict__ sp, float theta, const int* __restrict__ lat,
@@ -698,6 +702,10 @@ __global__ __launch_bounds__(kBhThreads) void k_bh_walk(
             w.pz = p.z - (float)lz * dz0;
             w.self = lx == 0 && ly == 0 && lz == 0;
         }
+        if (!COHERENT) {
+            while (w.node < nrec)
+                w.node = bh_step<SYM>(SP, theta, cb, sb, w.node, t.cq[w.node], t.aux[w.node], w, ncell, nion, nopen);
+        } else {
         // The next record is usually the following one (a leaf, an opened
         // cell): it is loaded while the current one is processed.
         int r = __builtin_amdgcn_readfirstlane(wave_min(w.node));
@@ -722,6 +730,7 @@ __global__ __launch_bounds__(kBhThreads) void k_bh_walk(
             }
             r = r2;
         }
+        }
         tx += (float)((double)w.ax * kc);
         ty += (float)((double)w.ay * kc);
         tz += (float)((double)w.az * kc);
@@ -742,6 +751,21 @@ __global__ __launch_bounds__(kBhThreads) void k_bh_walk(
         atomicAdd(&ctr[1], b);
         atomicAdd(&ctr[3], o);
     }
+}
+
+template <bool COHERENT>
+void rc_walk(int symt, dim3 g, dim3 b, hipStream_t stream, BhWalk w, int nrec, const float4* SP, const int* SI, int ns,
+             const int* st, const int* sp, float theta, const int* lat, int nimg, const float* dom, int i0, double* ax,
+             double* ay, double* az, unsigned long long* ctr) {
+    if (symt == SHP_SYM_NONE)
+        hipLaunchKernelGGL((k_bh_walk<SHP_SYM_NONE, COHERENT>), g, b, 0, stream, w, nrec, SP, SI, ns, st, sp, theta,
+                           lat, nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az, ctr);
+    else if (symt == SHP_SYM_ROTATIONAL)
+        hipLaunchKernelGGL((k_bh_walk<SHP_SYM_ROTATIONAL, COHERENT>), g, b, 0, stream, w, nrec, SP, SI, ns, st, sp,
+                           theta, lat, nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az, ctr);
+    else
+        hipLaunchKernelGGL((k_bh_walk<SHP_SYM_TRANSLATIONAL, COHERENT>), g, b, 0, stream, w, nrec, SP, SI, ns, st, sp,
+                           theta, lat, nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az, ctr);
 }
 
 __global__ void k_bh_zero(double* __restrict__ p, size_t n) {
@@ -938,16 +962,14 @@ int bh_eval(shp_handle h, const float4* P, const int* st, const int* sp, int i0,
         }
         SHP_HIP_CHECK(hipEventRecord(h->ev0, stream));
         const dim3 g(bdiv(ns, kBhThreads)), b(kBhThreads);
-        if (symt == SHP_SYM_NONE)
-            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_NONE>, g, b, 0, stream, w, nodes_total, SP, SI, ns, st, sp, theta, lat,
-                               nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az, ctr + 2);
-        else if (symt == SHP_SYM_ROTATIONAL)
-            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_ROTATIONAL>, g, b, 0, stream, w, nodes_total, SP, SI, ns, st, sp, theta,
-                               lat, nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az, ctr + 2);
+        // the coherent walk needs enough waves per SIMD to hide its per-step chain
+        const bool coherent = h->bh_opt.walk == 2 || (h->bh_opt.walk == 0 && (long)ns >= kBhCoherentMin);
+        if (coherent)
+            rc_walk<true>(symt, g, b, stream, w, nodes_total, SP, SI, ns, st, sp, theta, lat, nimg, dom, i0, ax, ay,
+                          az, ctr + 2);
         else
-            hipLaunchKernelGGL(k_bh_walk<SHP_SYM_TRANSLATIONAL>, g, b, 0, stream, w, nodes_total, SP, SI, ns, st, sp,
-                               theta, lat, nimg, dom[0], dom[1], dom[2], k_const_host(), i0, ax, ay, az,
-                               ctr + 2);
+            rc_walk<false>(symt, g, b, stream, w, nodes_total, SP, SI, ns, st, sp, theta, lat, nimg, dom, i0, ax, ay,
+                           az, ctr + 2);
         SHP_HIP_CHECK(hipGetLastError());
         SHP_HIP_CHECK(hipEventRecord(h->ev1, stream));
         h->timed = true;
@@ -990,6 +1012,7 @@ int shp_bh_set_options(shp_handle h, const shp_bh_options* o) {
         set_error("shp_bh_set_options: the outlier toggle is not supported (reference default: off)");
         return SHP_ERR_UNSUPPORTED;
     }
+    SHP_REQUIRE(o->walk >= 0 && o->walk <= 2, "shp_bh_set_options: walk %d (0, 1 or 2)", o->walk);
     h->bh_opt = *o;
     return SHP_OK;
 }

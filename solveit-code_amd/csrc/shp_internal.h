@@ -165,7 +165,7 @@ struct shp_context {
     bool timed = false, timed_update = false;
     // Barnes-Hut root-layout options (shp_bh_set_options; reference defaults)
     // and the last evaluation's tree summary (shp_bh_stats).
-    shp_bh_options bh_opt = {0, 0, {1, 1, 1}, 2, 0};
+    shp_bh_options bh_opt = {0, 0, {1, 1, 1}, 2, 0, 0};
     bool bh_have_stats = false;
     unsigned long long* bh_ctr = nullptr;  // device counters in the workspace
     double bh_host_stats[11] = {};

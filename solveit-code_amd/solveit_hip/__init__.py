@@ -63,6 +63,7 @@ class BhOptions(ctypes.Structure):
         ("box_dims", ctypes.c_int * 3),
         ("subcell_block", ctypes.c_int),
         ("outlier_toggle", ctypes.c_int),
+        ("walk", ctypes.c_int),
     ]
 
 
@@ -522,9 +523,10 @@ class Engine:
                "shp_state_fields_host")
         return e1, e2
 
-    def bh_set_options(self, prism=False, box_dims=None, subcell_block=2, outlier_toggle=0):
+    def bh_set_options(self, prism=False, box_dims=None, subcell_block=2, outlier_toggle=0, walk=0):
         """Root-layout options of later Barnes-Hut calls (BarnesHutForceCalculation's
-        setDomainToggle / setBoxDimVals / setSubcellBlockSize / setOutlierToggle)."""
+        setDomainToggle / setBoxDimVals / setSubcellBlockSize / setOutlierToggle), and the
+        GPU walk (0 by size, 1 per lane, 2 wave-coherent)."""
         o = BhOptions()
         o.domain_toggle = 1 if prism else 0
         o.force_box_dims = 0 if box_dims is None else 1
@@ -532,6 +534,7 @@ class Engine:
             o.box_dims[c] = 1 if box_dims is None else int(box_dims[c])
         o.subcell_block = int(subcell_block)
         o.outlier_toggle = int(outlier_toggle)
+        o.walk = int(walk)
         _check(lib.shp_bh_set_options(self.h, ctypes.byref(o)), "shp_bh_set_options")
 
     def bh_stats(self):

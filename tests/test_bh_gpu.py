@@ -85,6 +85,32 @@ def _same_tree(st, info, pos4, acc4, **lay):
     assert st["opened_cells"] == info["opened_cells"]
 
 
+@pytest.mark.parametrize("sym", ["none", "rotational", "translational"])
+def test_bh_walks_give_the_same_bits(eng, sym):
+    """The per-lane walk and the wave-coherent walk (shp_bh_options.walk 1 and
+    2) visit each target's nodes in the same order: equal bits, equal counts,
+    and both within the oracle's bar."""
+    import solveit_hip
+    d, kw = _case(sym)
+    pos4, vel4, acc4 = synth.to_reference_aos(d)
+    print(f"case bh walks sym={sym}", flush=True)
+    s = solveit_hip.make_symmetry(sym, nsec=4, lattice=kw.get("lattice"),
+                                  domain=kw.get("domain", (0.0, 0.0, 0.0)))
+    out, stats = [], []
+    try:
+        for walk in (1, 2):
+            eng.bh_set_options(walk=walk)
+            out.append(eng.bh_host(pos4, vel4, acc4, theta=0.5, symmetry=s))
+            stats.append(eng.bh_stats())
+    finally:
+        eng.bh_set_options()
+    np.testing.assert_array_equal(out[0], out[1])
+    assert stats[0] == stats[1]
+    ref, _, info = oracle.bh_ref(pos4, vel4, acc4, theta=0.5, sym=sym, info=True, **kw)
+    assert _ulps(out[1], ref).max() <= 2
+    _same_tree(stats[1], info, pos4, acc4)
+
+
 @pytest.mark.parametrize("opt", ["prism", "forced"])
 def test_bh_layout_options_match_oracle(eng, opt):
     """setDomainToggle(rectangular prism) and setBoxDimVals on the cone."""
