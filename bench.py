@@ -305,10 +305,14 @@ def run_config(k, args):
         msb, _ = _timed(lambda: eng.bh_dev(stb, theta=0.5))
         dsm = synth.cone(131072)
         oracle.fmm_set_shell_orders([0, 0, 0, 10, 8, 7])
-        t0 = time.perf_counter()
-        oracle.fmm_f64(dsm, 10, 6, ring=2)
-        tf = time.perf_counter() - t0
-        oracle.fmm_set_shell_orders(None)
+        oracle.set_threads(1)  # one host thread, like the Barnes-Hut leg
+        try:
+            t0 = time.perf_counter()
+            oracle.fmm_f64(dsm, 10, 6, ring=2)
+            tf = time.perf_counter() - t0
+        finally:
+            oracle.set_threads(0)
+            oracle.fmm_set_shell_orders(None)
         msf, _ = _timed(lambda: eng.fmm_dev(P.DeviceState.from_numpy(dsm), order=10, leaf_level=6, ring=2))
         res["cpu_baseline"] = {
             "bh_theta0.5_s_65536": tb, "gpu_bh_theta0.5_ms_65536": msb,
