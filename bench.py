@@ -457,8 +457,13 @@ def run_config5(eng, args, log):
     gen = torch.Generator(device="cpu").manual_seed(2026)
     h = 1e-6
     worst, fd_rows = 0.0, []
+    # directions fd_first .. fd_first + fd_dirs - 1 of the seeded sequence (a long
+    # check can be split over runs; every run draws the same sequence)
+    for _ in range(args.fd_first):
+        for _ in range(4):
+            torch.randn(n, generator=gen, dtype=torch.float64)
     t0 = time.perf_counter()
-    for i in range(args.fd_dirs):
+    for i in range(args.fd_first, args.fd_first + args.fd_dirs):
         dirs = [torch.randn(n, generator=gen, dtype=torch.float64).to(dev) for _ in range(4)]
         dq, dv = dirs[0] * 0.01, [x * 10.0 for x in dirs[1:]]
         ana = float(sum((g * x).sum() for g, x in zip(grads, [dq] + dv)))
@@ -484,7 +489,7 @@ def run_config5(eng, args, log):
             "pairs_per_step": pairs_step, "tuner_ms": tune_ms, "tuner_choice": chosen,
             "forward_s": fwd_s, "forward_particle_steps_per_s": n * K / fwd_s,
             "forward_fp32c_tflops_20flop": FLOP_PER_PAIR * pairs_step * K / fwd_s / 1e12,
-            "backward_s": bwd_s, "objective": J, "fd_dirs": args.fd_dirs, "fd_s": fd_s,
+            "backward_s": bwd_s, "objective": J, "fd_first": args.fd_first, "fd_dirs": args.fd_dirs, "fd_s": fd_s,
             "fd_worst_rel_diff": worst, "fd": fd_rows,
             "cpu_baseline": {"adjoint_particle_steps_per_s": 4096 * 20 / cpu_s, "cores": 1, "kind": "port",
                              "sample": f"oracle FP64 rollout + discrete adjoint, one 4096-ion cone, 20 steps "
@@ -508,6 +513,7 @@ def main():
     ap.add_argument("--config", type=int, default=0,
                     help="measure BASELINE config 1-5 instead of the headline step (one JSON line)")
     ap.add_argument("--fd-dirs", type=int, default=32, help="config 5: finite-difference directions")
+    ap.add_argument("--fd-first", type=int, default=0, help="config 5: index of the first direction checked")
     ap.add_argument("--adj-dt", type=float, default=1e-14,
                     help="config 5: time step of the differentiated rollout (s)")
     ap.add_argument("--copy-calibration", action="store_true",
