@@ -174,8 +174,12 @@ def _energy_total(eng, st, x0, y0, z0, nsteps_field):
 
 
 def run_config(k, args):
-    """One JSON line per BASELINE config (1-5) measured on this GPU; see the
-    docstring of each branch for what is timed."""
+    """One JSON line for workload k measured on this GPU: 1 the FP64 direct
+    leapfrog of a 4096-ion hemisphere with the tip field, 2 FP32C against FP64
+    direct on 65,536- and 262,144-ion cones, 3 direct / Barnes-Hut / FMM on
+    the 1M-ion cone, 4 FMM with symmetry images at 262,144 ions, 5 the
+    batched adjoint of 64 plumes; the comment of each branch says what is
+    timed."""
     import numpy as np
     import torch
     import solveit_hip
@@ -511,11 +515,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--config", type=int, default=0,
-                    help="measure BASELINE config 1-5 instead of the headline step (one JSON line)")
-    ap.add_argument("--fd-dirs", type=int, default=32, help="config 5: finite-difference directions")
-    ap.add_argument("--fd-first", type=int, default=0, help="config 5: index of the first direction checked")
+                    help="measure workload 1-5 (run_config) instead of the headline step (one JSON line)")
+    ap.add_argument("--fd-dirs", type=int, default=32, help="workload 5: finite-difference directions")
+    ap.add_argument("--fd-first", type=int, default=0, help="workload 5: index of the first direction checked")
     ap.add_argument("--adj-dt", type=float, default=1e-14,
-                    help="config 5: time step of the differentiated rollout (s)")
+                    help="workload 5: time step of the differentiated rollout (s)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy first (reference kernel for --pmc byte counts)")
     args = ap.parse_args()

@@ -263,9 +263,9 @@ def test_batch_forces_and_leapfrog_match_oracle(eng):
         np.testing.assert_array_equal(got["status"][off[k]:off[k + 1]], r["status"])
 
 
-def test_batched_adjoint_matches_oracle_config5_shape(eng):
-    """The batched checkpointed adjoint at the config-5 schedule (K = 200
-    leapfrog steps, a checkpoint every 20) on 8 plumes of uneven N drawn
+def test_batched_adjoint_matches_oracle_at_the_200_step_schedule(eng):
+    """The batched checkpointed adjoint at the 64-plume workload's schedule
+    (K = 200 leapfrog steps, a checkpoint every 20) on 8 plumes of uneven N drawn
     lognormally in [256, 2048]: the gradient of the mean sin^2 of the
     velocity divergence angle with respect to charges and initial velocities
     equals, instance by instance, the oracle's FP64 discrete adjoint

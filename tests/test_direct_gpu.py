@@ -132,7 +132,7 @@ def test_fp32c_dev_vs_fp64_oracle(eng, kind):
 
 
 def test_fp32c_vs_fp64_gpu_full_size(eng):
-    """Config-2 size: FP32C against the GPU FP64 path on 262,144 cone ions."""
+    """FP32C against the GPU FP64 path on 262,144 cone ions."""
     import solveit_hip
     d = synth.cone(262144)
     print("case fp32c-vs-fp64 n=262144", flush=True)
