@@ -167,6 +167,33 @@ int shp_fmm_host(shp_handle h, const float* pos4, const float* vel4, const float
                  int i0, int i1, int order, int leaf_level, int leaf_size, int ring,
                  const shp_symmetry* sym, double* out3);
 
+/* The reference's own FMM operators, the drop-in for
+ * MultipoleMethodForceCalculation::updateAcceleration with its spherical-
+ * harmonic expansions (multipole_method_force_calculation.cpp:148-522;
+ * p2m_kernel, m2m_kernel, m2l_kernel, l2l_kernel, l2p_kernel and rotation,
+ * fmm_cpukernel.cpp:10-1317; the tables of MultipoleMethodConstants::precalc,
+ * multipole_method_constants.cpp:40-226): expansions of 1/r translated by
+ * rotation, shift along z and rotation back, the softened centre distance in
+ * M2L (fmm_cpukernel.cpp:832), P2P over the 27 neighbour leaves.  order is
+ * numExpansions (1..10, setNumExpansions); max_level is setMaxLevel's value
+ * (-2: setOptimumLevel's choice from the largest domain's count).  The tree
+ * is the reference's: the domain grid of computeDomainBounds, maxLevel levels
+ * in every domain, float leaf placement (morton,
+ * multipole_method_force_calculation.cpp:976-1057); above the domain level,
+ * where the reference groups several domains into ternary supercells, octree
+ * levels of one enclosing cube.  Computed in FP64 (the translations as dense
+ * operators on the FP64 matrix cores).  Ranges and out3 as shp_fmm_host.
+ * SHP_ERR_UNSUPPORTED when a source lies outside the domain grid (the
+ * reference's outlier passes) or maxLevel < 2 (its single-box paths).  Waits. */
+int shp_fmm_sph_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4, int n,
+                     int i0, int i1, int order, int max_level, double* out3);
+/* Summary of the last shp_fmm_sph_host call, 12 values: [0] maxLevel, [1]
+ * levels above the domain level, [2..4] domains per axis, [5] leaf boxes,
+ * [6] M2L box pairs, [7] P2P box pairs, [8] sources, [9] boxes of all levels
+ * from level 2, [10..11] reserved.  No counterpart in the reference (its
+ * timing file records the level and domain counts). */
+int shp_fmm_sph_stats(shp_handle h, double* out);
+
 /* M2L expansion orders the engine uses on the Chebyshev shells 3, 4 and 5 of
  * the far band for an FMM of this order and near-field width (sym_type
  * SHP_SYM_ROTATIONAL: the rotational variant of shp_fmm_sym_dev, which keeps

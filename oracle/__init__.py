@@ -738,3 +738,119 @@ def rollout_grad_f64(d, dt, nsteps, grad_final):
         xb = xh
         vb = np.where(mv, w + 0.5 * dt * xh, w)
     return xb, vb, qb, cur
+
+
+# ---- the reference's rotation-based spherical-harmonic FMM (oracle/fmm_sph.c) ----
+
+lib.orc_domain_bounds.restype = _ci
+lib.orc_domain_bounds.argtypes = [_vp, _vp, _ci, _ci, _vp, _vp, _vp, _vp]
+lib.orc_fmm_sph.restype = _ci
+lib.orc_fmm_sph.argtypes = [_vp, _vp, _vp, _ci, _ci, _ci, _ci, _ci, _vp, _vp]
+lib.orc_sph_p2m.argtypes = [_ci, _ci, _vp, ctypes.c_double, _vp]
+lib.orc_sph_m2m.argtypes = [_ci, _ci, _vp, _vp]
+lib.orc_sph_m2l.argtypes = [_ci, _ci, _ci, _ci, ctypes.c_double, _vp, _vp]
+lib.orc_sph_l2l.argtypes = [_ci, _ci, _vp, _vp]
+lib.orc_sph_l2p.argtypes = [_ci, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_double, _vp]
+lib.orc_sph_rotate.restype = _ci
+lib.orc_sph_rotate.argtypes = [_ci, _ci, _ci, _ci, _ci, _vp, _vp]
+lib.orc_sph_set_cr_trig.argtypes = [_ci]
+lib.orc_sph_table.restype = ctypes.c_double
+lib.orc_sph_table.argtypes = [_ci, _ci]
+
+
+def domain_bounds(pos4, acc4, i0=0, i1=None):
+    """computeDomainBounds as the FMM calls it: (boxMin[3], dims[3], boxDimVals[3], nsrc)."""
+    pos4, acc4 = _f32(pos4), _f32(acc4)
+    i1 = pos4.shape[0] if i1 is None else i1
+    bm, dm, nd = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.int32)
+    ns = lib.orc_domain_bounds(pos4.ctypes.data, acc4.ctypes.data, i0, i1, None, bm.ctypes.data,
+                               dm.ctypes.data, nd.ctypes.data)
+    return bm, dm, nd, ns
+
+
+def fmm_sph(pos4, vel4, acc4, p=10, max_level=-2, i0=0, i1=None, cr_trig=False):
+    """The reference FMM's operators (rotation-based spherical harmonics) over the
+    reference's domain grid and leaf placement, in FP64; returns (acc[i1-i0,3], info).
+    max_level: setMaxLevel's value (-2: setOptimumLevel).  Raises on the cases
+    the restatement does not cover (sources outside the grid, maxLevel < 2).
+    cr_trig: correctly rounded float acos / atan in cart2sph_exact instead of
+    this host's acosf / atanf (which the reference calls)."""
+    pos4, vel4, acc4 = _f32(pos4), _f32(vel4), _f32(acc4)
+    n = pos4.shape[0]
+    i1 = n if i1 is None else i1
+    out = np.zeros((i1 - i0, 3))
+    info = np.zeros(12, np.int64)
+    lib.orc_sph_set_cr_trig(1 if cr_trig else 0)
+    try:
+        rc = lib.orc_fmm_sph(pos4.ctypes.data, vel4.ctypes.data, acc4.ctypes.data, n, i0, i1, p,
+                             max_level, out.ctypes.data, info.ctypes.data)
+    finally:
+        lib.orc_sph_set_cr_trig(0)
+    if rc != 0:
+        raise ValueError(f"orc_fmm_sph: unsupported case ({rc})")
+    keys = ["max_level", "k", "nd0", "nd1", "nd2", "leaves", "m2l_pairs", "p2p_pairs", "sources",
+            "boxes"]
+    return out, {k: int(info[i]) for i, k in enumerate(keys)}
+
+
+def _sph_nc(p):
+    return p * (p + 1) // 2
+
+
+def sph_p2m(p, xyzq, h):
+    """Multipole (complex, n(n+1)/2 + m order) of particles at xyzq[:, :3] from the centre."""
+    a = _f64(np.asarray(xyzq).reshape(-1, 4))
+    M = np.zeros(2 * _sph_nc(p))
+    lib.orc_sph_p2m(p, a.shape[0], a.ctypes.data, float(h), M.ctypes.data)
+    return M[0::2] + 1j * M[1::2]
+
+
+def _cz(v):
+    v = np.asarray(v, np.complex128)
+    out = np.zeros(2 * v.shape[0])
+    out[0::2], out[1::2] = v.real, v.imag
+    return out
+
+
+def sph_m2m(p, octant, Mc, Mp=None):
+    a = _cz(Mc)
+    b = _cz(np.zeros(_sph_nc(p)) if Mp is None else Mp)
+    lib.orc_sph_m2m(p, octant, a.ctypes.data, b.ctypes.data)
+    return b[0::2] + 1j * b[1::2]
+
+
+def sph_m2l(p, d, h, M, L=None):
+    a = _cz(M)
+    b = _cz(np.zeros(_sph_nc(p)) if L is None else L)
+    lib.orc_sph_m2l(p, int(d[0]), int(d[1]), int(d[2]), float(h), a.ctypes.data, b.ctypes.data)
+    return b[0::2] + 1j * b[1::2]
+
+
+def sph_l2l(p, octant, Lp):
+    a = _cz(Lp)
+    b = _cz(np.zeros(_sph_nc(p)))
+    lib.orc_sph_l2l(p, octant, a.ctypes.data, b.ctypes.data)
+    return b[0::2] + 1j * b[1::2]
+
+
+def sph_l2p(p, L, dxyz, h):
+    """The field vector the reference's L2P forms before scaling by -K q / m."""
+    a = _cz(L)
+    f = np.zeros(3)
+    lib.orc_sph_l2p(p, a.ctypes.data, float(dxyz[0]), float(dxyz[1]), float(dxyz[2]), float(h),
+                    f.ctypes.data)
+    return f
+
+
+def sph_rotate(p, d, C, backward=False):
+    a = _cz(C)
+    b = np.zeros_like(a)
+    if lib.orc_sph_rotate(p, int(d[0]), int(d[1]), int(d[2]), 1 if backward else 0, a.ctypes.data,
+                          b.ctypes.data) != 0:
+        raise ValueError("direction must have gcd 1")
+    return b[0::2] + 1j * b[1::2]
+
+
+def sph_table(which, index):
+    return lib.orc_sph_table({"factorial": 0, "anm": 1, "Anm": 2, "Ynm": 3}[which], int(index))

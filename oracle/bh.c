@@ -76,13 +76,16 @@ typedef struct {
     int nsrc;        /* ions of status 1/2 in the range */
 } bh_layout;
 
-/* computeDomainBounds (no outliers) followed by computeWorldLimits.
+/* DomainBoundsFunction::computeDomainBounds (no outliers): the corner of the
+ * first root cell (boxMin), the cell dimensions and the cells per axis.
  * toggle: 0 cubic cells, 1 rectangular-prism cells; forced: NULL or the
- * cell counts of setBoxDimVals. */
-static void bh_make_layout(const float* pos4, const float* acc4, int i0, int i1, int toggle,
-                           const int* forced, bh_layout* L) {
+ * cell counts of setBoxDimVals.  minO / maxO: the bounds of the sources
+ * seeded with the first ion of the range.  Returns the number of sources. */
+static int domain_bounds(const float* pos4, const float* acc4, int i0, int i1, int toggle,
+                         const int* forced, float boxMin[3], float dims[3], int bdv[3],
+                         float minO[3], float maxO[3]) {
     const float* seed = pos4 + 4 * (size_t)i0;
-    float boxMin[3], rootmax[3], minO[3], maxO[3], dims[3];
+    float rootmax[3];
     for (int c = 0; c < 3; c++) boxMin[c] = rootmax[c] = minO[c] = maxO[c] = seed[c];
     int npart = 0;
     for (int i = i0; i < i1; i++) {
@@ -102,7 +105,7 @@ static void bh_make_layout(const float* pos4, const float* acc4, int i0, int i1,
         boxMin[c] = boxMin[c] - subtr;
         dims[c] = (rootmax[c] - boxMin[c]) + subtr;
     }
-    int bdv[3] = {1, 1, 1};
+    bdv[0] = bdv[1] = bdv[2] = 1;
     if (toggle == 0) {
         float dimDiff[3] = {dims[0], dims[1], dims[2]};
         int dimInd[3] = {0, 1, 2};
@@ -166,6 +169,15 @@ static void bh_make_layout(const float* pos4, const float* acc4, int i0, int i1,
             dims[c] = rbs;
         }
     }
+    return npart;
+}
+
+/* computeDomainBounds followed by computeWorldLimits. */
+static void bh_make_layout(const float* pos4, const float* acc4, int i0, int i1, int toggle,
+                           const int* forced, bh_layout* L) {
+    float boxMin[3], dims[3], minO[3], maxO[3];
+    int bdv[3];
+    const int npart = domain_bounds(pos4, acc4, i0, i1, toggle, forced, boxMin, dims, bdv, minO, maxO);
     /* computeWorldLimits */
     float roots[3] = {boxMin[0], boxMin[1], boxMin[2]};
     for (int c = 0; c < 3; c++) {
@@ -468,6 +480,17 @@ int orc_bh_layout(const float* pos4, const float* acc4, int i0, int i1, int togg
         nd3[c] = L.nd[c];
     }
     return L.nsrc;
+}
+
+/* computeDomainBounds alone, as MultipoleMethodForceCalculation::setDomainSize
+ * calls it (multipole_method_force_calculation.cpp:648-668): cubic cells, no
+ * outliers.  Returns the number of sources; box_min3 / dims3 / nd3 as the
+ * reference's boxMin, dims and boxDimVals. */
+int orc_domain_bounds(const float* pos4, const float* acc4, int i0, int i1, const int* forced3,
+                      float* box_min3, float* dims3, int* nd3) {
+    float minO[3], maxO[3];
+    if (i1 <= i0) return 0;
+    return domain_bounds(pos4, acc4, i0, i1, 0, forced3, box_min3, dims3, nd3, minO, maxO);
 }
 
 /* The root cell an ion is placed in (generateTree :279-310), as a chain index;

@@ -15,7 +15,7 @@ OUT = os.path.join(HERE, "solveit_hip", "libsolveit_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["context.hip", "direct.hip", "leapfrog.hip", "fmm.hip", "bh.hip", "adjoint.hip", "batch.hip",
-           "field_io.hip", "writers.hip"]
+           "field_io.hip", "writers.hip", "fmm_sph.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

@@ -131,12 +131,16 @@ __host__ __device__ inline float bh_max3(float a, float b, float c) {
     return m;
 }
 
-// computeDomainBounds (outlier flags all 0) followed by computeWorldLimits,
-// from the sources' float bounds (mn, mx), the first ion of the range and
-// the number of sources.  toggle 0: cubic cells, 1: rectangular prism.
-void bh_layout_host(const float mn_src[3], const float mx_src[3], const float seed[3], int nsrc,
-                    int toggle, const int* forced, BhLayout* L) {
-    float boxMin[3], rootmax[3], minO[3], maxO[3], dims[3];
+}  // namespace
+
+// computeDomainBounds (outlier flags all 0) from the sources' float bounds
+// (mn, mx), the first ion of the range and the number of sources:
+// boxMin, the cell dimensions and the cells per axis, plus the bounds seeded
+// with the first ion (minO, maxO).  toggle 0: cubic cells, 1: rectangular prism.
+void domain_bounds_host(const float mn_src[3], const float mx_src[3], const float seed[3], int nsrc,
+                        int toggle, const int* forced, float boxMin[3], float dims[3], int bdv[3],
+                        float minO[3], float maxO[3]) {
+    float rootmax[3];
     for (int c = 0; c < 3; c++) {
         boxMin[c] = minO[c] = mn_src[c] < seed[c] ? mn_src[c] : seed[c];
         rootmax[c] = maxO[c] = mx_src[c] > seed[c] ? mx_src[c] : seed[c];
@@ -146,7 +150,7 @@ void bh_layout_host(const float mn_src[3], const float mx_src[3], const float se
         boxMin[c] = boxMin[c] - subtr;
         dims[c] = (rootmax[c] - boxMin[c]) + subtr;
     }
-    int bdv[3] = {1, 1, 1};
+    bdv[0] = bdv[1] = bdv[2] = 1;
     if (toggle == 0) {
         float dd[3] = {dims[0], dims[1], dims[2]};
         int di[3] = {0, 1, 2};
@@ -198,6 +202,16 @@ void bh_layout_host(const float mn_src[3], const float mx_src[3], const float se
             dims[c] = size;
         }
     }
+}
+
+namespace {
+
+// computeDomainBounds followed by computeWorldLimits.
+void bh_layout_host(const float mn_src[3], const float mx_src[3], const float seed[3], int nsrc,
+                    int toggle, const int* forced, BhLayout* L) {
+    float boxMin[3], minO[3], maxO[3], dims[3];
+    int bdv[3];
+    domain_bounds_host(mn_src, mx_src, seed, nsrc, toggle, forced, boxMin, dims, bdv, minO, maxO);
     float roots[3] = {boxMin[0], boxMin[1], boxMin[2]};
     for (int c = 0; c < 3; c++)
         while (roots[c] > minO[c]) {

@@ -110,6 +110,7 @@ int shp_destroy(shp_handle h) {
     if (h->res_evn) (void)hipFree(h->res_evn);
     shp::free_mesh(h);
     shp::fmm_release(h);
+    shp::sph_release(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return SHP_OK;

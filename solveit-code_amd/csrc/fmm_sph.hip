@@ -1,0 +1,1273 @@
+// The reference's own FMM operators on the GPU: spherical-harmonic expansions
+// of 1/r, translated by a rotation, a shift along z and the rotation back
+// (MultipoleMethodForceCalculation, fmm_cpukernel.cpp:10-1317, tables of
+// MultipoleMethodConstants::precalc, multipole_method_constants.cpp:40-226),
+// over the reference's domain grid and leaf placement, in FP64.
+//
+// MI355X mapping.  Every translation of the reference is a fixed real-linear
+// map of the 2 p(p+1)/2 real components of an expansion: M2M and L2L per child
+// octant (level independent), M2L per offset and level (the softened centre
+// distance depends on the box size).  The engine builds each map once per call
+// as a dense matrix by applying the reference's rotate / shift / rotate
+// sequence to unit vectors (one thread per column), and then runs the M2L of
+// a level as batched dense products on the FP64 matrix cores: a wave takes 16
+// target boxes of one child octant (they share the offset list), gathers the
+// 16 source expansions of each offset as the B operand and accumulates
+// T(offset) x M with v_mfma_f64_16x16x4f64, the operator pre-swizzled so each
+// A fragment is one coalesced 512-byte load.  P2M and L2P follow the
+// reference's Legendre recurrences per particle; P2P is the reference's direct
+// sum over the 27 neighbour leaves.
+//
+// Tree.  The reference lays a grid of cubic domains of side rootBoxSize
+// (computeDomainBounds, setDomainSize multipole_method_force_calculation.cpp:
+// 648-832), gives every domain an octree of depth maxLevel (setOptimumLevel
+// :835-881 or setMaxLevel) and places each particle with float arithmetic
+// (morton :976-1057).  Below the domain level its P2P and M2L lists
+// (getInteractionList :1685-2446) are those of one octree over a cube of side
+// 2^K rootBoxSize (2^K >= the largest domain count), which is the tree used
+// here at every level; for one domain (K = 0) it is the reference's tree.
+// Above the domain level the reference groups domains into ternary
+// supercells; here they are octree levels of the enclosing cube.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "shp_internal.h"
+
+namespace shp {
+namespace {
+
+constexpr int kSpPmax = 10;     // FMMConstant::maxNumExpansions (constants.h:82)
+constexpr int kSpNe2 = 100;     // numExpansion2
+constexpr int kSpNe4 = 10000;   // numExpansion4
+constexpr int kSpNrot = 1155;   // possibleRotations
+constexpr int kSpNrel = 1331;   // numRelativeBox = 11^3
+constexpr int kSpNcMax = 55;    // numCoefficients at p = 10
+constexpr int kSpOps = 343;     // M2L operator slots: offsets in [-3, 3]^3
+constexpr int kSpNoff = 189;    // M2L offsets per child octant
+constexpr int kSpMaxDomains = 1 << 22;
+constexpr double kPiHi = 3.141592653589793;       // long double pi as a double pair
+constexpr double kPiLo = 1.2246467991473532e-16;
+constexpr double kPiHalf = 1.5707963267948966;
+
+// ------------------------------------------------------------ host tables --
+
+struct SphHostTables {
+    float factorial[4 * kSpNe2];
+    float anm[4 * kSpNe2];
+    float Anm[kSpNe4];
+    double Ynm[4 * kSpNe2];  // the reference's complex Ynm table is real
+    double2 Dnm[kSpNrot][kSpPmax][kSpNe2];
+    int link[kSpNrel];
+};
+
+// cart2sph_exact (auxiliary_functions.cpp:160-186): float sqrtf / acosf /
+// atanf on double arguments, phi + pi formed in long double (PI is long
+// double, constants.h:28).
+__host__ __device__ inline void sp_cart2sph(double& r, double& th, double& ph, double dx, double dy, double dz) {
+    r = (double)(float)sqrt((double)(float)(dx * dx + dy * dy + dz * dz));
+    if (r != 0) {
+        th = (double)(float)acos((double)(float)(dz / r));
+        if (th == 0 && dz != 0) th = (double)(float)sqrt((double)(float)(dx * dx + dy * dy)) / dz;
+    } else {
+        th = 0;
+    }
+    if (dx == 0 && dy == 0) {
+        ph = 0;
+    } else if (dx == 0) {
+        ph = dy > 0 ? kPiHalf : -kPiHalf;
+    } else {
+        const double a = (double)(float)atan((double)(float)(dy / dx));
+        if (dx > 0) {
+            ph = a;
+        } else {
+            const double s = a + kPiHi;  // a + pi rounded once (two-sum of a and pi's leading part)
+            const double bb = s - a;
+            const double err = (a - (s - bb)) + (kPiHi - bb);
+            ph = s + (err + kPiLo);
+        }
+    }
+}
+
+__host__ __device__ inline int sp_gcd(int a, int b) {
+    a = a < 0 ? -a : a;
+    b = b < 0 ? -b : b;
+    while (b != 0) {
+        const int t = b;
+        b = a % b;
+        a = t;
+    }
+    return a;
+}
+
+inline int sp_nmk(int n, int m, int k) { return (4 * n * n * n + 6 * n * n + 5 * n) / 3 + m * (2 * n + 1) + k; }
+
+// MultipoleMethodConstants::precalc: the same recurrences and roundings
+// (float storage of factorial, anm and Anm).
+void sp_build_tables(SphHostTables* T) {
+    const int P = kSpPmax;
+    for (int n = 0; n < 2 * P; n++)
+        for (int m = -n; m <= n; m++) {
+            const int nm = n * n + n + m, am = std::abs(m);
+            double fnmm = 1, fnpm = 1, fnma = 1, fnpa = 1;
+            for (int i = 1; i <= n - m; i++) fnmm *= i;
+            for (int i = 1; i <= n + m; i++) fnpm *= i;
+            for (int i = 1; i <= n - am; i++) fnma *= i;
+            for (int i = 1; i <= n + am; i++) fnpa *= i;
+            T->factorial[nm] = (float)std::sqrt(fnma / fnpa);
+            T->anm[nm] = (float)(std::pow(-1.0, n) / std::sqrt(fnmm * fnpm));
+        }
+    std::memset(T->Anm, 0, sizeof(T->Anm));
+    for (int j = 0; j < P; j++)
+        for (int k = -j; k <= j; k++) {
+            const int jk = j * j + j + k;
+            for (int n = std::abs(k); n < P; n++) {
+                const int nk = n * n + n + k;
+                const int jnk = (j + n) * (j + n) + j + n;
+                T->Anm[jk * kSpNe2 + nk] = (float)(std::pow(-1.0, j + k) * T->anm[nk] * T->anm[jk] / T->anm[jnk]);
+            }
+        }
+    double pn = 1;
+    for (int m = 0; m < 2 * P; m++) {
+        double p = pn;
+        T->Ynm[m * m + 2 * m] = T->factorial[m * m + 2 * m] * p;
+        T->Ynm[m * m] = T->Ynm[m * m + 2 * m];
+        double p1 = p;
+        p = (2 * m + 1) * p;
+        for (int n = m + 1; n < 2 * P; n++) {
+            T->Ynm[n * n + n + m] = T->factorial[n * n + n + m] * p;
+            T->Ynm[n * n + n - m] = T->Ynm[n * n + n + m];
+            const double p2 = p1;
+            p1 = p;
+            p = ((2 * n + 1) * p1 - (n + m) * p2) / (n - m + 1);
+        }
+        pn = 0;
+    }
+    std::vector<double> a0(kSpNe4), a1(kSpNe4), D(kSpNe4);
+    for (int n = 0; n < P; n++)
+        for (int m = 1; m <= n; m++) {
+            const double anmd = n * (n + 1) - m * (m - 1);
+            for (int k = 1 - m; k < m; k++) {
+                const int i = sp_nmk(n, m, k);
+                a0[i] = -(m + k) / std::sqrt(anmd);
+                a1[i] = std::sqrt(((double)(n * (n + 1) - k * (k + 1))) / (n * (n + 1) - m * (m - 1)));
+            }
+        }
+    int nrot = 0;
+    for (int i = 0; i < kSpNrel; i++) {
+        // base-11 digits y, x, z (unmorton11)
+        const int by = i % 11, bx = (i / 11) % 11, bz = i / 121;
+        const int ix = bx - 5, iy = by - 5, iz = bz - 5;
+        if (sp_gcd(ix, sp_gcd(iy, iz)) > 1) {
+            T->link[i] = -1;
+            continue;
+        }
+        T->link[i] = nrot;
+        double rho, alpha, beta;
+        sp_cart2sph(rho, alpha, beta, ix, iy, iz);
+        // the host keeps the reference's long double phi + pi exactly
+        if (ix < 0) beta = (double)((long double)(float)std::atan((double)(float)((double)iy / ix)) +
+                                    3.14159265358979323846264338328L);
+        if (rho == 0) alpha = 0;
+        const double sc = std::sin(alpha) / (1 + std::cos(alpha));
+        double2 eb[4 * kSpPmax];
+        for (int n = 0; n < 4 * P - 3; n++) {
+            const double t = (n - 2 * P + 2) * beta;
+            eb[n] = make_double2(std::cos(t), std::sin(t));
+        }
+        const bool south = (rho == -(double)iz) && (rho > 0);
+        for (int n = 0; n < P; n++) {
+            D[sp_nmk(n, n, n)] = std::pow(std::cos(alpha * 0.5), 2 * n);
+            if (south && n > 0) D[sp_nmk(n, n, n)] = 0;
+            for (int k = n; k >= 1 - n; k--) {
+                const double ank = ((double)n + k) / (n - k + 1);
+                D[sp_nmk(n, n, k - 1)] = std::sqrt(ank) * std::tan(alpha * 0.5) * D[sp_nmk(n, n, k)];
+                if (south) D[sp_nmk(n, n, k - 1)] = (k == 1 - n) ? 1 : 0;
+            }
+            int count = 0;
+            for (int m = n; m >= 1; m--)
+                for (int k = m - 1; k >= 1 - m; k--) {
+                    const int a = sp_nmk(n, m, k);
+                    D[sp_nmk(n, m - 1, k)] = a1[a] * D[sp_nmk(n, m, k + 1)] + a0[a] * sc * D[a];
+                    if (south) {
+                        count++;
+                        D[sp_nmk(n, m - 1, k)] = (k == 1 - m) ? std::pow(-1.0, count) : 0;
+                    }
+                }
+        }
+        for (int n = 1; n < P; n++)
+            for (int m = 0; m <= n; m++) {
+                const int base = (4 * n * n * n + 6 * n * n + 5 * n) / 3;
+                for (int k = -m; k <= -1; k++) {
+                    const int a = sp_nmk(n, m, k);
+                    D[a] = std::pow(-1.0, k) * D[a];
+                    D[base - k * (2 * n + 1) - m] = std::pow(-1.0, m + k) * D[a];
+                }
+                for (int k = 0; k <= m; k++) {
+                    const int a = sp_nmk(n, m, k), b = sp_nmk(n, k, m);
+                    D[b] = std::pow(-1.0, m + k) * D[a];
+                    D[base - k * (2 * n + 1) - m] = D[b];
+                }
+            }
+        for (int n = 0; n < P; n++)
+            for (int m = 0; m <= n; m++)
+                for (int k = -n; k <= n; k++) {
+                    const double v = D[sp_nmk(n, m, k)];
+                    const double2 e = eb[k + m + 2 * P - 2];
+                    T->Dnm[nrot][m][n * (n + 1) + k] = make_double2(e.x * v, e.y * v);
+                }
+        nrot++;
+    }
+}
+
+const SphHostTables* sp_host_tables() {
+    static SphHostTables* T = nullptr;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        T = new SphHostTables();
+        sp_build_tables(T);
+    });
+    return T;
+}
+
+// ---------------------------------------------------------- device tables --
+
+struct SphTab {
+    const float* factorial;
+    const float* anm;
+    const float* Anm;
+    const double* Ynm;
+    const double2* Dnm;  // [rot][m][nk]
+    const int* link;
+};
+
+struct SphState {
+    void* buf = nullptr;
+    SphTab tab{};
+};
+
+// rotationMorton on the 11-grid (multipole_method_force_calculation.cpp:1610-1652).
+__device__ inline void sp_rot_index(const SphTab& t, int x, int y, int z, int& fwd, int& bwd) {
+    const int base = y + z * 121;
+    fwd = t.link[base + x * 11];
+    bwd = t.link[base + (10 - x) * 11];
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// MultipoleMethodForceCalculation::rotation (fmm_cpukernel.cpp:10-32).
+__device__ void sp_rotate(const SphTab& t, int p, const double2* C, double2* out, int rot) {
+    const double2* Dr = t.Dnm + (size_t)rot * kSpPmax * kSpNe2;
+    for (int n = 0; n < p; n++)
+        for (int m = 0; m <= n; m++) {
+            const double2* Dm = Dr + m * kSpNe2 + n * (n + 1);
+            double2 s = make_double2(0, 0);
+            for (int k = -n; k <= -1; k++) {
+                const double2 c = C[n * (n + 1) / 2 - k];
+                const double2 v = cmul(Dm[k], make_double2(c.x, -c.y));
+                s.x += v.x;
+                s.y += v.y;
+            }
+            for (int k = 0; k <= n; k++) {
+                const double2 v = cmul(Dm[k], C[n * (n + 1) / 2 + k]);
+                s.x += v.x;
+                s.y += v.y;
+            }
+            out[n * (n + 1) / 2 + m] = s;
+        }
+}
+
+// The z shifts of m2m_kernel (fmm_cpukernel.cpp:587-601), m2l_kernel (:878-897)
+// and l2l_kernel (:1100-1116), in their arithmetic order.
+__device__ void sp_shift_m2m(const SphTab& t, int p, const double2* B, double2* A) {
+    const double rho = sqrt(3.0) / 2;
+    for (int j = 0; j < p; j++) {
+        const double powTwo = pow(2.0, -j);
+        for (int k = 0; k <= j; k++) {
+            const int jk = j * j + j + k;
+            double2 s = make_double2(0, 0);
+            for (int n = 0; n <= j - k; n++) {
+                const int jnk = (j - n) * (j - n) + j - n + k;
+                const int nm = n * n + n;
+                const double c0 = pow(-1.0, n) * (double)t.anm[nm] * (double)t.anm[jnk] / (double)t.anm[jk] * pow(rho, n);
+                const double cr = t.Ynm[nm] * c0 * powTwo;
+                const double2 b = B[(j - n) * (j - n + 1) / 2 + k];
+                const double2 v = cmul(b, make_double2(cr, 0.0 * c0 * powTwo));
+                s.x += v.x;
+                s.y += v.y;
+            }
+            A[j * (j + 1) / 2 + k] = s;
+        }
+    }
+}
+
+__device__ void sp_shift_m2l(const SphTab& t, int p, double rho, const double2* A, double2* LA) {
+    double rhoj = 1;
+    for (int j = 0; j < p; j++) {
+        double rhojk = rhoj;
+        rhoj *= rho;
+        for (int k = 0; k <= j; k++) {
+            const int jk = j * j + j + k;
+            double2 s = make_double2(0, 0);
+            double rhojn = rhojk;
+            rhojk *= rho;
+            for (int n = k; n < p; n++) {
+                rhojn *= rho;
+                const int nk = n * n + n + k;
+                const int jnk = (j + n) * (j + n) + j + n;
+                const double c = (double)t.Anm[jk * kSpNe2 + nk] / rhojn;
+                const double2 v = cmul(A[n * (n + 1) / 2 + k], make_double2(t.Ynm[jnk] * c, 0.0 * c));
+                s.x += v.x;
+                s.y += v.y;
+            }
+            LA[j * (j + 1) / 2 + k] = s;
+        }
+    }
+}
+
+__device__ void sp_shift_l2l(const SphTab& t, int p, const double2* B, double2* A) {
+    const double rho = sqrt(3.0) / 2;
+    for (int j = 0; j < p; j++)
+        for (int k = 0; k <= j; k++) {
+            const int jk = j * j + j + k;
+            double2 s = make_double2(0, 0);
+            for (int n = j; n < p; n++) {
+                const int jnk = (n - j) * (n - j) + n - j;
+                const int nk = n * n + n + k;
+                const float f = t.anm[jnk] * t.anm[jk] / t.anm[nk];  // float arithmetic in the reference
+                const double c = (double)f * pow(rho, n - j);
+                const double pw = pow(2.0, -n - 1);
+                const double2 v = cmul(B[n * (n + 1) / 2 + k], make_double2(t.Ynm[jnk] * c * pw, 0.0 * c * pw));
+                s.x += v.x;
+                s.y += v.y;
+            }
+            A[j * (j + 1) / 2 + k] = s;
+        }
+}
+
+// One operator, applied to the unit vector of real component `col`.
+// kind 0: M2M of child octant a; 1: L2L into child octant a; 2: M2L of the
+// offset (a, b, c) = target - source at box size h.
+__device__ void sp_apply_op(const SphTab& t, int p, int kind, int a, int b, int c, double h, int col, double2* X,
+                            double2* Y) {
+    const int nc = p * (p + 1) / 2;
+    for (int i = 0; i < nc; i++) X[i] = make_double2(0, 0);
+    if (col & 1) X[col >> 1].y = 1.0;
+    else X[col >> 1].x = 1.0;
+    int fwd, bwd;
+    if (kind == 0) {
+        const int cx = (a >> 1) & 1, cy = a & 1, cz = (a >> 2) & 1;
+        sp_rot_index(t, 6 - 2 * cx, 6 - 2 * cy, 6 - 2 * cz, fwd, bwd);
+        sp_rotate(t, p, X, Y, fwd);
+        sp_shift_m2m(t, p, Y, X);
+        sp_rotate(t, p, X, Y, bwd);
+    } else if (kind == 1) {
+        const int cx = (a >> 1) & 1, cy = a & 1, cz = (a >> 2) & 1;
+        sp_rot_index(t, 2 * cx + 4, 2 * cy + 4, 2 * cz + 4, fwd, bwd);
+        sp_rotate(t, p, X, Y, fwd);
+        sp_shift_l2l(t, p, Y, X);
+        sp_rotate(t, p, X, Y, bwd);
+    } else {
+        const double fx = a, fy = b, fz = c;
+        const double rho = sqrt(fx * fx + fy * fy + fz * fz + kSoftening / (h * h));
+        int dx = a, dy = b, dz = c;
+        const int g = sp_gcd(dx, sp_gcd(dy, dz));
+        if (g == 3 || g == 5) {
+            dx = (dx > 0) - (dx < 0);
+            dy = (dy > 0) - (dy < 0);
+            dz = (dz > 0) - (dz < 0);
+        } else if (g > 1) {
+            dx /= g;
+            dy /= g;
+            dz /= g;
+        }
+        sp_rot_index(t, dx + 5, dy + 5, dz + 5, fwd, bwd);
+        sp_rotate(t, p, X, Y, fwd);
+        sp_shift_m2l(t, p, rho, Y, X);
+        sp_rotate(t, p, X, Y, bwd);
+    }
+}
+
+// ------------------------------------------------------------- tree keys --
+
+// Morton interleave in the reference's bit order (unmorton,
+// multipole_method_functions.cpp): y -> bit 3i, x -> 3i+1, z -> 3i+2.
+__host__ __device__ inline uint64_t sp_spread(uint32_t v) {
+    uint64_t x = v & 0x1fffffu;
+    x = (x | x << 32) & 0x1f00000000ffffULL;
+    x = (x | x << 16) & 0x1f0000ff0000ffULL;
+    x = (x | x << 8) & 0x100f00f00f00f00fULL;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+    x = (x | x << 2) & 0x1249249249249249ULL;
+    return x;
+}
+__host__ __device__ inline uint32_t sp_compact(uint64_t x) {
+    x &= 0x1249249249249249ULL;
+    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ULL;
+    x = (x ^ (x >> 4)) & 0x100f00f00f00f00fULL;
+    x = (x ^ (x >> 8)) & 0x1f0000ff0000ffULL;
+    x = (x ^ (x >> 16)) & 0x1f00000000ffffULL;
+    x = (x ^ (x >> 32)) & 0x1fffffULL;
+    return (uint32_t)x;
+}
+__host__ __device__ inline uint64_t sp_key(int x, int y, int z) {
+    return sp_spread((uint32_t)y) | (sp_spread((uint32_t)x) << 1) | (sp_spread((uint32_t)z) << 2);
+}
+__host__ __device__ inline void sp_unkey(uint64_t k, int& x, int& y, int& z) {
+    y = (int)sp_compact(k);
+    x = (int)sp_compact(k >> 1);
+    z = (int)sp_compact(k >> 2);
+}
+
+// index of key among the nb sorted keys, -1 if absent
+__device__ inline int sp_find(const uint64_t* __restrict__ keys, int nb, uint64_t key) {
+    int lo = 0, hi = nb;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < nb && keys[lo] == key) ? lo : -1;
+}
+
+// Geometry of the reference's grid: boxMin, rootBoxSize, the leaf box size
+// (float as morton() computes it, double as p2m / l2p do), 2^maxLevel and the
+// cells per axis; GL is the leaf level of the enclosing cube.
+struct SpGeom {
+    float bmin[3];
+    float root;
+    float bsf;
+    double bsd;
+    int side;
+    int bdv[3];
+    int GL;
+};
+
+__device__ inline int sp_f2i(float v) { return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)v : INT_MIN; }
+
+// Domain of a source (setDomainSize :714-737); false when outside the grid.
+__device__ inline bool sp_domain(const SpGeom& g, const float4& p, int nd[3]) {
+    const float c[3] = {p.x, p.y, p.z};
+    bool in = true;
+    for (int a = 0; a < 3; a++) {
+        const float t = (c[a] - g.bmin[a]) / g.root;
+        int v = sp_f2i(t);
+        if (t < 0) v--;
+        if (g.root == 0) v = 0;
+        nd[a] = v;
+        if (v < 0 || v >= g.bdv[a]) in = false;
+    }
+    return in;
+}
+
+// Centre of a leaf box as p2m / l2p form it: float corner of its domain plus
+// (local index + 0.5) leaf sizes in double, stored as float.
+__device__ inline void sp_leaf_centre(const SpGeom& g, uint64_t key, float cen[3]) {
+    int gc[3];
+    sp_unkey(key, gc[0], gc[1], gc[2]);
+    for (int a = 0; a < 3; a++) {
+        const int nd = gc[a] / g.side, nl = gc[a] % g.side;
+        const float dmin = g.bmin[a] + (float)nd * g.root;
+        cen[a] = (float)((double)dmin + (nl + 0.5) * g.bsd);
+    }
+}
+
+__global__ void k_sp_flags(const int* __restrict__ st, int i0, int m, char* __restrict__ f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) {
+        const int s = st[i0 + i];
+        f[i] = (s == 1 || s == 2) ? 1 : 0;
+    }
+}
+
+// Float bounds of the sources: per-block partials, then one block.
+__global__ void k_sp_bbox(const float4* __restrict__ P, const int* __restrict__ src, const int* __restrict__ nsrc,
+                          float* __restrict__ part) {
+    __shared__ float sh[6][256];
+    const int ns = *nsrc;
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
+        const float4 p = P[src[s]];
+        mn[0] = fminf(mn[0], p.x); mx[0] = fmaxf(mx[0], p.x);
+        mn[1] = fminf(mn[1], p.y); mx[1] = fmaxf(mx[1], p.y);
+        mn[2] = fminf(mn[2], p.z); mx[2] = fmaxf(mx[2], p.z);
+    }
+    for (int a = 0; a < 3; a++) {
+        sh[a][threadIdx.x] = mn[a];
+        sh[3 + a][threadIdx.x] = mx[a];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int a = 0; a < 3; a++) {
+                sh[a][threadIdx.x] = fminf(sh[a][threadIdx.x], sh[a][threadIdx.x + w]);
+                sh[3 + a][threadIdx.x] = fmaxf(sh[3 + a][threadIdx.x], sh[3 + a][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_sp_bbox_final(const float* __restrict__ part, int nparts, float* __restrict__ out6) {
+    const int a = threadIdx.x;
+    if (a >= 6) return;
+    float v = a < 3 ? INFINITY : -INFINITY;
+    for (int b = 0; b < nparts; b++) v = a < 3 ? fminf(v, part[b * 6 + a]) : fmaxf(v, part[b * 6 + a]);
+    out6[a] = v;
+}
+
+// Domain histogram (setOptimumLevel counts per domain) and sources outside the grid.
+__global__ void k_sp_domains(const float4* __restrict__ P, const int* __restrict__ src, int ns, SpGeom g,
+                             int* __restrict__ hist, int* __restrict__ ctr) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    int nd[3];
+    if (!sp_domain(g, P[src[s]], nd)) {
+        atomicAdd(ctr, 1);
+        return;
+    }
+    const int c = atomicAdd(&hist[(nd[0] * g.bdv[1] + nd[1]) * g.bdv[2] + nd[2]], 1) + 1;
+    atomicMax(ctr + 1, c);
+}
+
+// Leaf key of each source (morton :1014-1040 in global cell coordinates).
+__global__ void k_sp_keys(const float4* __restrict__ P, const int* __restrict__ src, int ns, SpGeom g,
+                          uint64_t* __restrict__ keys, int* __restrict__ vals) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int i = src[s];
+    const float4 p = P[i];
+    int nd[3];
+    sp_domain(g, p, nd);
+    const float c[3] = {p.x, p.y, p.z};
+    int gc[3];
+    for (int a = 0; a < 3; a++) {
+        const float dmin = g.bmin[a] + (float)nd[a] * g.root;
+        int nl = sp_f2i((c[a] - dmin) / g.bsf);
+        if (nl >= g.side) nl--;
+        gc[a] = nd[a] * g.side + nl;
+    }
+    keys[s] = sp_key(gc[0], gc[1], gc[2]);
+    vals[s] = i;
+}
+
+__global__ void k_sp_gather(const float4* __restrict__ P, const int* __restrict__ vals, int ns,
+                            const uint64_t* __restrict__ keys, float4* __restrict__ sp, int* __restrict__ flag) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= ns) return;
+    sp[a] = P[vals[a]];
+    flag[a] = (a > 0 && keys[a] != keys[a - 1]) ? 1 : 0;
+}
+
+__global__ void k_sp_shift(const uint64_t* __restrict__ in, int n, uint64_t* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) out[b] = in[b] >> 3;
+}
+
+// ------------------------------------------------------------- operators --
+
+// One dense operator per block, one real input component per thread (column).
+// ops[o] = (kind, a, b, c) as sp_apply_op; swz != 0 stores the MFMA A-fragment
+// order: element (row, col) at ((col/4) RT + row/16) 64 + (col%4) 16 + row%16.
+__global__ __launch_bounds__(128) void k_sp_build_ops(SphTab t, int p, int KP, const int4* __restrict__ ops,
+                                                      double h, int swz, double* __restrict__ T) {
+    const int o = blockIdx.x, col = threadIdx.x;
+    if (col >= KP) return;
+    const int4 d = ops[o];
+    double* To = T + (size_t)o * KP * KP;
+    const int nc2 = p * (p + 1);
+    const int RT = KP / 16;
+    double2 X[kSpNcMax], Y[kSpNcMax];
+    if (col < nc2 && d.x >= 0) sp_apply_op(t, p, d.x, d.y, d.z, d.w, h, col, X, Y);
+    for (int row = 0; row < KP; row++) {
+        double v = 0.0;
+        if (col < nc2 && row < nc2 && d.x >= 0) v = (row & 1) ? Y[row >> 1].y : Y[row >> 1].x;
+        const size_t at = swz ? ((size_t)((col >> 2) * RT + (row >> 4)) * 64 + (col & 3) * 16 + (row & 15))
+                              : (size_t)row * KP + col;
+        To[at] = v;
+    }
+}
+
+// P2M (p2m_kernel, fmm_cpukernel.cpp:281-384): one block per leaf box, lane m
+// accumulates the order-m coefficients of every degree over the box's
+// particles in key order.
+__global__ __launch_bounds__(64) void k_sp_p2m(SphTab t, int p, int KP, SpGeom g, const uint64_t* __restrict__ keys,
+                                               const int* __restrict__ first, const int* __restrict__ count,
+                                               const float4* __restrict__ sp, double* __restrict__ M) {
+    const int b = blockIdx.x, m = threadIdx.x;
+    double* Mb = M + (size_t)b * KP;
+    if (m == 0)
+        for (int c = p * (p + 1); c < KP; c++) Mb[c] = 0.0;
+    if (m >= p) return;
+    float cen[3];
+    sp_leaf_centre(g, keys[b], cen);
+    double2 acc[kSpPmax];
+    for (int n = 0; n < kSpPmax; n++) acc[n] = make_double2(0, 0);
+    for (int a = first[b]; a < first[b] + count[b]; a++) {
+        const float4 q = sp[a];
+        const float fx = q.x - cen[0], fy = q.y - cen[1], fz = q.z - cen[2];
+        double rho, alpha, beta;
+        sp_cart2sph(rho, alpha, beta, fx, fy, fz);
+        rho /= g.bsd;
+        const double xx = cos(alpha);
+        const double s2 = sqrt((1 - xx) * (1 + xx));
+        double fact = 1, pn = 1, rhom = 1.0;
+        for (int mm = 0; mm < m; mm++) {  // the reference's running P_m^m and rho^m
+            rhom *= rho;
+            pn = -pn * fact * s2;
+            fact += 2;
+        }
+        double pv = pn;
+        const double eb = -m * beta;
+        const double ce = cos(eb), se = sin(eb);
+        double Y = rhom * (double)t.factorial[m * m + 2 * m] * pv;
+        acc[m].x += (double)q.w * Y * ce;
+        acc[m].y += (double)q.w * Y * se;
+        double p1 = pv;
+        pv = xx * (2 * m + 1) * pv;
+        double rhon = rhom * rho;
+        for (int n = m + 1; n < p; n++) {
+            Y = rhon * (double)t.factorial[n * n + n + m] * pv;
+            acc[n].x += (double)q.w * Y * ce;
+            acc[n].y += (double)q.w * Y * se;
+            const double p2 = p1;
+            p1 = pv;
+            pv = (xx * (2 * n + 1) * p1 - (n + m) * p2) / (n - m + 1);
+            rhon *= rho;
+        }
+    }
+    for (int n = m; n < p; n++) {
+        Mb[2 * (n * (n + 1) / 2 + m)] = acc[n].x;
+        Mb[2 * (n * (n + 1) / 2 + m) + 1] = acc[n].y;
+    }
+}
+
+// M2M: each parent sums its children (key order) through the child-octant operators.
+__global__ __launch_bounds__(128) void k_sp_m2m(const double* __restrict__ Top, int KP, int nc2,
+                                                const uint64_t* __restrict__ ckeys, int ncb, const double* __restrict__ Mc,
+                                                const uint64_t* __restrict__ pkeys, double* __restrict__ Mp) {
+    const int pb = blockIdx.x, row = threadIdx.x;
+    if (row >= KP) return;
+    const uint64_t pk = pkeys[pb];
+    int lo = 0, hi = ncb;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((ckeys[mid] >> 3) < pk) lo = mid + 1;
+        else hi = mid;
+    }
+    double acc = 0.0;
+    if (row < nc2)
+        for (int c = lo; c < ncb && (ckeys[c] >> 3) == pk; c++) {
+            const double* T = Top + ((size_t)(ckeys[c] & 7u) * KP + row) * KP;
+            const double* m = Mc + (size_t)c * KP;
+            double s = 0.0;
+            for (int k = 0; k < nc2; k++) s += T[k] * m[k];
+            acc += s;
+        }
+    Mp[(size_t)pb * KP + row] = acc;
+}
+
+// L2L: each child takes its parent's local expansion through its octant's operator.
+__global__ __launch_bounds__(128) void k_sp_l2l(const double* __restrict__ Top, int KP, int nc2,
+                                                const uint64_t* __restrict__ pkeys, int npb, const double* __restrict__ Lp,
+                                                const uint64_t* __restrict__ ckeys, double* __restrict__ Lc) {
+    const int cb = blockIdx.x, row = threadIdx.x;
+    if (row >= KP) return;
+    const uint64_t ck = ckeys[cb];
+    const int pb = sp_find(pkeys, npb, ck >> 3);
+    double s = 0.0;
+    if (row < nc2 && pb >= 0) {
+        const double* T = Top + ((size_t)(ck & 7u) * KP + row) * KP;
+        const double* l = Lp + (size_t)pb * KP;
+        for (int k = 0; k < nc2; k++) s += T[k] * l[k];
+    }
+    Lc[(size_t)cb * KP + row] = s;
+}
+
+typedef double sp_d4 __attribute__((ext_vector_type(4)));
+
+// M2L of one level on the FP64 matrix cores.  Block = one wave = 16 target
+// boxes of one child octant (tiles[b] = octant, first, count); lane l serves
+// target l % 16 and k-slice l / 16 of every 16x16x4 product.  For each of the
+// octant's 189 offsets (offs[octant][o] = source - target, operator slot) the
+// wave gathers the 16 source expansions (zero where the source box is empty)
+// and accumulates T(slot) x M.  Fragments of v_mfma_f64_16x16x4f64 (measured
+// on gfx950): lane l holds A[l % 16][l / 16], B[l / 16][l % 16], and D rows
+// l / 16 + 4 r (r = 0..3) of column l % 16.  ctr[0] counts the box pairs that exist.
+template <int RT>
+__global__ __launch_bounds__(64) void k_sp_m2l(const double* __restrict__ T, const double* __restrict__ M,
+                                               double* __restrict__ L, const uint64_t* __restrict__ keys, int nb,
+                                               int level, const int* __restrict__ grp, const int4* __restrict__ tiles,
+                                               const int4* __restrict__ offs, unsigned long long* __restrict__ ctr) {
+    constexpr int KP = RT * 16, KS = KP / 4;
+    const int lane = threadIdx.x, j = lane & 15, kq = lane >> 4;
+    const int4 tl = tiles[blockIdx.x];
+    const int tb = j < tl.z ? grp[tl.y + j] : -1;
+    int tx = 0, ty = 0, tz = 0;
+    if (tb >= 0) sp_unkey(keys[tb], tx, ty, tz);
+    const int W = 1 << level;
+    sp_d4 acc[RT];
+    for (int r = 0; r < RT; r++) acc[r] = (sp_d4){0.0, 0.0, 0.0, 0.0};
+    const int4* od = offs + tl.x * kSpNoff;
+    unsigned long long pairs = 0;
+    for (int o = 0; o < kSpNoff; o++) {
+        const int4 d = od[o];
+        int sb = -1;
+        if (tb >= 0) {
+            const int sx = tx + d.x, sy = ty + d.y, sz = tz + d.z;
+            if (sx >= 0 && sy >= 0 && sz >= 0 && sx < W && sy < W && sz < W) sb = sp_find(keys, nb, sp_key(sx, sy, sz));
+        }
+        const unsigned long long have = __ballot(sb >= 0);
+        if (have == 0) continue;
+        pairs += __popcll(have & 0xffffull);
+        const double* Ms = M + (size_t)(sb >= 0 ? sb : 0) * KP;
+        const double* To = T + (size_t)d.w * KP * KP;
+        for (int s = 0; s < KS; s++) {
+            double bv = 0.0;
+            if (sb >= 0) bv = Ms[4 * s + kq];
+            for (int r = 0; r < RT; r++) {
+                const double av = To[(size_t)(s * RT + r) * 64 + lane];
+                acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[r], 0, 0, 0);
+            }
+        }
+    }
+    if (lane == 0 && pairs) atomicAdd(ctr, pairs);
+    if (tb >= 0) {
+        double* Lt = L + (size_t)tb * KP;
+        for (int r = 0; r < RT; r++)
+            for (int i = 0; i < 4; i++) Lt[16 * r + kq + 4 * i] += acc[r][i];  // D row kq + 4 i, column j
+    }
+}
+
+// L2P (l2p_kernel, fmm_cpukernel.cpp:1188-1317) and P2P (p2p_kernel :194-219 over
+// DirectForceFunction::direct, direct_force_method.cpp:153-190) per target in key
+// order: out[c][i - i0] = P2P - K q / m * L2P.  ctr[1] counts the P2P box pairs.
+__global__ __launch_bounds__(64) void k_sp_leaf(SphTab t, int p, int KP, SpGeom g, double kc,
+                                                const uint64_t* __restrict__ keys, int nb, const int* __restrict__ first,
+                                                const int* __restrict__ count, const int* __restrict__ sbox,
+                                                const float4* __restrict__ sp, const int* __restrict__ sorig, int ns,
+                                                const int* __restrict__ st, const int* __restrict__ spc,
+                                                const double* __restrict__ L, int i0, int m, double* __restrict__ out,
+                                                unsigned long long* __restrict__ ctr) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= ns) return;
+    const int i = sorig[a];
+    const int b = sbox[a];
+    int ix, iy, iz;
+    sp_unkey(keys[b], ix, iy, iz);
+    const int W = 1 << g.GL;
+    if (a == first[b]) {
+        int nnb = 0;
+        for (int jx = ix - 1; jx <= ix + 1; jx++)
+            for (int jy = iy - 1; jy <= iy + 1; jy++)
+                for (int jz = iz - 1; jz <= iz + 1; jz++)
+                    if (jx >= 0 && jy >= 0 && jz >= 0 && jx < W && jy < W && jz < W &&
+                        sp_find(keys, nb, sp_key(jx, jy, jz)) >= 0)
+                        nnb++;
+        atomicAdd(ctr + 1, (unsigned long long)nnb);
+    }
+    if (st[i] != 1) return;
+    const float4 pi = sp[a];
+    const double mass = species_mass(spc[i]);
+    const double ac = kc * (double)pi.w / mass;
+    double tot[3] = {0, 0, 0};
+    for (int jx = ix - 1; jx <= ix + 1; jx++)
+        for (int jy = iy - 1; jy <= iy + 1; jy++)
+            for (int jz = iz - 1; jz <= iz + 1; jz++) {
+                if (jx < 0 || jy < 0 || jz < 0 || jx >= W || jy >= W || jz >= W) continue;
+                const int sb = sp_find(keys, nb, sp_key(jx, jy, jz));
+                if (sb < 0) continue;
+                double ai[3] = {0, 0, 0};
+                for (int e = first[sb]; e < first[sb] + count[sb]; e++) {
+                    if (sorig[e] == i) continue;
+                    const float4 pj = sp[e];
+                    const double d0 = (double)(pi.x - pj.x), d1 = (double)(pi.y - pj.y), d2 = (double)(pi.z - pj.z);
+                    const double invD = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2 + kSoftening);
+                    const double invD3 = (double)pj.w * invD * invD * invD;
+                    ai[0] += d0 * invD3;
+                    ai[1] += d1 * invD3;
+                    ai[2] += d2 * invD3;
+                }
+                for (int c = 0; c < 3; c++) tot[c] += ac * ai[c];
+            }
+    // L2P
+    float cen[3];
+    sp_leaf_centre(g, keys[b], cen);
+    const float fx = pi.x - cen[0], fy = pi.y - cen[1], fz = pi.z - cen[2];
+    double r, theta, phi;
+    sp_cart2sph(r, theta, phi, fx, fy, fz);
+    const double xx = cos(theta), yy = sin(theta);
+    const double s2 = sqrt((1 - xx) * (1 + xx));
+    double Y[kSpNe2], Yt[kSpNe2];
+    double fact = 1, pn = 1;
+    for (int mm = 0; mm < p; mm++) {
+        double pv = pn;
+        int nm = mm * mm + 2 * mm;
+        Y[nm] = (double)t.factorial[nm] * pv;
+        double p1 = pv;
+        pv = xx * (2 * mm + 1) * pv;
+        Yt[nm] = (double)t.factorial[nm] * (pv - (mm + 1) * xx * p1) / yy;
+        for (int n = mm + 1; n < p; n++) {
+            nm = n * n + n + mm;
+            Y[nm] = (double)t.factorial[nm] * pv;
+            const double p2 = p1;
+            p1 = pv;
+            pv = (xx * (2 * n + 1) * p1 - (n + mm) * p2) / (n - mm + 1);
+            Yt[nm] = (double)t.factorial[nm] * ((n - mm + 1) * pv - (n + 1) * xx * p1) / yy;
+        }
+        pn = -pn * fact * s2;
+        fact += 2;
+    }
+    const double* Lb = L + (size_t)b * KP;
+    double aR = 0, aT = 0, aP = 0, rn = 1;
+    r /= g.bsd;
+    for (int n = 0; n < p; n++) {
+        int nm = n * n + n, nms = n * (n + 1) / 2;
+        const double rr = n * rn / r * Y[nm];
+        const double rt = rn * Yt[nm];
+        aR += rr * Lb[2 * nms];
+        aT += rt * Lb[2 * nms];
+        for (int mm = 1; mm <= n; mm++) {
+            nm = n * n + n + mm;
+            nms = n * (n + 1) / 2 + mm;
+            const double ce = cos(mm * phi), se = sin(mm * phi);
+            const double vr = n * rn / r * Y[nm], vt = rn * Yt[nm], vp = mm * rn * Y[nm];
+            const double lr = Lb[2 * nms], li = Lb[2 * nms + 1];
+            aR += 2 * (ce * vr * lr - se * vr * li);
+            aT += 2 * (ce * vt * lr - se * vt * li);
+            aP += 2 * ((-(se * vp)) * lr - (ce * vp) * li);
+        }
+        rn *= r;
+    }
+    const double h2 = g.bsd * g.bsd;
+    aR /= h2;
+    aT /= h2;
+    aP /= h2;
+    const double st_ = sin(theta), ct = cos(theta), sf = sin(phi), cf = cos(phi);
+    const double f[3] = {st_ * cf * aR + ct * cf / r * aT - sf / r / st_ * aP,
+                         st_ * sf * aR + ct * sf / r * aT + cf / r / st_ * aP, ct * aR - st_ / r * aT};
+    const int row = i - i0;
+    for (int c = 0; c < 3; c++) out[(size_t)c * m + row] = tot[c] - ac * f[c];
+}
+
+__global__ void k_sp_zero(double* __restrict__ a, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
+__global__ void k_sp_zero_i(int* __restrict__ a, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+__global__ void k_sp_octants(const uint64_t* __restrict__ keys, int nb, unsigned* __restrict__ okey,
+                             int* __restrict__ oval, int* __restrict__ ocnt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const unsigned o = (unsigned)(keys[b] & 7u);
+    okey[b] = o;
+    oval[b] = b;
+    atomicAdd(&ocnt[o], 1);
+}
+
+__global__ void k_sp_from_aos(const float4* __restrict__ vel, const float4* __restrict__ acc, int n,
+                              int* __restrict__ st, int* __restrict__ sp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        st[i] = (int)acc[i].w;
+        sp[i] = (int)vel[i].w;
+    }
+}
+
+__global__ void k_sp_interleave(const double* __restrict__ a, int m, double* __restrict__ out3) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) {
+        out3[3 * (size_t)i] = a[i];
+        out3[3 * (size_t)i + 1] = a[m + i];
+        out3[3 * (size_t)i + 2] = a[2 * (size_t)m + i];
+    }
+}
+
+inline int sp_bdiv(long a, int b) { return (int)((a + b - 1) / b); }
+
+// setOptimumLevel's level_switch (multipole_method_force_calculation.cpp:842-869).
+int sp_level_of_count(int c) {
+    static const float sw[8] = {200.0f, 1000.0f, 1.3e4f, 1e5f, 7e5f, 5e6f, 3e7f, 1.5e8f};
+    for (int i = 0; i < 8; i++)
+        if ((float)c < sw[i]) return i;
+    return 8;
+}
+
+int sp_state(shp_handle h, SphState** out) {
+    if (h->sph) {
+        *out = static_cast<SphState*>(h->sph);
+        return SHP_OK;
+    }
+    const SphHostTables* T = sp_host_tables();
+    SphState* S = new SphState();
+    const size_t bytes = sizeof(SphHostTables);
+    hipError_t e = hipMalloc(&S->buf, bytes);
+    if (e == hipSuccess) e = hipMemcpy(S->buf, T, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (S->buf) (void)hipFree(S->buf);
+        delete S;
+        set_error("spherical FMM tables: %s", hipGetErrorString(e));
+        return SHP_ERR_HIP;
+    }
+    const SphHostTables* D = static_cast<const SphHostTables*>(S->buf);
+    S->tab.factorial = D->factorial;
+    S->tab.anm = D->anm;
+    S->tab.Anm = D->Anm;
+    S->tab.Ynm = D->Ynm;
+    S->tab.Dnm = &D->Dnm[0][0][0];
+    S->tab.link = D->link;
+    h->sph = S;
+    *out = S;
+    return SHP_OK;
+}
+
+// The 189 offsets (source - target) of a target in child octant o, in the
+// reference's list order (parent neighbour, then child; x, y, z), each with the
+// operator slot of target - source.
+void sp_offsets(std::vector<int4>& offs) {
+    offs.clear();
+    for (int o = 0; o < 8; o++) {
+        const int c[3] = {(o >> 1) & 1, o & 1, (o >> 2) & 1};
+        for (int ex = -1; ex <= 1; ex++)
+            for (int ey = -1; ey <= 1; ey++)
+                for (int ez = -1; ez <= 1; ez++)
+                    for (int qx = 0; qx < 2; qx++)
+                        for (int qy = 0; qy < 2; qy++)
+                            for (int qz = 0; qz < 2; qz++) {
+                                const int dx = 2 * ex + qx - c[0], dy = 2 * ey + qy - c[1], dz = 2 * ez + qz - c[2];
+                                if (std::abs(dx) <= 1 && std::abs(dy) <= 1 && std::abs(dz) <= 1) continue;
+                                const int slot = (-dx + 3) * 49 + (-dy + 3) * 7 + (-dz + 3);
+                                offs.push_back(make_int4(dx, dy, dz, slot));
+                            }
+    }
+}
+
+template <int RT>
+void sp_launch_m2l(hipStream_t s, int ntiles, const double* T, const double* M, double* L, const uint64_t* keys,
+                   int nb, int level, const int* grp, const int4* tiles, const int4* offs,
+                   unsigned long long* ctr) {
+    hipLaunchKernelGGL(k_sp_m2l<RT>, dim3(ntiles), dim3(64), 0, s, T, M, L, keys, nb, level, grp, tiles, offs, ctr);
+}
+
+// Evaluates targets (status 1) in [i0, i1) from the sources (status 1 or 2) in
+// the same range; out[c * m + i - i0] for m = i1 - i0.
+int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i0, int i1, int p, int forced,
+             double* out) {
+    hipStream_t stream = h->stream;
+    const int m = i1 - i0;
+    h->sph_have_stats = false;
+    hipLaunchKernelGGL(k_sp_zero, dim3(sp_bdiv(3L * m, 256)), dim3(256), 0, stream, out, (size_t)3 * m);
+    if (m == 0) return SHP_OK;
+    SphState* S = nullptr;
+    int rc = sp_state(h, &S);
+    if (rc) return rc;
+    const int nc2 = p * (p + 1);
+    const int KP = (nc2 + 15) / 16 * 16;
+    const int RT = KP / 16;
+    // sources and their float bounds
+    SHP_WS(flags, char, h, WS_FLAGS, m);
+    SHP_WS(misc, int, h, WS_COUNTS, 64);
+    int* nsel = misc;                                       // [0]
+    int* dctr = misc + 2;                                   // [2] outside, [3] largest domain count
+    int* nruns = misc + 4;                                  // [4]
+    int* ocnt = misc + 8;                                   // [8..15]
+    float* box6 = (float*)(misc + 16);                      // [16..21]
+    unsigned long long* pctr = (unsigned long long*)(misc + 24);  // [24..27]: M2L pairs, P2P box pairs
+    hipLaunchKernelGGL(k_sp_flags, dim3(sp_bdiv(m, 256)), dim3(256), 0, stream, st, i0, m, flags);
+    hipLaunchKernelGGL(k_sp_zero_i, dim3(1), dim3(64), 0, stream, misc, (size_t)64);
+    // WS_SPH_KEYS: keys in/out (2 m u64), vals in/out (2 m int), selected sources (m int)
+    char* kb = static_cast<char*>(h->get(WS_SPH_KEYS, (size_t)m * (2 * sizeof(uint64_t) + 3 * sizeof(int))));
+    if (!kb) return SHP_ERR_NOMEM;
+    uint64_t* kin = (uint64_t*)kb;
+    uint64_t* kout = kin + m;
+    int* vin = (int*)(kout + m);
+    int* vout = vin + m;
+    int* sel = vout + m;
+    hipcub::CountingInputIterator<int> it(i0);
+    size_t tb = 0;
+    SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, flags, sel, nsel, m, stream));
+    SHP_WS(cub, char, h, WS_CUB, tb);
+    SHP_HIP_CHECK(hipcub::DeviceSelect::Flagged(cub, tb, it, flags, sel, nsel, m, stream));
+    const int nbp = std::min(sp_bdiv(m, 256), 512);
+    SHP_WS(part, float, h, WS_BBOX, 6 * (size_t)nbp);
+    hipLaunchKernelGGL(k_sp_bbox, dim3(nbp), dim3(256), 0, stream, P, sel, nsel, part);
+    hipLaunchKernelGGL(k_sp_bbox_final, dim3(1), dim3(64), 0, stream, part, nbp, box6);
+    int ns = 0;
+    float bb[6];
+    float4 seed4;
+    SHP_HIP_CHECK(hipMemcpyAsync(&ns, nsel, sizeof(int), hipMemcpyDeviceToHost, stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(bb, box6, sizeof(bb), hipMemcpyDeviceToHost, stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(&seed4, P + i0, sizeof(seed4), hipMemcpyDeviceToHost, stream));
+    SHP_HIP_CHECK(hipStreamSynchronize(stream));
+    if (ns == 0) return SHP_OK;
+    // the reference's domain grid (computeDomainBounds, host)
+    SpGeom g{};
+    {
+        const float seed[3] = {seed4.x, seed4.y, seed4.z};
+        float dims[3], minO[3], maxO[3];
+        domain_bounds_host(bb, bb + 3, seed, ns, 0, nullptr, g.bmin, dims, g.bdv, minO, maxO);
+        g.root = dims[0];
+    }
+    const long ndom = (long)g.bdv[0] * g.bdv[1] * g.bdv[2];
+    if (ndom < 1 || ndom > kSpMaxDomains) {
+        set_error("shp_fmm_sph: %ld domains (limit %d)", ndom, kSpMaxDomains);
+        return SHP_ERR_UNSUPPORTED;
+    }
+    SHP_WS(hist, int, h, WS_SPH_LVL, ndom);
+    hipLaunchKernelGGL(k_sp_zero_i, dim3(sp_bdiv(ndom, 256)), dim3(256), 0, stream, hist, (size_t)ndom);
+    hipLaunchKernelGGL(k_sp_domains, dim3(sp_bdiv(ns, 256)), dim3(256), 0, stream, P, sel, ns, g, hist, dctr);
+    int dres[2];
+    SHP_HIP_CHECK(hipMemcpyAsync(dres, dctr, sizeof(dres), hipMemcpyDeviceToHost, stream));
+    SHP_HIP_CHECK(hipStreamSynchronize(stream));
+    if (dres[0] > 0) {
+        set_error("shp_fmm_sph: %d sources outside the domain grid (the reference's outlier passes)", dres[0]);
+        return SHP_ERR_UNSUPPORTED;
+    }
+    const int maxLevel = forced != -2 ? forced : sp_level_of_count(dres[1]);
+    int bmax = std::max(g.bdv[0], std::max(g.bdv[1], g.bdv[2]));
+    int K = 0;
+    while ((1 << K) < bmax) K++;
+    const int GL = K + maxLevel;
+    if (maxLevel < 2 || GL > 21) {
+        set_error("shp_fmm_sph: maxLevel %d with %d levels above the domains (need 2 <= maxLevel, total <= 21)",
+                  maxLevel, K);
+        return SHP_ERR_UNSUPPORTED;
+    }
+    g.side = 1 << maxLevel;
+    g.bsf = g.root / (float)g.side;
+    g.bsd = (double)g.root / g.side;
+    g.GL = GL;
+    // keys, sort, leaf boxes
+    const dim3 gs(sp_bdiv(ns, 256)), bs(256);
+    hipLaunchKernelGGL(k_sp_keys, gs, bs, 0, stream, P, sel, ns, g, kin, vin);
+    size_t t1 = 0, t2 = 0, t3 = 0;
+    SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, kin, kout, vin, vout, ns, 0, 3 * GL, stream));
+    SHP_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, t2, kout, kin, vin, nruns, ns, stream));
+    SHP_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, t3, vin, vout, ns, stream));
+    SHP_WS(cub2, char, h, WS_CUB, std::max(t1, std::max(t2, t3)));
+    SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub2, t1, kin, kout, vin, vout, ns, 0, 3 * GL, stream));
+    // WS_SPH_PART: sorted sources (float4), leaf box of each, run flags/counts and starts
+    char* pb = static_cast<char*>(h->get(WS_SPH_PART, (size_t)ns * (sizeof(float4) + 3 * sizeof(int)) + 64));
+    if (!pb) return SHP_ERR_NOMEM;
+    float4* sp4 = (float4*)pb;
+    int* sbox = (int*)(sp4 + ns);
+    int* lcnt = sbox + ns;
+    int* lfirst = lcnt + ns;
+    hipLaunchKernelGGL(k_sp_gather, gs, bs, 0, stream, P, vout, ns, kout, sp4, lfirst);
+    SHP_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(cub2, t3, lfirst, sbox, ns, stream));
+    // leaf level: unique keys into the level store
+    std::vector<long> loff(GL + 2, 0);
+    std::vector<int> nbl(GL + 1, 0);
+    // level keys: at most ns per level
+    uint64_t* lk = static_cast<uint64_t*>(h->get(WS_SPH_LVL, sizeof(uint64_t) * (size_t)ns * (GL - 1) + 64));
+    if (!lk) return SHP_ERR_NOMEM;
+    SHP_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(cub2, t2, kout, lk, lcnt, nruns, ns, stream));
+    int nrun = 0;
+    SHP_HIP_CHECK(hipMemcpyAsync(&nrun, nruns, sizeof(int), hipMemcpyDeviceToHost, stream));
+    SHP_HIP_CHECK(hipStreamSynchronize(stream));
+    size_t t4 = 0;
+    SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t4, lcnt, lfirst, nrun, stream));
+    SHP_WS(cub3, char, h, WS_CUB, std::max(std::max(t1, t2), std::max(t3, t4)));
+    SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cub3, t4, lcnt, lfirst, nrun, stream));
+    nbl[GL] = nrun;
+    // level l occupies lk[loff[l] .. loff[l] + nbl[l]); leaf level first
+    loff[GL] = 0;
+    long used = nrun;
+    for (int l = GL - 1; l >= 2; l--) {
+        uint64_t* child = lk + loff[l + 1];
+        uint64_t* tmp = kin;  // free now
+        hipLaunchKernelGGL(k_sp_shift, dim3(sp_bdiv(nbl[l + 1], 256)), dim3(256), 0, stream, child, nbl[l + 1], tmp);
+        SHP_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(cub3, t2, tmp, lk + used, vin, nruns, nbl[l + 1], stream));
+        SHP_HIP_CHECK(hipMemcpyAsync(&nrun, nruns, sizeof(int), hipMemcpyDeviceToHost, stream));
+        SHP_HIP_CHECK(hipStreamSynchronize(stream));
+        loff[l] = used;
+        nbl[l] = nrun;
+        used += nrun;
+    }
+    // expansions: M and L of every level, KP doubles per box
+    std::vector<long> eoff(GL + 1, 0);
+    long etot = 0;
+    for (int l = 2; l <= GL; l++) {
+        eoff[l] = etot;
+        etot += (long)nbl[l] * KP;
+    }
+    double* E = static_cast<double*>(h->get(WS_SPH_EXP, sizeof(double) * 2 * (size_t)etot));
+    if (!E) return SHP_ERR_NOMEM;
+    double* Mx = E;
+    double* Lx = E + etot;
+    // operators: 8 M2M, 8 L2L, then the 343 M2L slots of one level
+    const int nops = 16 + kSpOps;
+    double* Tops = static_cast<double*>(h->get(WS_SPH_OPS, sizeof(double) * (size_t)nops * KP * KP));
+    if (!Tops) return SHP_ERR_NOMEM;
+    // descriptors and octant offsets (host, uploaded with the tiles)
+    std::vector<int4> desc(nops);
+    for (int o = 0; o < 8; o++) {
+        desc[o] = make_int4(0, o, 0, 0);
+        desc[8 + o] = make_int4(1, o, 0, 0);
+    }
+    for (int s = 0; s < kSpOps; s++) {
+        const int dx = s / 49 - 3, dy = (s / 7) % 7 - 3, dz = s % 7 - 3;
+        const bool far = std::abs(dx) > 1 || std::abs(dy) > 1 || std::abs(dz) > 1;
+        desc[16 + s] = far ? make_int4(2, dx, dy, dz) : make_int4(-1, 0, 0, 0);
+    }
+    std::vector<int4> offs;
+    sp_offsets(offs);
+    // tiles: at most nb / 16 + 8 per level
+    int maxnb = 0;
+    for (int l = 2; l <= GL; l++) maxnb = std::max(maxnb, nbl[l]);
+    const int maxtiles = maxnb / 16 + 8;
+    char* gb = static_cast<char*>(h->get(WS_SPH_GRP, sizeof(int4) * ((size_t)nops + offs.size() + maxtiles) +
+                                                          sizeof(int) * 4 * (size_t)maxnb + 256));
+    if (!gb) return SHP_ERR_NOMEM;
+    int4* ddesc = (int4*)gb;
+    int4* doffs = ddesc + nops;
+    int4* dtiles = doffs + offs.size();
+    unsigned* okey = (unsigned*)(dtiles + maxtiles);
+    unsigned* okey2 = okey + maxnb;
+    int* oval = (int*)(okey2 + maxnb);
+    int* grp = oval + maxnb;
+    SHP_HIP_CHECK(hipMemcpyAsync(ddesc, desc.data(), sizeof(int4) * nops, hipMemcpyHostToDevice, stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(doffs, offs.data(), sizeof(int4) * offs.size(), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_sp_build_ops, dim3(16), dim3(128), 0, stream, S->tab, p, KP, ddesc, 1.0, 0, Tops);
+    // P2M, M2M
+    hipLaunchKernelGGL(k_sp_p2m, dim3(nbl[GL]), dim3(64), 0, stream, S->tab, p, KP, g, lk + loff[GL], lfirst, lcnt,
+                       sp4, Mx + eoff[GL]);
+    for (int l = GL - 1; l >= 2; l--)
+        hipLaunchKernelGGL(k_sp_m2m, dim3(nbl[l]), dim3(128), 0, stream, Tops, KP, nc2, lk + loff[l + 1], nbl[l + 1],
+                           Mx + eoff[l + 1], lk + loff[l], Mx + eoff[l]);
+    // downward pass
+    double* Tm2l = Tops + (size_t)16 * KP * KP;
+    std::vector<int4> tiles;
+    for (int l = 2; l <= GL; l++) {
+        const int nb = nbl[l];
+        if (l == 2) {
+            hipLaunchKernelGGL(k_sp_zero, dim3(sp_bdiv((long)nb * KP, 256)), dim3(256), 0, stream, Lx + eoff[l],
+                               (size_t)nb * KP);
+        } else {
+            hipLaunchKernelGGL(k_sp_l2l, dim3(nb), dim3(128), 0, stream, Tops + (size_t)8 * KP * KP, KP, nc2,
+                               lk + loff[l - 1], nbl[l - 1], Lx + eoff[l - 1], lk + loff[l], Lx + eoff[l]);
+        }
+        const double hl = (double)g.root * std::ldexp(1.0, K - l);  // box size at level l
+        hipLaunchKernelGGL(k_sp_build_ops, dim3(kSpOps), dim3(128), 0, stream, S->tab, p, KP, ddesc + 16, hl, 1, Tm2l);
+        // boxes grouped by child octant (stable), tiles of 16
+        hipLaunchKernelGGL(k_sp_zero_i, dim3(1), dim3(64), 0, stream, ocnt, (size_t)8);
+        hipLaunchKernelGGL(k_sp_octants, dim3(sp_bdiv(nb, 256)), dim3(256), 0, stream, lk + loff[l], nb, okey, oval,
+                           ocnt);
+        size_t t5 = 0;
+        SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, t5, okey, okey2, oval, grp, nb, 0, 3, stream));
+        SHP_WS(cub4, char, h, WS_CUB, t5);
+        SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub4, t5, okey, okey2, oval, grp, nb, 0, 3, stream));
+        int oc[8];
+        SHP_HIP_CHECK(hipMemcpyAsync(oc, ocnt, sizeof(oc), hipMemcpyDeviceToHost, stream));
+        SHP_HIP_CHECK(hipStreamSynchronize(stream));
+        tiles.clear();
+        int start = 0;
+        for (int o = 0; o < 8; o++) {
+            for (int q = 0; q < oc[o]; q += 16) tiles.push_back(make_int4(o, start + q, std::min(16, oc[o] - q), 0));
+            start += oc[o];
+        }
+        if (tiles.empty()) continue;
+        SHP_HIP_CHECK(hipMemcpyAsync(dtiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, stream));
+        const int nt = (int)tiles.size();
+        const uint64_t* keys = lk + loff[l];
+        switch (RT) {
+        case 1: sp_launch_m2l<1>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 2: sp_launch_m2l<2>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 3: sp_launch_m2l<3>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 4: sp_launch_m2l<4>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 5: sp_launch_m2l<5>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 6: sp_launch_m2l<6>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        default: sp_launch_m2l<7>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        }
+        // the tile list is host memory reused by the next level: wait for the copy
+        SHP_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    // leaves: P2P + L2P
+    hipLaunchKernelGGL(k_sp_leaf, dim3(sp_bdiv(ns, 64)), dim3(64), 0, stream, S->tab, p, KP, g, k_const_host(),
+                       lk + loff[GL], nbl[GL], lfirst, lcnt, sbox, sp4, vout, ns, st, spc, Lx + eoff[GL], i0, m, out,
+                       pctr);
+    unsigned long long pc[2] = {0, 0};
+    SHP_HIP_CHECK(hipMemcpyAsync(pc, pctr, sizeof(pc), hipMemcpyDeviceToHost, stream));
+    SHP_HIP_CHECK(hipStreamSynchronize(stream));
+    long boxes = 0;
+    for (int l = 2; l <= GL; l++) boxes += nbl[l];
+    double* sts = h->sph_stats;
+    sts[0] = maxLevel;
+    sts[1] = K;
+    sts[2] = g.bdv[0];
+    sts[3] = g.bdv[1];
+    sts[4] = g.bdv[2];
+    sts[5] = nbl[GL];
+    sts[6] = (double)pc[0];
+    sts[7] = (double)pc[1];
+    sts[8] = ns;
+    sts[9] = (double)boxes;
+    sts[10] = 0;
+    sts[11] = 0;
+    h->sph_have_stats = true;
+    return SHP_OK;
+}
+
+}  // namespace
+
+void sph_release(shp_handle h) {
+    if (!h || !h->sph) return;
+    SphState* S = static_cast<SphState*>(h->sph);
+    if (S->buf) (void)hipFree(S->buf);
+    delete S;
+    h->sph = nullptr;
+}
+
+}  // namespace shp
+
+using namespace shp;
+
+extern "C" {
+
+int shp_fmm_sph_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4, int n, int i0, int i1,
+                     int order, int max_level, double* out3) {
+    SHP_REQUIRE(h && pos4 && vel4 && acc4 && out3, "shp_fmm_sph_host: null argument");
+    SHP_REQUIRE(n >= 0 && 0 <= i0 && i0 <= i1 && i1 <= n, "shp_fmm_sph_host: bad range n=%d [%d,%d)", n, i0, i1);
+    SHP_REQUIRE(order >= 1 && order <= kSpPmax, "shp_fmm_sph_host: order %d outside 1..%d", order, kSpPmax);
+    SHP_REQUIRE(max_level == -2 || max_level >= 0, "shp_fmm_sph_host: max_level %d (-2: from the counts)", max_level);
+    const int m = i1 - i0;
+    if (m == 0) return SHP_OK;
+    SHP_HIP_CHECK(hipSetDevice(h->device));
+    SHP_WS(dpos, float4, h, WS_HOST_IN, n);
+    SHP_WS(dvel, float4, h, WS_HOST_IN2, n);
+    SHP_WS(dacc, float4, h, WS_HOST_IN3, n);
+    SHP_WS(ints, int, h, WS_WIDE_I32, 2 * (size_t)n);
+    SHP_WS(dout, double, h, WS_HOST_OUT, 6 * (size_t)m);
+    SHP_HIP_CHECK(hipMemcpyAsync(dpos, pos4, sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(dvel, vel4, sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(dacc, acc4, sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_sp_from_aos, dim3(sp_bdiv(n, 256)), dim3(256), 0, h->stream, dvel, dacc, n, ints, ints + n);
+    int rc = sph_eval(h, dpos, ints, ints + n, i0, i1, order, max_level, dout);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sp_interleave, dim3(sp_bdiv(m, 256)), dim3(256), 0, h->stream, dout, m, dout + 3 * (size_t)m);
+    SHP_HIP_CHECK(hipMemcpyAsync(out3, dout + 3 * (size_t)m, sizeof(double) * 3 * (size_t)m, hipMemcpyDeviceToHost,
+                                 h->stream));
+    SHP_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return SHP_OK;
+}
+
+int shp_fmm_sph_stats(shp_handle h, double* out) {
+    SHP_REQUIRE(h && out, "shp_fmm_sph_stats: null argument");
+    SHP_REQUIRE(h->sph_have_stats, "shp_fmm_sph_stats: no spherical FMM evaluation on this handle yet");
+    std::memcpy(out, h->sph_stats, sizeof(h->sph_stats));
+    return SHP_OK;
+}
+
+}  // extern "C"

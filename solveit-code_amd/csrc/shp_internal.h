@@ -36,6 +36,15 @@ constexpr int kFmmStats = 11;
 
 void set_error(const char* fmt, ...);
 
+// DomainBoundsFunction::computeDomainBounds (domain_bounds_functions.cpp:121-316)
+// without outliers, from the sources' float bounds (mn, mx), the first ion of
+// the range (seed) and the number of sources: the first root cell's corner,
+// the cell dimensions and the cells per axis (bh.hip).  toggle 0 cubic cells,
+// 1 rectangular prism; forced: NULL or setBoxDimVals' counts.
+void domain_bounds_host(const float mn_src[3], const float mx_src[3], const float seed[3], int nsrc,
+                        int toggle, const int* forced, float boxMin[3], float dims[3], int bdv[3],
+                        float minO[3], float maxO[3]);
+
 // Leapfrog building blocks shared with the batched driver (leapfrog.hip).
 int lf_begin(shp_handle h, const shp_state* s, const shp_step_opts* o, double t0, double dt, int* eff);
 int lf_kick_drift(shp_handle h, const shp_state* s, const shp_step_opts* o, double t, const double* tdev,
@@ -43,6 +52,7 @@ int lf_kick_drift(shp_handle h, const shp_state* s, const shp_step_opts* o, doub
 void free_mesh(shp_handle h);
 int direct_scratch_setup(int device);
 void fmm_release(shp_handle h);
+void sph_release(shp_handle h);  // tables of the spherical-harmonic FMM (fmm_sph.hip)
 
 // Writers of the resident state (writers.hip).  One boundary-condition event
 // is kEventWords 32-bit words in the layout of writeToEventsFile
@@ -105,6 +115,12 @@ enum WsSlot {
     WS_FMM_PTS,     // FMM point targets of the M2P pass (treecode: every target)
     WS_FMM_OUTL,    // FMM outlier detection: sort keys, flags, the outlier list
     WS_CMP,         // accuracy metric: per-block partials
+    WS_SPH_KEYS,    // spherical FMM: leaf keys and sources (in, out), domain histogram
+    WS_SPH_LVL,     // spherical FMM: box keys of all levels, run counts and offsets
+    WS_SPH_PART,    // spherical FMM: sources in key order and their leaf boxes
+    WS_SPH_EXP,     // spherical FMM: multipole and local expansions of all levels
+    WS_SPH_OPS,     // spherical FMM: dense translation operators
+    WS_SPH_GRP,     // spherical FMM: boxes grouped by octant, M2L tiles
     WS_NSLOTS
 };
 
@@ -189,6 +205,11 @@ struct shp_context {
     bool res_events_on = false;
     long long drift_seq = 0;
     shp::StateWriter* writer = nullptr;
+    // Spherical-harmonic FMM (shp_fmm_sph_host): device tables, built on first
+    // use, and the last call's summary (shp_fmm_sph_stats).
+    void* sph = nullptr;
+    double sph_stats[12] = {};
+    bool sph_have_stats = false;
     void* ws[shp::WS_NSLOTS] = {};
     size_t ws_bytes[shp::WS_NSLOTS] = {};
     int num_cus = 256;

@@ -234,6 +234,8 @@ def _load():
         "shp_direct_dev_noalloc": (c_int, [c_void_p, ctypes.POINTER(Particles), c_int,
                                            ctypes.POINTER(Symmetry), c_void_p, c_void_p, c_void_p]),
         "shp_bh_stats": (c_int, [c_void_p, dp]),
+        "shp_fmm_sph_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, c_int, c_int, dp]),
+        "shp_fmm_sph_stats": (c_int, [c_void_p, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -469,6 +471,33 @@ class Engine:
                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         _check(rc, "shp_fmm_host")
         return out
+
+    def fmm_sph_host(self, pos4, vel4, acc4, i0=None, i1=None, order=10, max_level=-2):
+        """The reference FMM's own operators (rotation-based spherical harmonics,
+        FP64) on host AoS float32 arrays; returns the (i1-i0, 3) float64
+        acceleration.  max_level -2: the reference's setOptimumLevel choice."""
+        pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+        vel4 = np.ascontiguousarray(vel4, dtype=np.float32)
+        acc4 = np.ascontiguousarray(acc4, dtype=np.float32)
+        n = pos4.shape[0]
+        i0 = 0 if i0 is None else i0
+        i1 = n if i1 is None else i1
+        out = np.zeros((max(i1 - i0, 0), 3), dtype=np.float64)
+        fp = ctypes.POINTER(ctypes.c_float)
+        rc = lib.shp_fmm_sph_host(self.h, pos4.ctypes.data_as(fp), vel4.ctypes.data_as(fp),
+                                  acc4.ctypes.data_as(fp), n, i0, i1, int(order), int(max_level),
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        _check(rc, "shp_fmm_sph_host")
+        return out
+
+    def fmm_sph_stats(self):
+        """Summary of the last fmm_sph_host call."""
+        out = np.zeros(12)
+        _check(lib.shp_fmm_sph_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+               "shp_fmm_sph_stats")
+        keys = ["max_level", "k", "nd0", "nd1", "nd2", "leaves", "m2l_pairs", "p2p_pairs", "sources",
+                "boxes"]
+        return {k: int(out[i]) for i, k in enumerate(keys)}
 
     def state_upload_host(self, pos4, vel4, acc4, t_inj):
         """Copies the reference's host arrays into the engine's resident FP64 state."""

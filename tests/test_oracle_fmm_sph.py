@@ -1,0 +1,192 @@
+"""Pins of the oracle's restatement of the reference FMM's own operators
+(oracle/fmm_sph.c: MultipoleMethodConstants::precalc, rotation, p2m / m2m /
+m2l / l2l / l2p kernels of fmm_cpukernel.cpp) by what the mathematics fixes:
+closed-form tables, rotations that undo each other, translations that are
+exact (M2M, L2L), convergence to the exact Coulomb field, and the tree rules
+of setDomainSize / setOptimumLevel on hand cases."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+
+def _cube(n, side, seed, zscale=1.0):
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(0, side, (n, 3))
+    pos[:, 2] *= zscale
+    pos[:, 3] = np.where(rng.random(n) < 0.05, -1.0, 1.0)
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, 3] = rng.integers(0, 2, n)  # monomer / dimer
+    acc = np.zeros((n, 4), np.float32)
+    acc[:, 3] = 1.0
+    return pos, vel, acc
+
+
+def test_tables_closed_forms():
+    """factorial = sqrt((n-|m|)!/(n+|m|)!), anm = (-1)^n / sqrt((n-m)!(n+m)!)
+    (both stored in float), and Ynm on the z axis: 1 for m = 0, 0 otherwise."""
+    for n in range(20):
+        for m in range(-n, n + 1):
+            nm = n * n + n + m
+            f = math.sqrt(math.factorial(n - abs(m)) / math.factorial(n + abs(m)))
+            a = (-1) ** n / math.sqrt(math.factorial(n - m) * math.factorial(n + m))
+            assert oracle.sph_table("factorial", nm) == float(np.float32(f))
+            assert oracle.sph_table("anm", nm) == float(np.float32(a))
+            assert oracle.sph_table("Ynm", nm) == pytest.approx(1.0 if m == 0 else 0.0, abs=1e-7)
+
+
+def test_Anm_from_anm():
+    """Anm[j,k; n,k] = (-1)^(j+k) anm[n,k] anm[j,k] / anm[j+n,0] (float)."""
+    for j in range(10):
+        for k in range(-j, j + 1):
+            for n in range(abs(k), 10):
+                jk, nk, jn = j * j + j + k, n * n + n + k, (j + n) * (j + n) + j + n
+                want = (-1) ** (j + k) * oracle.sph_table("anm", nk) * oracle.sph_table("anm", jk) / \
+                    oracle.sph_table("anm", jn)
+                assert oracle.sph_table("Anm", jk * 100 + nk) == float(np.float32(want))
+
+
+def _multipole(p, seed, h=1e-8):
+    rng = np.random.default_rng(seed)
+    xyzq = np.c_[rng.uniform(-0.5, 0.5, (20, 3)) * h, rng.choice([-1.0, 1.0], 20)]
+    return oracle.sph_p2m(p, xyzq, h)
+
+
+@pytest.mark.parametrize("d", [(1, 0, 0), (1, 1, 1), (2, -1, 3), (-3, 1, 0), (1, -1, -1), (0, 1, 2),
+                               (-5, 4, 3)])
+def test_rotation_round_trip(d):
+    """The backward rotation undoes the forward one on a multipole of real charges."""
+    p = 10
+    C = _multipole(p, 1)
+    R = oracle.sph_rotate(p, d, C)
+    B = oracle.sph_rotate(p, d, R, backward=True)
+    assert np.max(np.abs(B - C)) <= 1e-14 * np.max(np.abs(C))
+    assert np.max(np.abs(R - C)) > 1e-3 * np.max(np.abs(C))  # the rotation does something
+
+
+def test_rotation_preserves_degree_norms():
+    """A rotation mixes orders within a degree only: sum over m of |C_n^m|^2
+    with weights 1 (m = 0) and 2 (m > 0) is invariant, up to the normalisation
+    factor of the reference's expansions, which is itself m-independent."""
+    p = 10
+    C = _multipole(p, 2)
+    R = oracle.sph_rotate(p, (2, -1, 3), C)
+
+    def norms(V):
+        out = []
+        for n in range(p):
+            s = abs(V[n * (n + 1) // 2]) ** 2
+            s += 2 * sum(abs(V[n * (n + 1) // 2 + m]) ** 2 for m in range(1, n + 1))
+            out.append(s)
+        return np.array(out)
+
+    np.testing.assert_allclose(norms(R), norms(C), rtol=1e-12)
+
+
+@pytest.mark.parametrize("octant", range(8))
+def test_m2m_and_l2l_are_exact(octant):
+    """M2M of a child's multipole equals the parent's multipole of the same
+    charges, and L2L re-expands a local expansion without loss, up to the
+    float storage of the tables and the float angles (below 5e-7)."""
+    p, H = 10, 1e-6
+    rng = np.random.default_rng(octant)
+    cx, cy, cz = (octant >> 1) & 1, octant & 1, (octant >> 2) & 1
+    cc = np.array([cx - 0.5, cy - 0.5, cz - 0.5]) * H / 2
+    pts = rng.uniform(-0.5, 0.5, (10, 3)) * H / 2
+    q = rng.choice([-1.0, 1.0], 10)
+    Mp = oracle.sph_m2m(p, octant, oracle.sph_p2m(p, np.c_[pts, q], H / 2))
+    Md = oracle.sph_p2m(p, np.c_[pts + cc, q], H)
+    assert np.max(np.abs(Mp - Md)) <= 5e-7 * np.max(np.abs(Md))
+    M = oracle.sph_p2m(p, [0.1 * H, 0, 0, 1.0], H)
+    Lp = oracle.sph_m2l(p, (3, 0, 1), H, M)
+    Lc = oracle.sph_l2l(p, octant, Lp)
+    x = rng.uniform(-0.5, 0.5, 3) * H / 2
+    f1 = oracle.sph_l2p(p, Lc, x, H / 2)
+    f2 = oracle.sph_l2p(p, Lp, x + cc, H)
+    assert np.linalg.norm(f1 - f2) <= 5e-7 * np.linalg.norm(f2)
+
+
+def test_single_source_converges_to_coulomb():
+    """P2M -> M2L -> L2P of one charge gives -q (r_t - r_s)/|r|^3 (the field the
+    reference's L2P subtracts), converging geometrically in p."""
+    h = 1e-6
+    rng = np.random.default_rng(2)
+    worst = {}
+    for p in (4, 6, 8, 10):
+        errs = []
+        for t in range(12):
+            src = rng.uniform(-0.5, 0.5, 3) * h
+            d = np.array([3, 1, -2]) if t % 2 else np.array([2, 2, 2])
+            off = rng.uniform(-0.5, 0.5, 3) * h
+            L = oracle.sph_m2l(p, d, h, oracle.sph_p2m(p, [*src, 1.0], h))
+            f = oracle.sph_l2p(p, L, off, h)
+            r = (d * h + off) - src
+            exact = r / np.linalg.norm(r) ** 3
+            errs.append(np.linalg.norm(-f - exact) / np.linalg.norm(exact))
+        worst[p] = max(errs)
+    assert worst[4] > worst[6] > worst[8] > worst[10]
+    assert worst[10] < 1e-5 and worst[4] < 5e-2
+
+
+def test_fmm_converges_to_direct_sum():
+    """On a 10 um cube (leaf boxes far larger than the softening length) the
+    whole FMM converges to the direct sum as p grows."""
+    pos, vel, acc = _cube(6000, 1e-5, 4)
+    ref = oracle.direct_ref(pos, vel, acc)
+    errs = []
+    for p in (2, 4, 6, 8, 10):
+        a, info = oracle.fmm_sph(pos, vel, acc, p=p, max_level=3)
+        assert (info["nd0"], info["nd1"], info["nd2"], info["k"]) == (1, 1, 1, 0)
+        errs.append(oracle.rel_l2(a, ref)[0])
+    assert all(e1 > e2 for e1, e2 in zip(errs, errs[1:])), errs
+    assert errs[-1] < 3e-5
+
+
+def test_softening_floor_of_the_reference_expansion():
+    """The reference expands plain 1/r and softens only the M2L centre distance,
+    so on a dense 100 nm cube (12.5 nm leaves against a 0.8 nm softening
+    length) the error stops falling with p (SURVEY F9's hemisphere floor)."""
+    pos, vel, acc = _cube(8000, 1e-7, 5)
+    ref = oracle.direct_ref(pos, vel, acc)
+    e8 = oracle.rel_l2(oracle.fmm_sph(pos, vel, acc, p=8)[0], ref)[0]
+    e10 = oracle.rel_l2(oracle.fmm_sph(pos, vel, acc, p=10)[0], ref)[0]
+    assert e10 > 1e-4 and e10 > 0.7 * e8
+
+
+def test_level_switch_and_domain_grid():
+    """setOptimumLevel: 6000 sources in one domain -> maxLevel 2, 20000 -> 3;
+    a 4:4:1 slab -> a 4 x 4 x 1 domain grid, two octree levels above it."""
+    for n, want in ((6000, 2), (20000, 3)):
+        pos, vel, acc = _cube(n, 1e-5, 6)
+        _, info = oracle.fmm_sph(pos, vel, acc, p=2)
+        assert info["max_level"] == want and info["sources"] == n
+        assert info["leaves"] <= 8 ** want
+    pos, vel, acc = _cube(6000, 1e-5, 7, zscale=0.25)
+    bm, dims, nd, ns = oracle.domain_bounds(pos, acc)
+    assert tuple(nd) == (4, 4, 1) and ns == 6000
+    _, info = oracle.fmm_sph(pos, vel, acc, p=4, max_level=2)
+    assert (info["nd0"], info["nd1"], info["nd2"], info["k"]) == (4, 4, 1, 2)
+
+
+def test_flattened_multi_domain_converges():
+    pos, vel, acc = _cube(6000, 1e-5, 7, zscale=0.25)
+    ref = oracle.direct_ref(pos, vel, acc)
+    e = [oracle.rel_l2(oracle.fmm_sph(pos, vel, acc, p=p, max_level=2)[0], ref)[0] for p in (4, 8, 10)]
+    assert e[0] > e[1] > e[2] and e[2] < 5e-5
+
+
+def test_status_rules_and_range():
+    """Targets are status 1 in [i0, i1); status 2 ions are sources only; status 0
+    ions and ions outside the range take no part."""
+    pos, vel, acc = _cube(3000, 1e-5, 8)
+    acc[::7, 3] = 2.0
+    acc[::11, 3] = 0.0
+    i0, i1 = 100, 2900
+    a, _ = oracle.fmm_sph(pos, vel, acc, p=10, max_level=2, i0=i0, i1=i1)
+    ref = oracle.direct_ref(pos, vel, acc, i0=i0, i1=i1, j0=i0, j1=i1)
+    tgt = acc[i0:i1, 3] == 1.0
+    assert np.all(a[~tgt] == 0.0)
+    assert oracle.rel_l2(a[tgt], ref[tgt])[0] < 5e-5
