@@ -269,9 +269,11 @@ def run_config(k, args):
             log(json.dumps(row))
         res.update(workload="cone plume, FP32C direct vs GPU FP64 direct", runs=out)
     elif k == 3:
-        # N=1M cone: Barnes-Hut theta 0.3/0.5/0.7 and FMM p=4/6/8/10 (leaf
-        # size 64): ms per force evaluation and rel L2 vs FP64 direct on the
-        # seeded 8192-target subsample.
+        # N=1M cone: Barnes-Hut theta 0.3/0.5/0.7, the reference FMM's operators
+        # and the engine's FMM at p=4/6/8/10 (leaf size 64): ms per force
+        # evaluation and rel L2 vs FP64 direct on the seeded 8192-target
+        # subsample; the engine's FMM also against the reference operators at
+        # the same order.
         n = 1048576
         d = synth.cone(n)
         st = P.DeviceState.from_numpy(d)
@@ -291,10 +293,36 @@ def run_config(k, args):
                          "interactions_per_s": inter / (walk * 1e-3), "node_visits_per_s": visits / (walk * 1e-3),
                          **_metric(eng, st, a, a64, sub)})
             log(json.dumps(rows[-1]))
+        # The reference FMM's own operators (spherical harmonics, rotation-based
+        # translations, FP64, M2L on the matrix cores; shp_fmm_sph_host) on the
+        # plume's float AoS with the reference's level choice: ms per call with
+        # the host arrays in and out, rel L2 against the FP64 direct sum of the
+        # same float inputs (the reference's direct contract, shp_direct_host).
+        pos4, vel4, acc4 = synth.to_reference_aos(d)
+        ms_ref, a_ref = _timed(lambda: eng.direct_host(pos4, vel4, acc4), reps=1)
+        subn = sub.cpu().numpy()
+        tgt = acc4[subn, 3] == 1.0
+
+        def rel_l2_np(a, b):
+            e = np.linalg.norm(a - b, axis=1) ** 2 / np.linalg.norm(b, axis=1) ** 2
+            return float(np.sqrt(np.mean(e))), float(np.sqrt(np.max(e)))
+
+        sph = {}
+        for p in (4, 6, 8, 10):
+            ms, a = _timed(lambda: eng.fmm_sph_host(pos4, vel4, acc4, order=p))
+            sph[p] = a
+            l2, mx = rel_l2_np(a[subn][tgt], a_ref[subn][tgt])
+            rows.append({"method": "fmm_reference_operators", "order": p, "ms_host_in_out": ms,
+                         **eng.fmm_sph_stats(), "rel_l2": l2, "max_rel": mx,
+                         "vs": "FP64 direct of the same float inputs"})
+            log(json.dumps(rows[-1]))
+        rows[0]["ref_contract_direct_ms"] = ms_ref
         for p in (4, 6, 8, 10):
             ms, a = _timed(lambda: eng.fmm_dev(st, order=p, leaf_size=64, ring=2))
+            at = torch.stack(a, 1)[sub].cpu().numpy()
             rows.append({"method": "fmm", "order": p, "ms": ms, "leaf_level": eng.fmm_leaf_level(),
-                         **_metric(eng, st, a, a64, sub)})
+                         **_metric(eng, st, a, a64, sub),
+                         "rel_l2_vs_reference_operators_same_order": rel_l2_np(at[tgt], sph[p][subn][tgt])[0]})
             log(json.dumps(rows[-1]))
         # CPU legs (one thread, bounded): the oracle's Barnes-Hut (reference
         # arithmetic) on a whole 65536-ion cone of the same shape, with the GPU

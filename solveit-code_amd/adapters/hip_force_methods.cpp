@@ -169,10 +169,19 @@ void HipMultipoleMethodForceCalculation::updateAcceleration() {
     ChargedParticles& cp = particleSystem->chargedParticles;
     const shp_symmetry sym = symmetryOf(*particleSystem, grid, nsec, lattice);
     out.assign(3 * (size_t)(lastParticle - firstParticle), 0.0);
-    if (lastParticle > firstParticle &&
-        shp_fmm_host(handle, &cp.position[0].x, &cp.velocity[0].x, &cp.acceleration[0].x, lastParticle,
-                     firstParticle, lastParticle, order, leafLevel, leafSize, ring, &sym, out.data()) != SHP_OK)
-        fail("shp_fmm_host");
+    if (lastParticle > firstParticle) {
+        if (referenceOperators) {
+            if (sym.type != SHP_SYM_NONE) fail("the reference operators are evaluated without symmetry");
+            if (shp_fmm_sph_host(handle, &cp.position[0].x, &cp.velocity[0].x, &cp.acceleration[0].x,
+                                 lastParticle, firstParticle, lastParticle, order, forcedMaxLevel,
+                                 out.data()) != SHP_OK)
+                fail("shp_fmm_sph_host");
+        } else if (shp_fmm_host(handle, &cp.position[0].x, &cp.velocity[0].x, &cp.acceleration[0].x,
+                                lastParticle, firstParticle, lastParticle, order, leafLevel, leafSize, ring,
+                                &sym, out.data()) != SHP_OK) {
+            fail("shp_fmm_host");
+        }
+    }
     addInto(cp, firstParticle, lastParticle, out);
 }
 

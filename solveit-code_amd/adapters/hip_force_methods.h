@@ -109,11 +109,20 @@ public:
     void setTimerName() override;
 
     void setNumExpansions(int p) { order = p; }
-    void setMaxLevel(int maxLevel) { leafLevel = maxLevel; }
+    void setMaxLevel(int maxLevel) {
+        leafLevel = maxLevel;
+        forcedMaxLevel = maxLevel;
+    }
     void setNumSectors(int n) { nsec = n; }
     // The reference's FmmType (its constructor's fType, structures.h:283-288):
     // treecode or FMM, with or without outliers held out of the tree.
     void setFmmType(FmmType fType);
+    // true: evaluate with the reference's own operators (spherical harmonics,
+    // rotation-based translations, FP64; shp_fmm_sph_host) on the reference's
+    // tree, with setMaxLevel as the reference's forcedMaxLevel; false (the
+    // default): the engine's Cartesian expansion of the softened kernel.
+    // Without symmetry only.
+    void setReferenceOperators(bool on) { referenceOperators = on; }
 
 protected:
     void allocateThreadMemory() override {}
@@ -122,6 +131,8 @@ protected:
 private:
     shp_handle handle = nullptr;
     int order, ring, leafSize, leafLevel = 0, nsec = 4;
+    int forcedMaxLevel = -2;
+    bool referenceOperators = false;
     std::vector<double> out;
     std::vector<int> lattice;
 };
