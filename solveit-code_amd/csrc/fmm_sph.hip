@@ -524,17 +524,39 @@ __global__ void k_sp_bbox_final(const float* __restrict__ part, int nparts, floa
 }
 
 // Domain histogram (setOptimumLevel counts per domain) and sources outside the grid.
-__global__ void k_sp_domains(const float4* __restrict__ P, const int* __restrict__ src, int ns, SpGeom g,
-                             int* __restrict__ hist, int* __restrict__ ctr) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns) return;
-    int nd[3];
-    if (!sp_domain(g, P[src[s]], nd)) {
-        atomicAdd(ctr, 1);
-        return;
+constexpr int kSpHistLds = 4096;  // domain grids up to this size are counted in LDS first
+
+// Domain histogram (setOptimumLevel counts per domain) and sources outside the
+// grid (ctr[0]); a block counts into LDS when the grid is small, so a grid of a
+// few domains does not serialise a million global atomics.
+__global__ __launch_bounds__(256) void k_sp_domains(const float4* __restrict__ P, const int* __restrict__ src, int ns,
+                                                    SpGeom g, int ndom, int* __restrict__ hist, int* __restrict__ ctr) {
+    __shared__ int lh[kSpHistLds];
+    const bool local = ndom <= kSpHistLds;
+    if (local)
+        for (int d = threadIdx.x; d < ndom; d += blockDim.x) lh[d] = 0;
+    __syncthreads();
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
+        int nd[3];
+        if (!sp_domain(g, P[src[s]], nd)) {
+            atomicAdd(ctr, 1);
+            continue;
+        }
+        const int d = (nd[0] * g.bdv[1] + nd[1]) * g.bdv[2] + nd[2];
+        if (local) atomicAdd(&lh[d], 1);
+        else atomicAdd(&hist[d], 1);
     }
-    const int c = atomicAdd(&hist[(nd[0] * g.bdv[1] + nd[1]) * g.bdv[2] + nd[2]], 1) + 1;
-    atomicMax(ctr + 1, c);
+    __syncthreads();
+    if (local)
+        for (int d = threadIdx.x; d < ndom; d += blockDim.x)
+            if (lh[d]) atomicAdd(&hist[d], lh[d]);
+}
+
+// ctr[1] = the largest domain count.
+__global__ void k_sp_hist_max(const int* __restrict__ hist, int ndom, int* __restrict__ ctr) {
+    int m = 0;
+    for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < ndom; d += gridDim.x * blockDim.x) m = max(m, hist[d]);
+    if (m) atomicMax(ctr + 1, m);
 }
 
 // Leaf key of each source (morton :1014-1040 in global cell coordinates).
@@ -595,58 +617,74 @@ __global__ __launch_bounds__(128) void k_sp_build_ops(SphTab t, int p, int KP, c
     }
 }
 
-// P2M (p2m_kernel, fmm_cpukernel.cpp:281-384): one block per leaf box, lane m
-// accumulates the order-m coefficients of every degree over the box's
-// particles in key order.
+// P2M (p2m_kernel, fmm_cpukernel.cpp:281-384): one block of 64 lanes per leaf
+// box.  Lane l serves order m = l % p of the particles l / p, l / p + G, ...
+// (G = 64 / p particle groups), running the reference's Legendre recurrence
+// of that order with its running powers of rho; the G group sums of each
+// coefficient are then added in group order.
 __global__ __launch_bounds__(64) void k_sp_p2m(SphTab t, int p, int KP, SpGeom g, const uint64_t* __restrict__ keys,
                                                const int* __restrict__ first, const int* __restrict__ count,
                                                const float4* __restrict__ sp, double* __restrict__ M) {
-    const int b = blockIdx.x, m = threadIdx.x;
+    __shared__ double2 part[64][kSpPmax];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const int G = 64 / p, m = lane % p, grp = lane / p;
     double* Mb = M + (size_t)b * KP;
-    if (m == 0)
-        for (int c = p * (p + 1); c < KP; c++) Mb[c] = 0.0;
-    if (m >= p) return;
-    float cen[3];
-    sp_leaf_centre(g, keys[b], cen);
     double2 acc[kSpPmax];
     for (int n = 0; n < kSpPmax; n++) acc[n] = make_double2(0, 0);
-    for (int a = first[b]; a < first[b] + count[b]; a++) {
-        const float4 q = sp[a];
-        const float fx = q.x - cen[0], fy = q.y - cen[1], fz = q.z - cen[2];
-        double rho, alpha, beta;
-        sp_cart2sph(rho, alpha, beta, fx, fy, fz);
-        rho /= g.bsd;
-        const double xx = cos(alpha);
-        const double s2 = sqrt((1 - xx) * (1 + xx));
-        double fact = 1, pn = 1, rhom = 1.0;
-        for (int mm = 0; mm < m; mm++) {  // the reference's running P_m^m and rho^m
-            rhom *= rho;
-            pn = -pn * fact * s2;
-            fact += 2;
-        }
-        double pv = pn;
-        const double eb = -m * beta;
-        const double ce = cos(eb), se = sin(eb);
-        double Y = rhom * (double)t.factorial[m * m + 2 * m] * pv;
-        acc[m].x += (double)q.w * Y * ce;
-        acc[m].y += (double)q.w * Y * se;
-        double p1 = pv;
-        pv = xx * (2 * m + 1) * pv;
-        double rhon = rhom * rho;
-        for (int n = m + 1; n < p; n++) {
-            Y = rhon * (double)t.factorial[n * n + n + m] * pv;
-            acc[n].x += (double)q.w * Y * ce;
-            acc[n].y += (double)q.w * Y * se;
-            const double p2 = p1;
-            p1 = pv;
-            pv = (xx * (2 * n + 1) * p1 - (n + m) * p2) / (n - m + 1);
-            rhon *= rho;
+    if (grp < G) {
+        float cen[3];
+        sp_leaf_centre(g, keys[b], cen);
+        for (int a = first[b] + grp; a < first[b] + count[b]; a += G) {
+            const float4 q = sp[a];
+            const float fx = q.x - cen[0], fy = q.y - cen[1], fz = q.z - cen[2];
+            double rho, alpha, beta;
+            sp_cart2sph(rho, alpha, beta, fx, fy, fz);
+            rho /= g.bsd;
+            const double xx = cos(alpha);
+            const double s2 = sqrt((1 - xx) * (1 + xx));
+            double fact = 1, pn = 1, rhom = 1.0;
+            for (int mm = 0; mm < m; mm++) {  // the reference's running P_m^m and rho^m
+                rhom *= rho;
+                pn = -pn * fact * s2;
+                fact += 2;
+            }
+            double pv = pn;
+            const double eb = -m * beta;
+            const double ce = cos(eb), se = sin(eb);
+            double Y = rhom * (double)t.factorial[m * m + 2 * m] * pv;
+            acc[m].x += (double)q.w * Y * ce;
+            acc[m].y += (double)q.w * Y * se;
+            double p1 = pv;
+            pv = xx * (2 * m + 1) * pv;
+            double rhon = rhom * rho;
+            for (int n = m + 1; n < p; n++) {
+                Y = rhon * (double)t.factorial[n * n + n + m] * pv;
+                acc[n].x += (double)q.w * Y * ce;
+                acc[n].y += (double)q.w * Y * se;
+                const double p2 = p1;
+                p1 = pv;
+                pv = (xx * (2 * n + 1) * p1 - (n + m) * p2) / (n - m + 1);
+                rhon *= rho;
+            }
         }
     }
-    for (int n = m; n < p; n++) {
-        Mb[2 * (n * (n + 1) / 2 + m)] = acc[n].x;
-        Mb[2 * (n * (n + 1) / 2 + m) + 1] = acc[n].y;
+    for (int n = 0; n < kSpPmax; n++) part[lane][n] = acc[n];
+    __syncthreads();
+    // lane c < p(p+1)/2 adds coefficient (n, mc) over the groups in order
+    const int nc = p * (p + 1) / 2;
+    for (int c = lane; c < nc; c += 64) {
+        int n = 0;
+        while ((n + 1) * (n + 2) / 2 <= c) n++;
+        const int mc = c - n * (n + 1) / 2;
+        double2 v = make_double2(0, 0);
+        for (int gg = 0; gg < G; gg++) {
+            v.x += part[gg * p + mc][n].x;
+            v.y += part[gg * p + mc][n].y;
+        }
+        Mb[2 * c] = v.x;
+        Mb[2 * c + 1] = v.y;
     }
+    for (int c = 2 * nc + lane; c < KP; c += 64) Mb[c] = 0.0;
 }
 
 // M2M: each parent sums its children (key order) through the child-octant operators.
@@ -701,14 +739,20 @@ typedef double sp_d4 __attribute__((ext_vector_type(4)));
 // and accumulates T(slot) x M.  Fragments of v_mfma_f64_16x16x4f64 (measured
 // on gfx950): lane l holds A[l % 16][l / 16], B[l / 16][l % 16], and D rows
 // l / 16 + 4 r (r = 0..3) of column l % 16.  ctr[0] counts the box pairs that exist.
+// Work unit = (tile, chunk of the offset list): with nchunk > 1 each unit
+// writes its 16 x KP partial sums to part[chunk][tile] and k_sp_m2l_reduce adds
+// them to L in chunk order, so a level with few tiles still fills the chip.
 template <int RT>
 __global__ __launch_bounds__(64) void k_sp_m2l(const double* __restrict__ T, const double* __restrict__ M,
                                                double* __restrict__ L, const uint64_t* __restrict__ keys, int nb,
                                                int level, const int* __restrict__ grp, const int4* __restrict__ tiles,
+                                               int ntiles, int nchunk, double* __restrict__ part,
                                                const int4* __restrict__ offs, unsigned long long* __restrict__ ctr) {
     constexpr int KP = RT * 16, KS = KP / 4;
     const int lane = threadIdx.x, j = lane & 15, kq = lane >> 4;
-    const int4 tl = tiles[blockIdx.x];
+    const int tile = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
+    const int o0 = chunk * kSpNoff / nchunk, o1 = (chunk + 1) * kSpNoff / nchunk;
+    const int4 tl = tiles[tile];
     const int tb = j < tl.z ? grp[tl.y + j] : -1;
     int tx = 0, ty = 0, tz = 0;
     if (tb >= 0) sp_unkey(keys[tb], tx, ty, tz);
@@ -717,7 +761,7 @@ __global__ __launch_bounds__(64) void k_sp_m2l(const double* __restrict__ T, con
     for (int r = 0; r < RT; r++) acc[r] = (sp_d4){0.0, 0.0, 0.0, 0.0};
     const int4* od = offs + tl.x * kSpNoff;
     unsigned long long pairs = 0;
-    for (int o = 0; o < kSpNoff; o++) {
+    for (int o = o0; o < o1; o++) {
         const int4 d = od[o];
         int sb = -1;
         if (tb >= 0) {
@@ -739,64 +783,156 @@ __global__ __launch_bounds__(64) void k_sp_m2l(const double* __restrict__ T, con
         }
     }
     if (lane == 0 && pairs) atomicAdd(ctr, pairs);
-    if (tb >= 0) {
-        double* Lt = L + (size_t)tb * KP;
+    if (nchunk == 1) {
+        if (tb >= 0) {
+            double* Lt = L + (size_t)tb * KP;
+            for (int r = 0; r < RT; r++)
+                for (int i = 0; i < 4; i++) Lt[16 * r + kq + 4 * i] += acc[r][i];  // D row kq + 4 i, column j
+        }
+    } else {
+        double* Pt = part + ((size_t)chunk * ntiles + tile) * 16 * KP + (size_t)j * KP;
         for (int r = 0; r < RT; r++)
-            for (int i = 0; i < 4; i++) Lt[16 * r + kq + 4 * i] += acc[r][i];  // D row kq + 4 i, column j
+            for (int i = 0; i < 4; i++) Pt[16 * r + kq + 4 * i] = acc[r][i];
     }
 }
 
-// L2P (l2p_kernel, fmm_cpukernel.cpp:1188-1317) and P2P (p2p_kernel :194-219 over
-// DirectForceFunction::direct, direct_force_method.cpp:153-190) per target in key
-// order: out[c][i - i0] = P2P - K q / m * L2P.  ctr[1] counts the P2P box pairs.
-__global__ __launch_bounds__(64) void k_sp_leaf(SphTab t, int p, int KP, SpGeom g, double kc,
-                                                const uint64_t* __restrict__ keys, int nb, const int* __restrict__ first,
-                                                const int* __restrict__ count, const int* __restrict__ sbox,
-                                                const float4* __restrict__ sp, const int* __restrict__ sorig, int ns,
-                                                const int* __restrict__ st, const int* __restrict__ spc,
-                                                const double* __restrict__ L, int i0, int m, double* __restrict__ out,
-                                                unsigned long long* __restrict__ ctr) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= ns) return;
-    const int i = sorig[a];
-    const int b = sbox[a];
+// L[target] += sum over chunks (in order) of the M2L partials; one thread per
+// (tile, target column, row).
+__global__ void k_sp_m2l_reduce(const double* __restrict__ part, int ntiles, int nchunk, int KP,
+                                const int4* __restrict__ tiles, const int* __restrict__ grp, double* __restrict__ L) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)ntiles * 16 * KP) return;
+    const int tile = (int)(i / (16 * KP)), j = (int)((i / KP) % 16), row = (int)(i % KP);
+    const int4 tl = tiles[tile];
+    if (j >= tl.z) return;
+    double v = 0.0;
+    for (int c = 0; c < nchunk; c++) v += part[((size_t)c * ntiles + tile) * 16 * KP + (size_t)j * KP + row];
+    L[(size_t)grp[tl.y + j] * KP + row] += v;
+}
+
+// The 27 neighbour leaves of leaf b in the reference's order (x, then y, then
+// z; -1 where absent).
+__device__ inline void sp_neighbours(const SpGeom& g, const uint64_t* __restrict__ keys, int nb, int b, int* nbr) {
     int ix, iy, iz;
     sp_unkey(keys[b], ix, iy, iz);
     const int W = 1 << g.GL;
-    if (a == first[b]) {
-        int nnb = 0;
-        for (int jx = ix - 1; jx <= ix + 1; jx++)
-            for (int jy = iy - 1; jy <= iy + 1; jy++)
-                for (int jz = iz - 1; jz <= iz + 1; jz++)
-                    if (jx >= 0 && jy >= 0 && jz >= 0 && jx < W && jy < W && jz < W &&
-                        sp_find(keys, nb, sp_key(jx, jy, jz)) >= 0)
-                        nnb++;
-        atomicAdd(ctr + 1, (unsigned long long)nnb);
+    for (int k = 0; k < 27; k++) {
+        const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
+        nbr[k] = (jx >= 0 && jy >= 0 && jz >= 0 && jx < W && jy < W && jz < W) ? sp_find(keys, nb, sp_key(jx, jy, jz))
+                                                                                   : -1;
     }
-    if (st[i] != 1) return;
-    const float4 pi = sp[a];
-    const double mass = species_mass(spc[i]);
-    const double ac = kc * (double)pi.w / mass;
+}
+
+// P2P work units: every leaf's targets in runs of 64.  ucnt[b] = runs of leaf
+// b; ucost[b] = 64 x its near-field sources (the sort key of its units);
+// ctr[1] counts the box pairs.
+__global__ void k_sp_unit_counts(SpGeom g, const uint64_t* __restrict__ keys, int nb, const int* __restrict__ count,
+                                 int* __restrict__ ucnt, unsigned* __restrict__ ucost,
+                                 unsigned long long* __restrict__ ctr) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int nbr[27];
+    sp_neighbours(g, keys, nb, b, nbr);
+    long near = 0;
+    int nnb = 0;
+    for (int k = 0; k < 27; k++)
+        if (nbr[k] >= 0) {
+            near += count[nbr[k]];
+            nnb++;
+        }
+    atomicAdd(ctr + 1, (unsigned long long)nnb);
+    ucnt[b] = (count[b] + 63) / 64;
+    ucost[b] = (unsigned)min(64L * near, 0xffffffffL);
+}
+
+// Unit u of leaf b at uoff[b] + u: value b << 32 | u, key ~cost so that an
+// ascending sort puts the longest units first.
+__global__ void k_sp_unit_fill(int nb, const int* __restrict__ ucnt, const int* __restrict__ uoff,
+                               const unsigned* __restrict__ ucost, unsigned* __restrict__ ukey,
+                               unsigned long long* __restrict__ uval) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    for (int u = 0; u < ucnt[b]; u++) {
+        ukey[uoff[b] + u] = ~ucost[b];
+        uval[uoff[b] + u] = ((unsigned long long)b << 32) | (unsigned)u;
+    }
+}
+
+// P2P (p2p_kernel, fmm_cpukernel.cpp:194-219, over DirectForceFunction::direct,
+// direct_force_method.cpp:153-190): one wave per unit of 64 targets of one
+// leaf, longest units first.  For each of the 27 neighbour leaves in the
+// reference's order the wave stages the leaf's sources in LDS, 64 at a time
+// (one coalesced load), every lane sums its target's pairs over the leaf in
+// FP64 (float displacement, as the reference forms it) and adds K q / m times
+// the leaf's sum.  near[c * ns + a] for the source at key-order position a
+// (status 1 only).
+__global__ __launch_bounds__(64) void k_sp_p2p(SpGeom g, double kc, const uint64_t* __restrict__ keys, int nb,
+                                               const int* __restrict__ first, const int* __restrict__ count,
+                                               const unsigned long long* __restrict__ units,
+                                               const float4* __restrict__ sp,
+                                               const int* __restrict__ sorig, int ns, const int* __restrict__ st,
+                                               const int* __restrict__ spc, double* __restrict__ near) {
+    __shared__ float4 tile[64];
+    __shared__ int nbr[27];
+    const int lane = threadIdx.x;
+    const unsigned long long uv = units[blockIdx.x];
+    const int b = (int)(uv >> 32), u = (int)(uv & 0xffffffffu);
+    if (lane == 0) sp_neighbours(g, keys, nb, b, nbr);
+    __syncthreads();
+    const int a = first[b] + 64 * u + lane;
+    const bool tgt = a < first[b] + count[b] && st[sorig[a]] == 1;
+    float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    double ac = 0.0;
+    if (tgt) {
+        pi = sp[a];
+        ac = kc * (double)pi.w / species_mass(spc[sorig[a]]);
+    }
     double tot[3] = {0, 0, 0};
-    for (int jx = ix - 1; jx <= ix + 1; jx++)
-        for (int jy = iy - 1; jy <= iy + 1; jy++)
-            for (int jz = iz - 1; jz <= iz + 1; jz++) {
-                if (jx < 0 || jy < 0 || jz < 0 || jx >= W || jy >= W || jz >= W) continue;
-                const int sb = sp_find(keys, nb, sp_key(jx, jy, jz));
-                if (sb < 0) continue;
-                double ai[3] = {0, 0, 0};
-                for (int e = first[sb]; e < first[sb] + count[sb]; e++) {
-                    if (sorig[e] == i) continue;
-                    const float4 pj = sp[e];
+    for (int k = 0; k < 27; k++) {
+        const int sb = nbr[k];
+        if (sb < 0) continue;
+        double ai[3] = {0, 0, 0};
+        const int e0 = first[sb], e1 = first[sb] + count[sb];
+        for (int s0 = e0; s0 < e1; s0 += 64) {
+            __syncthreads();
+            if (s0 + lane < e1) tile[lane] = sp[s0 + lane];
+            __syncthreads();
+            const int n = min(64, e1 - s0);
+            if (tgt)
+                for (int e = 0; e < n; e++) {
+                    if (s0 + e == a) continue;
+                    const float4 pj = tile[e];
                     const double d0 = (double)(pi.x - pj.x), d1 = (double)(pi.y - pj.y), d2 = (double)(pi.z - pj.z);
-                    const double invD = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2 + kSoftening);
+                    const double invD = rsqrt_f64(d0 * d0 + d1 * d1 + d2 * d2 + kSoftening);
                     const double invD3 = (double)pj.w * invD * invD * invD;
                     ai[0] += d0 * invD3;
                     ai[1] += d1 * invD3;
                     ai[2] += d2 * invD3;
                 }
-                for (int c = 0; c < 3; c++) tot[c] += ac * ai[c];
-            }
+        }
+        for (int c = 0; c < 3; c++) tot[c] += ac * ai[c];
+    }
+    if (tgt)
+        for (int c = 0; c < 3; c++) near[(size_t)c * ns + a] = tot[c];
+}
+
+// L2P (l2p_kernel, fmm_cpukernel.cpp:1188-1317) per target in key order, added
+// to its P2P sum: out[c][i - i0] = P2P - K q / m * L2P.
+__global__ __launch_bounds__(64) void k_sp_leaf(SphTab t, int p, int KP, SpGeom g, double kc,
+                                                const uint64_t* __restrict__ keys, const int* __restrict__ sbox,
+                                                const float4* __restrict__ sp, const int* __restrict__ sorig, int ns,
+                                                const int* __restrict__ st, const int* __restrict__ spc,
+                                                const double* __restrict__ near, const double* __restrict__ L, int i0,
+                                                int m, double* __restrict__ out) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= ns) return;
+    const int i = sorig[a];
+    const int b = sbox[a];
+    if (st[i] != 1) return;
+    const float4 pi = sp[a];
+    const double mass = species_mass(spc[i]);
+    const double ac = kc * (double)pi.w / mass;
+    const double tot[3] = {near[a], near[(size_t)ns + a], near[2 * (size_t)ns + a]};
     // L2P
     float cen[3];
     sp_leaf_centre(g, keys[b], cen);
@@ -955,10 +1091,14 @@ void sp_offsets(std::vector<int4>& offs) {
 }
 
 template <int RT>
-void sp_launch_m2l(hipStream_t s, int ntiles, const double* T, const double* M, double* L, const uint64_t* keys,
-                   int nb, int level, const int* grp, const int4* tiles, const int4* offs,
+void sp_launch_m2l(hipStream_t s, int ntiles, int nchunk, double* part, const double* T, const double* M, double* L,
+                   const uint64_t* keys, int nb, int level, const int* grp, const int4* tiles, const int4* offs,
                    unsigned long long* ctr) {
-    hipLaunchKernelGGL(k_sp_m2l<RT>, dim3(ntiles), dim3(64), 0, s, T, M, L, keys, nb, level, grp, tiles, offs, ctr);
+    hipLaunchKernelGGL(k_sp_m2l<RT>, dim3(ntiles * nchunk), dim3(64), 0, s, T, M, L, keys, nb, level, grp, tiles,
+                       ntiles, nchunk, part, offs, ctr);
+    if (nchunk > 1)
+        hipLaunchKernelGGL(k_sp_m2l_reduce, dim3(sp_bdiv((long)ntiles * 16 * RT * 16, 256)), dim3(256), 0, s, part,
+                           ntiles, nchunk, RT * 16, tiles, grp, L);
 }
 
 // Evaluates targets (status 1) in [i0, i1) from the sources (status 1 or 2) in
@@ -1027,7 +1167,10 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     }
     SHP_WS(hist, int, h, WS_SPH_LVL, ndom);
     hipLaunchKernelGGL(k_sp_zero_i, dim3(sp_bdiv(ndom, 256)), dim3(256), 0, stream, hist, (size_t)ndom);
-    hipLaunchKernelGGL(k_sp_domains, dim3(sp_bdiv(ns, 256)), dim3(256), 0, stream, P, sel, ns, g, hist, dctr);
+    hipLaunchKernelGGL(k_sp_domains, dim3(std::min(sp_bdiv(ns, 256), 4 * h->num_cus)), dim3(256), 0, stream, P, sel,
+                       ns, g, (int)ndom, hist, dctr);
+    hipLaunchKernelGGL(k_sp_hist_max, dim3(std::min(sp_bdiv(ndom, 256), 256)), dim3(256), 0, stream, hist, (int)ndom,
+                       dctr);
     int dres[2];
     SHP_HIP_CHECK(hipMemcpyAsync(dres, dctr, sizeof(dres), hipMemcpyDeviceToHost, stream));
     SHP_HIP_CHECK(hipStreamSynchronize(stream));
@@ -1182,22 +1325,61 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
         SHP_HIP_CHECK(hipMemcpyAsync(dtiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, stream));
         const int nt = (int)tiles.size();
         const uint64_t* keys = lk + loff[l];
+        // offset chunks per tile: about 8 waves per CU over the level, at most 27 (7 offsets each)
+        const int nch = std::max(1, std::min(27, (8 * h->num_cus + nt - 1) / nt));
+        double* part = nullptr;
+        if (nch > 1) {
+            part = static_cast<double*>(h->get(WS_SPH_M2LP, sizeof(double) * (size_t)nch * nt * 16 * KP));
+            if (!part) return SHP_ERR_NOMEM;
+        }
         switch (RT) {
-        case 1: sp_launch_m2l<1>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        case 2: sp_launch_m2l<2>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        case 3: sp_launch_m2l<3>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        case 4: sp_launch_m2l<4>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        case 5: sp_launch_m2l<5>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        case 6: sp_launch_m2l<6>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        default: sp_launch_m2l<7>(stream, nt, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 1: sp_launch_m2l<1>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 2: sp_launch_m2l<2>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 3: sp_launch_m2l<3>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 4: sp_launch_m2l<4>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 5: sp_launch_m2l<5>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 6: sp_launch_m2l<6>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        default: sp_launch_m2l<7>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
         }
         // the tile list is host memory reused by the next level: wait for the copy
         SHP_HIP_CHECK(hipStreamSynchronize(stream));
     }
-    // leaves: P2P + L2P
+    // leaves: P2P in units of 64 targets, longest first, then L2P
+    const int nleaf = nbl[GL];
+    // WS_SPH_NEAR: P2P sums [3][ns] doubles, then per leaf: unit count, offset,
+    // cost; per unit (at most ns / 64 + nleaf): keys and values, in and out
+    const size_t umax = (size_t)ns / 64 + nleaf + 1;
+    char* nbuf = static_cast<char*>(h->get(WS_SPH_NEAR, sizeof(double) * 3 * (size_t)ns +
+                                                             sizeof(unsigned long long) * 2 * umax +
+                                                             sizeof(int) * (3 * (size_t)nleaf + 2 * umax) + 64));
+    if (!nbuf) return SHP_ERR_NOMEM;
+    double* near = (double*)nbuf;
+    unsigned long long* uval = (unsigned long long*)(near + 3 * (size_t)ns);
+    unsigned long long* uval2 = uval + umax;
+    int* ucnt = (int*)(uval2 + umax);
+    int* uoff = ucnt + nleaf;
+    unsigned* ucost = (unsigned*)(uoff + nleaf);
+    unsigned* ukey = ucost + nleaf;
+    unsigned* ukey2 = ukey + umax;
+    hipLaunchKernelGGL(k_sp_unit_counts, dim3(sp_bdiv(nleaf, 64)), dim3(64), 0, stream, g, lk + loff[GL], nleaf,
+                       lcnt, ucnt, ucost, pctr);
+    size_t t6 = 0, t7 = 0;
+    SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t6, ucnt, uoff, nleaf, stream));
+    SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, t7, ukey, ukey2, uval, uval2, (int)umax, 0, 32, stream));
+    SHP_WS(cub5, char, h, WS_CUB, std::max(t6, t7));
+    SHP_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cub5, t6, ucnt, uoff, nleaf, stream));
+    int lastc[2];
+    SHP_HIP_CHECK(hipMemcpyAsync(&lastc[0], uoff + nleaf - 1, sizeof(int), hipMemcpyDeviceToHost, stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(&lastc[1], ucnt + nleaf - 1, sizeof(int), hipMemcpyDeviceToHost, stream));
+    SHP_HIP_CHECK(hipStreamSynchronize(stream));
+    const int nunits = lastc[0] + lastc[1];
+    hipLaunchKernelGGL(k_sp_unit_fill, dim3(sp_bdiv(nleaf, 64)), dim3(64), 0, stream, nleaf, ucnt, uoff, ucost, ukey,
+                       uval);
+    SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub5, t7, ukey, ukey2, uval, uval2, nunits, 0, 32, stream));
+    hipLaunchKernelGGL(k_sp_p2p, dim3(nunits), dim3(64), 0, stream, g, k_const_host(), lk + loff[GL], nleaf, lfirst,
+                       lcnt, uval2, sp4, vout, ns, st, spc, near);
     hipLaunchKernelGGL(k_sp_leaf, dim3(sp_bdiv(ns, 64)), dim3(64), 0, stream, S->tab, p, KP, g, k_const_host(),
-                       lk + loff[GL], nbl[GL], lfirst, lcnt, sbox, sp4, vout, ns, st, spc, Lx + eoff[GL], i0, m, out,
-                       pctr);
+                       lk + loff[GL], sbox, sp4, vout, ns, st, spc, near, Lx + eoff[GL], i0, m, out);
     unsigned long long pc[2] = {0, 0};
     SHP_HIP_CHECK(hipMemcpyAsync(pc, pctr, sizeof(pc), hipMemcpyDeviceToHost, stream));
     SHP_HIP_CHECK(hipStreamSynchronize(stream));

@@ -121,6 +121,8 @@ enum WsSlot {
     WS_SPH_EXP,     // spherical FMM: multipole and local expansions of all levels
     WS_SPH_OPS,     // spherical FMM: dense translation operators
     WS_SPH_GRP,     // spherical FMM: boxes grouped by octant, M2L tiles
+    WS_SPH_NEAR,    // spherical FMM: P2P sums per source in key order, P2P work units
+    WS_SPH_M2LP,    // spherical FMM: M2L partial sums of the offset chunks
     WS_NSLOTS
 };
 
