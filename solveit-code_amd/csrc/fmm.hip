@@ -1515,6 +1515,7 @@ __global__ __launch_bounds__(256) void k_fmm_level_cost(const LevelCostArgs A, i
 // that the dense leaves of a plume core spread over many waves.
 constexpr int kLeafUnit = 64;
 constexpr int kLeafTile = 256;  // sources staged in LDS per barrier pair
+constexpr int kLeafXcdRun = 16;  // consecutive leaf units dealt to one XCD
 
 // Sorted ion i relative to the centre of its own leaf in units of the leaf
 // half-width h (FP32), with its charge.  A source j of leaf S seen from leaf T
@@ -1719,8 +1720,14 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(
     // Units run longest first (uorder sorts them by targets x near-field
     // sources, descending), so the dense units of a plume core do not finish
     // last.
-    if ((int)blockIdx.x >= *nunits) return;
-    const int2 unit = units[uorder[blockIdx.x]];
+    // Blocks are dealt to the 8 XCDs in turn.  Each run of kLeafXcdRun
+    // consecutive units in that order (mostly units of one dense leaf, which
+    // share their near field) goes to one XCD, so its L2 fetches the near field
+    // once; the longest-first order holds at the granularity of 8 runs.
+    const int w = blockIdx.x, grp = w / (8 * kLeafXcdRun), rr = w % (8 * kLeafXcdRun);
+    const int un = grp * 8 * kLeafXcdRun + (rr & 7) * kLeafXcdRun + (rr >> 3);
+    if (un >= *nunits) return;
+    const int2 unit = units[uorder[un]];
     const int b = unit.x, lane = threadIdx.x;
     tile[kLeafTile + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const uint64_t key = lv.key[b];
@@ -2695,7 +2702,9 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
         hipLaunchKernelGGL(k_fmm_l2l, dim3(nbox[l]), dim3(kShiftLanes), 9 * nc * sizeof(float), st,
                            lv[l], lv[l + 1], nc, tb->mi, tb->l2l_term, tb->l2l_nt);
     SHP_HIP_CHECK(hipEventRecord(h->fev[4], st));
-    hipLaunchKernelGGL(k_fmm_leaf, dim3(umax), dim3(64), 0, st, lv[L], L, ring, leafgrid, root, sp, perm,
+    // whole runs of 8 x kLeafXcdRun blocks (the block -> unit map permutes within a run)
+    const int ublocks = fdiv_up(umax, 8 * kLeafXcdRun) * 8 * kLeafXcdRun;
+    hipLaunchKernelGGL(k_fmm_leaf, dim3(ublocks), dim3(64), 0, st, lv[L], L, ring, leafgrid, root, sp, perm,
                        pt->status, pt->species, spf, tb->mi, nc, order, k_const_host(), units, uorder, nunits,
                        h->fmm_stats,
                        ax, ay, az);
