@@ -532,7 +532,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--order", type=int, default=10)
     ap.add_argument("--leaf-size", type=int, default=20)
