@@ -366,11 +366,7 @@ def run_config(k, args):
                                       domain=(2e-6, 2e-6, 2e-6))
                 nimg = 27
             else:
-                d = synth.cone(n)
-                phi = np.arctan2(d["y"], d["x"])
-                kk = np.floor(np.mod(phi, 2 * np.pi) / (np.pi / 3))
-                c, s_ = np.cos(-kk * np.pi / 3), np.sin(-kk * np.pi / 3)
-                d["x"], d["y"] = c * d["x"] - s_ * d["y"], s_ * d["x"] + c * d["y"]
+                d = synth.fold_sectors(synth.cone(n), 6)
                 sym = P.make_symmetry("rotational", nsec=6)
                 nimg = 6
             st = P.DeviceState.from_numpy(d)

@@ -183,10 +183,19 @@ int shp_fmm_host(shp_handle h, const float* pos4, const float* vel4, const float
  * where the reference groups several domains into ternary supercells, octree
  * levels of one enclosing cube.  Computed in FP64 (the translations as dense
  * operators on the FP64 matrix cores).  Ranges and out3 as shp_fmm_host.
+ * sym NULL or SHP_SYM_NONE: updateAccelerationNoSym; SHP_SYM_ROTATIONAL with
+ * nsec 4: updateAccelerationRotational (multipole_method_force_calculation_
+ * rotational.cpp:9-331, fmm_cpukernel_rotational.cpp): the ions in the first
+ * quadrant of a domain column centred on the z axis (computeDomainBounds-
+ * Rotations, domain_bounds_functions.cpp:317-500), the other quadrants their
+ * images, mapped back by getInteractionListRotations (:568-1438).
  * SHP_ERR_UNSUPPORTED when a source lies outside the domain grid (the
- * reference's outlier passes) or maxLevel < 2 (its single-box paths).  Waits. */
+ * reference's outlier passes), maxLevel < 2 (its single-box paths), for the
+ * translational symmetry or other sector counts, and for a rotational grid
+ * with more than one domain in x or y (the reference's quadrant map works in
+ * domain-local x, y).  Waits. */
 int shp_fmm_sph_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4, int n,
-                     int i0, int i1, int order, int max_level, double* out3);
+                     int i0, int i1, int order, int max_level, const shp_symmetry* sym, double* out3);
 /* Summary of the last shp_fmm_sph_host call, 12 values: [0] maxLevel, [1]
  * levels above the domain level, [2..4] domains per axis, [5] leaf boxes,
  * [6] M2L box pairs, [7] P2P box pairs, [8] sources, [9] boxes of all levels

@@ -743,9 +743,9 @@ def rollout_grad_f64(d, dt, nsteps, grad_final):
 # ---- the reference's rotation-based spherical-harmonic FMM (oracle/fmm_sph.c) ----
 
 lib.orc_domain_bounds.restype = _ci
-lib.orc_domain_bounds.argtypes = [_vp, _vp, _ci, _ci, _vp, _vp, _vp, _vp]
+lib.orc_domain_bounds.argtypes = [_vp, _vp, _ci, _ci, _vp, _ci, _vp, _vp, _vp]
 lib.orc_fmm_sph.restype = _ci
-lib.orc_fmm_sph.argtypes = [_vp, _vp, _vp, _ci, _ci, _ci, _ci, _ci, _vp, _vp]
+lib.orc_fmm_sph.argtypes = [_vp, _vp, _vp, _ci, _ci, _ci, _ci, _ci, _ci, _vp, _vp]
 lib.orc_sph_p2m.argtypes = [_ci, _ci, _vp, ctypes.c_double, _vp]
 lib.orc_sph_m2m.argtypes = [_ci, _ci, _vp, _vp]
 lib.orc_sph_m2l.argtypes = [_ci, _ci, _ci, _ci, ctypes.c_double, _vp, _vp]
@@ -759,21 +759,24 @@ lib.orc_sph_table.restype = ctypes.c_double
 lib.orc_sph_table.argtypes = [_ci, _ci]
 
 
-def domain_bounds(pos4, acc4, i0=0, i1=None):
-    """computeDomainBounds as the FMM calls it: (boxMin[3], dims[3], boxDimVals[3], nsrc)."""
+def domain_bounds(pos4, acc4, i0=0, i1=None, rotational=False):
+    """computeDomainBounds as the FMM calls it (computeDomainBoundsRotations with
+    rotational=True): (boxMin[3], dims[3], boxDimVals[3], nsrc)."""
     pos4, acc4 = _f32(pos4), _f32(acc4)
     i1 = pos4.shape[0] if i1 is None else i1
     bm, dm, nd = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.int32)
-    ns = lib.orc_domain_bounds(pos4.ctypes.data, acc4.ctypes.data, i0, i1, None, bm.ctypes.data,
-                               dm.ctypes.data, nd.ctypes.data)
+    ns = lib.orc_domain_bounds(pos4.ctypes.data, acc4.ctypes.data, i0, i1, None, 1 if rotational else 0,
+                               bm.ctypes.data, dm.ctypes.data, nd.ctypes.data)
     return bm, dm, nd, ns
 
 
-def fmm_sph(pos4, vel4, acc4, p=10, max_level=-2, i0=0, i1=None, cr_trig=False):
+def fmm_sph(pos4, vel4, acc4, p=10, max_level=-2, i0=0, i1=None, cr_trig=False, rotational=False):
     """The reference FMM's operators (rotation-based spherical harmonics) over the
     reference's domain grid and leaf placement, in FP64; returns (acc[i1-i0,3], info).
-    max_level: setMaxLevel's value (-2: setOptimumLevel).  Raises on the cases
-    the restatement does not cover (sources outside the grid, maxLevel < 2).
+    max_level: setMaxLevel's value (-2: setOptimumLevel).  rotational: the
+    reference's rotational FMM (four quadrant images about z).  Raises on the
+    cases the restatement does not cover (sources outside the grid, maxLevel < 2,
+    a rotational grid with more than one domain in x or y).
     cr_trig: correctly rounded float acos / atan in cart2sph_exact instead of
     this host's acosf / atanf (which the reference calls)."""
     pos4, vel4, acc4 = _f32(pos4), _f32(vel4), _f32(acc4)
@@ -784,7 +787,7 @@ def fmm_sph(pos4, vel4, acc4, p=10, max_level=-2, i0=0, i1=None, cr_trig=False):
     lib.orc_sph_set_cr_trig(1 if cr_trig else 0)
     try:
         rc = lib.orc_fmm_sph(pos4.ctypes.data, vel4.ctypes.data, acc4.ctypes.data, n, i0, i1, p,
-                             max_level, out.ctypes.data, info.ctypes.data)
+                             max_level, 1 if rotational else 0, out.ctypes.data, info.ctypes.data)
     finally:
         lib.orc_sph_set_cr_trig(0)
     if rc != 0:

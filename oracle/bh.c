@@ -81,19 +81,20 @@ typedef struct {
  * toggle: 0 cubic cells, 1 rectangular-prism cells; forced: NULL or the
  * cell counts of setBoxDimVals.  minO / maxO: the bounds of the sources
  * seeded with the first ion of the range.  Returns the number of sources. */
-static int domain_bounds(const float* pos4, const float* acc4, int i0, int i1, int toggle,
+static int domain_bounds(const float* pos4, const float* acc4, int i0, int i1, int toggle, int rot,
                          const int* forced, float boxMin[3], float dims[3], int bdv[3],
                          float minO[3], float maxO[3]) {
     const float* seed = pos4 + 4 * (size_t)i0;
     float rootmax[3];
     for (int c = 0; c < 3; c++) boxMin[c] = rootmax[c] = minO[c] = maxO[c] = seed[c];
+    if (rot) boxMin[0] = boxMin[1] = 0.0f; /* computeDomainBoundsRotations: x, y minima not taken */
     int npart = 0;
     for (int i = i0; i < i1; i++) {
         const float s = acc4[4 * (size_t)i + 3];
         if (!(s == 1.0f || s == 2.0f)) continue;
         const float* p = pos4 + 4 * (size_t)i;
         for (int c = 0; c < 3; c++) {
-            if (p[c] < boxMin[c]) boxMin[c] = p[c];
+            if (p[c] < boxMin[c] && (!rot || c == 2)) boxMin[c] = p[c];
             if (p[c] > rootmax[c]) rootmax[c] = p[c];
             if (p[c] < minO[c]) minO[c] = p[c];
             if (p[c] > maxO[c]) maxO[c] = p[c];
@@ -101,9 +102,21 @@ static int domain_bounds(const float* pos4, const float* acc4, int i0, int i1, i
         npart++;
     }
     const float subtr = 0.0f;
-    for (int c = 0; c < 3; c++) {
-        boxMin[c] = boxMin[c] - subtr;
-        dims[c] = (rootmax[c] - boxMin[c]) + subtr;
+    if (rot) {
+        /* a square about the z axis: x, y in [-R, R], R the larger of the x and y maxima
+         * (domain_bounds_functions.cpp:393-403) */
+        rootmax[0] = rootmax[0] < rootmax[1] ? rootmax[1] : rootmax[0];
+        rootmax[1] = rootmax[0];
+        boxMin[0] = -rootmax[0];
+        boxMin[1] = -rootmax[1];
+        dims[0] = rootmax[0] * 2;
+        dims[1] = rootmax[1] * 2;
+        dims[2] = rootmax[2] - boxMin[2];
+    } else {
+        for (int c = 0; c < 3; c++) {
+            boxMin[c] = boxMin[c] - subtr;
+            dims[c] = (rootmax[c] - boxMin[c]) + subtr;
+        }
     }
     bdv[0] = bdv[1] = bdv[2] = 1;
     if (toggle == 0) {
@@ -177,7 +190,7 @@ static void bh_make_layout(const float* pos4, const float* acc4, int i0, int i1,
                            const int* forced, bh_layout* L) {
     float boxMin[3], dims[3], minO[3], maxO[3];
     int bdv[3];
-    const int npart = domain_bounds(pos4, acc4, i0, i1, toggle, forced, boxMin, dims, bdv, minO, maxO);
+    const int npart = domain_bounds(pos4, acc4, i0, i1, toggle, 0, forced, boxMin, dims, bdv, minO, maxO);
     /* computeWorldLimits */
     float roots[3] = {boxMin[0], boxMin[1], boxMin[2]};
     for (int c = 0; c < 3; c++) {
@@ -484,13 +497,16 @@ int orc_bh_layout(const float* pos4, const float* acc4, int i0, int i1, int togg
 
 /* computeDomainBounds alone, as MultipoleMethodForceCalculation::setDomainSize
  * calls it (multipole_method_force_calculation.cpp:648-668): cubic cells, no
- * outliers.  Returns the number of sources; box_min3 / dims3 / nd3 as the
+ * outliers.  rot != 0: computeDomainBoundsRotations (domain_bounds_functions.cpp:
+ * 317-500, called by setDomainSizeRotations, multipole_method_force_calculation_
+ * rotational.cpp:333-350), whose x, y extent is the square [-R, R]^2 about the
+ * z axis.  Returns the number of sources; box_min3 / dims3 / nd3 as the
  * reference's boxMin, dims and boxDimVals. */
-int orc_domain_bounds(const float* pos4, const float* acc4, int i0, int i1, const int* forced3,
+int orc_domain_bounds(const float* pos4, const float* acc4, int i0, int i1, const int* forced3, int rot,
                       float* box_min3, float* dims3, int* nd3) {
     float minO[3], maxO[3];
     if (i1 <= i0) return 0;
-    return domain_bounds(pos4, acc4, i0, i1, 0, forced3, box_min3, dims3, nd3, minO, maxO);
+    return domain_bounds(pos4, acc4, i0, i1, 0, rot, forced3, box_min3, dims3, nd3, minO, maxO);
 }
 
 /* The root cell an ion is placed in (generateTree :279-310), as a chain index;

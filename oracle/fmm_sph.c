@@ -38,6 +38,22 @@
  * levels 2 .. leaf (children of the parent's 27 neighbours that are not
  * neighbours), L2L down, L2P + 27-neighbour P2P at the leaves.
  *
+ * Rotational symmetry (updateAccelerationRotational, multipole_method_force_
+ * calculation_rotational.cpp:9-331): the ions fill one quadrant (x >= 0,
+ * y >= 0) of a domain column centred on the z axis (computeDomainBoundsRotations,
+ * domain_bounds_functions.cpp:317-500), and the other three quadrants hold
+ * its images rotated by k pi / 2, k = 1, 2, 3.  Below the domain level the
+ * P2P and M2L lists (getInteractionListRotations :568-1438) map a cell of
+ * quadrant k back to the real cell it images, with k: P2P rotates that cell's
+ * sources (DirectForceFunction::direct_rotational, direct_force_method_
+ * rotational.cpp:48-94), M2L multiplies its multipole by i^(-m k) (m2l_kernel_
+ * rotations, fmm_cpukernel_rotational.cpp:298-496, qrf_complex), and the M2M
+ * into a domain adds the four images of each child (m2m_kernel_rotations :107-
+ * 290).  Above the domain level there is no quadrant map ("supercells don't
+ * have rotational symmetry").  The quadrant map works in domain-local x, y,
+ * so the reference's lists are consistent only for one domain per x and y;
+ * other grids are not restated (-4).
+ *
  * Expansion storage: coefficient (n, m), 0 <= m <= n < p, at n(n+1)/2 + m,
  * complex double.  Morton bit order of the reference (multipole_method_
  * functions.cpp unmorton): y -> bit 3i, x -> bit 3i+1, z -> bit 3i+2.
@@ -47,7 +63,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-int orc_domain_bounds(const float* pos4, const float* acc4, int i0, int i1, const int* forced3,
+int orc_domain_bounds(const float* pos4, const float* acc4, int i0, int i1, const int* forced3, int rot,
                       float* box_min3, float* dims3, int* nd3);
 double orc_k_const(void);
 long double orc_species_mass_ld(int s);
@@ -547,6 +563,49 @@ static int sp_ilog2ceil(int v) {
     return k;
 }
 
+/* The quadrant map of getInteractionListRotations (multipole_method_force_
+ * calculation_rotational.cpp:616-634): a cell (x, y) of the local window of
+ * jd cells per axis lies in quadrant k of the axis at jd / 2 and images the
+ * real cell (*rx, *ry) of quadrant 0 rotated by k pi / 2. */
+static int sp_quadrant(int x, int y, int jd, int* rx, int* ry) {
+    const int half = jd >> 1;
+    if (x >= half && y >= half) {
+        *rx = x;
+        *ry = y;
+        return 0;
+    }
+    if (x < half && y >= half) {
+        *rx = y;
+        *ry = jd - x - 1;
+        return 1;
+    }
+    if (x < half && y < half) {
+        *rx = jd - x - 1;
+        *ry = jd - y - 1;
+        return 2;
+    }
+    *rx = jd - y - 1;
+    *ry = x;
+    return 3;
+}
+
+/* quadrantRotationFunction (auxiliary_functions.h:55-57): sin(n pi / 2). */
+static int sp_qrf(int n) {
+    const int odd = (n & 1) == 1 ? -1 : 1;
+    const int sgn = ((1 - odd) > 0) - ((1 - odd) < 0);
+    return sgn * ((((n - 1) >> 1) & 1) == 1 ? -1 : 1);
+}
+
+/* A multipole imaged into quadrant k: C_n^m qrf_complex(-m k) (fmm_cpukernel_
+ * rotational.cpp:367-372, :190-195). */
+static void sp_phase(int p, int k, const cz* C, cz* out) {
+    for (int n = 0; n < p; n++)
+        for (int m = 0; m <= n; m++) {
+            const cz r = cz_make((double)sp_qrf(-m * k + 1), (double)sp_qrf(-m * k));
+            out[n * (n + 1) / 2 + m] = cz_mul(C[n * (n + 1) / 2 + m], r);
+        }
+}
+
 /* setOptimumLevel's level_switch (multipole_method_force_calculation.cpp:842-869). */
 static int sp_level_of_count(int c) {
     static const float sw[8] = {200.0f, 1000.0f, 1.3e4f, 1e5f, 7e5f, 5e6f, 3e7f, 1.5e8f};
@@ -559,7 +618,8 @@ static int sp_level_of_count(int c) {
  * The reference's FMM (operators restated above) over the tree described at
  * the top, on the reference's float AoS.  Targets: status 1 in [i0, i1);
  * sources: status 1 or 2 in [i0, i1).  p: numExpansions (1..10).
- * forced_level: setMaxLevel's value (-2: setOptimumLevel).
+ * forced_level: setMaxLevel's value (-2: setOptimumLevel).  sym: 0 none,
+ * 1 rotational (the reference's four quadrants).
  * out3[3 (i - i0) + c]: the acceleration in double (zero for non-targets),
  * the reference's double terms before their float accumulation.
  * info (may be NULL, 12 entries): [0] maxLevel, [1] K, [2..4] boxDimVals,
@@ -567,20 +627,23 @@ static int sp_level_of_count(int c) {
  * [9] boxes over all levels.
  * Returns 0; -1 for p outside 1..10; -2 if a source lies outside the domain
  * grid (the reference then runs its outlier passes, not restated here);
- * -3 for maxLevel < 2 (the reference's single-box and supercell-only paths).
+ * -3 for maxLevel < 2 (the reference's single-box and supercell-only paths);
+ * -4 for a rotational grid with more than one domain in x or y.
  */
 int orc_fmm_sph(const float* pos4, const float* vel4, const float* acc4, int n, int i0, int i1, int p,
-                int forced_level, double* out3, long long* info) {
-    if (p < 1 || p > SP_PMAX) return -1;
+                int forced_level, int sym, double* out3, long long* info) {
+    if (p < 1 || p > SP_PMAX || sym < 0 || sym > 1) return -1;
+    const int rot = sym == 1;
     sph_precalc();
     const int nc = p * (p + 1) / 2;
     if (i1 < i0 || i0 < 0 || i1 > n) return -1;
     memset(out3, 0, sizeof(double) * 3 * (size_t)(i1 - i0));
     float boxMin[3], dims[3];
     int bdv[3];
-    const int nsrc = orc_domain_bounds(pos4, acc4, i0, i1, NULL, boxMin, dims, bdv);
+    const int nsrc = orc_domain_bounds(pos4, acc4, i0, i1, NULL, rot, boxMin, dims, bdv);
     if (info) memset(info, 0, sizeof(long long) * 12);
     if (nsrc == 0) return 0;
+    if (rot && (bdv[0] != 1 || bdv[1] != 1)) return -4;
     const float rootBoxSize = dims[0];
     const int ndom = bdv[0] * bdv[1] * bdv[2];
     /* domains (setDomainSize :714-752) */
@@ -702,7 +765,10 @@ int orc_fmm_sph(const float* pos4, const float* vel4, const float* acc4, int n, 
             sph_p2m_one(p, (double)q[3], dx, dy, dz, boxSizeD, M);
         }
     }
-    /* M2M: children in Morton order into each parent */
+    /* M2M: children in Morton order into each parent.  Rotational, into a
+     * domain (level K): each child's four quadrant images, the child placed at
+     * octant (x, y) = (1, 1), (0, 1), (0, 0), (1, 0) of quadrant k with its own
+     * z (m2m_kernel_rotations, fmm_cpukernel_rotational.cpp:148-215). */
     for (int l = GL - 1; l >= 2; l--) {
         const sp_level* c = &lv[l + 1];
         sp_level* q = &lv[l];
@@ -711,7 +777,16 @@ int orc_fmm_sph(const float* pos4, const float* vel4, const float* acc4, int n, 
             const uint64_t pk = c->key[b] >> 3;
             while (q->key[pb] != pk) pb++;
             const int o = (int)(c->key[b] & 7u);
-            sph_m2m(p, (o >> 1) & 1, o & 1, (o >> 2) & 1, c->M + (size_t)b * nc, q->M + (size_t)pb * nc);
+            if (rot && l == K) {
+                static const int qx[4] = {1, 0, 0, 1}, qy[4] = {1, 1, 0, 0};
+                cz Mq[SP_NCOEF];
+                for (int k = 0; k < 4; k++) {
+                    sp_phase(p, k, c->M + (size_t)b * nc, Mq);
+                    sph_m2m(p, qx[k], qy[k], (o >> 2) & 1, Mq, q->M + (size_t)pb * nc);
+                }
+            } else {
+                sph_m2m(p, (o >> 1) & 1, o & 1, (o >> 2) & 1, c->M + (size_t)b * nc, q->M + (size_t)pb * nc);
+            }
         }
     }
     /* downward: L2L then M2L at each level 2 .. GL (updateAccelerationNoSym :487-510) */
@@ -741,10 +816,22 @@ int orc_fmm_sph(const float* pos4, const float* vel4, const float* acc4, int n, 
                                 for (int jz = 2 * jpz; jz <= 2 * jpz + 1; jz++) {
                                     if (!(jx < ix - 1 || ix + 1 < jx || jy < iy - 1 || iy + 1 < jy || jz < iz - 1 || iz + 1 < jz))
                                         continue;
-                                    if (jx < 0 || jy < 0 || jz < 0 || jx >= width || jy >= width || jz >= width) continue;
-                                    const int sb = sp_find(q, sp_key((uint32_t)jx, (uint32_t)jy, (uint32_t)jz, l));
+                                    int sx = jx, sy = jy, k = 0, lim = width;
+                                    if (rot && l > K) {  /* quadrant map in the domain's local x, y */
+                                        lim = 1 << (l - K);
+                                        k = sp_quadrant(jx, jy, lim, &sx, &sy);
+                                    }
+                                    if (sx < 0 || sy < 0 || jz < 0 || sx >= lim || sy >= lim || jz >= width) continue;
+                                    const int sb = sp_find(q, sp_key((uint32_t)sx, (uint32_t)sy, (uint32_t)jz, l));
                                     if (sb < 0) continue;
-                                    sph_m2l(p, ix - jx, iy - jy, iz - jz, hsz, q->M + (size_t)sb * nc, q->L + (size_t)b * nc);
+                                    if (k) {
+                                        cz Mq[SP_NCOEF];
+                                        sp_phase(p, k, q->M + (size_t)sb * nc, Mq);
+                                        sph_m2l(p, ix - jx, iy - jy, iz - jz, hsz, Mq, q->L + (size_t)b * nc);
+                                    } else {
+                                        sph_m2l(p, ix - jx, iy - jy, iz - jz, hsz, q->M + (size_t)sb * nc,
+                                                q->L + (size_t)b * nc);
+                                    }
                                     m2l_pairs++;
                                 }
         }
@@ -762,13 +849,21 @@ int orc_fmm_sph(const float* pos4, const float* vel4, const float* acc4, int n, 
             const float dmin = boxMin[c] + (float)nd * rootBoxSize;
             cen[c] = (float)((double)dmin + (nl + 0.5) * boxSizeD);
         }
-        int nb_list[27], nnb = 0;
+        int nb_list[27], nb_quad[27], nnb = 0;
         for (int jx = ix - 1; jx <= ix + 1; jx++)
             for (int jy = iy - 1; jy <= iy + 1; jy++)
                 for (int jz = iz - 1; jz <= iz + 1; jz++) {
-                    if (jx < 0 || jy < 0 || jz < 0 || jx >= width || jy >= width || jz >= width) continue;
-                    const int sb = sp_find(leaf, sp_key((uint32_t)jx, (uint32_t)jy, (uint32_t)jz, GL));
-                    if (sb >= 0) nb_list[nnb++] = sb;
+                    int sx = jx, sy = jy, k = 0, lim = width;
+                    if (rot) {  /* the leaf level is below the domains */
+                        lim = side;
+                        k = sp_quadrant(jx, jy, lim, &sx, &sy);
+                    }
+                    if (sx < 0 || sy < 0 || jz < 0 || sx >= lim || sy >= lim || jz >= width) continue;
+                    const int sb = sp_find(leaf, sp_key((uint32_t)sx, (uint32_t)sy, (uint32_t)jz, GL));
+                    if (sb >= 0) {
+                        nb_list[nnb] = sb;
+                        nb_quad[nnb++] = k;
+                    }
                 }
         p2p_pairs += nnb;
         for (int a = leaf->first[b]; a < leaf->first[b] + leaf->count[b]; a++) {
@@ -779,16 +874,25 @@ int orc_fmm_sph(const float* pos4, const float* vel4, const float* acc4, int n, 
             const long double mass = sp == 3 ? 0.0L : orc_species_mass_ld(sp);
             const double ac = (double)((long double)(kc * (double)pi[3]) / mass);
             double tot[3] = {0, 0, 0};
-            /* P2P: DirectForceFunction::direct per neighbour box, in list order */
+            /* P2P: DirectForceFunction::direct per neighbour box, in list order;
+             * rotational: direct_rotational with the source rotated by k pi / 2
+             * in float (the self term of k = 0 is zero there, skipped here) */
             for (int t = 0; t < nnb; t++) {
                 const int sb = nb_list[t];
+                const int k = nb_quad[t];
+                const float ck = (float)sp_qrf(k + 1), sk = (float)sp_qrf(k);
                 double ai[3] = {0, 0, 0};
                 for (int e = leaf->first[sb]; e < leaf->first[sb] + leaf->count[sb]; e++) {
                     const int j = ki[e].idx;
-                    if (j == i) continue;
+                    if (j == i && k == 0) continue;
                     const float* pj = pos4 + 4 * (size_t)j;
-                    const double d0 = (double)(pi[0] - pj[0]);
-                    const double d1 = (double)(pi[1] - pj[1]);
+                    float xj = pj[0], yj = pj[1];
+                    if (k) {
+                        xj = ck * pj[0] - sk * pj[1];
+                        yj = sk * pj[0] + ck * pj[1];
+                    }
+                    const double d0 = (double)(pi[0] - xj);
+                    const double d1 = (double)(pi[1] - yj);
                     const double d2 = (double)(pi[2] - pj[2]);
                     const double invD = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2 + SP_SOFTENING);
                     const double invD3 = (double)pj[3] * invD * invD * invD;

@@ -171,9 +171,9 @@ void HipMultipoleMethodForceCalculation::updateAcceleration() {
     out.assign(3 * (size_t)(lastParticle - firstParticle), 0.0);
     if (lastParticle > firstParticle) {
         if (referenceOperators) {
-            if (sym.type != SHP_SYM_NONE) fail("the reference operators are evaluated without symmetry");
+            if (sym.type == SHP_SYM_TRANSLATIONAL) fail("the reference operators cover no translational symmetry");
             if (shp_fmm_sph_host(handle, &cp.position[0].x, &cp.velocity[0].x, &cp.acceleration[0].x,
-                                 lastParticle, firstParticle, lastParticle, order, forcedMaxLevel,
+                                 lastParticle, firstParticle, lastParticle, order, forcedMaxLevel, &sym,
                                  out.data()) != SHP_OK)
                 fail("shp_fmm_sph_host");
         } else if (shp_fmm_host(handle, &cp.position[0].x, &cp.velocity[0].x, &cp.acceleration[0].x,

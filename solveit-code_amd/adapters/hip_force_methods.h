@@ -121,7 +121,8 @@ public:
     // rotation-based translations, FP64; shp_fmm_sph_host) on the reference's
     // tree, with setMaxLevel as the reference's forcedMaxLevel; false (the
     // default): the engine's Cartesian expansion of the softened kernel.
-    // Without symmetry only.
+    // Without symmetry or with the rotational one of 4 quadrants
+    // (updateAccelerationRotational).
     void setReferenceOperators(bool on) { referenceOperators = on; }
 
 protected:

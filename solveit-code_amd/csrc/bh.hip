@@ -137,8 +137,10 @@ __host__ __device__ inline float bh_max3(float a, float b, float c) {
 // (mn, mx), the first ion of the range and the number of sources:
 // boxMin, the cell dimensions and the cells per axis, plus the bounds seeded
 // with the first ion (minO, maxO).  toggle 0: cubic cells, 1: rectangular prism.
+// rot != 0: computeDomainBoundsRotations (domain_bounds_functions.cpp:317-500),
+// whose x, y extent is the square [-R, R]^2, R the larger of the x and y maxima.
 void domain_bounds_host(const float mn_src[3], const float mx_src[3], const float seed[3], int nsrc,
-                        int toggle, const int* forced, float boxMin[3], float dims[3], int bdv[3],
+                        int toggle, int rot, const int* forced, float boxMin[3], float dims[3], int bdv[3],
                         float minO[3], float maxO[3]) {
     float rootmax[3];
     for (int c = 0; c < 3; c++) {
@@ -146,9 +148,19 @@ void domain_bounds_host(const float mn_src[3], const float mx_src[3], const floa
         rootmax[c] = maxO[c] = mx_src[c] > seed[c] ? mx_src[c] : seed[c];
     }
     const float subtr = 0.0f;
-    for (int c = 0; c < 3; c++) {
-        boxMin[c] = boxMin[c] - subtr;
-        dims[c] = (rootmax[c] - boxMin[c]) + subtr;
+    if (rot) {
+        rootmax[0] = rootmax[0] < rootmax[1] ? rootmax[1] : rootmax[0];
+        rootmax[1] = rootmax[0];
+        boxMin[0] = -rootmax[0];
+        boxMin[1] = -rootmax[1];
+        dims[0] = rootmax[0] * 2;
+        dims[1] = rootmax[1] * 2;
+        dims[2] = rootmax[2] - boxMin[2];
+    } else {
+        for (int c = 0; c < 3; c++) {
+            boxMin[c] = boxMin[c] - subtr;
+            dims[c] = (rootmax[c] - boxMin[c]) + subtr;
+        }
     }
     bdv[0] = bdv[1] = bdv[2] = 1;
     if (toggle == 0) {
@@ -211,7 +223,7 @@ void bh_layout_host(const float mn_src[3], const float mx_src[3], const float se
                     int toggle, const int* forced, BhLayout* L) {
     float boxMin[3], minO[3], maxO[3], dims[3];
     int bdv[3];
-    domain_bounds_host(mn_src, mx_src, seed, nsrc, toggle, forced, boxMin, dims, bdv, minO, maxO);
+    domain_bounds_host(mn_src, mx_src, seed, nsrc, toggle, 0, forced, boxMin, dims, bdv, minO, maxO);
     float roots[3] = {boxMin[0], boxMin[1], boxMin[2]};
     for (int c = 0; c < 3; c++)
         while (roots[c] > minO[c]) {

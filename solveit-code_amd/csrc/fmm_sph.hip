@@ -28,6 +28,17 @@
 // here at every level; for one domain (K = 0) it is the reference's tree.
 // Above the domain level the reference groups domains into ternary
 // supercells; here they are octree levels of the enclosing cube.
+//
+// Rotational symmetry (updateAccelerationRotational, multipole_method_force_
+// calculation_rotational.cpp:9-331): the ions fill quadrant 0 (x >= 0, y >= 0)
+// of one column of domains centred on the z axis (computeDomainBoundsRotations)
+// and quadrants 1-3 hold their images rotated by k pi / 2.  Below the domain
+// level a cell of the local window in quadrant k stands for the real cell it
+// images (getInteractionListRotations :568-1438): its P2P sources are rotated
+// in float (direct_rotational), its multipole is multiplied by i^(-m k) in the
+// M2L gather (m2l_kernel_rotations, fmm_cpukernel_rotational.cpp:298-496), and
+// the M2M into a domain adds the four images of each child (m2m_kernel_rotations
+// :107-290), one dense operator per child z.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -66,14 +77,31 @@ struct SphHostTables {
     int link[kSpNrel];
 };
 
+// a + b c with the product rounded before the sum, as the reference's x86-64
+// build computes it (no fused multiply-add): values that are rounded to float
+// afterwards (box corners, centres, cart2sph arguments) must not be fused, or
+// the float rounding can differ from the reference's.
+__host__ __device__ inline float sp_madd_unfused(float a, float b, float c) {
+#pragma clang fp contract(off)
+    return a + b * c;
+}
+__host__ __device__ inline double sp_madd_unfused(double a, double b, double c) {
+#pragma clang fp contract(off)
+    return a + b * c;
+}
+__host__ __device__ inline double sp_sq3_unfused(double x, double y, double z) {
+#pragma clang fp contract(off)
+    return x * x + y * y + z * z;
+}
+
 // cart2sph_exact (auxiliary_functions.cpp:160-186): float sqrtf / acosf /
 // atanf on double arguments, phi + pi formed in long double (PI is long
 // double, constants.h:28).
 __host__ __device__ inline void sp_cart2sph(double& r, double& th, double& ph, double dx, double dy, double dz) {
-    r = (double)(float)sqrt((double)(float)(dx * dx + dy * dy + dz * dz));
+    r = (double)(float)sqrt((double)(float)sp_sq3_unfused(dx, dy, dz));
     if (r != 0) {
         th = (double)(float)acos((double)(float)(dz / r));
-        if (th == 0 && dz != 0) th = (double)(float)sqrt((double)(float)(dx * dx + dy * dy)) / dz;
+        if (th == 0 && dz != 0) th = (double)(float)sqrt((double)(float)sp_sq3_unfused(dx, dy, 0.0)) / dz;
     } else {
         th = 0;
     }
@@ -262,6 +290,43 @@ __device__ inline double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// The quadrant map of getInteractionListRotations (multipole_method_force_
+// calculation_rotational.cpp:616-634): cell (x, y) of a local window of jd
+// cells per axis lies in quadrant k about jd / 2 and images the real cell
+// (rx, ry) of quadrant 0 rotated by k pi / 2.
+__host__ __device__ inline int sp_quadrant(int x, int y, int jd, int& rx, int& ry) {
+    const int half = jd >> 1;
+    if (x >= half && y >= half) {
+        rx = x;
+        ry = y;
+        return 0;
+    }
+    if (x < half && y >= half) {
+        rx = y;
+        ry = jd - x - 1;
+        return 1;
+    }
+    if (x < half && y < half) {
+        rx = jd - x - 1;
+        ry = jd - y - 1;
+        return 2;
+    }
+    rx = jd - y - 1;
+    ry = x;
+    return 3;
+}
+
+// C * qrf_complex(-m k) = C i^(-m k) (auxiliary_functions.cpp:83-88): a
+// multipole of order m imaged into quadrant k.
+__device__ inline double2 sp_phase(double2 c, int m, int k) {
+    switch ((-m * k) & 3) {
+    case 0: return c;
+    case 1: return make_double2(-c.y, c.x);
+    case 2: return make_double2(-c.x, -c.y);
+    default: return make_double2(c.y, -c.x);
+    }
+}
+
 // MultipoleMethodForceCalculation::rotation (fmm_cpukernel.cpp:10-32).
 __device__ void sp_rotate(const SphTab& t, int p, const double2* C, double2* out, int rot) {
     const double2* Dr = t.Dnm + (size_t)rot * kSpPmax * kSpNe2;
@@ -354,7 +419,10 @@ __device__ void sp_shift_l2l(const SphTab& t, int p, const double2* B, double2* 
 
 // One operator, applied to the unit vector of real component `col`.
 // kind 0: M2M of child octant a; 1: L2L into child octant a; 2: M2L of the
-// offset (a, b, c) = target - source at box size h.
+// offset (a, b, c) = target - source at box size h; 3: the rotational M2M of a
+// child with z bit a into its domain, the sum over its four quadrant images,
+// image k placed at octant (x, y) = (1, 1), (0, 1), (0, 0), (1, 0)
+// (fmm_cpukernel_rotational.cpp:148-215).
 __device__ void sp_apply_op(const SphTab& t, int p, int kind, int a, int b, int c, double h, int col, double2* X,
                             double2* Y) {
     const int nc = p * (p + 1) / 2;
@@ -362,6 +430,30 @@ __device__ void sp_apply_op(const SphTab& t, int p, int kind, int a, int b, int 
     if (col & 1) X[col >> 1].y = 1.0;
     else X[col >> 1].x = 1.0;
     int fwd, bwd;
+    if (kind == 3) {
+        double2 Z[kSpNcMax];
+        for (int i = 0; i < nc; i++) Z[i] = make_double2(0, 0);
+        const int ci = col >> 1;
+        int mc = 0;
+        while ((mc + 1) * (mc + 2) / 2 <= ci) mc++;
+        mc = ci - mc * (mc + 1) / 2;
+        const double2 unit = (col & 1) ? make_double2(0.0, 1.0) : make_double2(1.0, 0.0);
+        for (int k = 0; k < 4; k++) {
+            const int qx = (k == 0 || k == 3) ? 1 : 0, qy = (k < 2) ? 1 : 0;
+            for (int i = 0; i < nc; i++) X[i] = make_double2(0, 0);
+            X[ci] = sp_phase(unit, mc, k);
+            sp_rot_index(t, 6 - 2 * qx, 6 - 2 * qy, 6 - 2 * a, fwd, bwd);
+            sp_rotate(t, p, X, Y, fwd);
+            sp_shift_m2m(t, p, Y, X);
+            sp_rotate(t, p, X, Y, bwd);
+            for (int i = 0; i < nc; i++) {
+                Z[i].x += Y[i].x;
+                Z[i].y += Y[i].y;
+            }
+        }
+        for (int i = 0; i < nc; i++) Y[i] = Z[i];
+        return;
+    }
     if (kind == 0) {
         const int cx = (a >> 1) & 1, cy = a & 1, cz = (a >> 2) & 1;
         sp_rot_index(t, 6 - 2 * cx, 6 - 2 * cy, 6 - 2 * cz, fwd, bwd);
@@ -448,6 +540,7 @@ struct SpGeom {
     int side;
     int bdv[3];
     int GL;
+    int rot;  // rotational symmetry: the quadrant map below the domain level
 };
 
 __device__ inline int sp_f2i(float v) { return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)v : INT_MIN; }
@@ -474,8 +567,8 @@ __device__ inline void sp_leaf_centre(const SpGeom& g, uint64_t key, float cen[3
     sp_unkey(key, gc[0], gc[1], gc[2]);
     for (int a = 0; a < 3; a++) {
         const int nd = gc[a] / g.side, nl = gc[a] % g.side;
-        const float dmin = g.bmin[a] + (float)nd * g.root;
-        cen[a] = (float)((double)dmin + (nl + 0.5) * g.bsd);
+        const float dmin = sp_madd_unfused(g.bmin[a], (float)nd, g.root);
+        cen[a] = (float)sp_madd_unfused((double)dmin, nl + 0.5, g.bsd);
     }
 }
 
@@ -571,7 +664,7 @@ __global__ void k_sp_keys(const float4* __restrict__ P, const int* __restrict__ 
     const float c[3] = {p.x, p.y, p.z};
     int gc[3];
     for (int a = 0; a < 3; a++) {
-        const float dmin = g.bmin[a] + (float)nd[a] * g.root;
+        const float dmin = sp_madd_unfused(g.bmin[a], (float)nd[a], g.root);
         int nl = sp_f2i((c[a] - dmin) / g.bsf);
         if (nl >= g.side) nl--;
         gc[a] = nd[a] * g.side + nl;
@@ -687,8 +780,9 @@ __global__ __launch_bounds__(64) void k_sp_p2m(SphTab t, int p, int KP, SpGeom g
     for (int c = 2 * nc + lane; c < KP; c += 64) Mb[c] = 0.0;
 }
 
-// M2M: each parent sums its children (key order) through the child-octant operators.
-__global__ __launch_bounds__(128) void k_sp_m2m(const double* __restrict__ Top, int KP, int nc2,
+// M2M: each parent sums its children (key order) through the child-octant
+// operators; zop != 0: through the rotational operator of the child's z bit.
+__global__ __launch_bounds__(128) void k_sp_m2m(const double* __restrict__ Top, int KP, int nc2, int zop,
                                                 const uint64_t* __restrict__ ckeys, int ncb, const double* __restrict__ Mc,
                                                 const uint64_t* __restrict__ pkeys, double* __restrict__ Mp) {
     const int pb = blockIdx.x, row = threadIdx.x;
@@ -703,7 +797,8 @@ __global__ __launch_bounds__(128) void k_sp_m2m(const double* __restrict__ Top, 
     double acc = 0.0;
     if (row < nc2)
         for (int c = lo; c < ncb && (ckeys[c] >> 3) == pk; c++) {
-            const double* T = Top + ((size_t)(ckeys[c] & 7u) * KP + row) * KP;
+            const unsigned slot = zop ? (unsigned)((ckeys[c] >> 2) & 1u) : (unsigned)(ckeys[c] & 7u);
+            const double* T = Top + ((size_t)slot * KP + row) * KP;
             const double* m = Mc + (size_t)c * KP;
             double s = 0.0;
             for (int k = 0; k < nc2; k++) s += T[k] * m[k];
@@ -739,15 +834,21 @@ typedef double sp_d4 __attribute__((ext_vector_type(4)));
 // and accumulates T(slot) x M.  Fragments of v_mfma_f64_16x16x4f64 (measured
 // on gfx950): lane l holds A[l % 16][l / 16], B[l / 16][l % 16], and D rows
 // l / 16 + 4 r (r = 0..3) of column l % 16.  ctr[0] counts the box pairs that exist.
+// jd > 0 (rotational, below the domain level): a source cell of the window of
+// jd cells per axis in x, y is mapped to the real cell it images and its
+// multipole is gathered times i^(-m k) (m2l_kernel_rotations,
+// fmm_cpukernel_rotational.cpp:367-372); mord[c] = order m of real component c.
 // Work unit = (tile, chunk of the offset list): with nchunk > 1 each unit
 // writes its 16 x KP partial sums to part[chunk][tile] and k_sp_m2l_reduce adds
 // them to L in chunk order, so a level with few tiles still fills the chip.
 template <int RT>
 __global__ __launch_bounds__(64) void k_sp_m2l(const double* __restrict__ T, const double* __restrict__ M,
                                                double* __restrict__ L, const uint64_t* __restrict__ keys, int nb,
-                                               int level, const int* __restrict__ grp, const int4* __restrict__ tiles,
-                                               int ntiles, int nchunk, double* __restrict__ part,
-                                               const int4* __restrict__ offs, unsigned long long* __restrict__ ctr) {
+                                               int level, int jd, const int* __restrict__ grp,
+                                               const int4* __restrict__ tiles, int ntiles, int nchunk,
+                                               double* __restrict__ part, const int4* __restrict__ offs,
+                                               const signed char* __restrict__ mord,
+                                               unsigned long long* __restrict__ ctr) {
     constexpr int KP = RT * 16, KS = KP / 4;
     const int lane = threadIdx.x, j = lane & 15, kq = lane >> 4;
     const int tile = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
@@ -763,10 +864,17 @@ __global__ __launch_bounds__(64) void k_sp_m2l(const double* __restrict__ T, con
     unsigned long long pairs = 0;
     for (int o = o0; o < o1; o++) {
         const int4 d = od[o];
-        int sb = -1;
+        int sb = -1, quad = 0;
         if (tb >= 0) {
-            const int sx = tx + d.x, sy = ty + d.y, sz = tz + d.z;
-            if (sx >= 0 && sy >= 0 && sz >= 0 && sx < W && sy < W && sz < W) sb = sp_find(keys, nb, sp_key(sx, sy, sz));
+            int sx = tx + d.x, sy = ty + d.y;
+            const int sz = tz + d.z;
+            int lim = W;
+            if (jd > 0) {
+                lim = jd;
+                quad = sp_quadrant(tx + d.x, ty + d.y, jd, sx, sy);
+            }
+            if (sx >= 0 && sy >= 0 && sz >= 0 && sx < lim && sy < lim && sz < W)
+                sb = sp_find(keys, nb, sp_key(sx, sy, sz));
         }
         const unsigned long long have = __ballot(sb >= 0);
         if (have == 0) continue;
@@ -775,7 +883,15 @@ __global__ __launch_bounds__(64) void k_sp_m2l(const double* __restrict__ T, con
         const double* To = T + (size_t)d.w * KP * KP;
         for (int s = 0; s < KS; s++) {
             double bv = 0.0;
-            if (sb >= 0) bv = Ms[4 * s + kq];
+            if (sb >= 0) {
+                const int c = 4 * s + kq;
+                if (quad == 0) {
+                    bv = Ms[c];
+                } else {
+                    const double2 z = sp_phase(make_double2(Ms[c & ~1], Ms[c | 1]), mord[c], quad);
+                    bv = (c & 1) ? z.y : z.x;
+                }
+            }
             for (int r = 0; r < RT; r++) {
                 const double av = To[(size_t)(s * RT + r) * 64 + lane];
                 acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[r], 0, 0, 0);
@@ -811,15 +927,23 @@ __global__ void k_sp_m2l_reduce(const double* __restrict__ part, int ntiles, int
 }
 
 // The 27 neighbour leaves of leaf b in the reference's order (x, then y, then
-// z; -1 where absent).
+// z): 4 * leaf + quadrant (the quadrant map of the rotational lists, leaf
+// window of side cells per axis; 0 without symmetry), -1 where absent.
 __device__ inline void sp_neighbours(const SpGeom& g, const uint64_t* __restrict__ keys, int nb, int b, int* nbr) {
     int ix, iy, iz;
     sp_unkey(keys[b], ix, iy, iz);
     const int W = 1 << g.GL;
     for (int k = 0; k < 27; k++) {
-        const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
-        nbr[k] = (jx >= 0 && jy >= 0 && jz >= 0 && jx < W && jy < W && jz < W) ? sp_find(keys, nb, sp_key(jx, jy, jz))
-                                                                                   : -1;
+        int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, quad = 0, lim = W;
+        const int jz = iz + k % 3 - 1;
+        if (g.rot) {
+            lim = g.side;
+            quad = sp_quadrant(ix + k / 9 - 1, iy + (k / 3) % 3 - 1, lim, jx, jy);
+        }
+        const int f = (jx >= 0 && jy >= 0 && jz >= 0 && jx < lim && jy < lim && jz < W)
+                          ? sp_find(keys, nb, sp_key(jx, jy, jz))
+                          : -1;
+        nbr[k] = f >= 0 ? 4 * f + quad : -1;
     }
 }
 
@@ -837,7 +961,7 @@ __global__ void k_sp_unit_counts(SpGeom g, const uint64_t* __restrict__ keys, in
     int nnb = 0;
     for (int k = 0; k < 27; k++)
         if (nbr[k] >= 0) {
-            near += count[nbr[k]];
+            near += count[nbr[k] >> 2];
             nnb++;
         }
     atomicAdd(ctr + 1, (unsigned long long)nnb);
@@ -864,8 +988,10 @@ __global__ void k_sp_unit_fill(int nb, const int* __restrict__ ucnt, const int* 
 // reference's order the wave stages the leaf's sources in LDS, 64 at a time
 // (one coalesced load), every lane sums its target's pairs over the leaf in
 // FP64 (float displacement, as the reference forms it) and adds K q / m times
-// the leaf's sum.  near[c * ns + a] for the source at key-order position a
-// (status 1 only).
+// the leaf's sum.  A leaf imaged into quadrant k has its sources rotated by
+// k pi / 2 as direct_rotational rotates them (direct_force_method_rotational.
+// cpp:59-75: the float products by cos and sin in {0, 1, -1} are exact).
+// near[c * ns + a] for the source at key-order position a (status 1 only).
 __global__ __launch_bounds__(64) void k_sp_p2p(SpGeom g, double kc, const uint64_t* __restrict__ keys, int nb,
                                                const int* __restrict__ first, const int* __restrict__ count,
                                                const unsigned long long* __restrict__ units,
@@ -889,18 +1015,32 @@ __global__ __launch_bounds__(64) void k_sp_p2p(SpGeom g, double kc, const uint64
     }
     double tot[3] = {0, 0, 0};
     for (int k = 0; k < 27; k++) {
-        const int sb = nbr[k];
-        if (sb < 0) continue;
+        if (nbr[k] < 0) continue;
+        const int sb = nbr[k] >> 2, quad = nbr[k] & 3;
         double ai[3] = {0, 0, 0};
         const int e0 = first[sb], e1 = first[sb] + count[sb];
         for (int s0 = e0; s0 < e1; s0 += 64) {
             __syncthreads();
-            if (s0 + lane < e1) tile[lane] = sp[s0 + lane];
+            if (s0 + lane < e1) {
+                float4 q = sp[s0 + lane];
+                const float x = q.x, y = q.y;
+                if (quad == 1) {
+                    q.x = -y;
+                    q.y = x;
+                } else if (quad == 2) {
+                    q.x = -x;
+                    q.y = -y;
+                } else if (quad == 3) {
+                    q.x = y;
+                    q.y = -x;
+                }
+                tile[lane] = q;
+            }
             __syncthreads();
             const int n = min(64, e1 - s0);
             if (tgt)
                 for (int e = 0; e < n; e++) {
-                    if (s0 + e == a) continue;
+                    if (s0 + e == a && quad == 0) continue;
                     const float4 pj = tile[e];
                     const double d0 = (double)(pi.x - pj.x), d1 = (double)(pi.y - pj.y), d2 = (double)(pi.z - pj.z);
                     const double invD = rsqrt_f64(d0 * d0 + d1 * d1 + d2 * d2 + kSoftening);
@@ -1092,19 +1232,19 @@ void sp_offsets(std::vector<int4>& offs) {
 
 template <int RT>
 void sp_launch_m2l(hipStream_t s, int ntiles, int nchunk, double* part, const double* T, const double* M, double* L,
-                   const uint64_t* keys, int nb, int level, const int* grp, const int4* tiles, const int4* offs,
-                   unsigned long long* ctr) {
-    hipLaunchKernelGGL(k_sp_m2l<RT>, dim3(ntiles * nchunk), dim3(64), 0, s, T, M, L, keys, nb, level, grp, tiles,
-                       ntiles, nchunk, part, offs, ctr);
+                   const uint64_t* keys, int nb, int level, int jd, const int* grp, const int4* tiles,
+                   const int4* offs, const signed char* mord, unsigned long long* ctr) {
+    hipLaunchKernelGGL(k_sp_m2l<RT>, dim3(ntiles * nchunk), dim3(64), 0, s, T, M, L, keys, nb, level, jd, grp, tiles,
+                       ntiles, nchunk, part, offs, mord, ctr);
     if (nchunk > 1)
         hipLaunchKernelGGL(k_sp_m2l_reduce, dim3(sp_bdiv((long)ntiles * 16 * RT * 16, 256)), dim3(256), 0, s, part,
                            ntiles, nchunk, RT * 16, tiles, grp, L);
 }
 
 // Evaluates targets (status 1) in [i0, i1) from the sources (status 1 or 2) in
-// the same range; out[c * m + i - i0] for m = i1 - i0.
+// the same range; out[c * m + i - i0] for m = i1 - i0.  rot: the rotational FMM.
 int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i0, int i1, int p, int forced,
-             double* out) {
+             int rot, double* out) {
     hipStream_t stream = h->stream;
     const int m = i1 - i0;
     h->sph_have_stats = false;
@@ -1157,8 +1297,14 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     {
         const float seed[3] = {seed4.x, seed4.y, seed4.z};
         float dims[3], minO[3], maxO[3];
-        domain_bounds_host(bb, bb + 3, seed, ns, 0, nullptr, g.bmin, dims, g.bdv, minO, maxO);
+        domain_bounds_host(bb, bb + 3, seed, ns, 0, rot, nullptr, g.bmin, dims, g.bdv, minO, maxO);
         g.root = dims[0];
+        g.rot = rot;
+    }
+    if (rot && (g.bdv[0] != 1 || g.bdv[1] != 1)) {
+        set_error("shp_fmm_sph: rotational grid of %d x %d x %d domains (the reference's quadrant map works in "
+                  "domain-local x, y: one domain per x and y)", g.bdv[0], g.bdv[1], g.bdv[2]);
+        return SHP_ERR_UNSUPPORTED;
     }
     const long ndom = (long)g.bdv[0] * g.bdv[1] * g.bdv[2];
     if (ndom < 1 || ndom > kSpMaxDomains) {
@@ -1250,8 +1396,8 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     if (!E) return SHP_ERR_NOMEM;
     double* Mx = E;
     double* Lx = E + etot;
-    // operators: 8 M2M, 8 L2L, then the 343 M2L slots of one level
-    const int nops = 16 + kSpOps;
+    // operators: 8 M2M, 8 L2L, the 343 M2L slots of one level, 2 rotational M2M
+    const int nops = 16 + kSpOps + 2;
     double* Tops = static_cast<double*>(h->get(WS_SPH_OPS, sizeof(double) * (size_t)nops * KP * KP));
     if (!Tops) return SHP_ERR_NOMEM;
     // descriptors and octant offsets (host, uploaded with the tiles)
@@ -1265,6 +1411,16 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
         const bool far = std::abs(dx) > 1 || std::abs(dy) > 1 || std::abs(dz) > 1;
         desc[16 + s] = far ? make_int4(2, dx, dy, dz) : make_int4(-1, 0, 0, 0);
     }
+    desc[16 + kSpOps] = make_int4(3, 0, 0, 0);
+    desc[17 + kSpOps] = make_int4(3, 1, 0, 0);
+    // order m of each real component (the M2L gather's quadrant phase)
+    std::vector<signed char> mord(KP, 0);
+    for (int c = 0; c < nc2; c++) {
+        const int ci = c >> 1;
+        int n = 0;
+        while ((n + 1) * (n + 2) / 2 <= ci) n++;
+        mord[c] = (signed char)(ci - n * (n + 1) / 2);
+    }
     std::vector<int4> offs;
     sp_offsets(offs);
     // tiles: at most nb / 16 + 8 per level
@@ -1272,7 +1428,7 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     for (int l = 2; l <= GL; l++) maxnb = std::max(maxnb, nbl[l]);
     const int maxtiles = maxnb / 16 + 8;
     char* gb = static_cast<char*>(h->get(WS_SPH_GRP, sizeof(int4) * ((size_t)nops + offs.size() + maxtiles) +
-                                                          sizeof(int) * 4 * (size_t)maxnb + 256));
+                                                          sizeof(int) * 4 * (size_t)maxnb + 256 + 128));
     if (!gb) return SHP_ERR_NOMEM;
     int4* ddesc = (int4*)gb;
     int4* doffs = ddesc + nops;
@@ -1281,15 +1437,25 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     unsigned* okey2 = okey + maxnb;
     int* oval = (int*)(okey2 + maxnb);
     int* grp = oval + maxnb;
+    signed char* dmord = (signed char*)(grp + maxnb);
     SHP_HIP_CHECK(hipMemcpyAsync(ddesc, desc.data(), sizeof(int4) * nops, hipMemcpyHostToDevice, stream));
     SHP_HIP_CHECK(hipMemcpyAsync(doffs, offs.data(), sizeof(int4) * offs.size(), hipMemcpyHostToDevice, stream));
+    SHP_HIP_CHECK(hipMemcpyAsync(dmord, mord.data(), (size_t)KP, hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(k_sp_build_ops, dim3(16), dim3(128), 0, stream, S->tab, p, KP, ddesc, 1.0, 0, Tops);
+    double* Trot = Tops + (size_t)(16 + kSpOps) * KP * KP;
+    // the four-image M2M into the domains (level K) is used when it is a level of the tree
+    const bool rot_root = rot && K >= 2;
+    if (rot_root)
+        hipLaunchKernelGGL(k_sp_build_ops, dim3(2), dim3(128), 0, stream, S->tab, p, KP, ddesc + 16 + kSpOps, 1.0, 0,
+                           Trot);
     // P2M, M2M
     hipLaunchKernelGGL(k_sp_p2m, dim3(nbl[GL]), dim3(64), 0, stream, S->tab, p, KP, g, lk + loff[GL], lfirst, lcnt,
                        sp4, Mx + eoff[GL]);
-    for (int l = GL - 1; l >= 2; l--)
-        hipLaunchKernelGGL(k_sp_m2m, dim3(nbl[l]), dim3(128), 0, stream, Tops, KP, nc2, lk + loff[l + 1], nbl[l + 1],
-                           Mx + eoff[l + 1], lk + loff[l], Mx + eoff[l]);
+    for (int l = GL - 1; l >= 2; l--) {
+        const bool zop = rot_root && l == K;
+        hipLaunchKernelGGL(k_sp_m2m, dim3(nbl[l]), dim3(128), 0, stream, zop ? Trot : Tops, KP, nc2, zop ? 1 : 0,
+                           lk + loff[l + 1], nbl[l + 1], Mx + eoff[l + 1], lk + loff[l], Mx + eoff[l]);
+    }
     // downward pass
     double* Tm2l = Tops + (size_t)16 * KP * KP;
     std::vector<int4> tiles;
@@ -1325,6 +1491,7 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
         SHP_HIP_CHECK(hipMemcpyAsync(dtiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, stream));
         const int nt = (int)tiles.size();
         const uint64_t* keys = lk + loff[l];
+        const int jd = (rot && l > K) ? (1 << (l - K)) : 0;  // the quadrant window below the domains
         // offset chunks per tile: about 8 waves per CU over the level, at most 27 (7 offsets each)
         const int nch = std::max(1, std::min(27, (8 * h->num_cus + nt - 1) / nt));
         double* part = nullptr;
@@ -1333,13 +1500,13 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
             if (!part) return SHP_ERR_NOMEM;
         }
         switch (RT) {
-        case 1: sp_launch_m2l<1>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        case 2: sp_launch_m2l<2>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        case 3: sp_launch_m2l<3>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        case 4: sp_launch_m2l<4>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        case 5: sp_launch_m2l<5>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        case 6: sp_launch_m2l<6>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
-        default: sp_launch_m2l<7>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, grp, dtiles, doffs, pctr); break;
+        case 1: sp_launch_m2l<1>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
+        case 2: sp_launch_m2l<2>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
+        case 3: sp_launch_m2l<3>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
+        case 4: sp_launch_m2l<4>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
+        case 5: sp_launch_m2l<5>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
+        case 6: sp_launch_m2l<6>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
+        default: sp_launch_m2l<7>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
         }
         // the tile list is host memory reused by the next level: wait for the copy
         SHP_HIP_CHECK(hipStreamSynchronize(stream));
@@ -1419,11 +1586,19 @@ using namespace shp;
 extern "C" {
 
 int shp_fmm_sph_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4, int n, int i0, int i1,
-                     int order, int max_level, double* out3) {
+                     int order, int max_level, const shp_symmetry* sym, double* out3) {
     SHP_REQUIRE(h && pos4 && vel4 && acc4 && out3, "shp_fmm_sph_host: null argument");
     SHP_REQUIRE(n >= 0 && 0 <= i0 && i0 <= i1 && i1 <= n, "shp_fmm_sph_host: bad range n=%d [%d,%d)", n, i0, i1);
     SHP_REQUIRE(order >= 1 && order <= kSpPmax, "shp_fmm_sph_host: order %d outside 1..%d", order, kSpPmax);
     SHP_REQUIRE(max_level == -2 || max_level >= 0, "shp_fmm_sph_host: max_level %d (-2: from the counts)", max_level);
+    const int stype = sym ? sym->type : SHP_SYM_NONE;
+    SHP_REQUIRE(stype == SHP_SYM_NONE || stype == SHP_SYM_ROTATIONAL || stype == SHP_SYM_TRANSLATIONAL,
+                "shp_fmm_sph_host: symmetry type %d", stype);
+    if (stype == SHP_SYM_TRANSLATIONAL || (stype == SHP_SYM_ROTATIONAL && sym->nsec != 4)) {
+        set_error("shp_fmm_sph_host: the reference operators cover no symmetry and the rotational one with 4 "
+                  "sectors (got type %d, %d sectors)", stype, stype == SHP_SYM_ROTATIONAL ? sym->nsec : 0);
+        return SHP_ERR_UNSUPPORTED;
+    }
     const int m = i1 - i0;
     if (m == 0) return SHP_OK;
     SHP_HIP_CHECK(hipSetDevice(h->device));
@@ -1436,7 +1611,7 @@ int shp_fmm_sph_host(shp_handle h, const float* pos4, const float* vel4, const f
     SHP_HIP_CHECK(hipMemcpyAsync(dvel, vel4, sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
     SHP_HIP_CHECK(hipMemcpyAsync(dacc, acc4, sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
     hipLaunchKernelGGL(k_sp_from_aos, dim3(sp_bdiv(n, 256)), dim3(256), 0, h->stream, dvel, dacc, n, ints, ints + n);
-    int rc = sph_eval(h, dpos, ints, ints + n, i0, i1, order, max_level, dout);
+    int rc = sph_eval(h, dpos, ints, ints + n, i0, i1, order, max_level, stype == SHP_SYM_ROTATIONAL ? 1 : 0, dout);
     if (rc) return rc;
     hipLaunchKernelGGL(k_sp_interleave, dim3(sp_bdiv(m, 256)), dim3(256), 0, h->stream, dout, m, dout + 3 * (size_t)m);
     SHP_HIP_CHECK(hipMemcpyAsync(out3, dout + 3 * (size_t)m, sizeof(double) * 3 * (size_t)m, hipMemcpyDeviceToHost,

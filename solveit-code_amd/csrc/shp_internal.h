@@ -40,9 +40,10 @@ void set_error(const char* fmt, ...);
 // without outliers, from the sources' float bounds (mn, mx), the first ion of
 // the range (seed) and the number of sources: the first root cell's corner,
 // the cell dimensions and the cells per axis (bh.hip).  toggle 0 cubic cells,
-// 1 rectangular prism; forced: NULL or setBoxDimVals' counts.
+// 1 rectangular prism; rot != 0: computeDomainBoundsRotations (:317-500), the
+// x, y extent a square about the z axis; forced: NULL or setBoxDimVals' counts.
 void domain_bounds_host(const float mn_src[3], const float mx_src[3], const float seed[3], int nsrc,
-                        int toggle, const int* forced, float boxMin[3], float dims[3], int bdv[3],
+                        int toggle, int rot, const int* forced, float boxMin[3], float dims[3], int bdv[3],
                         float minO[3], float maxO[3]);
 
 // Leapfrog building blocks shared with the batched driver (leapfrog.hip).

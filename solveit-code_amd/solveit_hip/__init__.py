@@ -234,7 +234,8 @@ def _load():
         "shp_direct_dev_noalloc": (c_int, [c_void_p, ctypes.POINTER(Particles), c_int,
                                            ctypes.POINTER(Symmetry), c_void_p, c_void_p, c_void_p]),
         "shp_bh_stats": (c_int, [c_void_p, dp]),
-        "shp_fmm_sph_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, c_int, c_int, dp]),
+        "shp_fmm_sph_host": (c_int, [c_void_p, fp, fp, fp, c_int, c_int, c_int, c_int, c_int,
+                                     ctypes.POINTER(Symmetry), dp]),
         "shp_fmm_sph_stats": (c_int, [c_void_p, dp]),
     }
     for name, (res, args) in sig.items():
@@ -472,10 +473,13 @@ class Engine:
         _check(rc, "shp_fmm_host")
         return out
 
-    def fmm_sph_host(self, pos4, vel4, acc4, i0=None, i1=None, order=10, max_level=-2):
+    def fmm_sph_host(self, pos4, vel4, acc4, i0=None, i1=None, order=10, max_level=-2, symmetry=None):
         """The reference FMM's own operators (rotation-based spherical harmonics,
         FP64) on host AoS float32 arrays; returns the (i1-i0, 3) float64
-        acceleration.  max_level -2: the reference's setOptimumLevel choice."""
+        acceleration.  max_level -2: the reference's setOptimumLevel choice.
+        symmetry: None or make_symmetry("rotational", 4) (the reference's
+        rotational FMM)."""
+        sym = symmetry if symmetry is not None else make_symmetry("none")
         pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
         vel4 = np.ascontiguousarray(vel4, dtype=np.float32)
         acc4 = np.ascontiguousarray(acc4, dtype=np.float32)
@@ -486,7 +490,7 @@ class Engine:
         fp = ctypes.POINTER(ctypes.c_float)
         rc = lib.shp_fmm_sph_host(self.h, pos4.ctypes.data_as(fp), vel4.ctypes.data_as(fp),
                                   acc4.ctypes.data_as(fp), n, i0, i1, int(order), int(max_level),
-                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+                                  ctypes.byref(sym), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         _check(rc, "shp_fmm_sph_host")
         return out
 

@@ -10,6 +10,7 @@ follow structures.h:224-233; charges are in units of e.
   z ~ U(0, L), r = 100 nm * exp(N(0, 0.5^2)) rejected when r > z tan(20 deg),
   phi ~ U(0, 2 pi); 95 % +1e monomers, 5 % -1e BF4- (87 amu).
 * cell_fill(): uniform fill of a cubic cell plus the cone wrapped into it.
+* fold_sectors(): a plume rotated into the first of nsec sectors about z.
 """
 import numpy as np
 
@@ -83,6 +84,26 @@ def cell_fill(n, side=2e-6, cone_frac=0.5, seed=SEED_BASE + 4):
     q = np.concatenate([np.ones(n - nc), cpl["q"]])
     species = np.concatenate([np.zeros(n - nc, np.int32), cpl["species"]])
     return _pack(pos, q, species, _velocities(rng, n))
+
+
+def fold_sectors(d, nsec):
+    """A copy of plume d with every ion rotated about z into the first of nsec
+    equal sectors, azimuth in [0, 2 pi / nsec) (the state a rotationally
+    symmetric run keeps).  Quadrants (nsec = 4) are folded by exact coordinate
+    swaps and sign changes."""
+    out = dict(d)
+    x, y = np.asarray(d["x"], np.float64), np.asarray(d["y"], np.float64)
+    phi = np.mod(np.arctan2(y, x), 2 * np.pi)
+    k = np.floor(phi / (2 * np.pi / nsec))
+    if nsec == 4:
+        k = np.minimum(k, 3).astype(np.int64)
+        nx = np.choose(k, [x, y, -x, -y])
+        ny = np.choose(k, [y, -x, -y, x])
+    else:
+        c, s = np.cos(-k * 2 * np.pi / nsec), np.sin(-k * 2 * np.pi / nsec)
+        nx, ny = c * x - s * y, s * x + c * y
+    out["x"], out["y"] = np.ascontiguousarray(nx), np.ascontiguousarray(ny)
+    return out
 
 
 def to_reference_aos(d):

@@ -48,13 +48,17 @@ def _cube(n, side, seed, zscale=1.0):
     return pos, vel, acc
 
 
-def _check(eng, pos, vel, acc, p, level=-2, i0=0, i1=None, label=""):
+def _check(eng, pos, vel, acc, p, level=-2, i0=0, i1=None, label="", rotational=False):
+    import solveit_hip
     i1 = pos.shape[0] if i1 is None else i1
-    print(f"case {label}: n={pos.shape[0]} p={p} level={level} range=[{i0},{i1})", flush=True)
-    got = eng.fmm_sph_host(pos, vel, acc, i0=i0, i1=i1, order=p, max_level=level)
+    print(f"case {label}: n={pos.shape[0]} p={p} level={level} range=[{i0},{i1}) rotational={rotational}",
+          flush=True)
+    sym = solveit_hip.make_symmetry("rotational", nsec=4) if rotational else None
+    got = eng.fmm_sph_host(pos, vel, acc, i0=i0, i1=i1, order=p, max_level=level, symmetry=sym)
     st = eng.fmm_sph_stats()
-    ref, info = oracle.fmm_sph(pos, vel, acc, p=p, max_level=level, i0=i0, i1=i1, cr_trig=True)
-    ref_libm, _ = oracle.fmm_sph(pos, vel, acc, p=p, max_level=level, i0=i0, i1=i1)
+    ref, info = oracle.fmm_sph(pos, vel, acc, p=p, max_level=level, i0=i0, i1=i1, cr_trig=True,
+                               rotational=rotational)
+    ref_libm, _ = oracle.fmm_sph(pos, vel, acc, p=p, max_level=level, i0=i0, i1=i1, rotational=rotational)
     for k in ("max_level", "k", "nd0", "nd1", "nd2", "leaves", "m2l_pairs", "p2p_pairs", "sources",
               "boxes"):
         assert st[k] == info[k], (k, st[k], info[k])
@@ -85,6 +89,15 @@ def test_multi_domain_slab(eng):
     """A 4:4:1 slab: a 4 x 4 x 1 domain grid, octree levels above it."""
     pos, vel, acc = _cube(6000, 1e-5, 7, zscale=0.25)
     _check(eng, pos, vel, acc, 10, level=2, label="slab")
+
+
+def test_multi_domain_column(eng):
+    """A column of six domains along z: the float domain corners bmin + nd *
+    rootBoxSize, which the reference rounds product first; a fused
+    multiply-add gives other corners for nd = 3 and 5 on this input."""
+    pos, vel, acc = _cube(6000, 1e-5, 7)
+    pos[:, 2] *= 6.0
+    _check(eng, pos, vel, acc, 8, level=3, label="column, 6 domains")
 
 
 def test_cone_plume_and_accuracy(eng):
@@ -141,3 +154,57 @@ def test_engine_fmm_against_reference_operators(eng):
           flush=True)
     assert e_mine < e_ref
     assert between <= 2.0 * (e_ref + e_mine)
+
+
+# ---- rotational symmetry: the reference's four quadrant images ----
+
+def _quadrant_column(n, side, zlen, seed):
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, 0] = rng.uniform(0, side, n)
+    pos[:, 1] = rng.uniform(0, side, n)
+    pos[:, 2] = rng.uniform(0, zlen, n)
+    pos[:, 3] = np.where(rng.random(n) < 0.05, -1.0, 1.0)
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, 3] = rng.integers(0, 2, n)
+    acc = np.zeros((n, 4), np.float32)
+    acc[:, 3] = 1.0
+    return pos, vel, acc
+
+
+@pytest.mark.parametrize("p", [4, 10])
+def test_rotational_single_domain(eng, p):
+    pos, vel, acc = _quadrant_column(6000, 1e-5, 1.5e-5, 1)
+    _check(eng, pos, vel, acc, p, label="quadrant column, one domain", rotational=True)
+
+
+def test_rotational_domain_column(eng):
+    """Six domains along z (K = 3): the four-image M2M into each domain, plain
+    octree levels above it, the quadrant map below."""
+    pos, vel, acc = _quadrant_column(6000, 1e-5, 1.2e-4, 1)
+    _check(eng, pos, vel, acc, 8, level=3, label="quadrant column, 6 domains", rotational=True)
+
+
+def test_rotational_folded_cone(eng):
+    """The cone folded into the first quadrant: parity with the oracle, and
+    the reference expansion's accuracy against the direct sum over the four
+    quadrant images."""
+    d = synth.fold_sectors(synth.cone(16384, seed=11), 4)
+    pos, vel, acc = synth.to_reference_aos(d)
+    acc[::13, 3] = 2.0
+    got, _ = _check(eng, pos, vel, acc, 10, label="folded cone 16k", rotational=True)
+    direct = oracle.direct_ref(pos, vel, acc, sym="rotational", nsec=4)
+    tgt = acc[:, 3] == 1.0
+    err = oracle.rel_l2(got[tgt], direct[tgt])[0]
+    print(f"  folded cone 16k p=10 vs rotational direct: {err:.2e}", flush=True)
+    assert err < 1e-3
+
+
+def test_rotational_unsupported(eng):
+    import solveit_hip
+    pos, vel, acc = _quadrant_column(3000, 1e-5, 1e-5, 2)  # a 2 x 2 x 1 rotational grid
+    with pytest.raises(solveit_hip.ShpError):
+        eng.fmm_sph_host(pos, vel, acc, order=4, symmetry=solveit_hip.make_symmetry("rotational", nsec=4))
+    pos, vel, acc = _quadrant_column(6000, 1e-5, 1.5e-5, 1)
+    with pytest.raises(solveit_hip.ShpError):
+        eng.fmm_sph_host(pos, vel, acc, order=4, symmetry=solveit_hip.make_symmetry("rotational", nsec=6))

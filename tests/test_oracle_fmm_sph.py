@@ -190,3 +190,81 @@ def test_status_rules_and_range():
     tgt = acc[i0:i1, 3] == 1.0
     assert np.all(a[~tgt] == 0.0)
     assert oracle.rel_l2(a[tgt], ref[tgt])[0] < 5e-5
+
+
+# ---- rotational symmetry (updateAccelerationRotational) ----
+
+def _quadrant_column(n, side, zlen, seed):
+    """Ions in the first quadrant x, y in [0, side), z in [0, zlen)."""
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, 0] = rng.uniform(0, side, n)
+    pos[:, 1] = rng.uniform(0, side, n)
+    pos[:, 2] = rng.uniform(0, zlen, n)
+    pos[:, 3] = np.where(rng.random(n) < 0.05, -1.0, 1.0)
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, 3] = rng.integers(0, 2, n)
+    acc = np.zeros((n, 4), np.float32)
+    acc[:, 3] = 1.0
+    return pos, vel, acc
+
+
+def test_rotational_domain_bounds_hand_case():
+    """computeDomainBoundsRotations: a square [-R, R]^2 about the axis, R the
+    larger of the x and y maxima.  Sources with max x 3 um, max y 5 um and z in
+    [1, 13] um: extents (10, 10, 12) um, the smallest (x) is the first cell size,
+    y and z stay one cell (ratios 1 and 1.2), the size becomes the z extent,
+    then x 1.01 (domainEps) and the grid is centred on the axis in x, y."""
+    rng = np.random.default_rng(3)
+    n = 400
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, 0] = rng.uniform(0.5e-6, 3e-6, n)
+    pos[:, 1] = rng.uniform(0.5e-6, 5e-6, n)
+    pos[:, 2] = rng.uniform(1e-6, 13e-6, n)
+    pos[0, :3] = (1e-6, 1e-6, 1e-6)
+    pos[1, :3] = (3e-6, 5e-6, 13e-6)
+    acc = np.zeros((n, 4), np.float32)
+    acc[:, 3] = 1.0
+    bm, dims, nd, ns = oracle.domain_bounds(pos, acc, rotational=True)
+    f = np.float32
+    R = f(5e-6)
+    size = f(f(13e-6) - f(1e-6)) * f(f(1.0) + f(0.01) / f(1.0))
+    assert tuple(nd) == (1, 1, 1) and ns == n
+    assert dims[0] == size
+    assert bm[0] == f((R + (-R) - size) / f(2)) == bm[1]
+    assert bm[2] == f((f(13e-6) + f(1e-6) - size) / f(2))
+    # the plain bounds of the same ions are not centred on the axis
+    bm0, _, _, _ = oracle.domain_bounds(pos, acc)
+    assert bm0[0] != bm[0]
+
+
+@pytest.mark.parametrize("zlen,level,bar", [(1.5e-5, -2, 1e-5), (4e-5, -2, 1e-5), (1.2e-4, 3, 3e-5)])
+def test_rotational_fmm_converges_to_rotational_direct(zlen, level, bar):
+    """The rotational FMM (quadrant map in the P2P and M2L lists, i^(-mk) phase
+    of imaged multipoles, four-image M2M into the domains) converges with p to
+    the direct sum over the four quadrant images (direct_rotational).  The
+    cases: one domain, two domains along z, and six (K = 3, where the M2M into
+    the domains is a level of the tree)."""
+    pos, vel, acc = _quadrant_column(6000 if zlen != 4e-5 else 12000, 1e-5, zlen, 1)
+    ref = oracle.direct_ref(pos, vel, acc, sym="rotational", nsec=4)
+    errs = []
+    for p in (4, 8, 10):
+        a, info = oracle.fmm_sph(pos, vel, acc, p=p, max_level=level, rotational=True)
+        assert (info["nd0"], info["nd1"]) == (1, 1)
+        errs.append(oracle.rel_l2(a, ref)[0])
+    if zlen == 1.2e-4:
+        assert info["nd2"] == 6 and info["k"] == 3
+    assert errs[0] > errs[1] > errs[2] and errs[2] < bar, errs
+    # without the images the plain FMM misses them entirely
+    a0, _ = oracle.fmm_sph(pos, vel, acc, p=10, max_level=level)
+    assert oracle.rel_l2(a0, ref)[0] > 0.1
+
+
+def test_rotational_wide_grid_is_not_restated():
+    """A quadrant cube has a 2 x 2 x 1 rotational grid, where the reference's
+    quadrant map (domain-local x, y) does not hold."""
+    pos, vel, acc = _quadrant_column(3000, 1e-5, 1e-5, 2)
+    _, _, nd, _ = oracle.domain_bounds(pos, acc, rotational=True)
+    assert tuple(nd) == (2, 2, 1)
+    with pytest.raises(ValueError):
+        oracle.fmm_sph(pos, vel, acc, p=4, rotational=True)
