@@ -199,7 +199,8 @@ int shp_fmm_sph_host(shp_handle h, const float* pos4, const float* vel4, const f
 /* Summary of the last shp_fmm_sph_host call, 12 values: [0] maxLevel, [1]
  * levels above the domain level, [2..4] domains per axis, [5] leaf boxes,
  * [6] M2L box pairs, [7] P2P box pairs, [8] sources, [9] boxes of all levels
- * from level 2, [10..11] reserved.  No counterpart in the reference (its
+ * from level 2, [10] P2P ion pairs (each leaf's ions times its neighbour
+ * list's ions), [11] reserved.  No counterpart in the reference (its
  * timing file records the level and domain counts). */
 int shp_fmm_sph_stats(shp_handle h, double* out);
 

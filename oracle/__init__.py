@@ -793,7 +793,7 @@ def fmm_sph(pos4, vel4, acc4, p=10, max_level=-2, i0=0, i1=None, cr_trig=False, 
     if rc != 0:
         raise ValueError(f"orc_fmm_sph: unsupported case ({rc})")
     keys = ["max_level", "k", "nd0", "nd1", "nd2", "leaves", "m2l_pairs", "p2p_pairs", "sources",
-            "boxes"]
+            "boxes", "p2p_ion_pairs"]
     return out, {k: int(info[i]) for i, k in enumerate(keys)}
 
 

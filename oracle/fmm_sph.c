@@ -624,7 +624,7 @@ static int sp_level_of_count(int c) {
  * the reference's double terms before their float accumulation.
  * info (may be NULL, 12 entries): [0] maxLevel, [1] K, [2..4] boxDimVals,
  * [5] leaf boxes, [6] M2L box pairs, [7] P2P box pairs, [8] sources,
- * [9] boxes over all levels.
+ * [9] boxes over all levels, [10] P2P ion pairs (leaf ions x neighbour-list ions).
  * Returns 0; -1 for p outside 1..10; -2 if a source lies outside the domain
  * grid (the reference then runs its outlier passes, not restated here);
  * -3 for maxLevel < 2 (the reference's single-box and supercell-only paths);
@@ -743,7 +743,7 @@ int orc_fmm_sph(const float* pos4, const float* vel4, const float* acc4, int n, 
             lv[l].L = (cz*)calloc((size_t)lv[l].nb * nc, sizeof(cz));
         }
     }
-    long long m2l_pairs = 0, p2p_pairs = 0, nboxes = 0;
+    long long m2l_pairs = 0, p2p_pairs = 0, nboxes = 0, p2p_ions = 0;
     for (int l = 2; l <= GL; l++) nboxes += lv[l].nb;
 
     /* P2M (box centre in float, distance as a float difference, p2m :316-343) */
@@ -866,6 +866,7 @@ int orc_fmm_sph(const float* pos4, const float* vel4, const float* acc4, int n, 
                     }
                 }
         p2p_pairs += nnb;
+        for (int t = 0; t < nnb; t++) p2p_ions += (long long)leaf->count[b] * leaf->count[nb_list[t]];
         for (int a = leaf->first[b]; a < leaf->first[b] + leaf->count[b]; a++) {
             const int i = ki[a].idx;
             if (acc4[4 * (size_t)i + 3] != 1.0f) continue;
@@ -919,6 +920,7 @@ int orc_fmm_sph(const float* pos4, const float* vel4, const float* acc4, int n, 
         info[7] = p2p_pairs;
         info[8] = ns;
         info[9] = nboxes;
+        info[10] = p2p_ions;
     }
     for (int l = 2; l <= GL; l++) {
         free(lv[l].key);

@@ -61,6 +61,7 @@ constexpr int kSpNrel = 1331;   // numRelativeBox = 11^3
 constexpr int kSpNcMax = 55;    // numCoefficients at p = 10
 constexpr int kSpOps = 343;     // M2L operator slots: offsets in [-3, 3]^3
 constexpr int kSpNoff = 189;    // M2L offsets per child octant
+constexpr int kSpP2pTile = 256; // P2P sources staged in LDS per barrier pair
 constexpr int kSpMaxDomains = 1 << 22;
 constexpr double kPiHi = 3.141592653589793;       // long double pi as a double pair
 constexpr double kPiLo = 1.2246467991473532e-16;
@@ -949,7 +950,7 @@ __device__ inline void sp_neighbours(const SpGeom& g, const uint64_t* __restrict
 
 // P2P work units: every leaf's targets in runs of 64.  ucnt[b] = runs of leaf
 // b; ucost[b] = 64 x its near-field sources (the sort key of its units);
-// ctr[1] counts the box pairs.
+// ctr[1] counts the box pairs, ctr[2] the ion pairs.
 __global__ void k_sp_unit_counts(SpGeom g, const uint64_t* __restrict__ keys, int nb, const int* __restrict__ count,
                                  int* __restrict__ ucnt, unsigned* __restrict__ ucost,
                                  unsigned long long* __restrict__ ctr) {
@@ -965,6 +966,7 @@ __global__ void k_sp_unit_counts(SpGeom g, const uint64_t* __restrict__ keys, in
             nnb++;
         }
     atomicAdd(ctr + 1, (unsigned long long)nnb);
+    atomicAdd(ctr + 2, (unsigned long long)count[b] * (unsigned long long)near);
     ucnt[b] = (count[b] + 63) / 64;
     ucost[b] = (unsigned)min(64L * near, 0xffffffffL);
 }
@@ -985,8 +987,8 @@ __global__ void k_sp_unit_fill(int nb, const int* __restrict__ ucnt, const int* 
 // P2P (p2p_kernel, fmm_cpukernel.cpp:194-219, over DirectForceFunction::direct,
 // direct_force_method.cpp:153-190): one wave per unit of 64 targets of one
 // leaf, longest units first.  For each of the 27 neighbour leaves in the
-// reference's order the wave stages the leaf's sources in LDS, 64 at a time
-// (one coalesced load), every lane sums its target's pairs over the leaf in
+// reference's order the wave stages the leaf's sources in LDS, 256 at a time
+// (coalesced loads), every lane sums its target's pairs over the leaf in
 // FP64 (float displacement, as the reference forms it) and adds K q / m times
 // the leaf's sum.  A leaf imaged into quadrant k has its sources rotated by
 // k pi / 2 as direct_rotational rotates them (direct_force_method_rotational.
@@ -998,7 +1000,7 @@ __global__ __launch_bounds__(64) void k_sp_p2p(SpGeom g, double kc, const uint64
                                                const float4* __restrict__ sp,
                                                const int* __restrict__ sorig, int ns, const int* __restrict__ st,
                                                const int* __restrict__ spc, double* __restrict__ near) {
-    __shared__ float4 tile[64];
+    __shared__ float4 tile[kSpP2pTile];
     __shared__ int nbr[27];
     const int lane = threadIdx.x;
     const unsigned long long uv = units[blockIdx.x];
@@ -1019,28 +1021,33 @@ __global__ __launch_bounds__(64) void k_sp_p2p(SpGeom g, double kc, const uint64
         const int sb = nbr[k] >> 2, quad = nbr[k] & 3;
         double ai[3] = {0, 0, 0};
         const int e0 = first[sb], e1 = first[sb] + count[sb];
-        for (int s0 = e0; s0 < e1; s0 += 64) {
+        for (int s0 = e0; s0 < e1; s0 += kSpP2pTile) {
             __syncthreads();
-            if (s0 + lane < e1) {
-                float4 q = sp[s0 + lane];
-                const float x = q.x, y = q.y;
-                if (quad == 1) {
-                    q.x = -y;
-                    q.y = x;
-                } else if (quad == 2) {
-                    q.x = -x;
-                    q.y = -y;
-                } else if (quad == 3) {
-                    q.x = y;
-                    q.y = -x;
+            for (int t = lane; t < kSpP2pTile; t += 64) {
+                float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (s0 + t < e1) {
+                    q = sp[s0 + t];
+                    const float x = q.x, y = q.y;
+                    if (quad == 1) {
+                        q.x = -y;
+                        q.y = x;
+                    } else if (quad == 2) {
+                        q.x = -x;
+                        q.y = -y;
+                    } else if (quad == 3) {
+                        q.x = y;
+                        q.y = -x;
+                    }
                 }
-                tile[lane] = q;
+                tile[t] = q;
             }
             __syncthreads();
-            const int n = min(64, e1 - s0);
-            if (tgt)
+            // the self pair (zero displacement) adds exactly zero, as the
+            // reference's skip of j == i does; entries past the leaf have zero charge
+            const int n = min(kSpP2pTile, e1 - s0);
+            if (tgt) {
+#pragma unroll 4
                 for (int e = 0; e < n; e++) {
-                    if (s0 + e == a && quad == 0) continue;
                     const float4 pj = tile[e];
                     const double d0 = (double)(pi.x - pj.x), d1 = (double)(pi.y - pj.y), d2 = (double)(pi.z - pj.z);
                     const double invD = rsqrt_f64(d0 * d0 + d1 * d1 + d2 * d2 + kSoftening);
@@ -1049,6 +1056,7 @@ __global__ __launch_bounds__(64) void k_sp_p2p(SpGeom g, double kc, const uint64
                     ai[1] += d1 * invD3;
                     ai[2] += d2 * invD3;
                 }
+            }
         }
         for (int c = 0; c < 3; c++) tot[c] += ac * ai[c];
     }
@@ -1264,7 +1272,7 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     int* nruns = misc + 4;                                  // [4]
     int* ocnt = misc + 8;                                   // [8..15]
     float* box6 = (float*)(misc + 16);                      // [16..21]
-    unsigned long long* pctr = (unsigned long long*)(misc + 24);  // [24..27]: M2L pairs, P2P box pairs
+    unsigned long long* pctr = (unsigned long long*)(misc + 24);  // [24..29]: M2L pairs, P2P box and ion pairs
     hipLaunchKernelGGL(k_sp_flags, dim3(sp_bdiv(m, 256)), dim3(256), 0, stream, st, i0, m, flags);
     hipLaunchKernelGGL(k_sp_zero_i, dim3(1), dim3(64), 0, stream, misc, (size_t)64);
     // WS_SPH_KEYS: keys in/out (2 m u64), vals in/out (2 m int), selected sources (m int)
@@ -1547,7 +1555,7 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
                        lcnt, uval2, sp4, vout, ns, st, spc, near);
     hipLaunchKernelGGL(k_sp_leaf, dim3(sp_bdiv(ns, 64)), dim3(64), 0, stream, S->tab, p, KP, g, k_const_host(),
                        lk + loff[GL], sbox, sp4, vout, ns, st, spc, near, Lx + eoff[GL], i0, m, out);
-    unsigned long long pc[2] = {0, 0};
+    unsigned long long pc[3] = {0, 0, 0};
     SHP_HIP_CHECK(hipMemcpyAsync(pc, pctr, sizeof(pc), hipMemcpyDeviceToHost, stream));
     SHP_HIP_CHECK(hipStreamSynchronize(stream));
     long boxes = 0;
@@ -1563,7 +1571,7 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     sts[7] = (double)pc[1];
     sts[8] = ns;
     sts[9] = (double)boxes;
-    sts[10] = 0;
+    sts[10] = (double)pc[2];
     sts[11] = 0;
     h->sph_have_stats = true;
     return SHP_OK;

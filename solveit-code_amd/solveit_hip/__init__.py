@@ -500,7 +500,7 @@ class Engine:
         _check(lib.shp_fmm_sph_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
                "shp_fmm_sph_stats")
         keys = ["max_level", "k", "nd0", "nd1", "nd2", "leaves", "m2l_pairs", "p2p_pairs", "sources",
-                "boxes"]
+                "boxes", "p2p_ion_pairs"]
         return {k: int(out[i]) for i, k in enumerate(keys)}
 
     def state_upload_host(self, pos4, vel4, acc4, t_inj):

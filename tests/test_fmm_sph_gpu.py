@@ -60,7 +60,7 @@ def _check(eng, pos, vel, acc, p, level=-2, i0=0, i1=None, label="", rotational=
                                rotational=rotational)
     ref_libm, _ = oracle.fmm_sph(pos, vel, acc, p=p, max_level=level, i0=i0, i1=i1, rotational=rotational)
     for k in ("max_level", "k", "nd0", "nd1", "nd2", "leaves", "m2l_pairs", "p2p_pairs", "sources",
-              "boxes"):
+              "boxes", "p2p_ion_pairs"):
         assert st[k] == info[k], (k, st[k], info[k])
     tgt = acc[i0:i1, 3] == 1.0
     assert np.all(got[~tgt] == 0.0)
