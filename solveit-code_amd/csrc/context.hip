@@ -37,7 +37,10 @@ void* shp_context::get(int slot, size_t bytes) {
         ws[slot] = nullptr;
         ws_bytes[slot] = 0;
     }
-    size_t want = bytes + bytes / 8;  // headroom for slowly growing N
+    // Headroom: the FMM's box and unit counts grow from call to call as a
+    // plume expands, and each regrowth costs a stream synchronisation and a
+    // fresh allocation in the middle of an evaluation.
+    size_t want = bytes + bytes / 2;
     hipError_t e = hipMalloc(&ws[slot], want);
     if (e != hipSuccess) {
         ws[slot] = nullptr;
