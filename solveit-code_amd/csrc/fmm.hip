@@ -1345,15 +1345,27 @@ __global__ void k_fmm_scale(float* __restrict__ v, const float* __restrict__ sc,
     if (k < nc) v[(size_t)blockIdx.x * ld + k] *= sc[k];
 }
 
-__global__ void k_fmm_grid_scatter(const uint64_t* __restrict__ key, int nb, int lo0, int lo1, int lo2,
-                                   int d0, int d1, int d2, int* __restrict__ grid) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    const uint64_t k = key[b];
-    const int gx = (int)compact21(k) - lo0, gy = (int)compact21(k >> 1) - lo1,
-              gz = (int)compact21(k >> 2) - lo2;
-    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < d0 && gy < d1 && gz < d2)
-        grid[gx + d0 * (gy + d1 * gz)] = b;
+// Dense cell grids of the levels 0 .. nl-1 in one launch: box b of the
+// concatenated level key arrays (level l holds boxes [boff[l], boff[l+1]))
+// writes its index within its level into that level's grid.
+constexpr int kGridLevels = 24;
+struct GridScatter {
+    const uint64_t* key;  // all levels' keys, level after level
+    int nl;
+    int boff[kGridLevels + 1];
+    int lo[kGridLevels][3], dim[kGridLevels][3];
+    long long goff[kGridLevels];
+};
+__global__ void k_fmm_grid_scatter(const GridScatter S, int* __restrict__ grid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.boff[S.nl]) return;
+    int l = 0;
+    while (i >= S.boff[l + 1]) l++;
+    const uint64_t k = S.key[i];
+    const int gx = (int)compact21(k) - S.lo[l][0], gy = (int)compact21(k >> 1) - S.lo[l][1],
+              gz = (int)compact21(k >> 2) - S.lo[l][2];
+    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < S.dim[l][0] && gy < S.dim[l][1] && gz < S.dim[l][2])
+        grid[S.goff[l] + gx + (long long)S.dim[l][0] * (gy + (long long)S.dim[l][1] * gz)] = i - S.boff[l];
 }
 
 // ------------------------------------------------------------------- L2L ----
@@ -2464,14 +2476,26 @@ static int fmm_eval(shp_handle h, const shp_particles* pt, int order, int leaf_l
     if (unrolled) {
         SHP_WS(grid, int, h, WS_FMM_GRID, gtot);
         hipLaunchKernelGGL(k_fmm_ifill, dim3(fdiv_up((long)gtot, 256)), dim3(256), 0, st, grid, (int)gtot, -1);
+        SHP_REQUIRE(Lb + 1 <= kGridLevels, "shp_fmm_dev: %d levels", Lb + 1);
+        GridScatter S{};
+        S.key = lv[0].key;
+        S.nl = Lb + 1;
         size_t goff = 0;
+        int boff = 0;
         for (int l = 0; l <= Lb; l++) {
             gl[l] = grid + goff;
-            hipLaunchKernelGGL(k_fmm_grid_scatter, dim3(fdiv_up(nbox[l], 256)), dim3(256), 0, st, lv[l].key,
-                               nbox[l], glo[3 * l], glo[3 * l + 1], glo[3 * l + 2], gdim[3 * l],
-                               gdim[3 * l + 1], gdim[3 * l + 2], gl[l]);
+            S.boff[l] = boff;
+            S.goff[l] = (long long)goff;
+            for (int c = 0; c < 3; c++) {
+                S.lo[l][c] = glo[3 * l + c];
+                S.dim[l][c] = gdim[3 * l + c];
+            }
+            SHP_REQUIRE(lv[l].key == S.key + boff, "shp_fmm_dev: level key arrays not contiguous");
+            boff += nbox[l];
             goff += (size_t)gdim[3 * l] * gdim[3 * l + 1] * gdim[3 * l + 2];
         }
+        S.boff[Lb + 1] = boff;
+        hipLaunchKernelGGL(k_fmm_grid_scatter, dim3(fdiv_up(boff, 256)), dim3(256), 0, st, S, grid);
     }
     if (choose) {
         // P2P pairs and M2L far-band box pairs of the candidate leaf levels
