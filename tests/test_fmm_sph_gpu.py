@@ -200,6 +200,15 @@ def test_rotational_folded_cone(eng):
     assert err < 1e-3
 
 
+def test_rotational_status_rules_and_range(eng):
+    """Rotational with status-2 sources, status-0 ions and a sub-range: the
+    images are those of the sources in the range only."""
+    pos, vel, acc = _quadrant_column(5000, 1e-5, 1.5e-5, 3)
+    acc[::7, 3] = 2.0
+    acc[::11, 3] = 0.0
+    _check(eng, pos, vel, acc, 8, level=2, i0=150, i1=4800, label="rotational statuses", rotational=True)
+
+
 def test_rotational_unsupported(eng):
     import solveit_hip
     pos, vel, acc = _quadrant_column(3000, 1e-5, 1e-5, 2)  # a 2 x 2 x 1 rotational grid

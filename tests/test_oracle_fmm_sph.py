@@ -268,3 +268,18 @@ def test_rotational_wide_grid_is_not_restated():
     assert tuple(nd) == (2, 2, 1)
     with pytest.raises(ValueError):
         oracle.fmm_sph(pos, vel, acc, p=4, rotational=True)
+
+
+def test_rotational_status_rules_and_range():
+    """Rotational: status-2 ions are sources only (their images too), status-0
+    ions and ions outside [i0, i1) take no part, as in direct_rotational over
+    the same range."""
+    pos, vel, acc = _quadrant_column(5000, 1e-5, 1.5e-5, 3)
+    acc[::7, 3] = 2.0
+    acc[::11, 3] = 0.0
+    i0, i1 = 150, 4800
+    a, _ = oracle.fmm_sph(pos, vel, acc, p=10, max_level=2, i0=i0, i1=i1, rotational=True)
+    ref = oracle.direct_ref(pos, vel, acc, i0=i0, i1=i1, j0=i0, j1=i1, sym="rotational", nsec=4)
+    tgt = acc[i0:i1, 3] == 1.0
+    assert np.all(a[~tgt] == 0.0)
+    assert oracle.rel_l2(a[tgt], ref[tgt])[0] < 1e-5
