@@ -361,9 +361,9 @@ def run_config(k, args):
         rows = []
         for kind in ("translational", "rotational"):
             if kind == "translational":
-                d = synth.cell_fill(n)
-                sym = P.make_symmetry("translational", lattice=synth.full_lattice(1, (1, 1, 1)),
-                                      domain=(2e-6, 2e-6, 2e-6))
+                d = d_cell = synth.cell_fill(n)
+                sym = sym_cell = P.make_symmetry("translational", lattice=synth.full_lattice(1, (1, 1, 1)),
+                                                 domain=(2e-6, 2e-6, 2e-6))
                 nimg = 27
             else:
                 d = synth.fold_sectors(synth.cone(n), 6)
@@ -404,6 +404,26 @@ def run_config(k, args):
                      "ms_host_in_out": ms, "rel_l2": l2, "max_rel": mx,
                      "vs": "reference-contract direct sum over the 4 images"})
         log(json.dumps(rows[-1]))
+        # The reference's translational FMM with its own operators (ternary tree
+        # of the 2 um cell, lists wrapped periodically) on the cell fill, against
+        # the reference-contract direct sum over the 27 images.  A float
+        # coordinate that rounds onto the +face of [-L/2, L/2) is the same ion
+        # one period lower.
+        posc, velc, accc = synth.to_reference_aos(d_cell)
+        cell = np.float32(2e-6)
+        bmin = -cell / np.float32(2)
+        on_face = (posc[:, :3] - bmin) / cell >= np.float32(1)
+        posc[:, :3] = np.where(on_face, posc[:, :3] - cell, posc[:, :3])
+        ms_refc, a_refc = _timed(lambda: eng.direct_host(posc, velc, accc, symmetry=sym_cell), reps=1)
+        tgtc = accc[:, 3] == 1.0
+        for p in (4, 10):
+            ms, a = _timed(lambda: eng.fmm_sph_host(posc, velc, accc, order=p, symmetry=sym_cell))
+            l2, mx = rel_l2_np(a[tgtc], a_refc[tgtc])
+            rows.append({"kind": "translational_reference_operators", "n": n, "images": 27, "order": p,
+                         "ms_host_in_out": ms, **eng.fmm_sph_stats(), "rel_l2": l2, "max_rel": mx,
+                         "ref_contract_direct_ms": ms_refc,
+                         "vs": "reference-contract direct sum over the 27 images"})
+            log(json.dumps(rows[-1]))
         import oracle
         t0 = time.perf_counter()
         oracle.direct_f64(d, i0=0, i1=128, sym="rotational", nsec=6)

@@ -189,11 +189,20 @@ int shp_fmm_host(shp_handle h, const float* pos4, const float* vel4, const float
  * quadrant of a domain column centred on the z axis (computeDomainBounds-
  * Rotations, domain_bounds_functions.cpp:317-500), the other quadrants their
  * images, mapped back by getInteractionListRotations (:568-1438).
- * SHP_ERR_UNSUPPORTED when a source lies outside the domain grid (the
- * reference's outlier passes), maxLevel < 2 (its single-box paths), for the
- * translational symmetry or other sector counts, and for a rotational grid
- * with more than one domain in x or y (the reference's quadrant map works in
- * domain-local x, y).  Waits. */
+ * SHP_SYM_TRANSLATIONAL: updateAccelerationTranslational (multipole_method_
+ * force_calculation_translational.cpp:9-331, fmm_cpukernel_translational.cpp):
+ * one cubic periodic cell of side domain[0] (= domain[1] = domain[2]) centred
+ * on the origin, a ternary tree of levels 1 .. maxLevel (1..6), lists wrapped
+ * periodically (getInteractionListTranslations :1179-1834), so every target
+ * sees the sources of the 27 cells at shifts {-1, 0, 1}^3; the lattice must be
+ * those 27 points (any order).  Its P2P adds direct_translational over each
+ * wrapped neighbour leaf pair (the reference's p2p_kernel_translations passes
+ * the thread's box range as the ion range and adds nothing, :305-336).
+ * SHP_ERR_UNSUPPORTED when a source lies outside the domain grid or the cell
+ * (the reference's outlier passes), maxLevel < 2 (translational: < 1; its
+ * single-box and supercell paths), other sector counts or image sets, and for
+ * a rotational grid with more than one domain in x or y (the reference's
+ * quadrant map works in domain-local x, y).  Waits. */
 int shp_fmm_sph_host(shp_handle h, const float* pos4, const float* vel4, const float* acc4, int n,
                      int i0, int i1, int order, int max_level, const shp_symmetry* sym, double* out3);
 /* Summary of the last shp_fmm_sph_host call, 12 values: [0] maxLevel, [1]

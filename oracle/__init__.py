@@ -797,6 +797,36 @@ def fmm_sph(pos4, vel4, acc4, p=10, max_level=-2, i0=0, i1=None, cr_trig=False, 
     return out, {k: int(info[i]) for i, k in enumerate(keys)}
 
 
+lib.orc_fmm_sph_trans.restype = _ci
+lib.orc_fmm_sph_trans.argtypes = [_vp, _vp, _vp, _ci, _ci, _ci, _ci, _ci, ctypes.c_float, _ci, _vp, _vp]
+
+
+def fmm_sph_trans(pos4, vel4, acc4, cell, p=10, max_level=-2, i0=0, i1=None, near=True, cr_trig=False):
+    """The reference's translational FMM (ternary tree over one periodic cubic
+    cell of side `cell` on all three axes, lists wrapped periodically) with the
+    operators of fmm_sph; returns (acc[i1-i0,3], info) like fmm_sph.  Every
+    target sees the sources of the 27 cells at shifts {-1, 0, 1}^3.  near=False
+    gives the reference as it runs: its translational P2P passes the thread's
+    box range as the ion range and adds nothing (fmm_cpukernel_translational.
+    cpp:326-332).  Raises for a source outside the cell, maxLevel < 1 or > 6."""
+    pos4, vel4, acc4 = _f32(pos4), _f32(vel4), _f32(acc4)
+    n = pos4.shape[0]
+    i1 = n if i1 is None else i1
+    out = np.zeros((i1 - i0, 3))
+    info = np.zeros(12, np.int64)
+    lib.orc_sph_set_cr_trig(1 if cr_trig else 0)
+    try:
+        rc = lib.orc_fmm_sph_trans(pos4.ctypes.data, vel4.ctypes.data, acc4.ctypes.data, n, i0, i1, p,
+                                   max_level, float(cell), 1 if near else 0, out.ctypes.data, info.ctypes.data)
+    finally:
+        lib.orc_sph_set_cr_trig(0)
+    if rc != 0:
+        raise ValueError(f"orc_fmm_sph_trans: unsupported case ({rc})")
+    keys = ["max_level", "k", "nd0", "nd1", "nd2", "leaves", "m2l_pairs", "p2p_pairs", "sources",
+            "boxes", "p2p_ion_pairs"]
+    return out, {k: int(info[i]) for i, k in enumerate(keys)}
+
+
 def _sph_nc(p):
     return p * (p + 1) // 2
 

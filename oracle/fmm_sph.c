@@ -348,15 +348,19 @@ static void sph_p2m_one(int p, double q, double dx, double dy, double dz, double
         }
 }
 
-/* M2M of one child of octant (cx, cy, cz) into its parent (fmm_cpukernel.cpp:562-605). */
-static void sph_m2m(int p, int cx, int cy, int cz_, const cz* Mc, cz* Mp) {
+/* M2M of one child into its parent: rotation by the 11-grid direction
+ * (rx, ry, rz), the z shift by rho child sizes scaled base^-j, the rotation
+ * back, accumulated (m2m_kernel, fmm_cpukernel.cpp:562-605: octree,
+ * rho = sqrt(3) / 2, base 2; m2m_kernel_translations,
+ * fmm_cpukernel_translational.cpp:507-547: ternary, rho = sqrt(|x-1| + |y-1| +
+ * |z-1|), base 3). */
+static void sph_m2m_g(int p, int rx, int ry, int rz, double rho, double base, const cz* Mc, cz* Mp) {
     int fwd, bwd;
-    rotation_index(6 - 2 * cx, 6 - 2 * cy, 6 - 2 * cz_, &fwd, &bwd);
+    rotation_index(rx, ry, rz, &fwd, &bwd);
     cz A[SP_NCOEF], B[SP_NCOEF];
     sph_rotate(p, Mc, B, fwd);
-    const double rho = sqrt(3.0) / 2;
     for (int j = 0; j < p; j++) {
-        const double powTwo = pow(2.0, -j);
+        const double powB = pow(base, -j);
         for (int k = 0; k <= j; k++) {
             const int jk = j * j + j + k;
             cz s = cz_make(0, 0);
@@ -365,7 +369,7 @@ static void sph_m2m(int p, int cx, int cy, int cz_, const cz* Mc, cz* Mp) {
                 const int jnks = (j - n) * (j - n + 1) / 2 + k;
                 const int nm = n * n + n;
                 const double c0 = pow(-1.0, n) * g_anm[nm] * g_anm[jnk] / g_anm[jk] * pow(rho, n);
-                const cz cnm = cz_scale(cz_scale(g_Ynm[nm], c0), powTwo);
+                const cz cnm = cz_scale(cz_scale(g_Ynm[nm], c0), powB);
                 s = cz_add(s, cz_mul(B[jnks], cnm));
             }
             A[j * (j + 1) / 2 + k] = s;
@@ -373,6 +377,11 @@ static void sph_m2m(int p, int cx, int cy, int cz_, const cz* Mc, cz* Mp) {
     }
     sph_rotate(p, A, B, bwd);
     for (int j = 0; j < p * (p + 1) / 2; j++) Mp[j] = cz_add(Mp[j], B[j]);
+}
+
+/* M2M of one child of octant (cx, cy, cz) into its parent (fmm_cpukernel.cpp:562-605). */
+static void sph_m2m(int p, int cx, int cy, int cz_, const cz* Mc, cz* Mp) {
+    sph_m2m_g(p, 6 - 2 * cx, 6 - 2 * cy, 6 - 2 * cz_, sqrt(3.0) / 2, 2.0, Mc, Mp);
 }
 
 /* M2L of one source box at integer offset (dx, dy, dz) = target - source, in
@@ -422,14 +431,16 @@ static void sph_m2l(int p, int dx, int dy, int dz, double box_size, const cz* Ms
     for (int j = 0; j < p * (p + 1) / 2; j++) Lt[j] = cz_add(Lt[j], LB[j]);
 }
 
-/* L2L from the parent into its child of octant (cx, cy, cz) (fmm_cpukernel.cpp:1096-1125);
- * the child's expansion is overwritten. */
-static void sph_l2l(int p, int cx, int cy, int cz_, const cz* Lp, cz* Lc) {
+/* L2L from the parent into one child: rotation by the 11-grid direction
+ * (rx, ry, rz), the z shift by rho child sizes scaled base^(-n-1), the
+ * rotation back; the child's expansion is overwritten (l2l_kernel,
+ * fmm_cpukernel.cpp:1096-1125: octree; l2l_kernel_translations,
+ * fmm_cpukernel_translational.cpp:795-833: ternary). */
+static void sph_l2l_g(int p, int rx, int ry, int rz, double rho, double base, const cz* Lp, cz* Lc) {
     int fwd, bwd;
-    rotation_index(2 * cx + 4, 2 * cy + 4, 2 * cz_ + 4, &fwd, &bwd);
+    rotation_index(rx, ry, rz, &fwd, &bwd);
     cz A[SP_NCOEF], B[SP_NCOEF];
     sph_rotate(p, Lp, B, fwd);
-    const double rho = sqrt(3.0) / 2;
     for (int j = 0; j < p; j++) {
         for (int k = 0; k <= j; k++) {
             const int jk = j * j + j + k;
@@ -439,7 +450,7 @@ static void sph_l2l(int p, int cx, int cy, int cz_, const cz* Lp, cz* Lc) {
                 const int nk = n * n + n + k;
                 /* anm[jnk] * anm[jk] / anm[nk] is float arithmetic in the reference */
                 const float f = g_anm[jnk] * g_anm[jk] / g_anm[nk];
-                const cz cnm = cz_scale(cz_scale(g_Ynm[jnk], (double)f * pow(rho, n - j)), pow(2.0, -n - 1));
+                const cz cnm = cz_scale(cz_scale(g_Ynm[jnk], (double)f * pow(rho, n - j)), pow(base, -n - 1));
                 s = cz_add(s, cz_mul(B[n * (n + 1) / 2 + k], cnm));
             }
             A[j * (j + 1) / 2 + k] = s;
@@ -447,6 +458,11 @@ static void sph_l2l(int p, int cx, int cy, int cz_, const cz* Lp, cz* Lc) {
     }
     sph_rotate(p, A, B, bwd);
     memcpy(Lc, B, sizeof(cz) * (size_t)(p * (p + 1) / 2));
+}
+
+/* L2L from the parent into its child of octant (cx, cy, cz) (fmm_cpukernel.cpp:1096-1125). */
+static void sph_l2l(int p, int cx, int cy, int cz_, const cz* Lp, cz* Lc) {
+    sph_l2l_g(p, 2 * cx + 4, 2 * cy + 4, 2 * cz_ + 4, sqrt(3.0) / 2, 2.0, Lp, Lc);
 }
 
 /* L2P of one particle (fmm_cpukernel.cpp:1243-1303): the field vector accel
@@ -923,6 +939,299 @@ int orc_fmm_sph(const float* pos4, const float* vel4, const float* acc4, int n, 
         info[10] = p2p_ions;
     }
     for (int l = 2; l <= GL; l++) {
+        free(lv[l].key);
+        free(lv[l].M);
+        free(lv[l].L);
+    }
+    free(leaf->first);
+    free(leaf->count);
+    free(lv);
+    free(ki);
+    return 0;
+}
+
+/* ------------------------------------------------------ translational FMM -- */
+
+/* Base-27 cell index of the ternary tree (morton3, multipole_method_functions.cpp:49-65):
+ * level digit 9 z + 3 x + y, finest level least significant. */
+static uint64_t sp_tkey(int x, int y, int z, int levels) {
+    uint64_t k = 0, w = 1;
+    for (int i = 0; i < levels; i++) {
+        k += (uint64_t)(9 * (z % 3) + 3 * (x % 3) + (y % 3)) * w;
+        x /= 3;
+        y /= 3;
+        z /= 3;
+        w *= 27;
+    }
+    return k;
+}
+/* unmorton3 (multipole_method_functions.cpp:68-86). */
+static void sp_tunkey(uint64_t k, int* x, int* y, int* z) {
+    int a = 0, b = 0, c = 0, w = 1;
+    while (k) {
+        const int d = (int)(k % 27);
+        k /= 27;
+        b += (d % 3) * w;
+        a += ((d / 3) % 3) * w;
+        c += (d / 9) * w;
+        w *= 3;
+    }
+    *x = a;
+    *y = b;
+    *z = c;
+}
+static int sp_wrap(int v, int w, int* shift) {
+    *shift = 0;
+    while (v < 0) { v += w; (*shift)--; }
+    while (v >= w) { v -= w; (*shift)++; }
+    return v;
+}
+
+/*
+ * The reference's translational FMM (updateAccelerationTranslational,
+ * multipole_method_force_calculation_translational.cpp:9-331) for a cubic
+ * periodic cell of side `cell` on all three axes (symmetryAxes (1, 1, 1),
+ * numSymmetryLevels >= 1): one domain, boxMin = -cell / 2 on every axis
+ * (computeDomainBoundsTranslations, domain_bounds_functions.cpp:637-661),
+ * a ternary tree of levels 1 .. maxLevel (3^l cells per axis, float leaf
+ * placement of mortonTranslations :669-715, level from setOptimumLevel-
+ * Translations :521-567), lists that wrap periodically (getInteraction-
+ * ListTranslations :1217-1334 P2P, :1725-1834 M2L_Subcell) with the image
+ * shift in the P2P list, and the operators of fmm_cpukernel_translational.cpp:
+ * P2M :344-446, M2M :455-558 (ternary shift, 3^-j), M2L :567-748 (offsets up
+ * to 5 boxes, gcd-reduced rotation), L2L :756-835 (3^(-n-1)), L2P :843-973.
+ * Every target then sees the sources of the 27 cells at shifts {-1, 0, 1}^3.
+ * near != 0: the P2P adds direct_translational (direct_force_method_
+ * translational.cpp:320-367) over each listed leaf pair, the target shifted
+ * by -shift * cell in float.  near == 0: the reference as it runs, where
+ * p2p_kernel_translations (fmm_cpukernel_translational.cpp:305-336) passes
+ * the thread's box range in place of the leaf pair's ion bounds, so the
+ * source loop (j from numBoxIndex up to 0) is empty and P2P adds nothing.
+ * out3, info and the ranges as orc_fmm_sph (info[1] = 0, [2..4] = 1).
+ * Returns 0; -1 bad arguments; -2 a source outside the cell (the
+ * reference's outlier passes); -3 maxLevel < 1 (its supercell-only path)
+ * or > 6 (its int box index overflows at 27^7).
+ */
+int orc_fmm_sph_trans(const float* pos4, const float* vel4, const float* acc4, int n, int i0, int i1, int p,
+                      int forced_level, float cell, int near, double* out3, long long* info) {
+    if (p < 1 || p > SP_PMAX || i1 < i0 || i0 < 0 || i1 > n || !(cell > 0)) return -1;
+    sph_precalc();
+    const int nc = p * (p + 1) / 2;
+    memset(out3, 0, sizeof(double) * 3 * (size_t)(i1 - i0));
+    if (info) memset(info, 0, sizeof(long long) * 12);
+    const float rootBoxSize = cell;
+    const float bmin = -rootBoxSize / 2;
+    int nsrc = 0;
+    for (int i = i0; i < i1; i++) {
+        const float s = acc4[4 * (size_t)i + 3];
+        if (!(s == 1.0f || s == 2.0f)) continue;
+        for (int c = 0; c < 3; c++) {
+            const float t = (pos4[4 * (size_t)i + c] - bmin) / rootBoxSize;
+            int v = (int)t;
+            if (t < 0) v--;
+            if (v != 0) return -2;
+        }
+        nsrc++;
+    }
+    if (nsrc == 0) return 0;
+    int maxLevel = sp_level_of_count(nsrc);
+    if (forced_level != -2) maxLevel = forced_level;
+    if (maxLevel < 1 || maxLevel > 6) return -3;
+    const int GL = maxLevel;
+    int side = 1;
+    for (int l = 0; l < GL; l++) side *= 3;
+    const float numTotalBoxes = (float)pow(3.0, GL);
+    const float boxSizeF = rootBoxSize / numTotalBoxes;          /* mortonTranslations: float */
+    const double boxSizeD = rootBoxSize / pow(3.0, GL);          /* p2m / l2p: double */
+
+    sp_ki* ki = (sp_ki*)malloc(sizeof(sp_ki) * (size_t)nsrc);
+    int ns = 0;
+    for (int i = i0; i < i1; i++) {
+        const float s = acc4[4 * (size_t)i + 3];
+        if (!(s == 1.0f || s == 2.0f)) continue;
+        int g[3];
+        for (int c = 0; c < 3; c++) {
+            int v = (int)((pos4[4 * (size_t)i + c] - bmin) / boxSizeF);
+            if (v >= numTotalBoxes) v--;
+            g[c] = v;
+        }
+        ki[ns].key = sp_tkey(g[0], g[1], g[2], GL);
+        ki[ns].idx = i;
+        ns++;
+    }
+    qsort(ki, (size_t)ns, sizeof(sp_ki), sp_cmp);
+
+    sp_level* lv = (sp_level*)calloc((size_t)GL + 1, sizeof(sp_level));
+    sp_level* leaf = &lv[GL];
+    leaf->key = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)ns);
+    leaf->first = (int*)malloc(sizeof(int) * (size_t)ns);
+    leaf->count = (int*)malloc(sizeof(int) * (size_t)ns);
+    {
+        int nb = 0;
+        for (int a = 0; a < ns; a++) {
+            if (a == 0 || ki[a].key != ki[a - 1].key) {
+                leaf->key[nb] = ki[a].key;
+                leaf->first[nb] = a;
+                leaf->count[nb] = 0;
+                nb++;
+            }
+            leaf->count[nb - 1]++;
+        }
+        leaf->nb = nb;
+    }
+    for (int l = GL - 1; l >= 1; l--) {
+        sp_level* c = &lv[l + 1];
+        sp_level* q = &lv[l];
+        q->key = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)c->nb);
+        int m = 0;
+        for (int b = 0; b < c->nb; b++) {
+            const uint64_t k = c->key[b] / 27;
+            if (m == 0 || q->key[m - 1] != k) q->key[m++] = k;
+        }
+        q->nb = m;
+    }
+    long long m2l_pairs = 0, p2p_pairs = 0, nboxes = 0, p2p_ions = 0;
+    for (int l = 1; l <= GL; l++) {
+        lv[l].M = (cz*)calloc((size_t)lv[l].nb * nc, sizeof(cz));
+        lv[l].L = (cz*)calloc((size_t)lv[l].nb * nc, sizeof(cz));
+        nboxes += lv[l].nb;
+    }
+    /* P2M: the leaf centre as p2m_kernel_translations forms it (:387-392) */
+    for (int b = 0; b < leaf->nb; b++) {
+        int g[3];
+        sp_tunkey(leaf->key[b], &g[0], &g[1], &g[2]);
+        float cen[3];
+        for (int c = 0; c < 3; c++) cen[c] = (float)((double)bmin + (g[c] + 0.5) * boxSizeD);
+        cz* M = leaf->M + (size_t)b * nc;
+        for (int a = leaf->first[b]; a < leaf->first[b] + leaf->count[b]; a++) {
+            const float* q = pos4 + 4 * (size_t)ki[a].idx;
+            const float dx = q[0] - cen[0], dy = q[1] - cen[1], dz = q[2] - cen[2];
+            sph_p2m_one(p, (double)q[3], dx, dy, dz, boxSizeD, M);
+        }
+    }
+    /* M2M down to level 1 (the loop of :180-210 stops at max(0, 2 - numLevelSC) = 1) */
+    for (int l = GL - 1; l >= 1; l--) {
+        const sp_level* c = &lv[l + 1];
+        sp_level* q = &lv[l];
+        int pb = 0;
+        for (int b = 0; b < c->nb; b++) {
+            const uint64_t pk = c->key[b] / 27;
+            while (q->key[pb] != pk) pb++;
+            int x, y, z;
+            sp_tunkey(c->key[b] % 27, &x, &y, &z);
+            const double rho = sqrt((double)(abs(x - 1) + abs(y - 1) + abs(z - 1)));
+            sph_m2m_g(p, 6 - x, 6 - y, 6 - z, rho, 3.0, c->M + (size_t)b * nc, q->M + (size_t)pb * nc);
+        }
+    }
+    /* downward: M2L at level 1, then L2L and M2L at 2 .. maxLevel (:242-313) */
+    for (int l = 1; l <= GL; l++) {
+        sp_level* q = &lv[l];
+        int width = 1;
+        for (int i = 0; i < l; i++) width *= 3;
+        const double hsz = rootBoxSize / pow(3.0, l);
+        if (l > 1) {
+            const sp_level* up = &lv[l - 1];
+            int pb = 0;
+            for (int b = 0; b < q->nb; b++) {
+                const uint64_t pk = q->key[b] / 27;
+                while (up->key[pb] != pk) pb++;
+                int x, y, z;
+                sp_tunkey(q->key[b] % 27, &x, &y, &z);
+                const double rho = sqrt((double)(abs(x - 1) + abs(y - 1) + abs(z - 1)));
+                sph_l2l_g(p, x + 4, y + 4, z + 4, rho, 3.0, up->L + (size_t)pb * nc, q->L + (size_t)b * nc);
+            }
+        }
+        for (int b = 0; b < q->nb; b++) {
+            int ix, iy, iz;
+            sp_tunkey(q->key[b], &ix, &iy, &iz);
+            const int px = ix / 3, py = iy / 3, pz = iz / 3;
+            for (int jpx = px - 1; jpx <= px + 1; jpx++)
+                for (int jpy = py - 1; jpy <= py + 1; jpy++)
+                    for (int jpz = pz - 1; jpz <= pz + 1; jpz++)
+                        for (int jx = 3 * jpx; jx <= 3 * jpx + 2; jx++)
+                            for (int jy = 3 * jpy; jy <= 3 * jpy + 2; jy++)
+                                for (int jz = 3 * jpz; jz <= 3 * jpz + 2; jz++) {
+                                    if (!(jx < ix - 1 || ix + 1 < jx || jy < iy - 1 || iy + 1 < jy || jz < iz - 1 || iz + 1 < jz))
+                                        continue;
+                                    int s0, s1, s2;
+                                    const int sx = sp_wrap(jx, width, &s0), sy = sp_wrap(jy, width, &s1),
+                                              sz = sp_wrap(jz, width, &s2);
+                                    const int sb = sp_find(q, sp_tkey(sx, sy, sz, l));
+                                    if (sb < 0) continue;
+                                    sph_m2l(p, ix - jx, iy - jy, iz - jz, hsz, q->M + (size_t)sb * nc,
+                                            q->L + (size_t)b * nc);
+                                    m2l_pairs++;
+                                }
+        }
+    }
+    /* L2P and P2P at the leaves */
+    const double kc = orc_k_const();
+    for (int b = 0; b < leaf->nb; b++) {
+        int ix, iy, iz;
+        sp_tunkey(leaf->key[b], &ix, &iy, &iz);
+        const int g[3] = {ix, iy, iz};
+        float cen[3];
+        for (int c = 0; c < 3; c++) cen[c] = (float)((double)bmin + (g[c] + 0.5) * boxSizeD);
+        int nb_list[27], nb_shift[27][3], nnb = 0;
+        for (int jx = ix - 1; jx <= ix + 1; jx++)
+            for (int jy = iy - 1; jy <= iy + 1; jy++)
+                for (int jz = iz - 1; jz <= iz + 1; jz++) {
+                    int s0, s1, s2;
+                    const int sx = sp_wrap(jx, side, &s0), sy = sp_wrap(jy, side, &s1), sz = sp_wrap(jz, side, &s2);
+                    const int sb = sp_find(leaf, sp_tkey(sx, sy, sz, GL));
+                    if (sb < 0) continue;
+                    nb_list[nnb] = sb;
+                    nb_shift[nnb][0] = s0;
+                    nb_shift[nnb][1] = s1;
+                    nb_shift[nnb][2] = s2;
+                    nnb++;
+                }
+        p2p_pairs += nnb;
+        for (int t = 0; t < nnb; t++) p2p_ions += (long long)leaf->count[b] * leaf->count[nb_list[t]];
+        for (int a = leaf->first[b]; a < leaf->first[b] + leaf->count[b]; a++) {
+            const int i = ki[a].idx;
+            if (acc4[4 * (size_t)i + 3] != 1.0f) continue;
+            const float* pi = pos4 + 4 * (size_t)i;
+            const int sp = (int)vel4[4 * (size_t)i + 3];
+            const long double mass = sp == 3 ? 0.0L : orc_species_mass_ld(sp);
+            const double ac = (double)((long double)(kc * (double)pi[3]) / mass);
+            double tot[3] = {0, 0, 0};
+            for (int t = 0; near && t < nnb; t++) {
+                const int sb = nb_list[t];
+                const float off[3] = {(float)nb_shift[t][0] * cell, (float)nb_shift[t][1] * cell,
+                                      (float)nb_shift[t][2] * cell};
+                const float xi = pi[0] - off[0], yi = pi[1] - off[1], zi = pi[2] - off[2];
+                double ai[3] = {0, 0, 0};
+                for (int e = leaf->first[sb]; e < leaf->first[sb] + leaf->count[sb]; e++) {
+                    const float* pj = pos4 + 4 * (size_t)ki[e].idx;
+                    const double d0 = (double)(xi - pj[0]);
+                    const double d1 = (double)(yi - pj[1]);
+                    const double d2 = (double)(zi - pj[2]);
+                    const double invD = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2 + SP_SOFTENING);
+                    const double invD3 = (double)pj[3] * invD * invD * invD;
+                    ai[0] += d0 * invD3;
+                    ai[1] += d1 * invD3;
+                    ai[2] += d2 * invD3;
+                }
+                for (int c = 0; c < 3; c++) tot[c] += ac * ai[c];
+            }
+            double f[3];
+            const float dx = pi[0] - cen[0], dy = pi[1] - cen[1], dz = pi[2] - cen[2];
+            sph_l2p_one(p, leaf->L + (size_t)b * nc, dx, dy, dz, boxSizeD, f);
+            for (int c = 0; c < 3; c++) out3[3 * (size_t)(i - i0) + c] = tot[c] - ac * f[c];
+        }
+    }
+    if (info) {
+        info[0] = maxLevel;
+        info[2] = info[3] = info[4] = 1;
+        info[5] = leaf->nb;
+        info[6] = m2l_pairs;
+        info[7] = p2p_pairs;
+        info[8] = ns;
+        info[9] = nboxes;
+        info[10] = p2p_ions;
+    }
+    for (int l = 1; l <= GL; l++) {
         free(lv[l].key);
         free(lv[l].M);
         free(lv[l].L);

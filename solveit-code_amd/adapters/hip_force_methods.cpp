@@ -171,9 +171,21 @@ void HipMultipoleMethodForceCalculation::updateAcceleration() {
     out.assign(3 * (size_t)(lastParticle - firstParticle), 0.0);
     if (lastParticle > firstParticle) {
         if (referenceOperators) {
-            if (sym.type == SHP_SYM_TRANSLATIONAL) fail("the reference operators cover no translational symmetry");
+            // updateAccelerationTranslational reaches the 27 cells at shifts
+            // {-1, 0, 1}^3 whatever numSymmetryLevels (>= 1) the Grid expands to
+            shp_symmetry rs = sym;
+            int lat27[81];
+            if (sym.type == SHP_SYM_TRANSLATIONAL) {
+                for (int k = 0; k < 27; k++) {
+                    lat27[3 * k] = k / 9 - 1;
+                    lat27[3 * k + 1] = (k / 3) % 3 - 1;
+                    lat27[3 * k + 2] = k % 3 - 1;
+                }
+                rs.nimg = 27;
+                rs.lattice = lat27;
+            }
             if (shp_fmm_sph_host(handle, &cp.position[0].x, &cp.velocity[0].x, &cp.acceleration[0].x,
-                                 lastParticle, firstParticle, lastParticle, order, forcedMaxLevel, &sym,
+                                 lastParticle, firstParticle, lastParticle, order, forcedMaxLevel, &rs,
                                  out.data()) != SHP_OK)
                 fail("shp_fmm_sph_host");
         } else if (shp_fmm_host(handle, &cp.position[0].x, &cp.velocity[0].x, &cp.acceleration[0].x,

@@ -352,10 +352,12 @@ __device__ void sp_rotate(const SphTab& t, int p, const double2* C, double2* out
 
 // The z shifts of m2m_kernel (fmm_cpukernel.cpp:587-601), m2l_kernel (:878-897)
 // and l2l_kernel (:1100-1116), in their arithmetic order.
-__device__ void sp_shift_m2m(const SphTab& t, int p, const double2* B, double2* A) {
-    const double rho = sqrt(3.0) / 2;
+// z shift of a multipole by rho child sizes, scaled base^-j (octree: sqrt(3) / 2
+// and 2, m2m_kernel; ternary: sqrt(|x-1| + |y-1| + |z-1|) and 3,
+// m2m_kernel_translations, fmm_cpukernel_translational.cpp:526-543).
+__device__ void sp_shift_m2m(const SphTab& t, int p, double rho, double base, const double2* B, double2* A) {
     for (int j = 0; j < p; j++) {
-        const double powTwo = pow(2.0, -j);
+        const double powTwo = pow(base, -j);
         for (int k = 0; k <= j; k++) {
             const int jk = j * j + j + k;
             double2 s = make_double2(0, 0);
@@ -398,8 +400,9 @@ __device__ void sp_shift_m2l(const SphTab& t, int p, double rho, const double2* 
     }
 }
 
-__device__ void sp_shift_l2l(const SphTab& t, int p, const double2* B, double2* A) {
-    const double rho = sqrt(3.0) / 2;
+// z shift of a local expansion by rho child sizes, scaled base^(-n-1) (l2l_kernel;
+// l2l_kernel_translations, fmm_cpukernel_translational.cpp:812-826).
+__device__ void sp_shift_l2l(const SphTab& t, int p, double rho, double base, const double2* B, double2* A) {
     for (int j = 0; j < p; j++)
         for (int k = 0; k <= j; k++) {
             const int jk = j * j + j + k;
@@ -409,7 +412,7 @@ __device__ void sp_shift_l2l(const SphTab& t, int p, const double2* B, double2* 
                 const int nk = n * n + n + k;
                 const float f = t.anm[jnk] * t.anm[jk] / t.anm[nk];  // float arithmetic in the reference
                 const double c = (double)f * pow(rho, n - j);
-                const double pw = pow(2.0, -n - 1);
+                const double pw = pow(base, -n - 1);
                 const double2 v = cmul(B[n * (n + 1) / 2 + k], make_double2(t.Ynm[jnk] * c * pw, 0.0 * c * pw));
                 s.x += v.x;
                 s.y += v.y;
@@ -419,7 +422,8 @@ __device__ void sp_shift_l2l(const SphTab& t, int p, const double2* B, double2* 
 }
 
 // One operator, applied to the unit vector of real component `col`.
-// kind 0: M2M of child octant a; 1: L2L into child octant a; 2: M2L of the
+// kind 0: M2M of child octant a; 1: L2L into child octant a; 4 / 5: M2M / L2L
+// of the ternary child of base-27 digit a; 2: M2L of the
 // offset (a, b, c) = target - source at box size h; 3: the rotational M2M of a
 // child with z bit a into its domain, the sum over its four quadrant images,
 // image k placed at octant (x, y) = (1, 1), (0, 1), (0, 0), (1, 0)
@@ -445,7 +449,7 @@ __device__ void sp_apply_op(const SphTab& t, int p, int kind, int a, int b, int 
             X[ci] = sp_phase(unit, mc, k);
             sp_rot_index(t, 6 - 2 * qx, 6 - 2 * qy, 6 - 2 * a, fwd, bwd);
             sp_rotate(t, p, X, Y, fwd);
-            sp_shift_m2m(t, p, Y, X);
+            sp_shift_m2m(t, p, sqrt(3.0) / 2, 2.0, Y, X);
             sp_rotate(t, p, X, Y, bwd);
             for (int i = 0; i < nc; i++) {
                 Z[i].x += Y[i].x;
@@ -459,13 +463,23 @@ __device__ void sp_apply_op(const SphTab& t, int p, int kind, int a, int b, int 
         const int cx = (a >> 1) & 1, cy = a & 1, cz = (a >> 2) & 1;
         sp_rot_index(t, 6 - 2 * cx, 6 - 2 * cy, 6 - 2 * cz, fwd, bwd);
         sp_rotate(t, p, X, Y, fwd);
-        sp_shift_m2m(t, p, Y, X);
+        sp_shift_m2m(t, p, sqrt(3.0) / 2, 2.0, Y, X);
         sp_rotate(t, p, X, Y, bwd);
     } else if (kind == 1) {
         const int cx = (a >> 1) & 1, cy = a & 1, cz = (a >> 2) & 1;
         sp_rot_index(t, 2 * cx + 4, 2 * cy + 4, 2 * cz + 4, fwd, bwd);
         sp_rotate(t, p, X, Y, fwd);
-        sp_shift_l2l(t, p, Y, X);
+        sp_shift_l2l(t, p, sqrt(3.0) / 2, 2.0, Y, X);
+        sp_rotate(t, p, X, Y, bwd);
+    } else if (kind == 4 || kind == 5) {
+        // ternary child of base-27 digit a = 9 z + 3 x + y (unmorton3 of nfjc / nfic)
+        const int cx = (a / 3) % 3, cy = a % 3, cz = a / 9;
+        const double rho = sqrt((double)(abs(cx - 1) + abs(cy - 1) + abs(cz - 1)));
+        if (kind == 4) sp_rot_index(t, 6 - cx, 6 - cy, 6 - cz, fwd, bwd);
+        else sp_rot_index(t, cx + 4, cy + 4, cz + 4, fwd, bwd);
+        sp_rotate(t, p, X, Y, fwd);
+        if (kind == 4) sp_shift_m2m(t, p, rho, 3.0, Y, X);
+        else sp_shift_l2l(t, p, rho, 3.0, Y, X);
         sp_rotate(t, p, X, Y, bwd);
     } else {
         const double fx = a, fy = b, fz = c;
@@ -519,6 +533,42 @@ __host__ __device__ inline void sp_unkey(uint64_t k, int& x, int& y, int& z) {
     z = (int)sp_compact(k >> 2);
 }
 
+// Base-27 cell index of the ternary tree (morton3, multipole_method_functions.
+// cpp:49-86): level digit 9 z + 3 x + y, finest level least significant.
+__host__ __device__ inline uint64_t sp_tkey(int x, int y, int z, int levels) {
+    uint64_t k = 0, w = 1;
+    for (int i = 0; i < levels; i++) {
+        k += (uint64_t)(9 * (z % 3) + 3 * (x % 3) + (y % 3)) * w;
+        x /= 3;
+        y /= 3;
+        z /= 3;
+        w *= 27;
+    }
+    return k;
+}
+__host__ __device__ inline void sp_tunkey(uint64_t k, int& x, int& y, int& z) {
+    x = y = z = 0;
+    int w = 1;
+    while (k) {
+        const int d = (int)(k % 27);
+        k /= 27;
+        y += (d % 3) * w;
+        x += ((d / 3) % 3) * w;
+        z += (d / 9) * w;
+        w *= 3;
+    }
+}
+__host__ __device__ inline int sp_pow3(int l) {
+    int w = 1;
+    for (int i = 0; i < l; i++) w *= 3;
+    return w;
+}
+// v in [-w, 2w) wrapped into [0, w); *shift = the cell image it came from
+__device__ inline int sp_wrap(int v, int w, int& shift) {
+    shift = v < 0 ? -1 : (v >= w ? 1 : 0);
+    return v - shift * w;
+}
+
 // index of key among the nb sorted keys, -1 if absent
 __device__ inline int sp_find(const uint64_t* __restrict__ keys, int nb, uint64_t key) {
     int lo = 0, hi = nb;
@@ -541,7 +591,8 @@ struct SpGeom {
     int side;
     int bdv[3];
     int GL;
-    int rot;  // rotational symmetry: the quadrant map below the domain level
+    int rot;   // rotational symmetry: the quadrant map below the domain level
+    int tern;  // translational: ternary tree over one periodic cell, base-27 keys
 };
 
 __device__ inline int sp_f2i(float v) { return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)v : INT_MIN; }
@@ -565,7 +616,8 @@ __device__ inline bool sp_domain(const SpGeom& g, const float4& p, int nd[3]) {
 // (local index + 0.5) leaf sizes in double, stored as float.
 __device__ inline void sp_leaf_centre(const SpGeom& g, uint64_t key, float cen[3]) {
     int gc[3];
-    sp_unkey(key, gc[0], gc[1], gc[2]);
+    if (g.tern) sp_tunkey(key, gc[0], gc[1], gc[2]);
+    else sp_unkey(key, gc[0], gc[1], gc[2]);
     for (int a = 0; a < 3; a++) {
         const int nd = gc[a] / g.side, nl = gc[a] % g.side;
         const float dmin = sp_madd_unfused(g.bmin[a], (float)nd, g.root);
@@ -670,7 +722,7 @@ __global__ void k_sp_keys(const float4* __restrict__ P, const int* __restrict__ 
         if (nl >= g.side) nl--;
         gc[a] = nd[a] * g.side + nl;
     }
-    keys[s] = sp_key(gc[0], gc[1], gc[2]);
+    keys[s] = g.tern ? sp_tkey(gc[0], gc[1], gc[2], g.GL) : sp_key(gc[0], gc[1], gc[2]);
     vals[s] = i;
 }
 
@@ -682,9 +734,9 @@ __global__ void k_sp_gather(const float4* __restrict__ P, const int* __restrict_
     flag[a] = (a > 0 && keys[a] != keys[a - 1]) ? 1 : 0;
 }
 
-__global__ void k_sp_shift(const uint64_t* __restrict__ in, int n, uint64_t* __restrict__ out) {
+__global__ void k_sp_shift(const uint64_t* __restrict__ in, int n, int tern, uint64_t* __restrict__ out) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < n) out[b] = in[b] >> 3;
+    if (b < n) out[b] = tern ? in[b] / 27 : in[b] >> 3;
 }
 
 // ------------------------------------------------------------- operators --
@@ -782,8 +834,9 @@ __global__ __launch_bounds__(64) void k_sp_p2m(SphTab t, int p, int KP, SpGeom g
 }
 
 // M2M: each parent sums its children (key order) through the child-octant
-// operators; zop != 0: through the rotational operator of the child's z bit.
-__global__ __launch_bounds__(128) void k_sp_m2m(const double* __restrict__ Top, int KP, int nc2, int zop,
+// operators; zop != 0: through the rotational operator of the child's z bit;
+// tern: the 27 ternary children (parent key = key / 27, operator key % 27).
+__global__ __launch_bounds__(128) void k_sp_m2m(const double* __restrict__ Top, int KP, int nc2, int zop, int tern,
                                                 const uint64_t* __restrict__ ckeys, int ncb, const double* __restrict__ Mc,
                                                 const uint64_t* __restrict__ pkeys, double* __restrict__ Mp) {
     const int pb = blockIdx.x, row = threadIdx.x;
@@ -792,13 +845,15 @@ __global__ __launch_bounds__(128) void k_sp_m2m(const double* __restrict__ Top, 
     int lo = 0, hi = ncb;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if ((ckeys[mid] >> 3) < pk) lo = mid + 1;
+        if ((tern ? ckeys[mid] / 27 : ckeys[mid] >> 3) < pk) lo = mid + 1;
         else hi = mid;
     }
     double acc = 0.0;
     if (row < nc2)
-        for (int c = lo; c < ncb && (ckeys[c] >> 3) == pk; c++) {
-            const unsigned slot = zop ? (unsigned)((ckeys[c] >> 2) & 1u) : (unsigned)(ckeys[c] & 7u);
+        for (int c = lo; c < ncb && (tern ? ckeys[c] / 27 : ckeys[c] >> 3) == pk; c++) {
+            const unsigned slot = tern  ? (unsigned)(ckeys[c] % 27)
+                                  : zop ? (unsigned)((ckeys[c] >> 2) & 1u)
+                                        : (unsigned)(ckeys[c] & 7u);
             const double* T = Top + ((size_t)slot * KP + row) * KP;
             const double* m = Mc + (size_t)c * KP;
             double s = 0.0;
@@ -808,17 +863,18 @@ __global__ __launch_bounds__(128) void k_sp_m2m(const double* __restrict__ Top, 
     Mp[(size_t)pb * KP + row] = acc;
 }
 
-// L2L: each child takes its parent's local expansion through its octant's operator.
-__global__ __launch_bounds__(128) void k_sp_l2l(const double* __restrict__ Top, int KP, int nc2,
+// L2L: each child takes its parent's local expansion through its octant's
+// (tern: ternary child's) operator.
+__global__ __launch_bounds__(128) void k_sp_l2l(const double* __restrict__ Top, int KP, int nc2, int tern,
                                                 const uint64_t* __restrict__ pkeys, int npb, const double* __restrict__ Lp,
                                                 const uint64_t* __restrict__ ckeys, double* __restrict__ Lc) {
     const int cb = blockIdx.x, row = threadIdx.x;
     if (row >= KP) return;
     const uint64_t ck = ckeys[cb];
-    const int pb = sp_find(pkeys, npb, ck >> 3);
+    const int pb = sp_find(pkeys, npb, tern ? ck / 27 : ck >> 3);
     double s = 0.0;
     if (row < nc2 && pb >= 0) {
-        const double* T = Top + ((size_t)(ck & 7u) * KP + row) * KP;
+        const double* T = Top + ((size_t)(tern ? ck % 27 : ck & 7u) * KP + row) * KP;
         const double* l = Lp + (size_t)pb * KP;
         for (int k = 0; k < nc2; k++) s += T[k] * l[k];
     }
@@ -839,13 +895,17 @@ typedef double sp_d4 __attribute__((ext_vector_type(4)));
 // jd cells per axis in x, y is mapped to the real cell it images and its
 // multipole is gathered times i^(-m k) (m2l_kernel_rotations,
 // fmm_cpukernel_rotational.cpp:367-372); mord[c] = order m of real component c.
+// tern: the ternary tree of one periodic cell; offs holds the 702 offsets of
+// each of the 27 child positions and a source cell outside the 3^level cells
+// per axis is its periodic image (getInteractionListTranslations M2L_Subcell,
+// multipole_method_force_calculation_translational.cpp:1725-1834).
 // Work unit = (tile, chunk of the offset list): with nchunk > 1 each unit
 // writes its 16 x KP partial sums to part[chunk][tile] and k_sp_m2l_reduce adds
 // them to L in chunk order, so a level with few tiles still fills the chip.
 template <int RT>
 __global__ __launch_bounds__(64) void k_sp_m2l(const double* __restrict__ T, const double* __restrict__ M,
                                                double* __restrict__ L, const uint64_t* __restrict__ keys, int nb,
-                                               int level, int jd, const int* __restrict__ grp,
+                                               int level, int jd, int tern, int noff, const int* __restrict__ grp,
                                                const int4* __restrict__ tiles, int ntiles, int nchunk,
                                                double* __restrict__ part, const int4* __restrict__ offs,
                                                const signed char* __restrict__ mord,
@@ -853,20 +913,27 @@ __global__ __launch_bounds__(64) void k_sp_m2l(const double* __restrict__ T, con
     constexpr int KP = RT * 16, KS = KP / 4;
     const int lane = threadIdx.x, j = lane & 15, kq = lane >> 4;
     const int tile = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
-    const int o0 = chunk * kSpNoff / nchunk, o1 = (chunk + 1) * kSpNoff / nchunk;
+    const int o0 = chunk * noff / nchunk, o1 = (chunk + 1) * noff / nchunk;
     const int4 tl = tiles[tile];
     const int tb = j < tl.z ? grp[tl.y + j] : -1;
     int tx = 0, ty = 0, tz = 0;
-    if (tb >= 0) sp_unkey(keys[tb], tx, ty, tz);
-    const int W = 1 << level;
+    if (tb >= 0) {
+        if (tern) sp_tunkey(keys[tb], tx, ty, tz);
+        else sp_unkey(keys[tb], tx, ty, tz);
+    }
+    const int W = tern ? sp_pow3(level) : 1 << level;
     sp_d4 acc[RT];
     for (int r = 0; r < RT; r++) acc[r] = (sp_d4){0.0, 0.0, 0.0, 0.0};
-    const int4* od = offs + tl.x * kSpNoff;
+    const int4* od = offs + tl.x * noff;
     unsigned long long pairs = 0;
     for (int o = o0; o < o1; o++) {
         const int4 d = od[o];
         int sb = -1, quad = 0;
-        if (tb >= 0) {
+        if (tb >= 0 && tern) {
+            int e0, e1, e2;
+            const int sx = sp_wrap(tx + d.x, W, e0), sy = sp_wrap(ty + d.y, W, e1), sz = sp_wrap(tz + d.z, W, e2);
+            sb = sp_find(keys, nb, sp_tkey(sx, sy, sz, level));
+        } else if (tb >= 0) {
             int sx = tx + d.x, sy = ty + d.y;
             const int sz = tz + d.z;
             int lim = W;
@@ -930,8 +997,22 @@ __global__ void k_sp_m2l_reduce(const double* __restrict__ part, int ntiles, int
 // The 27 neighbour leaves of leaf b in the reference's order (x, then y, then
 // z): 4 * leaf + quadrant (the quadrant map of the rotational lists, leaf
 // window of side cells per axis; 0 without symmetry), -1 where absent.
+// tern: 27 * leaf + image code (sx + 1) 9 + (sy + 1) 3 + (sz + 1) of the
+// periodic wrap (getInteractionListTranslations P2P, multipole_method_force_
+// calculation_translational.cpp:1217-1334).
 __device__ inline void sp_neighbours(const SpGeom& g, const uint64_t* __restrict__ keys, int nb, int b, int* nbr) {
     int ix, iy, iz;
+    if (g.tern) {
+        sp_tunkey(keys[b], ix, iy, iz);
+        for (int k = 0; k < 27; k++) {
+            int e0, e1, e2;
+            const int jx = sp_wrap(ix + k / 9 - 1, g.side, e0), jy = sp_wrap(iy + (k / 3) % 3 - 1, g.side, e1),
+                      jz = sp_wrap(iz + k % 3 - 1, g.side, e2);
+            const int f = sp_find(keys, nb, sp_tkey(jx, jy, jz, g.GL));
+            nbr[k] = f >= 0 ? 27 * f + (e0 + 1) * 9 + (e1 + 1) * 3 + (e2 + 1) : -1;
+        }
+        return;
+    }
     sp_unkey(keys[b], ix, iy, iz);
     const int W = 1 << g.GL;
     for (int k = 0; k < 27; k++) {
@@ -962,7 +1043,7 @@ __global__ void k_sp_unit_counts(SpGeom g, const uint64_t* __restrict__ keys, in
     int nnb = 0;
     for (int k = 0; k < 27; k++)
         if (nbr[k] >= 0) {
-            near += count[nbr[k] >> 2];
+            near += count[g.tern ? nbr[k] / 27 : nbr[k] >> 2];
             nnb++;
         }
     atomicAdd(ctr + 1, (unsigned long long)nnb);
@@ -992,7 +1073,9 @@ __global__ void k_sp_unit_fill(int nb, const int* __restrict__ ucnt, const int* 
 // FP64 (float displacement, as the reference forms it) and adds K q / m times
 // the leaf's sum.  A leaf imaged into quadrant k has its sources rotated by
 // k pi / 2 as direct_rotational rotates them (direct_force_method_rotational.
-// cpp:59-75: the float products by cos and sin in {0, 1, -1} are exact).
+// cpp:59-75: the float products by cos and sin in {0, 1, -1} are exact); a
+// periodic image leaf shifts the target by -shift * cell in float as
+// direct_translational forms it (direct_force_method_translational.cpp:324-345).
 // near[c * ns + a] for the source at key-order position a (status 1 only).
 __global__ __launch_bounds__(64) void k_sp_p2p(SpGeom g, double kc, const uint64_t* __restrict__ keys, int nb,
                                                const int* __restrict__ first, const int* __restrict__ count,
@@ -1018,7 +1101,11 @@ __global__ __launch_bounds__(64) void k_sp_p2p(SpGeom g, double kc, const uint64
     double tot[3] = {0, 0, 0};
     for (int k = 0; k < 27; k++) {
         if (nbr[k] < 0) continue;
-        const int sb = nbr[k] >> 2, quad = nbr[k] & 3;
+        const int sb = g.tern ? nbr[k] / 27 : nbr[k] >> 2, quad = g.tern ? 0 : nbr[k] & 3;
+        const int code = g.tern ? nbr[k] % 27 : 13;
+        const float ox = (float)(code / 9 - 1) * g.root, oy = (float)((code / 3) % 3 - 1) * g.root,
+                    oz = (float)(code % 3 - 1) * g.root;
+        const float px = pi.x - ox, py = pi.y - oy, pz = pi.z - oz;
         double ai[3] = {0, 0, 0};
         const int e0 = first[sb], e1 = first[sb] + count[sb];
         for (int s0 = e0; s0 < e1; s0 += kSpP2pTile) {
@@ -1049,7 +1136,7 @@ __global__ __launch_bounds__(64) void k_sp_p2p(SpGeom g, double kc, const uint64
 #pragma unroll 4
                 for (int e = 0; e < n; e++) {
                     const float4 pj = tile[e];
-                    const double d0 = (double)(pi.x - pj.x), d1 = (double)(pi.y - pj.y), d2 = (double)(pi.z - pj.z);
+                    const double d0 = (double)(px - pj.x), d1 = (double)(py - pj.y), d2 = (double)(pz - pj.z);
                     const double invD = rsqrt_f64(d0 * d0 + d1 * d1 + d2 * d2 + kSoftening);
                     const double invD3 = (double)pj.w * invD * invD * invD;
                     ai[0] += d0 * invD3;
@@ -1151,11 +1238,11 @@ __global__ void k_sp_zero_i(int* __restrict__ a, size_t n) {
     if (i < n) a[i] = 0;
 }
 
-__global__ void k_sp_octants(const uint64_t* __restrict__ keys, int nb, unsigned* __restrict__ okey,
+__global__ void k_sp_octants(const uint64_t* __restrict__ keys, int nb, int tern, unsigned* __restrict__ okey,
                              int* __restrict__ oval, int* __restrict__ ocnt) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    const unsigned o = (unsigned)(keys[b] & 7u);
+    const unsigned o = tern ? (unsigned)(keys[b] % 27) : (unsigned)(keys[b] & 7u);
     okey[b] = o;
     oval[b] = b;
     atomicAdd(&ocnt[o], 1);
@@ -1238,21 +1325,44 @@ void sp_offsets(std::vector<int4>& offs) {
     }
 }
 
+// The 702 offsets (source - target) of a ternary cell at child position c
+// (base-27 digit, c = 9 z + 3 x + y), in the list order of M2L_Subcell
+// (parent neighbour, then child; x, y, z), each with the operator slot of
+// target - source in [-5, 5]^3.
+void sp_offsets_tern(std::vector<int4>& offs) {
+    offs.clear();
+    for (int o = 0; o < 27; o++) {
+        const int c[3] = {(o / 3) % 3, o % 3, o / 9};
+        for (int ex = -1; ex <= 1; ex++)
+            for (int ey = -1; ey <= 1; ey++)
+                for (int ez = -1; ez <= 1; ez++)
+                    for (int qx = 0; qx < 3; qx++)
+                        for (int qy = 0; qy < 3; qy++)
+                            for (int qz = 0; qz < 3; qz++) {
+                                const int dx = 3 * ex + qx - c[0], dy = 3 * ey + qy - c[1], dz = 3 * ez + qz - c[2];
+                                if (std::abs(dx) <= 1 && std::abs(dy) <= 1 && std::abs(dz) <= 1) continue;
+                                const int slot = (-dx + 5) * 121 + (-dy + 5) * 11 + (-dz + 5);
+                                offs.push_back(make_int4(dx, dy, dz, slot));
+                            }
+    }
+}
+
 template <int RT>
 void sp_launch_m2l(hipStream_t s, int ntiles, int nchunk, double* part, const double* T, const double* M, double* L,
-                   const uint64_t* keys, int nb, int level, int jd, const int* grp, const int4* tiles,
-                   const int4* offs, const signed char* mord, unsigned long long* ctr) {
-    hipLaunchKernelGGL(k_sp_m2l<RT>, dim3(ntiles * nchunk), dim3(64), 0, s, T, M, L, keys, nb, level, jd, grp, tiles,
-                       ntiles, nchunk, part, offs, mord, ctr);
+                   const uint64_t* keys, int nb, int level, int jd, int tern, int noff, const int* grp,
+                   const int4* tiles, const int4* offs, const signed char* mord, unsigned long long* ctr) {
+    hipLaunchKernelGGL(k_sp_m2l<RT>, dim3(ntiles * nchunk), dim3(64), 0, s, T, M, L, keys, nb, level, jd, tern, noff,
+                       grp, tiles, ntiles, nchunk, part, offs, mord, ctr);
     if (nchunk > 1)
         hipLaunchKernelGGL(k_sp_m2l_reduce, dim3(sp_bdiv((long)ntiles * 16 * RT * 16, 256)), dim3(256), 0, s, part,
                            ntiles, nchunk, RT * 16, tiles, grp, L);
 }
 
 // Evaluates targets (status 1) in [i0, i1) from the sources (status 1 or 2) in
-// the same range; out[c * m + i - i0] for m = i1 - i0.  rot: the rotational FMM.
+// the same range; out[c * m + i - i0] for m = i1 - i0.  rot: the rotational FMM;
+// cell > 0: the translational FMM over the periodic cube of that side.
 int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i0, int i1, int p, int forced,
-             int rot, double* out) {
+             int rot, float cell, double* out) {
     hipStream_t stream = h->stream;
     const int m = i1 - i0;
     h->sph_have_stats = false;
@@ -1266,15 +1376,15 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     const int RT = KP / 16;
     // sources and their float bounds
     SHP_WS(flags, char, h, WS_FLAGS, m);
-    SHP_WS(misc, int, h, WS_COUNTS, 64);
+    SHP_WS(misc, int, h, WS_COUNTS, 128);
     int* nsel = misc;                                       // [0]
     int* dctr = misc + 2;                                   // [2] outside, [3] largest domain count
     int* nruns = misc + 4;                                  // [4]
-    int* ocnt = misc + 8;                                   // [8..15]
+    int* ocnt = misc + 64;                                  // [64..90]: boxes per child position
     float* box6 = (float*)(misc + 16);                      // [16..21]
     unsigned long long* pctr = (unsigned long long*)(misc + 24);  // [24..29]: M2L pairs, P2P box and ion pairs
     hipLaunchKernelGGL(k_sp_flags, dim3(sp_bdiv(m, 256)), dim3(256), 0, stream, st, i0, m, flags);
-    hipLaunchKernelGGL(k_sp_zero_i, dim3(1), dim3(64), 0, stream, misc, (size_t)64);
+    hipLaunchKernelGGL(k_sp_zero_i, dim3(1), dim3(128), 0, stream, misc, (size_t)128);
     // WS_SPH_KEYS: keys in/out (2 m u64), vals in/out (2 m int), selected sources (m int)
     char* kb = static_cast<char*>(h->get(WS_SPH_KEYS, (size_t)m * (2 * sizeof(uint64_t) + 3 * sizeof(int))));
     if (!kb) return SHP_ERR_NOMEM;
@@ -1302,7 +1412,17 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     if (ns == 0) return SHP_OK;
     // the reference's domain grid (computeDomainBounds, host)
     SpGeom g{};
-    {
+    const int tern = cell > 0 ? 1 : 0;
+    if (tern) {
+        // computeDomainBoundsTranslations (domain_bounds_functions.cpp:637-661): one
+        // domain, boxMin = -rootBoxSize / 2 on every periodic axis
+        for (int a = 0; a < 3; a++) {
+            g.bmin[a] = -cell / 2;
+            g.bdv[a] = 1;
+        }
+        g.root = cell;
+        g.tern = 1;
+    } else {
         const float seed[3] = {seed4.x, seed4.y, seed4.z};
         float dims[3], minO[3], maxO[3];
         domain_bounds_host(bb, bb + 3, seed, ns, 0, rot, nullptr, g.bmin, dims, g.bdv, minO, maxO);
@@ -1337,24 +1457,32 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     int K = 0;
     while ((1 << K) < bmax) K++;
     const int GL = K + maxLevel;
-    if (maxLevel < 2 || GL > 21) {
+    if (tern && (maxLevel < 1 || maxLevel > 6)) {
+        set_error("shp_fmm_sph: translational maxLevel %d (the reference's ternary tree: 1..6; 0 is its "
+                  "supercell-only path, 7 overflows its int box index)", maxLevel);
+        return SHP_ERR_UNSUPPORTED;
+    }
+    if (!tern && (maxLevel < 2 || GL > 21)) {
         set_error("shp_fmm_sph: maxLevel %d with %d levels above the domains (need 2 <= maxLevel, total <= 21)",
                   maxLevel, K);
         return SHP_ERR_UNSUPPORTED;
     }
-    g.side = 1 << maxLevel;
+    // levels lmin .. GL carry expansions: the octree from 2, the ternary tree from 1
+    const int lmin = tern ? 1 : 2;
+    g.side = tern ? sp_pow3(maxLevel) : 1 << maxLevel;
     g.bsf = g.root / (float)g.side;
-    g.bsd = (double)g.root / g.side;
+    g.bsd = tern ? (double)g.root / std::pow(3.0, maxLevel) : (double)g.root / g.side;
     g.GL = GL;
     // keys, sort, leaf boxes
     const dim3 gs(sp_bdiv(ns, 256)), bs(256);
     hipLaunchKernelGGL(k_sp_keys, gs, bs, 0, stream, P, sel, ns, g, kin, vin);
     size_t t1 = 0, t2 = 0, t3 = 0;
-    SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, kin, kout, vin, vout, ns, 0, 3 * GL, stream));
+    const int kbits = tern ? 5 * GL : 3 * GL;  // 27^GL < 2^(5 GL)
+    SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, kin, kout, vin, vout, ns, 0, kbits, stream));
     SHP_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, t2, kout, kin, vin, nruns, ns, stream));
     SHP_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, t3, vin, vout, ns, stream));
     SHP_WS(cub2, char, h, WS_CUB, std::max(t1, std::max(t2, t3)));
-    SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub2, t1, kin, kout, vin, vout, ns, 0, 3 * GL, stream));
+    SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub2, t1, kin, kout, vin, vout, ns, 0, kbits, stream));
     // WS_SPH_PART: sorted sources (float4), leaf box of each, run flags/counts and starts
     char* pb = static_cast<char*>(h->get(WS_SPH_PART, (size_t)ns * (sizeof(float4) + 3 * sizeof(int)) + 64));
     if (!pb) return SHP_ERR_NOMEM;
@@ -1382,10 +1510,11 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     // level l occupies lk[loff[l] .. loff[l] + nbl[l]); leaf level first
     loff[GL] = 0;
     long used = nrun;
-    for (int l = GL - 1; l >= 2; l--) {
+    for (int l = GL - 1; l >= lmin; l--) {
         uint64_t* child = lk + loff[l + 1];
         uint64_t* tmp = kin;  // free now
-        hipLaunchKernelGGL(k_sp_shift, dim3(sp_bdiv(nbl[l + 1], 256)), dim3(256), 0, stream, child, nbl[l + 1], tmp);
+        hipLaunchKernelGGL(k_sp_shift, dim3(sp_bdiv(nbl[l + 1], 256)), dim3(256), 0, stream, child, nbl[l + 1], tern,
+                           tmp);
         SHP_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(cub3, t2, tmp, lk + used, vin, nruns, nbl[l + 1], stream));
         SHP_HIP_CHECK(hipMemcpyAsync(&nrun, nruns, sizeof(int), hipMemcpyDeviceToHost, stream));
         SHP_HIP_CHECK(hipStreamSynchronize(stream));
@@ -1396,7 +1525,7 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     // expansions: M and L of every level, KP doubles per box
     std::vector<long> eoff(GL + 1, 0);
     long etot = 0;
-    for (int l = 2; l <= GL; l++) {
+    for (int l = lmin; l <= GL; l++) {
         eoff[l] = etot;
         etot += (long)nbl[l] * KP;
     }
@@ -1404,23 +1533,27 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     if (!E) return SHP_ERR_NOMEM;
     double* Mx = E;
     double* Lx = E + etot;
-    // operators: 8 M2M, 8 L2L, the 343 M2L slots of one level, 2 rotational M2M
-    const int nops = 16 + kSpOps + 2;
+    // operators: nch M2M, nch L2L (8 octants or 27 ternary children), the M2L
+    // slots of one level (offsets in [-3, 3]^3 or [-5, 5]^3), 2 rotational M2M
+    const int nch = tern ? 27 : 8;
+    const int nslot = tern ? 1331 : kSpOps;
+    const int nops = 2 * nch + nslot + 2;
     double* Tops = static_cast<double*>(h->get(WS_SPH_OPS, sizeof(double) * (size_t)nops * KP * KP));
     if (!Tops) return SHP_ERR_NOMEM;
     // descriptors and octant offsets (host, uploaded with the tiles)
     std::vector<int4> desc(nops);
-    for (int o = 0; o < 8; o++) {
-        desc[o] = make_int4(0, o, 0, 0);
-        desc[8 + o] = make_int4(1, o, 0, 0);
+    for (int o = 0; o < nch; o++) {
+        desc[o] = make_int4(tern ? 4 : 0, o, 0, 0);
+        desc[nch + o] = make_int4(tern ? 5 : 1, o, 0, 0);
     }
-    for (int s = 0; s < kSpOps; s++) {
-        const int dx = s / 49 - 3, dy = (s / 7) % 7 - 3, dz = s % 7 - 3;
+    const int sw = tern ? 11 : 7, sh = sw / 2;
+    for (int s = 0; s < nslot; s++) {
+        const int dx = s / (sw * sw) - sh, dy = (s / sw) % sw - sh, dz = s % sw - sh;
         const bool far = std::abs(dx) > 1 || std::abs(dy) > 1 || std::abs(dz) > 1;
-        desc[16 + s] = far ? make_int4(2, dx, dy, dz) : make_int4(-1, 0, 0, 0);
+        desc[2 * nch + s] = far ? make_int4(2, dx, dy, dz) : make_int4(-1, 0, 0, 0);
     }
-    desc[16 + kSpOps] = make_int4(3, 0, 0, 0);
-    desc[17 + kSpOps] = make_int4(3, 1, 0, 0);
+    desc[2 * nch + nslot] = make_int4(3, 0, 0, 0);
+    desc[2 * nch + nslot + 1] = make_int4(3, 1, 0, 0);
     // order m of each real component (the M2L gather's quadrant phase)
     std::vector<signed char> mord(KP, 0);
     for (int c = 0; c < nc2; c++) {
@@ -1430,11 +1563,13 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
         mord[c] = (signed char)(ci - n * (n + 1) / 2);
     }
     std::vector<int4> offs;
-    sp_offsets(offs);
-    // tiles: at most nb / 16 + 8 per level
+    if (tern) sp_offsets_tern(offs);
+    else sp_offsets(offs);
+    const int noff = tern ? 702 : kSpNoff;
+    // tiles: at most nb / 16 + one per child position per level
     int maxnb = 0;
-    for (int l = 2; l <= GL; l++) maxnb = std::max(maxnb, nbl[l]);
-    const int maxtiles = maxnb / 16 + 8;
+    for (int l = lmin; l <= GL; l++) maxnb = std::max(maxnb, nbl[l]);
+    const int maxtiles = maxnb / 16 + nch;
     char* gb = static_cast<char*>(h->get(WS_SPH_GRP, sizeof(int4) * ((size_t)nops + offs.size() + maxtiles) +
                                                           sizeof(int) * 4 * (size_t)maxnb + 256 + 128));
     if (!gb) return SHP_ERR_NOMEM;
@@ -1449,49 +1584,52 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     SHP_HIP_CHECK(hipMemcpyAsync(ddesc, desc.data(), sizeof(int4) * nops, hipMemcpyHostToDevice, stream));
     SHP_HIP_CHECK(hipMemcpyAsync(doffs, offs.data(), sizeof(int4) * offs.size(), hipMemcpyHostToDevice, stream));
     SHP_HIP_CHECK(hipMemcpyAsync(dmord, mord.data(), (size_t)KP, hipMemcpyHostToDevice, stream));
-    hipLaunchKernelGGL(k_sp_build_ops, dim3(16), dim3(128), 0, stream, S->tab, p, KP, ddesc, 1.0, 0, Tops);
-    double* Trot = Tops + (size_t)(16 + kSpOps) * KP * KP;
+    hipLaunchKernelGGL(k_sp_build_ops, dim3(2 * nch), dim3(128), 0, stream, S->tab, p, KP, ddesc, 1.0, 0, Tops);
+    double* Trot = Tops + (size_t)(2 * nch + nslot) * KP * KP;
     // the four-image M2M into the domains (level K) is used when it is a level of the tree
     const bool rot_root = rot && K >= 2;
     if (rot_root)
-        hipLaunchKernelGGL(k_sp_build_ops, dim3(2), dim3(128), 0, stream, S->tab, p, KP, ddesc + 16 + kSpOps, 1.0, 0,
-                           Trot);
+        hipLaunchKernelGGL(k_sp_build_ops, dim3(2), dim3(128), 0, stream, S->tab, p, KP, ddesc + 2 * nch + nslot, 1.0,
+                           0, Trot);
     // P2M, M2M
     hipLaunchKernelGGL(k_sp_p2m, dim3(nbl[GL]), dim3(64), 0, stream, S->tab, p, KP, g, lk + loff[GL], lfirst, lcnt,
                        sp4, Mx + eoff[GL]);
-    for (int l = GL - 1; l >= 2; l--) {
+    for (int l = GL - 1; l >= lmin; l--) {
         const bool zop = rot_root && l == K;
-        hipLaunchKernelGGL(k_sp_m2m, dim3(nbl[l]), dim3(128), 0, stream, zop ? Trot : Tops, KP, nc2, zop ? 1 : 0,
+        hipLaunchKernelGGL(k_sp_m2m, dim3(nbl[l]), dim3(128), 0, stream, zop ? Trot : Tops, KP, nc2, zop ? 1 : 0, tern,
                            lk + loff[l + 1], nbl[l + 1], Mx + eoff[l + 1], lk + loff[l], Mx + eoff[l]);
     }
     // downward pass
-    double* Tm2l = Tops + (size_t)16 * KP * KP;
+    double* Tm2l = Tops + (size_t)2 * nch * KP * KP;
     std::vector<int4> tiles;
-    for (int l = 2; l <= GL; l++) {
+    for (int l = lmin; l <= GL; l++) {
         const int nb = nbl[l];
-        if (l == 2) {
+        if (l == lmin) {
             hipLaunchKernelGGL(k_sp_zero, dim3(sp_bdiv((long)nb * KP, 256)), dim3(256), 0, stream, Lx + eoff[l],
                                (size_t)nb * KP);
         } else {
-            hipLaunchKernelGGL(k_sp_l2l, dim3(nb), dim3(128), 0, stream, Tops + (size_t)8 * KP * KP, KP, nc2,
+            hipLaunchKernelGGL(k_sp_l2l, dim3(nb), dim3(128), 0, stream, Tops + (size_t)nch * KP * KP, KP, nc2, tern,
                                lk + loff[l - 1], nbl[l - 1], Lx + eoff[l - 1], lk + loff[l], Lx + eoff[l]);
         }
-        const double hl = (double)g.root * std::ldexp(1.0, K - l);  // box size at level l
-        hipLaunchKernelGGL(k_sp_build_ops, dim3(kSpOps), dim3(128), 0, stream, S->tab, p, KP, ddesc + 16, hl, 1, Tm2l);
-        // boxes grouped by child octant (stable), tiles of 16
-        hipLaunchKernelGGL(k_sp_zero_i, dim3(1), dim3(64), 0, stream, ocnt, (size_t)8);
-        hipLaunchKernelGGL(k_sp_octants, dim3(sp_bdiv(nb, 256)), dim3(256), 0, stream, lk + loff[l], nb, okey, oval,
-                           ocnt);
+        // box size at level l (m2l_kernel_translations: rootBoxSize / 3^l)
+        const double hl = tern ? (double)g.root / std::pow(3.0, l) : (double)g.root * std::ldexp(1.0, K - l);
+        hipLaunchKernelGGL(k_sp_build_ops, dim3(nslot), dim3(128), 0, stream, S->tab, p, KP, ddesc + 2 * nch, hl, 1,
+                           Tm2l);
+        // boxes grouped by child position (stable), tiles of 16
+        hipLaunchKernelGGL(k_sp_zero_i, dim3(1), dim3(64), 0, stream, ocnt, (size_t)nch);
+        hipLaunchKernelGGL(k_sp_octants, dim3(sp_bdiv(nb, 256)), dim3(256), 0, stream, lk + loff[l], nb, tern, okey,
+                           oval, ocnt);
+        const int obits = tern ? 5 : 3;
         size_t t5 = 0;
-        SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, t5, okey, okey2, oval, grp, nb, 0, 3, stream));
+        SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, t5, okey, okey2, oval, grp, nb, 0, obits, stream));
         SHP_WS(cub4, char, h, WS_CUB, t5);
-        SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub4, t5, okey, okey2, oval, grp, nb, 0, 3, stream));
-        int oc[8];
-        SHP_HIP_CHECK(hipMemcpyAsync(oc, ocnt, sizeof(oc), hipMemcpyDeviceToHost, stream));
+        SHP_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub4, t5, okey, okey2, oval, grp, nb, 0, obits, stream));
+        int oc[27];
+        SHP_HIP_CHECK(hipMemcpyAsync(oc, ocnt, sizeof(int) * nch, hipMemcpyDeviceToHost, stream));
         SHP_HIP_CHECK(hipStreamSynchronize(stream));
         tiles.clear();
         int start = 0;
-        for (int o = 0; o < 8; o++) {
+        for (int o = 0; o < nch; o++) {
             for (int q = 0; q < oc[o]; q += 16) tiles.push_back(make_int4(o, start + q, std::min(16, oc[o] - q), 0));
             start += oc[o];
         }
@@ -1500,21 +1638,21 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
         const int nt = (int)tiles.size();
         const uint64_t* keys = lk + loff[l];
         const int jd = (rot && l > K) ? (1 << (l - K)) : 0;  // the quadrant window below the domains
-        // offset chunks per tile: about 8 waves per CU over the level, at most 27 (7 offsets each)
-        const int nch = std::max(1, std::min(27, (8 * h->num_cus + nt - 1) / nt));
+        // offset chunks per tile: about 8 waves per CU over the level, at most 27
+        const int nck = std::max(1, std::min(27, (8 * h->num_cus + nt - 1) / nt));
         double* part = nullptr;
-        if (nch > 1) {
-            part = static_cast<double*>(h->get(WS_SPH_M2LP, sizeof(double) * (size_t)nch * nt * 16 * KP));
+        if (nck > 1) {
+            part = static_cast<double*>(h->get(WS_SPH_M2LP, sizeof(double) * (size_t)nck * nt * 16 * KP));
             if (!part) return SHP_ERR_NOMEM;
         }
         switch (RT) {
-        case 1: sp_launch_m2l<1>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
-        case 2: sp_launch_m2l<2>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
-        case 3: sp_launch_m2l<3>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
-        case 4: sp_launch_m2l<4>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
-        case 5: sp_launch_m2l<5>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
-        case 6: sp_launch_m2l<6>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
-        default: sp_launch_m2l<7>(stream, nt, nch, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, grp, dtiles, doffs, dmord, pctr); break;
+        case 1: sp_launch_m2l<1>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
+        case 2: sp_launch_m2l<2>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
+        case 3: sp_launch_m2l<3>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
+        case 4: sp_launch_m2l<4>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
+        case 5: sp_launch_m2l<5>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
+        case 6: sp_launch_m2l<6>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
+        default: sp_launch_m2l<7>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
         }
         // the tile list is host memory reused by the next level: wait for the copy
         SHP_HIP_CHECK(hipStreamSynchronize(stream));
@@ -1559,7 +1697,7 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
     SHP_HIP_CHECK(hipMemcpyAsync(pc, pctr, sizeof(pc), hipMemcpyDeviceToHost, stream));
     SHP_HIP_CHECK(hipStreamSynchronize(stream));
     long boxes = 0;
-    for (int l = 2; l <= GL; l++) boxes += nbl[l];
+    for (int l = lmin; l <= GL; l++) boxes += nbl[l];
     double* sts = h->sph_stats;
     sts[0] = maxLevel;
     sts[1] = K;
@@ -1602,10 +1740,29 @@ int shp_fmm_sph_host(shp_handle h, const float* pos4, const float* vel4, const f
     const int stype = sym ? sym->type : SHP_SYM_NONE;
     SHP_REQUIRE(stype == SHP_SYM_NONE || stype == SHP_SYM_ROTATIONAL || stype == SHP_SYM_TRANSLATIONAL,
                 "shp_fmm_sph_host: symmetry type %d", stype);
-    if (stype == SHP_SYM_TRANSLATIONAL || (stype == SHP_SYM_ROTATIONAL && sym->nsec != 4)) {
-        set_error("shp_fmm_sph_host: the reference operators cover no symmetry and the rotational one with 4 "
-                  "sectors (got type %d, %d sectors)", stype, stype == SHP_SYM_ROTATIONAL ? sym->nsec : 0);
+    if (stype == SHP_SYM_ROTATIONAL && sym->nsec != 4) {
+        set_error("shp_fmm_sph_host: the reference's rotational FMM has 4 sectors (got %d)", sym->nsec);
         return SHP_ERR_UNSUPPORTED;
+    }
+    float cell = 0.0f;
+    if (stype == SHP_SYM_TRANSLATIONAL) {
+        // the reference's translational FMM: one cubic periodic cell on all three
+        // axes (rootBoxSize asserted equal on the periodic axes, domain_bounds_
+        // functions.cpp:600-610) and the 27 images of shifts {-1, 0, 1}^3
+        cell = (float)sym->domain[0];
+        bool ok = cell > 0 && (float)sym->domain[1] == cell && (float)sym->domain[2] == cell && sym->nimg == 27 &&
+                  sym->lattice != nullptr;
+        int seen[27] = {0};
+        for (int k = 0; ok && k < 27; k++) {
+            const int* q = sym->lattice + 3 * k;
+            if (std::abs(q[0]) > 1 || std::abs(q[1]) > 1 || std::abs(q[2]) > 1) ok = false;
+            else if (seen[(q[0] + 1) * 9 + (q[1] + 1) * 3 + q[2] + 1]++) ok = false;
+        }
+        if (!ok) {
+            set_error("shp_fmm_sph_host: the reference's translational FMM needs a cubic cell on all three axes and "
+                      "the 27 lattice points {-1, 0, 1}^3 (got %d images)", sym->nimg);
+            return SHP_ERR_UNSUPPORTED;
+        }
     }
     const int m = i1 - i0;
     if (m == 0) return SHP_OK;
@@ -1619,7 +1776,8 @@ int shp_fmm_sph_host(shp_handle h, const float* pos4, const float* vel4, const f
     SHP_HIP_CHECK(hipMemcpyAsync(dvel, vel4, sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
     SHP_HIP_CHECK(hipMemcpyAsync(dacc, acc4, sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
     hipLaunchKernelGGL(k_sp_from_aos, dim3(sp_bdiv(n, 256)), dim3(256), 0, h->stream, dvel, dacc, n, ints, ints + n);
-    int rc = sph_eval(h, dpos, ints, ints + n, i0, i1, order, max_level, stype == SHP_SYM_ROTATIONAL ? 1 : 0, dout);
+    int rc = sph_eval(h, dpos, ints, ints + n, i0, i1, order, max_level, stype == SHP_SYM_ROTATIONAL ? 1 : 0, cell,
+                      dout);
     if (rc) return rc;
     hipLaunchKernelGGL(k_sp_interleave, dim3(sp_bdiv(m, 256)), dim3(256), 0, h->stream, dout, m, dout + 3 * (size_t)m);
     SHP_HIP_CHECK(hipMemcpyAsync(out3, dout + 3 * (size_t)m, sizeof(double) * 3 * (size_t)m, hipMemcpyDeviceToHost,

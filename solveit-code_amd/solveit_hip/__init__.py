@@ -477,8 +477,10 @@ class Engine:
         """The reference FMM's own operators (rotation-based spherical harmonics,
         FP64) on host AoS float32 arrays; returns the (i1-i0, 3) float64
         acceleration.  max_level -2: the reference's setOptimumLevel choice.
-        symmetry: None or make_symmetry("rotational", 4) (the reference's
-        rotational FMM)."""
+        symmetry: None, make_symmetry("rotational", 4) (the reference's
+        rotational FMM) or make_symmetry("translational", lattice=the 27 points
+        {-1, 0, 1}^3, domain=(L, L, L)) (its translational FMM over the periodic
+        cell [-L/2, L/2)^3, P2P included)."""
         sym = symmetry if symmetry is not None else make_symmetry("none")
         pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
         vel4 = np.ascontiguousarray(vel4, dtype=np.float32)

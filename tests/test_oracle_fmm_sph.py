@@ -283,3 +283,97 @@ def test_rotational_status_rules_and_range():
     tgt = acc[i0:i1, 3] == 1.0
     assert np.all(a[~tgt] == 0.0)
     assert oracle.rel_l2(a[tgt], ref[tgt])[0] < 1e-5
+
+
+# ---- translational FMM (ternary tree over one periodic cell) ----
+
+LAT27 = np.array([(x, y, z) for x in (-1, 0, 1) for y in (-1, 0, 1) for z in (-1, 0, 1)], np.int32)
+
+
+def _cell(n, cell, seed, clump=0):
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(-cell / 2, cell / 2, (n, 3))
+    if clump:
+        pos[:clump, :3] = cell / 2 - np.abs(rng.normal(0, cell / 40, (clump, 3)))
+    pos[:, 3] = np.where(rng.random(n) < 0.1, -1.0, 1.0)
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, 3] = rng.integers(0, 2, n)
+    acc = np.zeros((n, 4), np.float32)
+    acc[:, 3] = 1.0
+    return pos, vel, acc
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_translational_fmm_converges_to_27_image_direct(level):
+    """Every target of the translational FMM sees the sources of the 27 cells
+    at shifts {-1, 0, 1}^3 (near field through the wrapped P2P list, far field
+    through the wrapped M2L lists of every level): the error against the FP64
+    direct sum over those images falls with the order.  A wrong ternary shift
+    (rho, 3^-j), rotation index or list wrap stalls it."""
+    L = 2e-6
+    pos, vel, acc = _cell(1500, L, 5, clump=300)
+    acc[::7, 3] = 2.0
+    ref = oracle.direct_ref(pos, vel, acc, sym="translational", lattice=LAT27, domain=(L, L, L))
+    tgt = acc[:, 3] == 1.0
+    errs = []
+    for p in (4, 6, 8, 10):
+        out, info = oracle.fmm_sph_trans(pos, vel, acc, L, p=p, max_level=level, cr_trig=True)
+        errs.append(oracle.rel_l2(out[tgt], ref[tgt])[0])
+    assert info["max_level"] == level and info["leaves"] <= 27 ** level
+    assert all(b < a / 3 for a, b in zip(errs, errs[1:])), errs
+    assert errs[-1] < 5e-4, errs
+
+
+def test_translational_near_field_and_reference_p2p():
+    """near=False is the reference as it runs (its translational P2P passes the
+    thread's box range as the ion range and adds nothing); the difference to
+    near=True is the softened pair sum over the 27 wrapped neighbour leaves,
+    restated here by brute force from the leaf indices of mortonTranslations."""
+    L = np.float32(2e-6)
+    pos, vel, acc = _cell(400, float(L), 6, clump=80)
+    acc[::5, 3] = 2.0
+    acc[::13, 3] = 0.0
+    full, info = oracle.fmm_sph_trans(pos, vel, acc, float(L), p=6, max_level=2)
+    far, _ = oracle.fmm_sph_trans(pos, vel, acc, float(L), p=6, max_level=2, near=False)
+    side = 9
+    bmin = np.float32(-L / np.float32(2))
+    bs = np.float32(L / np.float32(side))
+    cellidx = np.minimum(((pos[:, :3] - bmin) / bs).astype(np.int64), side - 1)
+    src = (acc[:, 3] == 1.0) | (acc[:, 3] == 2.0)
+    kc = oracle.k_const()
+    near = np.zeros((pos.shape[0], 3))
+    for i in np.nonzero(acc[:, 3] == 1.0)[0]:
+        ac = kc * float(pos[i, 3]) / oracle.species_mass(int(vel[i, 3]))
+        for j in np.nonzero(src)[0]:
+            d = cellidx[j] - cellidx[i]
+            d = (d + 1) % side - 1  # periodic leaf distance in [-1, side - 2]
+            if np.any(np.abs(d) > 1):
+                continue
+            # image shift of the source leaf as the wrap of cellidx[i] + d finds it
+            s = np.floor_divide(cellidx[i] + d, side)
+            xi = pos[i, :3] - s.astype(np.float32) * L  # float, as direct_translational
+            dd = (xi - pos[j, :3]).astype(np.float64)
+            inv = 1.0 / math.sqrt(float(dd @ dd) + 6.4e-19)
+            near[i] += ac * float(pos[j, 3]) * inv ** 3 * dd
+    tgt = acc[:, 3] == 1.0
+    assert np.all(far[~tgt] == 0.0) and np.all(full[~tgt] == 0.0)
+    np.testing.assert_allclose(full[tgt] - far[tgt], near[tgt], rtol=1e-9, atol=1e-9 * np.abs(near).max())
+    assert oracle.rel_l2(far[tgt], full[tgt])[0] > 0.1
+
+
+def test_translational_level_and_unsupported():
+    L = 2e-6
+    pos, vel, acc = _cell(1500, L, 7)
+    _, info = oracle.fmm_sph_trans(pos, vel, acc, L, p=2)
+    assert info["max_level"] == 2 and (info["nd0"], info["nd1"], info["nd2"]) == (1, 1, 1)
+    _, info = oracle.fmm_sph_trans(pos[:150], vel[:150], acc[:150], L, p=2, max_level=1)
+    assert info["leaves"] <= 27 and info["boxes"] == info["leaves"]
+    with pytest.raises(ValueError):  # 150 sources: level 0, the supercell-only path
+        oracle.fmm_sph_trans(pos[:150], vel[:150], acc[:150], L, p=2)
+    with pytest.raises(ValueError):
+        oracle.fmm_sph_trans(pos, vel, acc, L, p=2, max_level=7)
+    out = pos.copy()
+    out[3, 1] = np.float32(L / 2)  # on the +y face: outside [-L/2, L/2)
+    with pytest.raises(ValueError):
+        oracle.fmm_sph_trans(out, vel, acc, L, p=2)
