@@ -895,6 +895,8 @@ typedef double sp_d4 __attribute__((ext_vector_type(4)));
 // jd cells per axis in x, y is mapped to the real cell it images and its
 // multipole is gathered times i^(-m k) (m2l_kernel_rotations,
 // fmm_cpukernel_rotational.cpp:367-372); mord[c] = order m of real component c.
+// cells (tern, may be null): box of each cell of the level, (x W + y) W + z,
+// -1 where empty; replaces the binary search of the level's keys.
 // tern: the ternary tree of one periodic cell; offs holds the 702 offsets of
 // each of the 27 child positions and a source cell outside the 3^level cells
 // per axis is its periodic image (getInteractionListTranslations M2L_Subcell,
@@ -908,7 +910,7 @@ __global__ __launch_bounds__(64) void k_sp_m2l(const double* __restrict__ T, con
                                                int level, int jd, int tern, int noff, const int* __restrict__ grp,
                                                const int4* __restrict__ tiles, int ntiles, int nchunk,
                                                double* __restrict__ part, const int4* __restrict__ offs,
-                                               const signed char* __restrict__ mord,
+                                               const signed char* __restrict__ mord, const int* __restrict__ cells,
                                                unsigned long long* __restrict__ ctr) {
     constexpr int KP = RT * 16, KS = KP / 4;
     const int lane = threadIdx.x, j = lane & 15, kq = lane >> 4;
@@ -932,7 +934,7 @@ __global__ __launch_bounds__(64) void k_sp_m2l(const double* __restrict__ T, con
         if (tb >= 0 && tern) {
             int e0, e1, e2;
             const int sx = sp_wrap(tx + d.x, W, e0), sy = sp_wrap(ty + d.y, W, e1), sz = sp_wrap(tz + d.z, W, e2);
-            sb = sp_find(keys, nb, sp_tkey(sx, sy, sz, level));
+            sb = cells ? cells[((size_t)sx * W + sy) * W + sz] : sp_find(keys, nb, sp_tkey(sx, sy, sz, level));
         } else if (tb >= 0) {
             int sx = tx + d.x, sy = ty + d.y;
             const int sz = tz + d.z;
@@ -1238,6 +1240,20 @@ __global__ void k_sp_zero_i(int* __restrict__ a, size_t n) {
     if (i < n) a[i] = 0;
 }
 
+__global__ void k_sp_fill_i(int* __restrict__ a, size_t n, int v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// cells[(x W + y) W + z] = b for every box b of a ternary level of W cells per axis.
+__global__ void k_sp_cells_set(const uint64_t* __restrict__ keys, int nb, int W, int* __restrict__ cells) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int x, y, z;
+    sp_tunkey(keys[b], x, y, z);
+    if (x < W && y < W && z < W) cells[((size_t)x * W + y) * W + z] = b;
+}
+
 __global__ void k_sp_octants(const uint64_t* __restrict__ keys, int nb, int tern, unsigned* __restrict__ okey,
                              int* __restrict__ oval, int* __restrict__ ocnt) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1350,9 +1366,10 @@ void sp_offsets_tern(std::vector<int4>& offs) {
 template <int RT>
 void sp_launch_m2l(hipStream_t s, int ntiles, int nchunk, double* part, const double* T, const double* M, double* L,
                    const uint64_t* keys, int nb, int level, int jd, int tern, int noff, const int* grp,
-                   const int4* tiles, const int4* offs, const signed char* mord, unsigned long long* ctr) {
+                   const int4* tiles, const int4* offs, const signed char* mord, const int* cells,
+                   unsigned long long* ctr) {
     hipLaunchKernelGGL(k_sp_m2l<RT>, dim3(ntiles * nchunk), dim3(64), 0, s, T, M, L, keys, nb, level, jd, tern, noff,
-                       grp, tiles, ntiles, nchunk, part, offs, mord, ctr);
+                       grp, tiles, ntiles, nchunk, part, offs, mord, cells, ctr);
     if (nchunk > 1)
         hipLaunchKernelGGL(k_sp_m2l_reduce, dim3(sp_bdiv((long)ntiles * 16 * RT * 16, 256)), dim3(256), 0, s, part,
                            ntiles, nchunk, RT * 16, tiles, grp, L);
@@ -1638,6 +1655,16 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
         const int nt = (int)tiles.size();
         const uint64_t* keys = lk + loff[l];
         const int jd = (rot && l > K) ? (1 << (l - K)) : 0;  // the quadrant window below the domains
+        // ternary levels of up to 243 cells per axis: a dense cell -> box table (<= 57 MB)
+        int* cells = nullptr;
+        if (tern && l <= 5) {
+            const int W = sp_pow3(l);
+            const size_t ncell = (size_t)W * W * W;
+            cells = static_cast<int*>(h->get(WS_SPH_CELLS, sizeof(int) * ncell));
+            if (!cells) return SHP_ERR_NOMEM;
+            hipLaunchKernelGGL(k_sp_fill_i, dim3(sp_bdiv((long)ncell, 256)), dim3(256), 0, stream, cells, ncell, -1);
+            hipLaunchKernelGGL(k_sp_cells_set, dim3(sp_bdiv(nb, 256)), dim3(256), 0, stream, keys, nb, W, cells);
+        }
         // offset chunks per tile: about 8 waves per CU over the level, at most 27
         const int nck = std::max(1, std::min(27, (8 * h->num_cus + nt - 1) / nt));
         double* part = nullptr;
@@ -1646,13 +1673,13 @@ int sph_eval(shp_handle h, const float4* P, const int* st, const int* spc, int i
             if (!part) return SHP_ERR_NOMEM;
         }
         switch (RT) {
-        case 1: sp_launch_m2l<1>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
-        case 2: sp_launch_m2l<2>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
-        case 3: sp_launch_m2l<3>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
-        case 4: sp_launch_m2l<4>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
-        case 5: sp_launch_m2l<5>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
-        case 6: sp_launch_m2l<6>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
-        default: sp_launch_m2l<7>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, pctr); break;
+        case 1: sp_launch_m2l<1>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, cells, pctr); break;
+        case 2: sp_launch_m2l<2>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, cells, pctr); break;
+        case 3: sp_launch_m2l<3>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, cells, pctr); break;
+        case 4: sp_launch_m2l<4>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, cells, pctr); break;
+        case 5: sp_launch_m2l<5>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, cells, pctr); break;
+        case 6: sp_launch_m2l<6>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, cells, pctr); break;
+        default: sp_launch_m2l<7>(stream, nt, nck, part, Tm2l, Mx + eoff[l], Lx + eoff[l], keys, nb, l, jd, tern, noff, grp, dtiles, doffs, dmord, cells, pctr); break;
         }
         // the tile list is host memory reused by the next level: wait for the copy
         SHP_HIP_CHECK(hipStreamSynchronize(stream));

@@ -124,6 +124,7 @@ enum WsSlot {
     WS_SPH_GRP,     // spherical FMM: boxes grouped by octant, M2L tiles
     WS_SPH_NEAR,    // spherical FMM: P2P sums per source in key order, P2P work units
     WS_SPH_M2LP,    // spherical FMM: M2L partial sums of the offset chunks
+    WS_SPH_CELLS,   // spherical FMM: cell -> box table of one ternary level
     WS_NSLOTS
 };
 
