@@ -54,6 +54,14 @@
  * so the reference's lists are consistent only for one domain per x and y;
  * other grids are not restated (-4).
  *
+ * Translational symmetry (orc_fmm_sph_trans; updateAccelerationTranslational,
+ * multipole_method_force_calculation_translational.cpp:9-331): one cubic cell
+ * periodic on all three axes, a ternary tree with base-27 keys, lists that
+ * wrap periodically, the ternary M2M / L2L shifts.  Pinned by convergence to
+ * the direct sum over the 27 images and by a brute-force near field; the
+ * supercell passes of several domains along a non-periodic axis are not
+ * restated.
+ *
  * Expansion storage: coefficient (n, m), 0 <= m <= n < p, at n(n+1)/2 + m,
  * complex double.  Morton bit order of the reference (multipole_method_
  * functions.cpp unmorton): y -> bit 3i, x -> bit 3i+1, z -> bit 3i+2.
